@@ -1,0 +1,51 @@
+/*
+ * ptk_oracle.h -- CPU restatements of the reference algorithms on the PTask hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  These functions are the checker: tests, the smoke check and the
+ * CPU-baseline leg of the benchmark call them to produce expected results.  Nothing in the
+ * product (ptask_amd/, include/) includes, links or loads them.
+ */
+#ifndef PTK_ORACLE_H
+#define PTK_ORACLE_H
+
+#include <stdint.h>
+
+#include "ptk_fdtd.h"
+#include "ptk_prng.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FDTD (oracle/fdtd.c) */
+void ptk_oracle_fdtd_hx_f32(const float *Hx, const float *Ey, const float *Ez, const FDTD_PARAMS *p, float *o);
+void ptk_oracle_fdtd_hy_f32(const float *Hy, const float *Ex, const float *Ez, const FDTD_PARAMS *p, float *o);
+void ptk_oracle_fdtd_hz_f32(const float *Hz, const float *Ex, const float *Ey, const FDTD_PARAMS *p, float *o);
+void ptk_oracle_fdtd_ex_f32(const float *Ex, const float *Hy, const float *Hz, const FDTD_PARAMS *p, float *o);
+void ptk_oracle_fdtd_ey_f32(const float *Ey, const float *Hx, const float *Hz, const FDTD_PARAMS *p, float *o);
+void ptk_oracle_fdtd_ez_f32(const float *Ez, const float *Hx, const float *Hy, const FDTD_PARAMS *p, float *o);
+void ptk_oracle_fdtd_hx_f64(const double *Hx, const double *Ey, const double *Ez, const FDTD_PARAMS *p, double *o);
+void ptk_oracle_fdtd_hy_f64(const double *Hy, const double *Ex, const double *Ez, const FDTD_PARAMS *p, double *o);
+void ptk_oracle_fdtd_hz_f64(const double *Hz, const double *Ex, const double *Ey, const FDTD_PARAMS *p, double *o);
+void ptk_oracle_fdtd_ex_f64(const double *Ex, const double *Hy, const double *Hz, const FDTD_PARAMS *p, double *o);
+void ptk_oracle_fdtd_ey_f64(const double *Ey, const double *Hx, const double *Hz, const FDTD_PARAMS *p, double *o);
+void ptk_oracle_fdtd_ez_f64(const double *Ez, const double *Hx, const double *Hy, const FDTD_PARAMS *p, double *o);
+void ptk_oracle_fdtd_run_f32(float *f[6], float *scratch[6], const FDTD_PARAMS *p, int steps);
+void ptk_oracle_fdtd_run_f64(double *f[6], double *scratch[6], const FDTD_PARAMS *p, int steps);
+void ptk_oracle_fdtd_run_ptrs_f32(float **f, float **scratch, const FDTD_PARAMS *p, int steps);
+void ptk_oracle_fdtd_run_ptrs_f64(double **f, double **scratch, const FDTD_PARAMS *p, int steps);
+void ptk_oracle_fdtd_init_field(float *a, int field, int Nx, int Ny, int Nz, uint64_t seed);
+
+/* Host-task ABI forms (void f(UINT, void**)), loaded by name by the HostAccelerator. */
+void HxComputation(unsigned int n, void **a);
+void HyComputation(unsigned int n, void **a);
+void HzComputation(unsigned int n, void **a);
+void ExComputation(unsigned int n, void **a);
+void EyComputation(unsigned int n, void **a);
+void EzComputation(unsigned int n, void **a);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTK_ORACLE_H */

@@ -1,0 +1,134 @@
+"""Parity of the six Yee task kernels (fdtd.hsaco) with the oracle, called through the C ABI.
+
+Bar: bit-exact against the binary32 oracle (the kernels perform the same IEEE operations in the
+same order); the <=1e-5 max relative error gate of the workload is asserted as well, and the
+binary64 distance is reported.  Covers several launch geometries (the reference's
+one-thread-per-cell CUDA geometry, the default column walk, odd shapes, a single block) and
+degenerate grids with a dimension of 1.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from ptask_amd import fdtd as FD
+from ptask_amd import paths
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mod(hip):
+    return hip.Module(paths.kernel_path("fdtd.hsaco"))
+
+
+def upload_fields(hip, F):
+    bufs = []
+    for a in F:
+        b = hip.DeviceBuffer(a.nbytes)
+        b.upload(a)
+        bufs.append(b)
+    return bufs
+
+
+def run_steps(hip, mod, F, p, steps, geometry):
+    nx, ny, nz = p.Nx, p.Ny, p.Nz
+    cur = upload_fields(hip, F)
+    nxt = [hip.DeviceBuffer(a.nbytes) for a in F]
+    s = hip.Stream()
+    for _ in range(steps):
+        for half in ((0, 1, 2), (3, 4, 5)):
+            for t in half:
+                name, own, a, b = FD.TASKS[t]
+                grid, block = geometry(*FD.extent(own, nx, ny, nz))
+                args = FD.pack_task_args(hip, cur[own].ptr, cur[a].ptr, cur[b].ptr, p,
+                                         nxt[own].ptr)
+                hip.launch(mod.function(name), grid, block, args, s)
+            for t in half:
+                cur[t], nxt[t] = nxt[t], cur[t]
+    s.synchronize()
+    out = []
+    for f, a in enumerate(F):
+        h = np.empty_like(a)
+        cur[f].download(h)
+        out.append(h)
+    s.destroy()
+    return out
+
+
+GEOMS = {
+    "default": FD.default_geometry,
+    "reference": FD.reference_geometry,
+    "odd": lambda d1, d2, d3: ((5, 2, 3), (64, 3, 1)),
+    "one_block": lambda d1, d2, d3: ((1, 1, 1), (64, 1, 1)),
+    "thin_slabs": lambda d1, d2, d3: FD.default_geometry(d1, d2, d3, block=128, target_slab=1),
+}
+
+
+def assert_parity(got, ref, ref64=None):
+    for f in range(6):
+        assert got[f].shape == ref[f].shape
+        if not np.array_equal(got[f].view(np.uint32), ref[f].view(np.uint32)):
+            bad = np.flatnonzero(got[f] != ref[f])
+            raise AssertionError(f"{FD.FIELDS[f]}: {bad.size} cells differ, first {bad[:5]}")
+        rel = np.abs(got[f] - ref[f]).max() / max(np.abs(ref[f]).max(), 1e-30)
+        assert rel <= 1e-5
+
+
+@pytest.mark.parametrize("geom", sorted(GEOMS))
+@pytest.mark.parametrize("shape", [(8, 8, 8), (13, 7, 5)])
+def test_six_task_step_bitexact(hip, mod, shape, geom):
+    nx, ny, nz = shape
+    print(f"case shape={shape} geom={geom}", flush=True)
+    p = oracle.make_params(nx, ny, nz, 0.5)
+    F = oracle.init_fields(nx, ny, nz, seed=100 + nx)
+    got = run_steps(hip, mod, F, FD.make_params(nx, ny, nz), 4, GEOMS[geom])
+    ref = oracle.fdtd_run(F, p, 4)
+    assert_parity(got, ref)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 5, 3), (6, 1, 2), (4, 3, 1), (2, 2, 2)])
+def test_degenerate_grids(hip, mod, shape):
+    nx, ny, nz = shape
+    print(f"case degenerate shape={shape}", flush=True)
+    p = oracle.make_params(nx, ny, nz, 0.5)
+    F = oracle.init_fields(nx, ny, nz, seed=7)
+    got = run_steps(hip, mod, F, FD.make_params(nx, ny, nz), 3, FD.default_geometry)
+    assert_parity(got, oracle.fdtd_run(F, p, 3))
+
+
+def test_params_bytes_match_oracle():
+    for shape in [(64, 64, 64), (384, 384, 384), (17, 33, 9)]:
+        a = FD.make_params(*shape)
+        b = oracle.make_params(*shape, 0.5)
+        assert bytes(a) == bytes(b)
+
+
+def test_long_run_64cubed_against_fp32_and_fp64(hip, mod):
+    """64^3 for 10 steps: the CPU configuration of the workload, checked at the gate and
+    reported against binary64."""
+    nx = ny = nz = 64
+    print("case 64^3 x 10 steps", flush=True)
+    p = oracle.make_params(nx, ny, nz, 0.5)
+    F = oracle.init_fields(nx, ny, nz, seed=2024)
+    got = run_steps(hip, mod, F, FD.make_params(nx, ny, nz), 10, FD.default_geometry)
+    ref = oracle.fdtd_run(F, p, 10)
+    assert_parity(got, ref)
+    ref64 = oracle.fdtd_run(F, p, 10, dtype=np.float64)
+    for f in range(6):
+        e64 = np.abs(got[f] - ref64[f]).max() / np.abs(ref64[f]).max()
+        print(f"{FD.FIELDS[f]} max rel err vs binary64: {e64:.3e}")
+        assert e64 < 1e-4
+
+
+def test_device_init_matches_host_init(hip, mod):
+    nx, ny, nz = 9, 6, 5
+    F = oracle.init_fields(nx, ny, nz, seed=0xABCDEF)
+    s = hip.Stream()
+    for f, a in enumerate(F):
+        b = hip.DeviceBuffer(a.nbytes)
+        args = hip.ArgPack().ptr(b.ptr).u64(a.size).i32(f).u64(0xABCDEF)
+        hip.launch(mod.function("FdtdInitField"), (3, 1, 1), (64, 1, 1), args, s)
+        s.synchronize()
+        h = np.empty_like(a)
+        b.download(h)
+        assert np.array_equal(h.view(np.uint32), a.view(np.uint32)), FD.FIELDS[f]
