@@ -1,0 +1,129 @@
+/* PTaskRuntime.h -- runtime-wide entry points: initialisation, factories and configuration
+ * (reference: ptask/PTaskRuntime.h).  Signatures follow the reference so application code
+ * compiles unchanged; configuration setters take effect before Initialize() unless noted. */
+#ifndef PTK_PTASKRUNTIME_H
+#define PTK_PTASKRUNTIME_H
+
+#include <string>
+
+#include "primitive_types.h"
+
+namespace PTask {
+
+class Accelerator;
+class Channel;
+class CompiledKernel;
+class Datablock;
+class DatablockTemplate;
+class Port;
+class Task;
+
+namespace Runtime {
+
+PTRESULT Initialize();
+BOOL IsInitialized();
+PTRESULT Terminate();
+
+/* ---- templates (reference PTaskRuntime.h:392-429) ---------------------------------------- */
+DatablockTemplate *GetDatablockTemplate(char *lpszType, unsigned int uiStride, unsigned int x,
+                                        unsigned int y, unsigned int z, bool bRecordStream = false,
+                                        bool bRaw = false);
+DatablockTemplate *GetDatablockTemplate(char *lpszTemplateName, unsigned int uiElementStride,
+                                        unsigned int uiElementsX, unsigned int uiElementsY,
+                                        unsigned int uiElementsZ, unsigned int uiPitch,
+                                        bool bIsRecordStream, bool bIsByteAddressable);
+DatablockTemplate *GetDatablockTemplate(char *lpszTemplateName, BUFFERDIMENSIONS *pBufferDims,
+                                        unsigned int uiNumBufferDims, bool bIsRecordStream,
+                                        bool bIsByteAddressable);
+DatablockTemplate *GetDatablockTemplate(char *lpszType, unsigned int uiStride,
+                                        PTASK_PARM_TYPE pttype);
+DatablockTemplate *GetDatablockTemplate(char *lpszType);
+
+/* ---- datablocks (reference PTaskRuntime.h:431-464) --------------------------------------- */
+Datablock *AllocateDatablock(DatablockTemplate *pTemplate, void *pInitData, UINT cbInitData,
+                             Channel *pDestChannel, BUFFERACCESSFLAGS flags = 0,
+                             CONTROLSIGNAL uiBlockControlCode = 0);
+Datablock *AllocateDatablock(DatablockTemplate *pTemplate, UINT uiDataSize, UINT uiMetaSize,
+                             UINT uiTemplateSize, CONTROLSIGNAL uiBlockControlCode = 0);
+Datablock *AllocateDatablock(CONTROLSIGNAL uiBlockControlCode);
+void FreeDatablock(Datablock *pBlock);
+
+/* ---- kernels and ports (reference PTaskRuntime.h:205-237) -------------------------------- */
+/* The accelerator class follows the file extension: ".hsaco"/".co" -> HIP (a gfx950 code
+ * object), ".so"/".dll" -> host (a shared library of host-task functions). */
+CompiledKernel *GetCompiledKernel(char *lpszFile, char *lpszOperation,
+                                  char *lpszCompilerOutput = NULL, int uiCompilerOutput = 0,
+                                  int tgx = 1, int tgy = 1, int tgz = 1);
+CompiledKernel *GetHostFunctionCompiledKernel(char *lpszOperation, FARPROC lpfn);
+Port *CreatePort(PORTTYPE type, DatablockTemplate *pTemplate, UINT uiId,
+                 char *lpszVariableBinding = NULL, UINT nKernelParameterIndex = (UINT)-1,
+                 UINT nInOutParmOutputIndex = (UINT)-1);
+PTRESULT SetTaskAffinity(Task *pTask, Accelerator *pAccelerator, AFFINITYTYPE eAffinityType);
+
+/* ---- accelerators --------------------------------------------------------------------------- */
+/* The uiEnumerationIndex-th enabled accelerator of a class (reference:
+ * Runtime::EnumerateAccelerators); the descriptor is owned by the caller (delete). */
+PTRESULT EnumerateAccelerators(ACCELERATOR_CLASS eClass, UINT uiEnumerationIndex,
+                               ACCELERATOR_DESCRIPTOR **ppDescriptor);
+Accelerator *GetAccelerator(ACCELERATOR_CLASS eClass, UINT uiIndex);
+UINT GetAcceleratorCount(ACCELERATOR_CLASS eClass);
+
+/* ---- configuration ------------------------------------------------------------------------ */
+void SetUseHost(BOOL b);
+void SetUseHIP(BOOL b);
+void SetUseCUDA(BOOL b);      // accepted for source compatibility; there is no CUDA backend
+void SetUseOpenCL(BOOL b);    // accepted for source compatibility; there is no OpenCL backend
+void SetUseDirectX(BOOL b);   // accepted for source compatibility; there is no DirectX backend
+BOOL GetUseHost();
+BOOL GetUseHIP();
+void SetMaximumConcurrency(int nGPUs);
+int GetMaximumConcurrency();
+void SetMaximumHostConcurrency(int nHostAccelerators);
+int GetMaximumHostConcurrency();
+void SetDefaultChannelCapacity(int n);
+int GetDefaultChannelCapacity();
+void SetTaskThreadPoolSize(UINT uiThreadPoolSize);
+UINT GetTaskThreadPoolSize();
+void SetTaskThreadPoolPolicy(THREADPOOLPOLICY ePolicy);
+void SetSchedulingMode(int mode);
+int GetSchedulingMode();
+void SetForceSynchronous(BOOL b);
+BOOL GetForceSynchronous();
+void SetProfilePSDispatch(BOOL b);
+BOOL GetProfilePSDispatch();
+void SetDebugMode(BOOL b);
+void SetUseGraphMonitorWatchdog(BOOL b);
+void SetDefaultViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY e);
+void SetDefaultOutputViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY e);
+VIEWMATERIALIZATIONPOLICY GetDefaultOutputViewMaterializationPolicy();
+void SetLoggingLevel(UINT nLevel);
+UINT GetLoggingLevel();
+void SetExitOnRuntimeFailure(BOOL bExit);
+BOOL GetExitOnRuntimeFailure();
+/* Blocks kept per output-port pool before recycled blocks are freed instead. */
+void SetDefaultBlockPoolSize(UINT n);
+UINT GetDefaultBlockPoolSize();
+
+/* ---- diagnostics -------------------------------------------------------------------------- */
+/* Prints the message; then exits the process unless SetExitOnRuntimeFailure(FALSE), in which
+ * case the error is recorded (GetLastError) and FALSE returned (reference:
+ * PTaskRuntime.cpp:3884-3899). */
+BOOL HandleError(const char *szMessage, ...);
+BOOL HandleError(const std::string &strError);
+const char *GetLastErrorText();
+void ClearLastError();
+void Inform(const char *szMessage, ...);
+void Warning(const char *szMessage, UINT nLogLevel = 0xFFFFFFFF);
+BOOL MandatoryInform(const char *szMessage, ...);
+void Trace(const char *szMessage, ...);
+
+/* Host<->device traffic since Initialize (or the last reset): counts and bytes of block
+ * populations in each direction.  Used to show that device-resident graphs move nothing. */
+void GetTransferStatistics(unsigned long long *pHtoDCount, unsigned long long *pHtoDBytes,
+                           unsigned long long *pDtoHCount, unsigned long long *pDtoHBytes);
+void ResetTransferStatistics();
+
+}  // namespace Runtime
+}  // namespace PTask
+
+#endif

@@ -1,0 +1,122 @@
+/* graph.h -- a dataflow graph of tasks, ports and channels (reference: ptask/graph.h).
+ *
+ * Execution model: Run() starts runner threads that serve a ready queue.  A task is queued
+ * whenever something that could make it ready happens (a block arrives on one of its inputs, a
+ * downstream channel drains); a runner re-checks readiness under the graph lock and dispatches.
+ * One lock per graph guards all channel queues and port state; backends run outside it. */
+#ifndef PTK_GRAPH_H
+#define PTK_GRAPH_H
+
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "Lockable.h"
+#include "ReferenceCounted.h"
+#include "channel.h"
+#include "primitive_types.h"
+
+namespace PTask {
+
+class CompiledKernel;
+class Task;
+class Port;
+class Datablock;
+
+class Graph : public ReferenceCounted {
+public:
+    Graph(const char *lpszName = NULL);
+    virtual ~Graph();
+    static BOOL DestroyGraph(Graph *pGraph, BOOL bGCSweep = FALSE);
+
+    virtual void Run(BOOL bSingleThreaded = FALSE);
+    virtual void Reset();
+    virtual void Stop();
+    virtual void Teardown();
+    virtual GRAPHSTATE GetState() { return m_eState; }
+    BOOL IsRunning() { return m_eState == PTGS_RUNNING; }
+    BOOL IsStopped() { return m_eState != PTGS_RUNNING; }
+    BOOL IsTorndown() { return m_eState == PTGS_TEARDOWNCOMPLETE; }
+    const char *GetName() { return m_strName.c_str(); }
+
+    virtual Task *AddTask(CompiledKernel *pKernel, UINT uiInputPortCount, Port **pvInputPorts,
+                          UINT uiOutputPorts, Port **pvOutputPorts, const char *lpszTaskName);
+    virtual Task *GetTask(const char *lpszTaskName);
+    virtual GraphInputChannel *AddInputChannel(Port *pPort, const char *lpszChannelName = NULL,
+                                               BOOL bSwitchChannel = FALSE,
+                                               Port *pDefaultTriggerPort = NULL,
+                                               CONTROLSIGNAL luiTriggerSignal = DBCTLC_NONE);
+    virtual InitializerChannel *AddInitializerChannel(Port *pPort, const char *lpszChannelName = NULL,
+                                                      BOOL bSwitchChannel = FALSE,
+                                                      Port *pDefaultTriggerPort = NULL,
+                                                      CONTROLSIGNAL luiTriggerSignal = DBCTLC_NONE);
+    virtual GraphOutputChannel *AddOutputChannel(Port *pPort, const char *lpszChannelName = NULL,
+                                                 Task *pTriggerTask = NULL,
+                                                 CONTROLSIGNAL luiTriggerSignal = DBCTLC_NONE);
+    virtual InternalChannel *AddInternalChannel(Port *pSrc, Port *pDst,
+                                                const char *lpszChannelName = NULL,
+                                                BOOL bSwitchChannel = FALSE);
+    virtual MultiChannel *AddMultiChannel(Port **ppDst, int nDestPorts,
+                                          const char *lpszChannelName = NULL,
+                                          BOOL bSwitchChannel = FALSE,
+                                          Port *pDefaultTriggerPort = NULL,
+                                          CONTROLSIGNAL luiTriggerSignal = DBCTLC_NONE);
+    virtual Channel *GetChannel(const char *lpszChannelName);
+    virtual void BindDescriptorPort(Port *pDescribedPort, Port *pDescriberPort,
+                                    DESCRIPTORFUNC func = DF_SIZE);
+    virtual void BindControlPort(Port *pController, Port *pGatedPort, BOOL bInitiallyOpen = FALSE);
+    virtual void BindControlPropagationPort(Port *pInputPort, Port *pOutputPort);
+    virtual void BindIterationScope(Port *pMetaPort, Port *pScopedPort);
+    virtual void BindControlPropagationChannel(Port *pInputPort, Channel *pControlledChannel);
+    /* Graphviz rendering of tasks and channels. */
+    virtual void WriteDOTFile(const char *lpszFileName, BOOL bDrawPorts = FALSE);
+    /* Structural checks: every formal parameter bound, every input port fed, indices dense. */
+    virtual PTRESULT CheckGraphSemantics(BOOL bVerbose = TRUE, BOOL bFailOnWarning = FALSE);
+    UINT GetTaskCount() { return (UINT)m_vTasks.size(); }
+    Task *GetTaskByIndex(UINT i) { return m_vTasks[i]; }
+    /* Total dispatches since Run(), across all tasks. */
+    unsigned long long GetDispatchCount();
+    /* Blocks until no task is ready or dispatching and every runner is idle (or the timeout in
+     * ms elapses); returns TRUE when quiescent. */
+    BOOL WaitQuiescent(DWORD dwTimeout = INFINITE);
+
+    /* ---- runtime internals ---------------------------------------------------------------- */
+    std::recursive_mutex &GraphLock() { return m_lock; }
+    std::condition_variable_any &DataArrived() { return m_cvData; }
+    std::condition_variable_any &RoomAvailable() { return m_cvRoom; }
+    /* Queue a task for a readiness check (graph lock held). */
+    void SignalTask(Task *pTask);
+    void SignalPortTask(Port *pPort);
+    /* Wake waiters on output channels / input-channel capacity (graph lock held). */
+    void NotifyDataArrived() { m_cvData.notify_all(); }
+    void NotifyRoom() { m_cvRoom.notify_all(); }
+    BOOL IsAlive() { return m_eState != PTGS_TEARINGDOWN && m_eState != PTGS_TEARDOWNCOMPLETE; }
+
+private:
+    void RunnerProc(UINT uiIndex);
+    void RegisterChannel(Channel *pChannel);
+
+    std::string m_strName;
+    GRAPHSTATE m_eState;
+    std::recursive_mutex m_lock;
+    std::condition_variable_any m_cvReady;
+    std::condition_variable_any m_cvData;
+    std::condition_variable_any m_cvRoom;
+    std::condition_variable_any m_cvIdle;
+    std::deque<Task *> m_qReady;
+    std::vector<Task *> m_vTasks;
+    std::vector<Channel *> m_vChannels;
+    std::map<std::string, Channel *> m_mChannels;
+    std::vector<std::thread> m_vRunners;
+    BOOL m_bStopRequested;
+    UINT m_uiBusyRunners;
+    BOOL m_bFinalized;
+};
+
+}  // namespace PTask
+
+#endif

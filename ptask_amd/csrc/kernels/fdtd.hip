@@ -38,6 +38,7 @@ PTK_FDTD_KERNEL HxComputation(const float *__restrict__ Hx, const float *__restr
         const float *ez = Ez + i + (long long)j * (p.Nx + 1) + s.k0 * PEz;
         float *o = out + q + s.k0 * P;
         float ey0 = ey[0];
+#pragma unroll PTK_FDTD_UNROLL
         for (int k = s.k0; k < s.k1; ++k) {
             const float ey1 = ey[P];
             const float ez0 = ez[0], ez1 = ez[p.Nx + 1];
@@ -66,6 +67,7 @@ PTK_FDTD_KERNEL HyComputation(const float *__restrict__ Hy, const float *__restr
         const float *ez = Ez + i + (long long)j * (p.Nx + 1) + s.k0 * PEz;
         float *o = out + q + s.k0 * P;
         float ex0 = ex[0];
+#pragma unroll PTK_FDTD_UNROLL
         for (int k = s.k0; k < s.k1; ++k) {
             const float ex1 = ex[P];
             const float ez0 = ez[0], ez1 = ez[1];
@@ -94,6 +96,7 @@ PTK_FDTD_KERNEL HzComputation(const float *__restrict__ Hz, const float *__restr
         const float *ex = Ex + i + (long long)j * p.Nx + s.k0 * PEx;
         const float *ey = Ey + i + (long long)j * (p.Nx + 1) + s.k0 * PEy;
         float *o = out + q + s.k0 * P;
+#pragma unroll PTK_FDTD_UNROLL
         for (int k = s.k0; k < s.k1; ++k) {
             const float ex0 = ex[0], ex1 = ex[p.Nx];
             const float ey0 = ey[0], ey1 = ey[1];
@@ -127,6 +130,7 @@ PTK_FDTD_KERNEL ExComputation(const float *__restrict__ Ex, const float *__restr
         const float *hz = Hz + i + (long long)j * p.Nx + s.k0 * PHz;
         const float *hy = Hy + q + s.k0 * P;
         float hym = (s.k0 >= 1) ? hy[-P] : 0.0f;
+#pragma unroll PTK_FDTD_UNROLL
         for (int k = s.k0; k < s.k1; ++k) {
             const float hy0 = (k <= p.Nz - 1) ? hy[0] : 0.0f;
             if (k < 1 || k > p.Nz - 1) {
@@ -164,6 +168,7 @@ PTK_FDTD_KERNEL EyComputation(const float *__restrict__ Ey, const float *__restr
         const float *hx = Hx + q + s.k0 * P;
         const float *hz = Hz + i + (long long)j * p.Nx + s.k0 * PHz;
         float hxm = (s.k0 >= 1) ? hx[-P] : 0.0f;
+#pragma unroll PTK_FDTD_UNROLL
         for (int k = s.k0; k < s.k1; ++k) {
             const float hx0 = (k <= p.Nz - 1) ? hx[0] : 0.0f;
             if (k < 1 || k > p.Nz - 1) {
@@ -201,6 +206,7 @@ PTK_FDTD_KERNEL EzComputation(const float *__restrict__ Ez, const float *__restr
         }
         const float *hy = Hy + i + (long long)j * p.Nx + s.k0 * PHy;
         const float *hx = Hx + i + (long long)j * (p.Nx + 1) + s.k0 * PHx;
+#pragma unroll PTK_FDTD_UNROLL
         for (int k = s.k0; k < s.k1; ++k) {
             const float hy0 = hy[0], hym = hy[-1];
             const float hx0 = hx[0], hxm = hx[-(p.Nx + 1)];

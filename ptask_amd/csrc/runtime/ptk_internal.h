@@ -1,0 +1,207 @@
+// ptk_internal.h -- runtime-private classes: the Host and HIP backends, block pools and the
+// process-wide runtime state.  Not installed; application code sees include/ptask/*.h only.
+#pragma once
+
+#include <atomic>
+#include <condition_variable>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "ptaskapi.h"
+#include "ptk_hip.h"
+
+namespace PTask {
+
+// ---------------------------------------------------------------------------------------------
+// Block pools (reference: BlockPool.cpp, OutputPort::GetPooledDestinationBuffer
+// OutputPort.cpp:603).  A pool keeps blocks whose last reference was released so the next
+// dispatch reuses their device storage: no allocation and no migration in steady state.
+class BlockPool : public ReferenceCounted {
+public:
+    BlockPool(DatablockTemplate *pTemplate, UINT uiMaxFree);
+    ~BlockPool() override;
+    // A free block with at least cb bytes, or a new one; returned with one reference.
+    Datablock *Request(UINT cb);
+    // Called by Datablock::OnLastRelease for blocks owned by this pool.
+    void Return(Datablock *pBlock);
+    // No more recycling: free blocks are deleted now, returning blocks are deleted later.
+    void Close();
+    size_t FreeCount();
+
+private:
+    std::mutex m_lock;
+    DatablockTemplate *m_pTemplate;
+    std::vector<Datablock *> m_vFree;
+    UINT m_uiMaxFree;
+    BOOL m_bClosed;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Host backend: memory space 0, host functions called in place (reference: hostaccelerator.cpp,
+// hosttask.cpp).  The oracle path of every parity check runs here.
+typedef void(__stdcall *LPFNHOSTTASK)(UINT nArguments, void **ppArguments);
+
+class HostAsyncContext : public AsyncContext {
+public:
+    explicit HostAsyncContext(Accelerator *pAcc, ASYNCCONTEXTTYPE e) : AsyncContext(pAcc, e) {}
+    void *GetPlatformContextObject() override { return nullptr; }
+    SyncPoint *CreateSyncPoint() override;
+    BOOL InsertFence(SyncPoint *) override { return TRUE; }
+    BOOL SynchronizeContext() override { return TRUE; }
+    void RecycleEvent(void *) override {}
+};
+
+class PHBuffer : public PBuffer {
+public:
+    PHBuffer(Accelerator *pAcc, size_t cb);
+    ~PHBuffer() override;
+};
+
+class HostAccelerator : public Accelerator {
+public:
+    HostAccelerator(UINT uiId);
+    ~HostAccelerator() override;
+    BOOL Open() override;
+    BOOL Compile(const char *lpszFile, const char *lpszOperation, CompiledKernel *pKernel,
+                 char *lpszCompilerOutput, int cbCompilerOutput) override;
+    BOOL Synchronize() override { return TRUE; }
+    BOOL MakeDeviceContextCurrent() override { return TRUE; }
+    BOOL SupportsByvalArguments() override { return FALSE; }
+    AsyncContext *CreateAsyncContext(ASYNCCONTEXTTYPE e) override;
+    AsyncContext *GetTransferContext() override { return m_pXfer; }
+    PBuffer *NewPlatformSpecificBuffer(size_t cb) override;
+    BOOL CopyHostToDevice(PBuffer *, const void *, size_t, AsyncContext *) override { return FALSE; }
+    BOOL CopyDeviceToHost(void *, PBuffer *, size_t) override { return FALSE; }
+    BOOL CopyDeviceToDevice(PBuffer *, PBuffer *, size_t, AsyncContext *) override { return FALSE; }
+
+private:
+    AsyncContext *m_pXfer;
+    std::map<std::string, void *> m_mLibraries;
+};
+
+class HostTask : public Task {
+public:
+    HostTask(Graph *pGraph, CompiledKernel *pKernel, const char *lpszName);
+
+protected:
+    BOOL PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vector<Datablock *> &vOut,
+                                  UINT uiIterations) override;
+};
+
+// ---------------------------------------------------------------------------------------------
+// HIP backend: one MI355X, memory space 1 (reference roles: cuaccelerator.cpp, cutask.cpp,
+// PCUBuffer.cpp, CUAsyncContext.cpp).  Every device call goes through include/ptk_hip.h.
+class HIPAsyncContext : public AsyncContext {
+public:
+    HIPAsyncContext(Accelerator *pAcc, ASYNCCONTEXTTYPE e);
+    ~HIPAsyncContext() override;
+    void *GetPlatformContextObject() override { return m_hStream; }
+    ptk_hip_stream GetStream() { return m_hStream; }
+    SyncPoint *CreateSyncPoint() override;
+    BOOL InsertFence(SyncPoint *pSP) override;
+    BOOL SynchronizeContext() override;
+    void RecycleEvent(void *pEvent) override;
+
+private:
+    ptk_hip_stream m_hStream;
+    std::mutex m_evLock;
+    std::vector<ptk_hip_event> m_vFreeEvents;
+};
+
+class PHIPBuffer : public PBuffer {
+public:
+    PHIPBuffer(Accelerator *pAcc, size_t cb);
+    ~PHIPBuffer() override;
+};
+
+class HIPAccelerator : public Accelerator {
+public:
+    HIPAccelerator(UINT uiId, int nDevice, UINT uiMemSpace);
+    ~HIPAccelerator() override;
+    BOOL Open() override;
+    BOOL Compile(const char *lpszFile, const char *lpszOperation, CompiledKernel *pKernel,
+                 char *lpszCompilerOutput, int cbCompilerOutput) override;
+    BOOL Synchronize() override;
+    BOOL MakeDeviceContextCurrent() override;
+    BOOL SupportsByvalArguments() override { return TRUE; }
+    AsyncContext *CreateAsyncContext(ASYNCCONTEXTTYPE e) override;
+    AsyncContext *GetTransferContext() override { return m_pXfer; }
+    PBuffer *NewPlatformSpecificBuffer(size_t cb) override;
+    BOOL CopyHostToDevice(PBuffer *pDst, const void *pSrc, size_t cb, AsyncContext *pCtx) override;
+    BOOL CopyDeviceToHost(void *pDst, PBuffer *pSrc, size_t cb) override;
+    BOOL CopyDeviceToDevice(PBuffer *pDst, PBuffer *pSrc, size_t cb, AsyncContext *pCtx) override;
+    size_t GetMemorySize() override { return m_props.total_global_mem; }
+    int GetDeviceOrdinal() { return m_nDevice; }
+    const ptk_hip_device_props_t &GetProps() { return m_props; }
+
+private:
+    int m_nDevice;
+    ptk_hip_device_props_t m_props;
+    AsyncContext *m_pXfer;
+    std::mutex m_modLock;
+    std::map<std::string, ptk_hip_module> m_mModules;
+};
+
+class HIPTask : public Task {
+public:
+    HIPTask(Graph *pGraph, CompiledKernel *pKernel, const char *lpszName);
+    ~HIPTask() override;
+    void ReleaseDeviceResources() override;
+    // The stream this task launches on (created on first dispatch).
+    HIPAsyncContext *GetContext() { return m_pContext; }
+
+protected:
+    BOOL PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vector<Datablock *> &vOut,
+                                  UINT uiIterations) override;
+
+private:
+    HIPAccelerator *m_pAccelerator;
+    HIPAsyncContext *m_pContext;
+    std::vector<unsigned char> m_vArgs;
+    ptk_hip_event m_hProfStart, m_hProfEnd;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Process-wide state.
+struct RuntimeState {
+    std::recursive_mutex lock;
+    bool initialized = false;
+    BOOL useHost = TRUE, useHIP = TRUE;
+    int maxConcurrency = 0;         // HIP devices to use (0 = all)
+    int maxHostConcurrency = 4;     // host tasks running at once
+    int channelCapacity = 4;
+    UINT threadPoolSize = 0;        // graph runner threads (0 = automatic)
+    int schedulingMode = SCHEDMODE_PRIORITY;
+    BOOL forceSynchronous = FALSE;
+    BOOL profilePSDispatch = FALSE;
+    BOOL exitOnFailure = TRUE;
+    UINT loggingLevel = 0;
+    UINT blockPoolSize = 8;
+    VIEWMATERIALIZATIONPOLICY outputViewPolicy = VIEWMATERIALIZATIONPOLICY_ON_DEMAND;
+    std::vector<Accelerator *> hostAccelerators;
+    std::vector<Accelerator *> hipAccelerators;
+    std::map<std::string, DatablockTemplate *> templates;
+    std::vector<CompiledKernel *> kernels;
+    std::vector<Port *> ports;
+    std::string lastError;
+    std::atomic<unsigned long long> h2dCount{0}, h2dBytes{0}, d2hCount{0}, d2hBytes{0};
+    std::atomic<UINT> nextDBUID{1};
+    // host-task concurrency limiter
+    std::mutex hostSlotsLock;
+    std::condition_variable hostSlotsCv;
+    int hostSlotsInUse = 0;
+    // templates used for descriptor blocks
+    DatablockTemplate *sizeDescriptorTemplate = nullptr;
+    DatablockTemplate *controlTemplate = nullptr;
+};
+
+RuntimeState &RT();
+Accelerator *HostAcceleratorInstance();
+HIPAccelerator *DefaultHIPAccelerator();
+Accelerator *AcceleratorForMemorySpace(UINT uiMemSpace);
+// Fails loudly: HandleError with the HIP C ABI's last error text.
+BOOL CheckHip(int status, const char *what);
+
+}  // namespace PTask

@@ -64,7 +64,7 @@ def make_params(nx: int, ny: int, nz: int, courant: float = 0.5) -> Params:
 
 
 def default_geometry(d1: int, d2: int, d3: int, block: int = 256,
-                     target_slab: int = 48) -> tuple[tuple[int, int, int], tuple[int, int, int]]:
+                     target_slab: int = 8) -> tuple[tuple[int, int, int], tuple[int, int, int]]:
     """Grid/block for the column-walk kernels: one lane per (i,j) column, planes split into
     slabs of about `target_slab`."""
     ncol = d1 * d2

@@ -1,0 +1,129 @@
+"""The FDTD Yee graph built against ptaskapi.h (ptask_amd/csrc/apps/ptk_fdtd.cpp).
+
+CPU part: the graph run on the HostAccelerator with the oracle's host-task functions equals the
+oracle's direct run bit for bit, in both graph shapes (looped general iteration and unrolled),
+with pipelined blocks, repeated runs and a single runner thread.
+
+GPU part: the same graph on HIPAccelerator with the gfx950 Yee kernels equals the host graph
+bit for bit, and moves exactly one host->device copy per initial field and one device->host
+copy per result, independent of the number of time steps (datablocks stay device-resident).
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from ptask_amd import paths
+
+APP = paths.bin_path("ptk_fdtd")
+HOSTLIB = os.path.join(oracle.ORACLE_DIR, "_build", "libptk_oracle.so")
+HSACO = paths.kernel_path("fdtd.hsaco")
+
+
+def run_app(*args, dump=None, timeout=300):
+    cmd = [APP, *map(str, args)]
+    if dump:
+        cmd += ["--dump", dump]
+    print("run:", " ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    assert p.returncode == 0, p.stderr + p.stdout
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
+def load_dump(path, nx, ny, nz):
+    raw = np.fromfile(path, dtype=np.float32)
+    out, off = [], 0
+    for f in range(6):
+        n = int(np.prod(oracle.extent(f, nx, ny, nz)))
+        out.append(raw[off:off + n])
+        off += n
+    assert off == raw.size
+    return out
+
+
+def direct(nx, ny, nz, steps, seed):
+    F = oracle.init_fields(nx, ny, nz, seed)
+    return oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), steps)
+
+
+@pytest.mark.parametrize("mode", ["looped", "unrolled"])
+@pytest.mark.parametrize("shape", [(8, 7, 6), (5, 9, 3)])
+def test_host_graph_equals_direct_oracle(tmp_path, mode, shape):
+    nx, ny, nz = shape
+    d = str(tmp_path / "f.bin")
+    r = run_app("--kernels", HOSTLIB, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", 5,
+                "--mode", mode, "--seed", 99, dump=d)
+    assert r["ok"] and r["backend"] == "host"
+    assert r["dispatches_last_run"] == 6 * 5
+    got = load_dump(d, nx, ny, nz)
+    ref = direct(nx, ny, nz, 5, 99)
+    for f in range(6):
+        assert np.array_equal(got[f].view(np.uint32), ref[f].view(np.uint32)), oracle.FIELDS[f]
+
+
+def test_unrolled_pipelined_blocks_and_single_thread(tmp_path):
+    a = run_app("--kernels", HOSTLIB, "--n", 6, "--steps", 3, "--mode", "unrolled", "--blocks", 3)
+    b = run_app("--kernels", HOSTLIB, "--n", 6, "--steps", 3, "--mode", "looped",
+                "--single-threaded")
+    assert a["ok"] and b["ok"]
+    assert a["dispatches_last_run"] == 6 * 3 * 3
+    assert [c["fnv1a"] for c in a["checksums"]] == [c["fnv1a"] for c in b["checksums"]]
+
+
+def test_looped_graph_reruns_reset_iteration_state():
+    r = run_app("--kernels", HOSTLIB, "--n", 5, "--steps", 4, "--mode", "looped", "--repeat", 3)
+    one = run_app("--kernels", HOSTLIB, "--n", 5, "--steps", 4, "--mode", "looped")
+    assert r["runs"] == 3 and r["dispatches_last_run"] == 24
+    assert [c["fnv1a"] for c in r["checksums"]] == [c["fnv1a"] for c in one["checksums"]]
+
+
+def test_missing_kernel_file_fails_loudly():
+    p = subprocess.run([APP, "--kernels", "/nonexistent/fdtd.hsaco", "--n", "4", "--steps", "1"],
+                       capture_output=True, text=True, timeout=60)
+    assert p.returncode != 0
+    assert "GetCompiledKernel" in p.stderr or "no ACCELERATOR_CLASS_HIP" in p.stderr \
+        or "accelerator" in p.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("geometry", ["default", "reference", "compute"])
+def test_hip_looped_graph_equals_host_graph(tmp_path, geometry):
+    nx, ny, nz, steps = 16, 12, 10, 6
+    dh, dg = str(tmp_path / "h.bin"), str(tmp_path / "g.bin")
+    h = run_app("--kernels", HOSTLIB, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
+                "--seed", 7, dump=dh)
+    g = run_app("--kernels", HSACO, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
+                "--seed", 7, "--geometry", geometry, dump=dg)
+    assert g["ok"] and g["backend"] == "hip"
+    H, G = load_dump(dh, nx, ny, nz), load_dump(dg, nx, ny, nz)
+    for f in range(6):
+        assert np.array_equal(H[f].view(np.uint32), G[f].view(np.uint32)), oracle.FIELDS[f]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["looped", "unrolled"])
+def test_hip_graph_keeps_blocks_device_resident(mode):
+    """One H2D per initial field and one D2H per result, whatever the step count."""
+    for steps in (2, 9):
+        g = run_app("--kernels", HSACO, "--n", 12, "--steps", steps, "--mode", mode)
+        h = run_app("--kernels", HOSTLIB, "--n", 12, "--steps", steps, "--mode", mode)
+        assert [c["fnv1a"] for c in g["checksums"]] == [c["fnv1a"] for c in h["checksums"]]
+        assert g["h2d_count"] == 6, g
+        assert g["d2h_count"] == 6, g
+        assert g["dispatches_last_run"] == 6 * steps
+
+
+@pytest.mark.gpu
+def test_hip_graph_64cubed_10_steps_matches_oracle(tmp_path):
+    """The CPU workload's size (64^3, 10 steps) through the HIP graph, at the 1e-5 gate."""
+    d = str(tmp_path / "g.bin")
+    g = run_app("--kernels", HSACO, "--n", 64, "--steps", 10, "--seed", 11, "--eager", dump=d)
+    got = load_dump(d, 64, 64, 64)
+    ref = direct(64, 64, 64, 10, 11)
+    for f in range(6):
+        rel = np.abs(got[f] - ref[f]).max() / np.abs(ref[f]).max()
+        assert rel <= 1e-5
+        assert np.array_equal(got[f].view(np.uint32), ref[f].view(np.uint32))
