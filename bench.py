@@ -2,18 +2,27 @@
 """Benchmark: FDTD Mcell-updates/s of the PTask Yee graph on one MI355X.
 
 Workload: a 384^3 binary32 Yee grid (six field arrays, 1.36 GB) advanced 200 time steps, fields
-seeded U[-1,1) on the device, Courant number 0.5, faces copied through.  One benchmark "step"
-is one full 200-time-step run (6 task kernels per time step).  Inputs are resident in HBM
-before timing starts; nothing crosses PCIe inside the timed region.
+seeded U[-1,1), Courant number 0.5, tangential E faces copied through.  The graph is the looped
+PTask graph (general-iteration meta ports, predicated back-channels) built against ptaskapi.h
+(ptask_amd/csrc/apps/ptk_fdtd.cpp) and dispatched task by task by the runtime.
 
-Prints one JSON line (see the repository's benchmark contract).  `--gpus N` with N > 1 exits:
-this build measures one GPU.
+One benchmark "step" is one 200-time-step run of the graph.  Runs are chained: each run starts
+from the previous run's output datablocks, which never leave HBM, so the timed region moves
+nothing over PCIe (the first, untimed warmup run uploads; a final untimed run downloads).
+
+Graph forms measured (all bit-identical to the six reference updates):
+  halves -- 2 tasks/step: FdtdHFields, FdtdEFields (72 B of HBM traffic per cell-step) [value]
+  six    -- the reference's six tasks/step (96 B per cell-step)
+  fused  -- 1 task/step, whole Yee step (48 B per cell-step)
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: this build measures one GPU.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -21,16 +30,17 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0   # MI355X published HBM3E bandwidth
+METRIC = "FDTD Mcell-updates/s; served graphs/s + p50/p99 latency vs plain dispatch"
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed 200-time-step runs")
+    ap.add_argument("--steps", type=int, default=3, help="timed 200-time-step graph runs")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--edge", type=int, default=384)
     ap.add_argument("--fdtd-steps", type=int, default=200)
-    ap.add_argument("--seed", type=int, default=0x5EED)
+    ap.add_argument("--forms", default="halves,six,fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
@@ -40,82 +50,61 @@ def progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-class FdtdDevice:
-    """Six resident field arrays plus their out-of-place partners, and the six task kernels."""
-
-    def __init__(self, H, FD, n: int, seed: int):
-        from ptask_amd import paths
-        self.H, self.FD, self.n = H, FD, n
-        self.mod = H.Module(paths.kernel_path("fdtd.hsaco"))
-        self.stream = H.Stream()
-        self.params = FD.make_params(n, n, n, 0.5)
-        self.cur, self.nxt = [], []
-        init = self.mod.function("FdtdInitField")
-        for f in range(6):
-            cnt = FD.elems(f, n, n, n)
-            a, b = H.DeviceBuffer(4 * cnt), H.DeviceBuffer(4 * cnt)
-            args = H.ArgPack().ptr(a.ptr).u64(cnt).i32(f).u64(seed)
-            H.launch(init, (2048, 1, 1), (256, 1, 1), args, self.stream)
-            self.cur.append(a)
-            self.nxt.append(b)
-        self.stream.synchronize()
-        self.fns = [self.mod.function(t[0]) for t in FD.TASKS]
-        self.geo = [FD.default_geometry(*FD.extent(t[1], n, n, n)) for t in FD.TASKS]
-        # pre-packed argument buffers for both ping-pong parities
-        self.packs = {}
-        for parity in (0, 1):
-            cur, nxt = (self.cur, self.nxt) if parity == 0 else (self.nxt, self.cur)
-            for t, (_, own, a, b) in enumerate(FD.TASKS):
-                # H tasks read the old E from `cur`; E tasks read the new H from `nxt`
-                ins = nxt if t >= 3 else cur
-                args = FD.pack_task_args(H, cur[own].ptr, ins[a].ptr, ins[b].ptr, self.params,
-                                         nxt[own].ptr)
-                self.packs[(parity, t)] = args.finish()
-        self.parity = 0
-
-    def launch_task(self, t: int) -> None:
-        import ctypes as C
-        cbuf, nbytes = self.packs[(self.parity, t)]
-        (gx, gy, gz), (bx, by, bz) = self.geo[t]
-        self.H.check(self.H.lib().ptk_hip_launch(self.fns[t], gx, gy, gz, bx, by, bz, 0,
-                                                 self.stream.handle, C.cast(cbuf, C.c_void_p),
-                                                 nbytes), "launch")
-
-    def time_step(self) -> None:
-        for t in range(6):
-            self.launch_task(t)
-        self.parity ^= 1
-
-    def run(self, steps: int) -> None:
-        for _ in range(steps):
-            self.time_step()
+def field_elems(n: int) -> list[int]:
+    from ptask_amd import fdtd as FD
+    return [FD.elems(f, n, n, n) for f in range(6)]
 
 
-def measure_kernels(dev: FdtdDevice, H, steps: int) -> dict:
-    """Average duration of each task kernel, from HIP events on the launch stream."""
-    evs = [[(H.Event(), H.Event()) for _ in range(6)] for _ in range(steps)]
-    for s in range(steps):
-        for t in range(6):
-            evs[s][t][0].record(dev.stream)
-            dev.launch_task(t)
-            evs[s][t][1].record(dev.stream)
-        dev.parity ^= 1
-    dev.stream.synchronize()
-    out = {}
-    for t, (name, own, _, _) in enumerate(dev.FD.TASKS):
-        ms = [evs[s][t][0].elapsed_ms(evs[s][t][1]) for s in range(steps)]
-        out[name] = sum(ms) / len(ms)
-    for row in evs:
-        for a, b in row:
-            a.destroy()
-            b.destroy()
-    return out
+def algorithmic_bytes(task: str, n: int) -> int:
+    """HBM bytes one launch of `task` must move: each input array read once, each output array
+    written once (binary32)."""
+    E = field_elems(n)
+    reads = {"FdtdHFields": range(6), "FdtdEFields": range(6), "FdtdStepFused": range(6)}
+    writes = {"FdtdHFields": (0, 1, 2), "FdtdEFields": (3, 4, 5), "FdtdStepFused": range(6)}
+    six = {"HxComputation": (0, (0, 4, 5)), "HyComputation": (1, (1, 3, 5)),
+           "HzComputation": (2, (2, 3, 4)), "ExComputation": (3, (3, 1, 2)),
+           "EyComputation": (4, (4, 0, 2)), "EzComputation": (5, (5, 0, 1))}
+    base = task.split("_")[0]
+    if base in six:
+        own, ins = six[base]
+        return 4 * (sum(E[f] for f in ins) + E[own])
+    return 4 * (sum(E[f] for f in reads[base]) + sum(E[f] for f in writes[base]))
+
+
+def run_graph(form: str, n: int, steps: int, repeat: int, warmup: int) -> dict:
+    from ptask_amd import paths
+    kernels = paths.kernel_path("fdtd.hsaco" if form == "six" else "fdtd_fused.hsaco")
+    cmd = [paths.bin_path("ptk_fdtd"), "--kernels", kernels, "--form", form, "--n", str(n),
+           "--steps", str(steps), "--repeat", str(repeat), "--warmup", str(warmup), "--chain",
+           "--profile"]
+    progress("run: " + " ".join(cmd))
+    t0 = time.time()
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    if p.returncode != 0:
+        raise RuntimeError(f"ptk_fdtd failed ({p.returncode}): {p.stderr[-2000:]}")
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    progress(f"{form}: {r['mcups']:.1f} Mcell-updates/s, {r['ms_per_run']:.2f} ms per "
+             f"{steps}-step run ({time.time() - t0:.1f} s wall incl. setup)")
+    return r
+
+
+def kernel_roofline(r: dict, n: int) -> dict:
+    """The dominant task kernel of a run: average launch duration from the HIP events the
+    runtime recorded on the task's stream during the timed runs."""
+    tasks = [t for t in r["tasks"] if t["dispatches"] > 0]
+    dom = max(tasks, key=lambda t: t["device_ms"])
+    avg_ms = dom["device_ms"] / dom["dispatches"]
+    alg = algorithmic_bytes(dom["task"], n)
+    achieved = alg / (avg_ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["task"],
+            "kernel_avg_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": alg,
+            "launches_timed": int(dom["dispatches"])}
 
 
 def cpu_baseline(n: int, seconds: float) -> dict:
     """The oracle (oracle/fdtd.c, binary32, one core) timed on the same 384^3 grid for as many
     whole time steps as fit in about `seconds`."""
-    import numpy as np
     import oracle
     p = oracle.make_params(n, n, n, 0.5)
     F = oracle.init_fields(n, n, n, seed=1)
@@ -141,71 +130,40 @@ def main() -> int:
               "(every GPU box here has a single MI355X)", file=sys.stderr)
         return 2
     from ptask_amd import hip as H
-    from ptask_amd import fdtd as FD
     if H.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
-    H.set_device(0)
     props = H.device_props(0)
-    n, T = args.edge, args.fdtd_steps
     progress(f"device {props.name.decode()} ({props.arch.decode()}), {props.compute_units} CUs")
-    dev = FdtdDevice(H, FD, n, args.seed)
-    progress(f"fields resident: {n}^3, {sum(b.nbytes for b in dev.cur) / 1e9:.3f} GB x2")
-
-    for w in range(args.warmup):
-        dev.run(T)
-        dev.stream.synchronize()
-        progress(f"warmup {w + 1}/{args.warmup} done")
-
-    e0, e1 = H.Event(), H.Event()
-    H.synchronize()
-    t0 = time.perf_counter()
-    e0.record(dev.stream)
-    for s in range(args.steps):
-        dev.run(T)
-    e1.record(dev.stream)
-    e1.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = e0.elapsed_ms(e1)
-    H.synchronize()
-    progress(f"timed {args.steps} x {T} time steps: {gpu_ms:.1f} ms (wall {wall * 1e3:.1f} ms)")
-
-    cells = n ** 3
-    ms_per_step = gpu_ms / args.steps
-    mcups = cells * T * args.steps / (gpu_ms / 1e3) / 1e6
-
-    kt = measure_kernels(dev, H, 10)
-    dom = max(kt, key=kt.get)
-    t_dom = [t for t in FD.TASKS if t[0] == dom][0]
-    alg_bytes = 16 * FD.elems(t_dom[1], n, n, n)   # 3 reads + 1 write of binary32 per cell
-    achieved = alg_bytes / (kt[dom] / 1e3) / 1e9
-    six_task_gbs = 96 * cells * T * args.steps / (gpu_ms / 1e3) / 1e9
-    progress("kernel ms: " + ", ".join(f"{k}={v:.4f}" for k, v in kt.items()))
-
+    n, T = args.edge, args.fdtd_steps
+    forms = [f for f in args.forms.split(",") if f]
+    results = {f: run_graph(f, n, T, args.steps, args.warmup) for f in forms}
+    head = results.get("halves") or results[forms[0]]
+    head_form = "halves" if "halves" in results else forms[0]
     line = {
-        "metric": "FDTD Mcell-updates/s; served graphs/s + p50/p99 latency vs plain dispatch",
-        "value": round(mcups, 1),
+        "metric": METRIC,
+        "value": round(head["mcups"], 1),
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(head["ms_per_run"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded U[-1,1) fields generated on the device)",
-        "config": {"workload": f"FDTD Yee {n}^3 binary32, {T} time steps, six-task form, "
-                               "plain per-task dispatch through the HIP C ABI",
-                   "grid": [n, n, n], "time_steps": T, "courant": 0.5,
+        "data": "synthetic (seeded U[-1,1) fields from the shared counter-based generator)",
+        "config": {"workload": f"FDTD Yee {n}^3 binary32, {T} time steps per step, looped PTask "
+                               f"graph, {head_form} form, per-task dispatch by the runtime, "
+                               "device-resident datablocks",
+                   "grid": [n, n, n], "time_steps": T, "courant": 0.5, "form": head_form,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": dom,
-                     "kernel_avg_ms": round(kt[dom], 5),
-                     "algorithmic_bytes_per_launch": alg_bytes},
-        "six_task_effective_GBs": round(six_task_gbs, 1),
-        "kernel_avg_ms": {k: round(v, 5) for k, v in kt.items()},
+        "roofline": kernel_roofline(head, n),
+        "forms": {f: {"mcups": round(r["mcups"], 1), "ms_per_run": round(r["ms_per_run"], 3),
+                      "tasks_per_step": len(r["tasks"]), "dispatches_per_run": r["dispatches_last_run"],
+                      "roofline": kernel_roofline(r, n)} for f, r in results.items()},
+        "checksums_equal_across_forms": len({tuple(c["fnv1a"] for c in r["checksums"])
+                                             for r in results.values()}) == 1,
     }
     if not args.no_cpu:
         progress("cpu baseline ...")

@@ -100,6 +100,14 @@ void SetLoggingLevel(UINT nLevel);
 UINT GetLoggingLevel();
 void SetExitOnRuntimeFailure(BOOL bExit);
 BOOL GetExitOnRuntimeFailure();
+/* Stream assignment of HIP tasks (no reference counterpart; the CUDA backend gives every task
+ * its own context, cutask.cpp:667-675).  PER_TASK lets independent tasks overlap on the device;
+ * SHARED puts every task on one in-order stream, which removes the cross-stream event waits of
+ * chains of dependent tasks.  Takes effect for tasks that have not dispatched yet. */
+#define PTK_STREAMS_PER_TASK 0
+#define PTK_STREAMS_SHARED 1
+void SetTaskStreamPolicy(int ePolicy);
+int GetTaskStreamPolicy();
 /* Blocks kept per output-port pool before recycled blocks are freed instead. */
 void SetDefaultBlockPoolSize(UINT n);
 UINT GetDefaultBlockPoolSize();

@@ -57,7 +57,8 @@ public:
     /* Device time of the last dispatch in ms (filled when Runtime::SetProfilePSDispatch(TRUE),
      * from events around the launches, as ptask/cutask.cpp:709-776). */
     double GetLastDispatchDeviceMs() { return m_dLastDeviceMs; }
-    double GetTotalDispatchDeviceMs() { return m_dTotalDeviceMs; }
+    /* Waits for the profiled dispatches still in flight. */
+    virtual double GetTotalDispatchDeviceMs() { return m_dTotalDeviceMs; }
 
     /* ---- runtime internals ---------------------------------------------------------------- */
     void AttachPorts(UINT nIn, Port **ppIn, UINT nOut, Port **ppOut);

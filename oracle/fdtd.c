@@ -159,3 +159,33 @@ HOSTTASK(HzComputation, ptk_oracle_fdtd_hz_f32)
 HOSTTASK(ExComputation, ptk_oracle_fdtd_ex_f32)
 HOSTTASK(EyComputation, ptk_oracle_fdtd_ey_f32)
 HOSTTASK(EzComputation, ptk_oracle_fdtd_ez_f32)
+
+/* Host-task forms of the fused graphs: the same updates grouped as the HIP task kernels group
+ * them.  FdtdHFields(Hx,Hy,Hz,Ex,Ey,Ez,&p,oHx,oHy,oHz); FdtdEFields(Ex,Ey,Ez,Hx,Hy,Hz,&p,oEx,oEy,
+ * oEz) with the new H; FdtdStepFused(Hx..Ez,&p,oHx..oEz) one whole step. */
+void FdtdHFields(unsigned int n, void **a) {
+    (void)n;
+    const FDTD_PARAMS *p = (const FDTD_PARAMS *)a[6];
+    ptk_oracle_fdtd_hx_f32((const float *)a[0], (const float *)a[4], (const float *)a[5], p, (float *)a[7]);
+    ptk_oracle_fdtd_hy_f32((const float *)a[1], (const float *)a[3], (const float *)a[5], p, (float *)a[8]);
+    ptk_oracle_fdtd_hz_f32((const float *)a[2], (const float *)a[3], (const float *)a[4], p, (float *)a[9]);
+}
+
+void FdtdEFields(unsigned int n, void **a) {
+    (void)n;
+    const FDTD_PARAMS *p = (const FDTD_PARAMS *)a[6];
+    ptk_oracle_fdtd_ex_f32((const float *)a[0], (const float *)a[4], (const float *)a[5], p, (float *)a[7]);
+    ptk_oracle_fdtd_ey_f32((const float *)a[1], (const float *)a[3], (const float *)a[5], p, (float *)a[8]);
+    ptk_oracle_fdtd_ez_f32((const float *)a[2], (const float *)a[3], (const float *)a[4], p, (float *)a[9]);
+}
+
+void FdtdStepFused(unsigned int n, void **a) {
+    (void)n;
+    const FDTD_PARAMS *p = (const FDTD_PARAMS *)a[6];
+    ptk_oracle_fdtd_hx_f32((const float *)a[0], (const float *)a[4], (const float *)a[5], p, (float *)a[7]);
+    ptk_oracle_fdtd_hy_f32((const float *)a[1], (const float *)a[3], (const float *)a[5], p, (float *)a[8]);
+    ptk_oracle_fdtd_hz_f32((const float *)a[2], (const float *)a[3], (const float *)a[4], p, (float *)a[9]);
+    ptk_oracle_fdtd_ex_f32((const float *)a[3], (const float *)a[8], (const float *)a[9], p, (float *)a[10]);
+    ptk_oracle_fdtd_ey_f32((const float *)a[4], (const float *)a[7], (const float *)a[9], p, (float *)a[11]);
+    ptk_oracle_fdtd_ez_f32((const float *)a[5], (const float *)a[7], (const float *)a[8], p, (float *)a[12]);
+}

@@ -43,6 +43,9 @@ void HzComputation(unsigned int n, void **a);
 void ExComputation(unsigned int n, void **a);
 void EyComputation(unsigned int n, void **a);
 void EzComputation(unsigned int n, void **a);
+void FdtdHFields(unsigned int n, void **a);
+void FdtdEFields(unsigned int n, void **a);
+void FdtdStepFused(unsigned int n, void **a);
 
 #ifdef __cplusplus
 }

@@ -30,14 +30,31 @@ using namespace PTask;
 namespace {
 
 const char *kFieldNames[6] = {"Hx", "Hy", "Hz", "Ex", "Ey", "Ez"};
-const char *kKernelNames[6] = {"HxComputation", "HyComputation", "HzComputation",
-                               "ExComputation", "EyComputation", "EzComputation"};
-// (own, in1, in2) per task, in step order (graphfdtd.cpp:448-560)
-const int kTaskInputs[6][3] = {{0, 4, 5}, {1, 3, 5}, {2, 3, 4}, {3, 1, 2}, {4, 0, 2}, {5, 0, 1}};
+
+// A graph form: the tasks of one time step, in order, with the fields each reads and writes.
+// Formal parameters: the input fields (0..n-1), the FDTD_PARAMS block (n), the outputs (n+1..).
+struct TaskSpec {
+    const char *kernel;
+    std::vector<int> in;
+    std::vector<int> out;
+};
+
+std::vector<TaskSpec> FormTasks(const std::string &form) {
+    if (form == "halves")  // all H updates, then all E updates (72 B per cell-step)
+        return {{"FdtdHFields", {0, 1, 2, 3, 4, 5}, {0, 1, 2}},
+                {"FdtdEFields", {3, 4, 5, 0, 1, 2}, {3, 4, 5}}};
+    if (form == "fused")  // one whole step (48 B per cell-step)
+        return {{"FdtdStepFused", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
+    // the reference's six tasks (graphfdtd.cpp:448-560): (own, in1, in2) -> own
+    return {{"HxComputation", {0, 4, 5}, {0}}, {"HyComputation", {1, 3, 5}, {1}},
+            {"HzComputation", {2, 3, 4}, {2}}, {"ExComputation", {3, 1, 2}, {3}},
+            {"EyComputation", {4, 0, 2}, {4}}, {"EzComputation", {5, 0, 1}, {5}}};
+}
 
 struct Options {
     int nx = 32, ny = 32, nz = 32, steps = 10, blocks = 1, repeat = 1;
     std::string mode = "looped";
+    std::string form = "six";
     std::string kernelFile;
     std::string geometry = "default";
     std::string dump;
@@ -47,6 +64,9 @@ struct Options {
     bool singleThreaded = false;
     bool eager = false;
     bool profile = false;
+    bool chain = false;  // run r+1 starts from run r's device-resident outputs
+    int warmup = 1;      // chained runs before the timed ones (the first one uploads)
+    std::string streams = "task";  // HIP stream policy: one per task, or one shared
 };
 
 bool Parse(int argc, char **argv, Options &o) {
@@ -67,6 +87,7 @@ bool Parse(int argc, char **argv, Options &o) {
         else if (a == "--blocks") o.blocks = atoi(next("--blocks"));
         else if (a == "--repeat") o.repeat = atoi(next("--repeat"));
         else if (a == "--mode") o.mode = next("--mode");
+        else if (a == "--form") o.form = next("--form");
         else if (a == "--kernels") o.kernelFile = next("--kernels");
         else if (a == "--geometry") o.geometry = next("--geometry");
         else if (a == "--dump") o.dump = next("--dump");
@@ -76,6 +97,9 @@ bool Parse(int argc, char **argv, Options &o) {
         else if (a == "--single-threaded") o.singleThreaded = true;
         else if (a == "--eager") o.eager = true;
         else if (a == "--profile") o.profile = true;
+        else if (a == "--chain") o.chain = true;
+        else if (a == "--warmup") o.warmup = atoi(next("--warmup"));
+        else if (a == "--streams") o.streams = next("--streams");
         else {
             fprintf(stderr, "unknown option %s\n", a.c_str());
             return false;
@@ -97,6 +121,7 @@ uint64_t Fnv1a(const void *p, size_t n) {
 struct FdtdGraph {
     Options o;
     bool hip = false;
+    std::vector<TaskSpec> spec;
     Graph *g = nullptr;
     DatablockTemplate *field[6] = {};
     DatablockTemplate *params = nullptr;
@@ -107,12 +132,21 @@ struct FdtdGraph {
     GraphOutputChannel *outputs[6] = {};
     UINT uid = 0;
 
-    void Geometry(Task *t, int f) {
-        int d1, d2, d3;
-        ptk_fdtd_extent(f, o.nx, o.ny, o.nz, &d1, &d2, &d3);
+    void Geometry(Task *t, const TaskSpec &ts) {
         if (!hip) {
-            t->SetComputeGeometry(1, 1, 1);  // a host task computes the whole array per call
+            t->SetComputeGeometry(1, 1, 1);  // a host task computes its whole arrays per call
             return;
+        }
+        if (ts.out.size() == 6) {  // whole-step kernel: 64x8 tiles, slabs of 16 planes
+            t->SetBlockAndGridSize(PTASKDIM3((o.nx + 64) / 64, (o.ny + 8) / 8, (o.nz + 16) / 16),
+                                   PTASKDIM3(64, 8, 1));
+            return;
+        }
+        int d1, d2, d3;
+        if (ts.out.size() == 3) {  // half steps walk the union domain
+            d1 = o.nx + 1, d2 = o.ny + 1, d3 = o.nz + 1;
+        } else {
+            ptk_fdtd_extent(ts.out[0], o.nx, o.ny, o.nz, &d1, &d2, &d3);
         }
         if (o.geometry == "reference") {
             // one thread per cell, as graphfdtd.cpp:439-446 sets for its CUDA kernels
@@ -121,33 +155,42 @@ struct FdtdGraph {
             t->SetComputeGeometry(d1, d2, d3);
         } else {
             // column walk: one lane per (i,j) column, k split in slabs of ~8 planes
-            int gx = (d1 * d2 + 255) / 256, slabs = (d3 + 7) / 8;
-            t->SetBlockAndGridSize(PTASKDIM3(gx, slabs, 1), PTASKDIM3(256, 1, 1));
+            const int bx = ts.out.size() == 3 ? 512 : 256;
+            int gx = (d1 * d2 + bx - 1) / bx, slabs = (d3 + 7) / 8;
+            t->SetBlockAndGridSize(PTASKDIM3(gx, slabs, 1), PTASKDIM3(bx, 1, 1));
         }
     }
 
-    // One Yee update task with ports (own, in1, in2, Config) -> out (+ a meta port if looped).
-    Task *AddUpdate(int t, bool looped, Port **ins, Port **outs, int step) {
-        CompiledKernel *k = Runtime::GetCompiledKernel((char *)o.kernelFile.c_str(), (char *)kKernelNames[t]);
+    // One task of the form: input ports (fields), the sticky Config port, an iteration meta port
+    // when looped, output ports.
+    Task *AddTask(const TaskSpec &ts, bool looped, std::vector<Port *> &ins, std::vector<Port *> &outs,
+                  Port *&meta, int step) {
+        CompiledKernel *k = Runtime::GetCompiledKernel((char *)o.kernelFile.c_str(), (char *)ts.kernel);
         if (!k) return nullptr;
-        const int own = kTaskInputs[t][0];
-        for (int a = 0; a < 3; ++a) {
-            int f = kTaskInputs[t][a];
-            ins[a] = Runtime::CreatePort(INPUT_PORT, field[f], uid++, (char *)kFieldNames[f], a);
+        const UINT n = (UINT)ts.in.size();
+        ins.clear();
+        outs.clear();
+        for (UINT a = 0; a < n; ++a)
+            ins.push_back(Runtime::CreatePort(INPUT_PORT, field[ts.in[a]], uid++, (char *)kFieldNames[ts.in[a]], a));
+        ins.push_back(Runtime::CreatePort(STICKY_PORT, params, uid++, (char *)"Config", n));
+        meta = nullptr;
+        if (looped) {
+            meta = Runtime::CreatePort(META_PORT, iters, uid++, (char *)"iterations");
+            meta->SetMetaFunction(MF_GENERAL_ITERATOR);
+            ins.push_back(meta);
         }
-        ins[3] = Runtime::CreatePort(STICKY_PORT, params, uid++, (char *)"Config", 3);
-        UINT nIn = 4;
-        if (looped) ins[nIn++] = Runtime::CreatePort(META_PORT, iters, uid++, (char *)"iterations");
-        outs[0] = Runtime::CreatePort(OUTPUT_PORT, field[own], uid++, (char *)kFieldNames[own], 4);
-        if (looped) outs[0]->SetMetaFunction(MF_NONE), ins[4]->SetMetaFunction(MF_GENERAL_ITERATOR);
-        std::string name = std::string(kFieldNames[own]) + "Comp" + (step >= 0 ? "_" + std::to_string(step) : "");
-        Task *task = g->AddTask(k, nIn, ins, 1, outs, name.c_str());
-        if (task) Geometry(task, own);
+        for (size_t b = 0; b < ts.out.size(); ++b)
+            outs.push_back(Runtime::CreatePort(OUTPUT_PORT, field[ts.out[b]], uid++,
+                                               (char *)kFieldNames[ts.out[b]], n + 1 + (UINT)b));
+        std::string name = std::string(ts.kernel) + (step >= 0 ? "_" + std::to_string(step) : "");
+        Task *task = g->AddTask(k, (UINT)ins.size(), ins.data(), (UINT)outs.size(), outs.data(), name.c_str());
+        if (task) Geometry(task, ts);
         return task;
     }
 
     bool Build() {
         hip = ptaskutils::SelectAcceleratorClass(o.kernelFile.c_str()) == ACCELERATOR_CLASS_HIP;
+        spec = FormTasks(o.form);
         for (int f = 0; f < 6; ++f) {
             int d1, d2, d3;
             ptk_fdtd_extent(f, o.nx, o.ny, o.nz, &d1, &d2, &d3);
@@ -160,65 +203,84 @@ struct FdtdGraph {
         return o.mode == "looped" ? BuildLooped() : BuildUnrolled();
     }
 
+    // producer[f] = (task index, output slot) of field f within one step
+    void Producers(int prodTask[6], int prodSlot[6]) {
+        for (size_t t = 0; t < spec.size(); ++t)
+            for (size_t b = 0; b < spec[t].out.size(); ++b) {
+                prodTask[spec[t].out[b]] = (int)t;
+                prodSlot[spec[t].out[b]] = (int)b;
+            }
+    }
+
+    // Looped: every task carries a MF_GENERAL_ITERATOR meta port.  A field produced earlier in
+    // the step arrives on an always-open channel; a field produced by the same or a later task
+    // is loop-carried: the application's switch channel feeds the first iteration, a
+    // back-channel closed on ENDITERATION feeds the rest (iteration.cpp:887-1060).
     bool BuildLooped() {
-        Port *ins[6][5], *outs[6][1];
-        for (int t = 0; t < 6; ++t)
-            if (!AddUpdate(t, true, ins[t], outs[t], -1)) return false;
-        for (int t = 0; t < 6; ++t) {
-            const int own = kTaskInputs[t][0];
-            // first iteration: every field input from the application (switch channels win)
-            for (int a = 0; a < 3; ++a) {
-                int f = kTaskInputs[t][a];
-                std::string nm = std::string(kFieldNames[t]) + "Comp." + kFieldNames[f] + ".in";
-                if (t < 3 || a == 0) fieldInputs[f].push_back(g->AddInputChannel(ins[t][a], nm.c_str(), TRUE));
+        const size_t T = spec.size();
+        std::vector<std::vector<Port *>> ins(T), outs(T);
+        std::vector<Port *> metas(T);
+        for (size_t t = 0; t < T; ++t)
+            if (!AddTask(spec[t], true, ins[t], outs[t], metas[t], -1)) return false;
+        int prodTask[6], prodSlot[6];
+        Producers(prodTask, prodSlot);
+        for (size_t t = 0; t < T; ++t) {
+            std::string tn = spec[t].kernel;
+            for (size_t a = 0; a < spec[t].in.size(); ++a) {
+                const int f = spec[t].in[a];
+                const int pt = prodTask[f];
+                Port *src = outs[pt][prodSlot[f]];
+                std::string nm = std::string(kFieldNames[f]) + "->" + tn + "." + std::to_string(a);
+                if (pt < (int)t) {
+                    g->AddInternalChannel(src, ins[t][a], nm.c_str());
+                } else {
+                    fieldInputs[f].push_back(g->AddInputChannel(ins[t][a], (nm + ".init").c_str(), TRUE));
+                    InternalChannel *c = g->AddInternalChannel(src, ins[t][a], (nm + ".back").c_str());
+                    c->SetPredicationType(CE_SRC, CGATEFN_CLOSE_ON_ENDITERATION);
+                }
             }
-            paramInputs.push_back(g->AddInputChannel(ins[t][3], (std::string(kFieldNames[own]) + ".cfg").c_str()));
-            iterInputs.push_back(g->AddInputChannel(ins[t][4], (std::string(kFieldNames[own]) + ".iters").c_str()));
-            g->BindControlPropagationPort(ins[t][4], outs[t][0]);
-            // loop-carried: own field back to itself
-            InternalChannel *back = g->AddInternalChannel(outs[t][0], ins[t][0], (std::string(kFieldNames[own]) + ".back").c_str());
-            back->SetPredicationType(CE_SRC, CGATEFN_CLOSE_ON_ENDITERATION);
-            outputs[own] = g->AddOutputChannel(outs[t][0], (std::string(kFieldNames[own]) + ".out").c_str());
-            outputs[own]->SetPredicationType(CE_SRC, CGATEFN_OPEN_ON_ENDITERATION);
-            if (o.eager) outputs[own]->SetViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY_EAGER);
+            paramInputs.push_back(g->AddInputChannel(ins[t][spec[t].in.size()], (tn + ".cfg").c_str()));
+            iterInputs.push_back(g->AddInputChannel(metas[t], (tn + ".iters").c_str()));
+            for (size_t b = 0; b < spec[t].out.size(); ++b) {
+                g->BindControlPropagationPort(metas[t], outs[t][b]);
+                const int f = spec[t].out[b];
+                outputs[f] = g->AddOutputChannel(outs[t][b], (std::string(kFieldNames[f]) + ".out").c_str());
+                outputs[f]->SetPredicationType(CE_SRC, CGATEFN_OPEN_ON_ENDITERATION);
+                if (o.eager) outputs[f]->SetViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY_EAGER);
+            }
         }
-        // H -> E within a step (always open); E -> H into the next step (closed at the end)
-        for (int t = 3; t < 6; ++t)
-            for (int a = 1; a < 3; ++a) {
-                int f = kTaskInputs[t][a];
-                g->AddInternalChannel(outs[f][0], ins[t][a], (std::string(kFieldNames[f]) + "->" + kKernelNames[t]).c_str());
-            }
-        for (int t = 0; t < 3; ++t)
-            for (int a = 1; a < 3; ++a) {
-                int f = kTaskInputs[t][a];
-                InternalChannel *c = g->AddInternalChannel(outs[f][0], ins[t][a], (std::string(kFieldNames[f]) + "->" + kKernelNames[t]).c_str());
-                c->SetPredicationType(CE_SRC, CGATEFN_CLOSE_ON_ENDITERATION);
-            }
         return true;
     }
 
+    // Unrolled: the tasks of every step are stamped out (graphfdtd.cpp:735-739); a field comes
+    // from this step's producer if it ran earlier in the step, else from the previous step.
     bool BuildUnrolled() {
-        Port *prevOut[6] = {};
+        const size_t T = spec.size();
+        int prodTask[6], prodSlot[6];
+        Producers(prodTask, prodSlot);
+        std::vector<std::vector<Port *>> prevOuts;
         for (int s = 0; s < o.steps; ++s) {
-            Port *ins[6][5], *outs[6][1];
-            for (int t = 0; t < 6; ++t)
-                if (!AddUpdate(t, false, ins[t], outs[t], s)) return false;
-            std::string u = "_" + std::to_string(s);
-            for (int t = 0; t < 6; ++t) {
-                paramInputs.push_back(g->AddInputChannel(ins[t][3], (std::string(kKernelNames[t]) + ".cfg" + u).c_str()));
-                for (int a = 0; a < 3; ++a) {
-                    int f = kTaskInputs[t][a];
-                    bool fromThisStep = t >= 3 && a > 0;  // E tasks read the new H
-                    std::string nm = std::string(kKernelNames[t]) + "." + kFieldNames[f] + u;
-                    if (fromThisStep) g->AddInternalChannel(outs[f][0], ins[t][a], nm.c_str());
+            std::vector<std::vector<Port *>> ins(T), outs(T);
+            Port *meta = nullptr;
+            for (size_t t = 0; t < T; ++t)
+                if (!AddTask(spec[t], false, ins[t], outs[t], meta, s)) return false;
+            const std::string u = "_" + std::to_string(s);
+            for (size_t t = 0; t < T; ++t) {
+                std::string tn = spec[t].kernel;
+                paramInputs.push_back(g->AddInputChannel(ins[t][spec[t].in.size()], (tn + ".cfg" + u).c_str()));
+                for (size_t a = 0; a < spec[t].in.size(); ++a) {
+                    const int f = spec[t].in[a];
+                    const int pt = prodTask[f];
+                    std::string nm = tn + "." + std::to_string(a) + "." + kFieldNames[f] + u;
+                    if (pt < (int)t) g->AddInternalChannel(outs[pt][prodSlot[f]], ins[t][a], nm.c_str());
                     else if (s == 0) fieldInputs[f].push_back(g->AddInputChannel(ins[t][a], nm.c_str()));
-                    else g->AddInternalChannel(prevOut[f], ins[t][a], nm.c_str());
+                    else g->AddInternalChannel(prevOuts[pt][prodSlot[f]], ins[t][a], nm.c_str());
                 }
             }
-            for (int t = 0; t < 6; ++t) prevOut[kTaskInputs[t][0]] = outs[t][0];
+            prevOuts = outs;
         }
         for (int f = 0; f < 6; ++f) {
-            outputs[f] = g->AddOutputChannel(prevOut[f], (std::string(kFieldNames[f]) + ".out").c_str());
+            outputs[f] = g->AddOutputChannel(prevOuts[prodTask[f]][prodSlot[f]], (std::string(kFieldNames[f]) + ".out").c_str());
             if (o.eager) outputs[f]->SetViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY_EAGER);
         }
         for (GraphInputChannel *c : paramInputs) c->SetNoDraw();
@@ -233,12 +295,14 @@ int main(int argc, char **argv) {
     if (!Parse(argc, argv, o)) {
         fprintf(stderr, "usage: ptk_fdtd --kernels <fdtd.hsaco|hosttasks.so> [--n N | --nx --ny --nz] "
                         "[--steps S] [--mode looped|unrolled] [--blocks B] [--repeat R] [--seed S] "
-                        "[--geometry default|reference|compute] [--dump file] [--eager] [--profile]\n");
+                        "[--form six|halves|fused] [--geometry default|reference|compute] [--dump file] "
+                        "[--eager] [--profile] [--chain]\n");
         return 2;
     }
     Runtime::SetExitOnRuntimeFailure(FALSE);
     if (o.threads) Runtime::SetTaskThreadPoolSize((UINT)o.threads);
     if (o.profile) Runtime::SetProfilePSDispatch(TRUE);
+    Runtime::SetTaskStreamPolicy(o.streams == "shared" ? PTK_STREAMS_SHARED : PTK_STREAMS_PER_TASK);
     if (o.mode == "unrolled") Runtime::SetDefaultChannelCapacity(std::max(4, o.blocks));
     if (PTFAILED(Runtime::Initialize())) {
         fprintf(stderr, "Runtime::Initialize failed\n");
@@ -267,21 +331,45 @@ int main(int argc, char **argv) {
     std::vector<double> runMs;
     unsigned long long h2d0 = 0, d2h0 = 0, h2db = 0, d2hb = 0;
     std::vector<float> final[6];
-    const int runs = o.mode == "looped" ? o.repeat : 1;
+    // chained: warmup runs, then the timed runs, then one run whose outputs are downloaded
+    const int runs = o.mode == "looped" ? (o.chain ? o.warmup + o.repeat + 1 : o.repeat) : 1;
+    std::vector<std::pair<double, UINT>> profileBase;
     const int blocks = o.mode == "looped" ? 1 : o.blocks;
     unsigned long long dispatch0 = 0;
     bool ok = true;
+    std::vector<Datablock *> carried;  // chained runs: last run's outputs, still on the device
+    Accelerator *hipAcc = G.hip ? Runtime::GetAccelerator(ACCELERATOR_CLASS_HIP, 0) : nullptr;
+    double chainedMs = 0;
+    CHighResolutionTimer chainTimer(gran_msec);
     for (int r = 0; r < runs && ok; ++r) {
+        const bool last = r == runs - 1;
+        if (o.chain && r == o.warmup) {
+            // timed region of chained runs: from a drained device to the last run's completion
+            G.g->WaitQuiescent(600000);
+            if (hipAcc) hipAcc->Synchronize();
+            for (UINT t = 0; t < G.g->GetTaskCount(); ++t) G.g->GetTaskByIndex(t)->GetTotalDispatchDeviceMs();
+            profileBase.clear();
+            for (UINT t = 0; t < G.g->GetTaskCount(); ++t)
+                profileBase.push_back(std::make_pair(G.g->GetTaskByIndex(t)->GetTotalDispatchDeviceMs(),
+                                                     G.g->GetTaskByIndex(t)->GetDispatchCount()));
+            chainTimer.reset();
+        }
         Runtime::ResetTransferStatistics();
         dispatch0 = G.g->GetDispatchCount();
         timer.reset();
         for (int b = 0; b < blocks; ++b) {
             for (int f = 0; f < 6; ++f) {
-                Datablock *db = Runtime::AllocateDatablock(G.field[f], init[f].data(),
-                                                           (UINT)(init[f].size() * sizeof(float)), nullptr);
+                Datablock *db = nullptr;
+                if (!carried.empty()) {
+                    db = carried[f];
+                } else {
+                    db = Runtime::AllocateDatablock(G.field[f], init[f].data(),
+                                                    (UINT)(init[f].size() * sizeof(float)), nullptr);
+                }
                 for (GraphInputChannel *c : G.fieldInputs[f]) c->Push(db);
                 db->Release();
             }
+            carried.clear();
             if (o.mode == "looped") {
                 Datablock *it = Runtime::AllocateDatablock(G.iters, &o.steps, sizeof(int), nullptr);
                 for (GraphInputChannel *c : G.iterInputs) c->Push(it);
@@ -296,6 +384,11 @@ int main(int argc, char **argv) {
                     ok = false;
                     break;
                 }
+                if (o.chain && !last) {
+                    res->ClearAllControlSignals();
+                    carried.push_back(res);  // keep our reference; never touched on the host
+                    continue;
+                }
                 res->Lock();
                 const float *p = (const float *)res->GetDataPointer(FALSE);
                 final[f].assign(p, p + init[f].size());
@@ -303,13 +396,38 @@ int main(int argc, char **argv) {
                 res->Release();
             }
         }
+        if (o.chain && r == runs - 2) {
+            // every timed run has been issued and pulled; wait for the device to finish them
+            G.g->WaitQuiescent(600000);
+            if (hipAcc) hipAcc->Synchronize();
+            chainedMs = chainTimer.elapsed(false);
+        }
         runMs.push_back(timer.elapsed(false));
         Runtime::GetTransferStatistics(&h2d0, &h2db, &d2h0, &d2hb);
+        if (o.chain && !last && r > 0)
+            fprintf(stderr, "run %d: %.3f ms, h2d %llu, d2h %llu\n", r, runMs.back(), h2d0, d2h0);
     }
     unsigned long long dispatches = G.g->GetDispatchCount() - dispatch0;
+    // per-task device time of the timed runs (profiling events resolved now, off the clock)
+    std::string taskJson;
     double devMs = 0;
-    if (o.profile)
-        for (UINT t = 0; t < G.g->GetTaskCount(); ++t) devMs += G.g->GetTaskByIndex(t)->GetTotalDispatchDeviceMs();
+    if (o.profile) {
+        for (UINT t = 0; t < G.g->GetTaskCount(); ++t) {
+            Task *task = G.g->GetTaskByIndex(t);
+            double ms = task->GetTotalDispatchDeviceMs();
+            double n = task->GetDispatchCount();
+            if (t < profileBase.size()) {
+                ms -= profileBase[t].first;
+                n -= profileBase[t].second;
+            }
+            if (o.chain) n -= blocks;  // the untimed download run
+            devMs += ms;
+            char buf[256];
+            snprintf(buf, sizeof(buf), "%s{\"task\": \"%s\", \"dispatches\": %.0f, \"device_ms\": %.4f}",
+                     taskJson.empty() ? "" : ", ", task->GetTaskName(), n, ms);
+            taskJson += buf;
+        }
+    }
     G.g->Stop();
     G.g->Teardown();
     Graph::DestroyGraph(G.g);
@@ -319,17 +437,22 @@ int main(int argc, char **argv) {
         for (int f = 0; fp && f < 6; ++f) fwrite(final[f].data(), sizeof(float), final[f].size(), fp);
         if (fp) fclose(fp);
     }
+    const double cells = (double)o.nx * o.ny * o.nz;
     double best = runMs.empty() ? 0 : runMs[0];
     for (double m : runMs) best = std::min(best, m);
-    const double cells = (double)o.nx * o.ny * o.nz;
-    printf("{\"ok\": %s, \"mode\": \"%s\", \"backend\": \"%s\", \"grid\": [%d, %d, %d], \"steps\": %d, "
-           "\"blocks\": %d, \"runs\": %d, \"best_ms\": %.4f, \"last_ms\": %.4f, "
+    // chained: the timed region covers o.repeat runs from a drained device to completion
+    double timedMs = o.chain ? chainedMs : best;
+    int timedRuns = o.chain ? o.repeat : 1;
+    printf("{\"ok\": %s, \"mode\": \"%s\", \"form\": \"%s\", \"backend\": \"%s\", \"grid\": [%d, %d, %d], \"steps\": %d, "
+           "\"blocks\": %d, \"runs\": %d, \"chained\": %s, \"timed_runs\": %d, \"timed_ms\": %.4f, "
+           "\"ms_per_run\": %.4f, \"best_ms\": %.4f, "
            "\"mcups\": %.3f, \"dispatches_last_run\": %llu, \"h2d_count\": %llu, \"h2d_bytes\": %llu, "
-           "\"d2h_count\": %llu, \"d2h_bytes\": %llu, \"device_ms_profiled\": %.4f, \"checksums\": [",
-           ok ? "true" : "false", o.mode.c_str(), G.hip ? "hip" : "host", o.nx, o.ny, o.nz, o.steps,
-           blocks, (int)runMs.size(), best, runMs.empty() ? 0.0 : runMs.back(),
-           best > 0 ? cells * o.steps * blocks / (best / 1e3) / 1e6 : 0.0, dispatches, h2d0, h2db,
-           d2h0, d2hb, devMs);
+           "\"d2h_count\": %llu, \"d2h_bytes\": %llu, \"device_ms_profiled\": %.4f, \"tasks\": [%s], \"checksums\": [",
+           ok ? "true" : "false", o.mode.c_str(), o.form.c_str(), G.hip ? "hip" : "host", o.nx, o.ny, o.nz, o.steps,
+           blocks, (int)runMs.size(), o.chain ? "true" : "false", timedRuns, timedMs,
+           timedRuns ? timedMs / timedRuns : 0.0, best,
+           timedMs > 0 ? cells * o.steps * blocks * timedRuns / (timedMs / 1e3) / 1e6 : 0.0, dispatches, h2d0, h2db,
+           d2h0, d2hb, devMs, taskJson.c_str());
     for (int f = 0; f < 6; ++f) {
         double s = 0;
         for (float v : final[f]) s += v;

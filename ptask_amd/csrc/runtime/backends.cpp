@@ -224,6 +224,7 @@ HIPAccelerator::HIPAccelerator(UINT uiId, int nDevice, UINT uiMemSpace)
 
 HIPAccelerator::~HIPAccelerator() {
     MakeDeviceContextCurrent();
+    if (m_pShared) m_pShared->Release();
     if (m_pXfer) m_pXfer->Release();
     for (auto &kv : m_mModules) ptk_hip_module_unload(kv.second);
 }
@@ -264,6 +265,13 @@ BOOL HIPAccelerator::Compile(const char *lpszFile, const char *lpszOperation, Co
     }
     pKernel->SetPlatformSpecificBinary(this, (void *)f, (void *)m);
     return TRUE;
+}
+
+AsyncContext *HIPAccelerator::GetSharedTaskContext() {
+    if (RT().streamPolicy != PTK_STREAMS_SHARED) return nullptr;
+    std::lock_guard<std::mutex> g(m_modLock);
+    if (!m_pShared) m_pShared = CreateAsyncContext(ASYNCCTXT_TASK);
+    return m_pShared;
 }
 
 BOOL HIPAccelerator::Synchronize() {
@@ -325,15 +333,13 @@ BOOL HIPAccelerator::CopyDeviceToDevice(PBuffer *pDst, PBuffer *pSrc, size_t cb,
 
 HIPTask::HIPTask(Graph *pGraph, CompiledKernel *pKernel, const char *lpszName)
     : Task(pGraph, pKernel, ACCELERATOR_CLASS_HIP, lpszName), m_pAccelerator(nullptr),
-      m_pContext(nullptr), m_hProfStart(nullptr), m_hProfEnd(nullptr) {}
+      m_pContext(nullptr) {}
 
 HIPTask::~HIPTask() { ReleaseDeviceResources(); }
 
 void HIPTask::ReleaseDeviceResources() {
     if (m_pAccelerator) m_pAccelerator->MakeDeviceContextCurrent();
-    if (m_hProfStart) ptk_hip_event_destroy(m_hProfStart);
-    if (m_hProfEnd) ptk_hip_event_destroy(m_hProfEnd);
-    m_hProfStart = m_hProfEnd = nullptr;
+    ResolveProfile();
     if (m_pContext) {
         m_pContext->SynchronizeContext();
         m_pContext->Release();
@@ -353,7 +359,9 @@ BOOL HIPTask::PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vecto
     }
     m_pAccelerator->MakeDeviceContextCurrent();
     if (!m_pContext) {
-        m_pContext = static_cast<HIPAsyncContext *>(m_pAccelerator->CreateAsyncContext(ASYNCCTXT_TASK));
+        m_pContext = static_cast<HIPAsyncContext *>(m_pAccelerator->GetSharedTaskContext());
+        if (m_pContext) m_pContext->AddRef();
+        else m_pContext = static_cast<HIPAsyncContext *>(m_pAccelerator->CreateAsyncContext(ASYNCCTXT_TASK));
         if (!m_pContext) return Runtime::HandleError("%s: stream creation failed", GetTaskName());
     }
     ptk_hip_function fn = (ptk_hip_function)m_pKernel->GetPlatformSpecificBinary(m_pAccelerator);
@@ -388,11 +396,12 @@ BOOL HIPTask::PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vecto
     GeometryFor(vIn, &grid, &block);
     ptk_hip_stream s = m_pContext->GetStream();
     const BOOL prof = Runtime::GetProfilePSDispatch();
-    if (prof && !m_hProfStart) {
-        CheckHip(ptk_hip_event_create(&m_hProfStart, 1), "event create");
-        CheckHip(ptk_hip_event_create(&m_hProfEnd, 1), "event create");
+    ptk_hip_event evStart = nullptr, evEnd = nullptr;
+    if (prof) {
+        CheckHip(ptk_hip_event_create(&evStart, 1), "event create");
+        CheckHip(ptk_hip_event_create(&evEnd, 1), "event create");
+        ptk_hip_event_record(evStart, s);
     }
-    if (prof) ptk_hip_event_record(m_hProfStart, s);
     for (UINT it = 0; it < uiIterations; ++it) {
         int rc = ptk_hip_launch(fn, (unsigned)grid.x, (unsigned)grid.y, (unsigned)grid.z,
                                 (unsigned)block.x, (unsigned)block.y, (unsigned)block.z, 0, s,
@@ -402,21 +411,40 @@ BOOL HIPTask::PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vecto
                                         GetTaskName(), m_pKernel->GetOperation(), grid.x, grid.y,
                                         grid.z, block.x, block.y, block.z, ptk_hip_last_error_string());
     }
-    if (prof) ptk_hip_event_record(m_hProfEnd, s);
+    if (prof) {
+        ptk_hip_event_record(evEnd, s);
+        m_vProfEvents.push_back(std::make_pair(evStart, evEnd));
+    }
     // (4) one sync point orders every later access to the buffers touched here
     SyncPoint *sp = m_pContext->CreateSyncPoint();
     if (!sp) return FALSE;
     for (Touch &t : touched) t.buf->AddOutstanding(sp, t.write);
     sp->Release();
-    if (prof) {
-        float ms = 0.f;
-        ptk_hip_event_synchronize(m_hProfEnd);
-        ptk_hip_event_elapsed_ms(&ms, m_hProfStart, m_hProfEnd);
-        m_dLastDeviceMs = ms;
-        m_dTotalDeviceMs += ms;
-    }
     if (RT().forceSynchronous) m_pContext->SynchronizeContext();
     return TRUE;
+}
+
+// Device time of the profiled dispatches: waits for their end events, accumulates the
+// start->end intervals (reference: the ProfilePSDispatch timing of cutask.cpp:709-776, without
+// its per-dispatch synchronisation) and recycles the events.
+void HIPTask::ResolveProfile() {
+    for (auto &pr : m_vProfEvents) {
+        float ms = 0.f;
+        if (ptk_hip_event_synchronize(pr.second) == PTK_HIP_OK &&
+            ptk_hip_event_elapsed_ms(&ms, pr.first, pr.second) == PTK_HIP_OK) {
+            m_dLastDeviceMs = ms;
+            m_dTotalDeviceMs += ms;
+        }
+        ptk_hip_event_destroy(pr.first);
+        ptk_hip_event_destroy(pr.second);
+    }
+    m_vProfEvents.clear();
+}
+
+double HIPTask::GetTotalDispatchDeviceMsResolved() {
+    if (m_pAccelerator) m_pAccelerator->MakeDeviceContextCurrent();
+    ResolveProfile();
+    return m_dTotalDeviceMs;
 }
 
 }  // namespace PTask

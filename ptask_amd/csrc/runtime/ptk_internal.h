@@ -134,6 +134,9 @@ public:
     BOOL CopyDeviceToDevice(PBuffer *pDst, PBuffer *pSrc, size_t cb, AsyncContext *pCtx) override;
     size_t GetMemorySize() override { return m_props.total_global_mem; }
     int GetDeviceOrdinal() { return m_nDevice; }
+    // The one stream every task shares when Runtime::SetTaskStreamPolicy(PTK_STREAMS_SHARED),
+    // else NULL (each task then gets its own stream).
+    AsyncContext *GetSharedTaskContext();
     const ptk_hip_device_props_t &GetProps() { return m_props; }
 
 private:
@@ -142,6 +145,7 @@ private:
     AsyncContext *m_pXfer;
     std::mutex m_modLock;
     std::map<std::string, ptk_hip_module> m_mModules;
+    AsyncContext *m_pShared = nullptr;
 };
 
 class HIPTask : public Task {
@@ -160,7 +164,14 @@ private:
     HIPAccelerator *m_pAccelerator;
     HIPAsyncContext *m_pContext;
     std::vector<unsigned char> m_vArgs;
-    ptk_hip_event m_hProfStart, m_hProfEnd;
+    // per-dispatch (start, end) events while dispatch profiling is on; resolved lazily so the
+    // profiled dispatches run at full speed
+    std::vector<std::pair<ptk_hip_event, ptk_hip_event>> m_vProfEvents;
+    void ResolveProfile();
+
+public:
+    double GetTotalDispatchDeviceMsResolved();
+    double GetTotalDispatchDeviceMs() override { return GetTotalDispatchDeviceMsResolved(); }
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -179,6 +190,7 @@ struct RuntimeState {
     BOOL exitOnFailure = TRUE;
     UINT loggingLevel = 0;
     UINT blockPoolSize = 8;
+    int streamPolicy = 0;           // PTK_STREAMS_PER_TASK or PTK_STREAMS_SHARED
     VIEWMATERIALIZATIONPOLICY outputViewPolicy = VIEWMATERIALIZATIONPOLICY_ON_DEMAND;
     std::vector<Accelerator *> hostAccelerators;
     std::vector<Accelerator *> hipAccelerators;

@@ -461,6 +461,8 @@ void SetLoggingLevel(UINT n) { RT().loggingLevel = n; }
 UINT GetLoggingLevel() { return RT().loggingLevel; }
 void SetExitOnRuntimeFailure(BOOL b) { RT().exitOnFailure = b; }
 BOOL GetExitOnRuntimeFailure() { return RT().exitOnFailure; }
+void SetTaskStreamPolicy(int e) { RT().streamPolicy = e; }
+int GetTaskStreamPolicy() { return RT().streamPolicy; }
 void SetDefaultBlockPoolSize(UINT n) { RT().blockPoolSize = n; }
 UINT GetDefaultBlockPoolSize() { return RT().blockPoolSize; }
 

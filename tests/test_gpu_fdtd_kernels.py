@@ -132,3 +132,114 @@ def test_device_init_matches_host_init(hip, mod):
         h = np.empty_like(a)
         b.download(h)
         assert np.array_equal(h.view(np.uint32), a.view(np.uint32)), FD.FIELDS[f]
+
+
+# ---- fused whole-step kernel (fdtd_fused.hsaco) ----------------------------------------------
+
+FUSED = {"FdtdStepFused2": 2, "FdtdStepFused4": 4, "FdtdStepFused": 8}
+
+
+def fused_geometry(nx, ny, nz, tj, slabs=None, max_tiles=None):
+    ti = -(-(nx + 1) // 64)
+    tjn = -(-(ny + 1) // tj)
+    if max_tiles:
+        ti, tjn = min(ti, max_tiles), min(tjn, max_tiles)
+    if slabs is None:
+        slabs = max(1, -(-(nz + 1) // 16))
+    return (ti, tjn, slabs), (64, tj, 1)
+
+
+def run_fused(hip, mod, name, F, p, steps, geo):
+    cur = upload_fields(hip, F)
+    nxt = [hip.DeviceBuffer(a.nbytes) for a in F]
+    s = hip.Stream()
+    for _ in range(steps):
+        args = hip.ArgPack()
+        for b in cur:
+            args.ptr(b.ptr)
+        args.raw(bytes(p), 4)
+        for b in nxt:
+            args.ptr(b.ptr)
+        hip.launch(mod.function(name), geo[0], geo[1], args, s)
+        cur, nxt = nxt, cur
+    s.synchronize()
+    out = []
+    for f, a in enumerate(F):
+        h = np.empty_like(a)
+        cur[f].download(h)
+        out.append(h)
+    s.destroy()
+    return out
+
+
+@pytest.fixture(scope="module")
+def fmod(hip):
+    return hip.Module(paths.kernel_path("fdtd_fused.hsaco"))
+
+
+@pytest.mark.parametrize("name", sorted(FUSED))
+@pytest.mark.parametrize("shape", [(8, 8, 8), (70, 9, 5), (13, 21, 17), (1, 1, 1), (3, 1, 2)])
+def test_fused_step_bitexact(hip, fmod, name, shape):
+    nx, ny, nz = shape
+    print(f"case fused {name} shape={shape}", flush=True)
+    F = oracle.init_fields(nx, ny, nz, seed=31 + nz)
+    p = FD.make_params(nx, ny, nz)
+    got = run_fused(hip, fmod, name, F, p, 5, fused_geometry(nx, ny, nz, FUSED[name]))
+    assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 5))
+
+
+@pytest.mark.parametrize("variant", ["strided_tiles", "one_plane_slabs", "single_slab"])
+def test_fused_step_any_grid_covers_domain(hip, fmod, variant):
+    nx, ny, nz = 150, 11, 9
+    print(f"case fused grid variant={variant}", flush=True)
+    geo = {"strided_tiles": fused_geometry(nx, ny, nz, 8, slabs=3, max_tiles=2),
+           "one_plane_slabs": fused_geometry(nx, ny, nz, 8, slabs=nz + 1),
+           "single_slab": fused_geometry(nx, ny, nz, 8, slabs=1)}[variant]
+    F = oracle.init_fields(nx, ny, nz, seed=5)
+    got = run_fused(hip, fmod, "FdtdStepFused", F, FD.make_params(nx, ny, nz), 3, geo)
+    assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 3))
+
+
+def run_halves(hip, mod, F, p, steps, geometry):
+    """FdtdHFields then FdtdEFields per step (72 B/cell-step form)."""
+    nx, ny, nz = p.Nx, p.Ny, p.Nz
+    cur = upload_fields(hip, F)
+    nxt = [hip.DeviceBuffer(a.nbytes) for a in F]
+    s = hip.Stream()
+    grid, block = geometry(nx + 1, ny + 1, nz + 1)
+    for _ in range(steps):
+        a = hip.ArgPack()
+        for f in (0, 1, 2, 3, 4, 5):
+            a.ptr(cur[f].ptr)
+        a.raw(bytes(p), 4)
+        for f in (0, 1, 2):
+            a.ptr(nxt[f].ptr)
+        hip.launch(mod.function("FdtdHFields"), grid, block, a, s)
+        e = hip.ArgPack()
+        for f in (3, 4, 5):
+            e.ptr(cur[f].ptr)
+        for f in (0, 1, 2):
+            e.ptr(nxt[f].ptr)
+        e.raw(bytes(p), 4)
+        for f in (3, 4, 5):
+            e.ptr(nxt[f].ptr)
+        hip.launch(mod.function("FdtdEFields"), grid, block, e, s)
+        cur, nxt = nxt, cur
+    s.synchronize()
+    out = []
+    for f, arr in enumerate(F):
+        h = np.empty_like(arr)
+        cur[f].download(h)
+        out.append(h)
+    s.destroy()
+    return out
+
+
+@pytest.mark.parametrize("geom", sorted(GEOMS))
+@pytest.mark.parametrize("shape", [(8, 8, 8), (13, 7, 5), (1, 4, 1), (5, 1, 3)])
+def test_half_step_kernels_bitexact(hip, fmod, shape, geom):
+    nx, ny, nz = shape
+    print(f"case halves shape={shape} geom={geom}", flush=True)
+    F = oracle.init_fields(nx, ny, nz, seed=77 + ny)
+    got = run_halves(hip, fmod, F, FD.make_params(nx, ny, nz), 4, GEOMS[geom])
+    assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 4))
