@@ -90,6 +90,40 @@ def fdtd_run(fields: list[np.ndarray], params: FdtdParams, steps: int,
     return [keep[fp[i]] for i in range(6)]
 
 
+def pfxsum(a: np.ndarray) -> np.ndarray:
+    """Exclusive scan with wrap-around (host_pfxsum, ptaskUnitTest/pfxsum.cpp:56-69)."""
+    L = lib()
+    if a.dtype == np.int64:
+        fn = L.ptk_oracle_pfxsum_i64
+    else:
+        a = a.astype(np.int32)
+        fn = L.ptk_oracle_pfxsum_i32
+    a = np.ascontiguousarray(a)
+    out = np.empty_like(a)
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong]
+    fn(a.ctypes.data, out.ctypes.data, a.size)
+    return out
+
+
+def groupby(keys: np.ndarray, vals: np.ndarray, num_keys: int):
+    """The groupby pipeline (oracle/groupby.c): returns (groups[G,3] of key,count,sum int64,
+    header (G, rows, errors), offsets[num_keys+1], out_keys, out_vals) with groups in ascending
+    key order and rows of a group in input order."""
+    L = lib()
+    keys = np.ascontiguousarray(keys, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.int32)
+    n = keys.size
+    groups = np.zeros(3 + 3 * (num_keys + 1), dtype=np.int64)
+    offsets = np.zeros(num_keys + 1, dtype=np.int64)
+    ok, ov = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+    fn = L.ptk_oracle_groupby
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_int] + [C.c_void_p] * 4
+    fn(keys.ctypes.data, vals.ctypes.data, n, num_keys, groups.ctypes.data, offsets.ctypes.data,
+       ok.ctypes.data, ov.ctypes.data)
+    G = int(groups[0])
+    return groups[3:3 + 3 * G].reshape(G, 3), groups[:3].copy(), offsets, ok, ov
+
+
 def fdtd_update(field: int, own: np.ndarray, a: np.ndarray, b: np.ndarray,
                 params: FdtdParams, dtype=np.float32) -> np.ndarray:
     """One of the six half-step updates (graphfdtd.cpp:185-312) with the reference's argument

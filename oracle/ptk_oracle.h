@@ -36,7 +36,18 @@ void ptk_oracle_fdtd_run_ptrs_f32(float **f, float **scratch, const FDTD_PARAMS 
 void ptk_oracle_fdtd_run_ptrs_f64(double **f, double **scratch, const FDTD_PARAMS *p, int steps);
 void ptk_oracle_fdtd_init_field(float *a, int field, int Nx, int Ny, int Nz, uint64_t seed);
 
+/* Prefix sum and groupby (oracle/groupby.c) */
+void ptk_oracle_pfxsum_i32(const int32_t *in, int32_t *out, long long n);
+void ptk_oracle_pfxsum_i64(const int64_t *in, int64_t *out, long long n);
+void ptk_oracle_groupby(const int32_t *keys, const int32_t *vals, long long n, int numKeys,
+                        int64_t *groups, int64_t *offsets, int32_t *outKeys, int32_t *outVals);
+
 /* Host-task ABI forms (void f(UINT, void**)), loaded by name by the HostAccelerator. */
+void pfxsum(unsigned int n, void **a);
+void GroupbyClear(unsigned int n, void **a);
+void GroupbyAccumulate(unsigned int n, void **a);
+void GroupbyOffsets(unsigned int n, void **a);
+void GroupbyReshuffle(unsigned int n, void **a);
 void HxComputation(unsigned int n, void **a);
 void HyComputation(unsigned int n, void **a);
 void HzComputation(unsigned int n, void **a);

@@ -1,0 +1,325 @@
+// ptk_groupby.cpp -- the groupby and prefix-sum graphs as PTask applications (the roles of
+// ptaskUnitTest/graphgroupby.cpp and ptaskUnitTest/cpfxsum.cpp), against ptaskapi.h only.
+//
+// --workload groupby: four tasks per run (graphgroupby.cpp builds the same stages:
+//   allocate/count/prefix-sum/reshuffle, :106-387):
+//     GroupbyClear      (pacing: keys; numKeys)                -> table
+//     GroupbyAccumulate (keys, vals, n, numKeys, table in/out) -> table
+//     GroupbyOffsets    (table, numKeys)                       -> offsets, groups
+//     GroupbyReshuffle  (keys, vals, n, numKeys, offsets, table in/out) -> outKeys, outVals
+// --workload pfxsum: one task pfxsum(pin, pout, N) wired as cpfxsum.cpp:204-226 wires it.
+// --workload scan2: the two-task large scan PfxScanReduce -> PfxScanApply.
+//
+// Keys: Zipf(s) over K distinct keys (rank r drawn by inverse CDF, key = r*A mod K for a fixed A
+// coprime to K, so frequent keys are spread over the key space), or uniform on [0, numKeys] as
+// the reference draws them (graphgroupby.cpp:428).  Values: int32 from the shared generator.
+// Repeated runs reuse the same input datablocks, which stay valid on the device, so only the
+// first run uploads.  Prints one JSON object; --dump writes inputs and outputs.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ptaskapi.h"
+#include "ptk_prng.h"
+
+using namespace PTask;
+
+namespace {
+
+struct Options {
+    std::string workload = "groupby";
+    std::string kernelFile, scanKernelFile;
+    long long rows = 1 << 20;
+    int numKeys = 65535;  // keys lie in [0, numKeys]
+    std::string dist = "zipf";
+    double zipfS = 1.1;
+    unsigned long long seed = 0x4fd478;
+    int repeat = 1, warmup = 0;
+    std::string dump;
+    bool profile = false;
+};
+
+uint64_t Fnv1a(const void *p, size_t n) {
+    const unsigned char *b = (const unsigned char *)p;
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+void MakeInputs(const Options &o, std::vector<int32_t> &keys, std::vector<int32_t> &vals) {
+    const long long n = o.rows;
+    keys.resize((size_t)n);
+    vals.resize((size_t)n);
+    const long long K = (long long)o.numKeys + 1;
+    std::vector<double> cdf;
+    long long A = 40503;
+    if (o.dist == "zipf") {
+        cdf.resize((size_t)K);
+        double acc = 0;
+        for (long long r = 0; r < K; ++r) {
+            acc += std::pow((double)(r + 1), -o.zipfS);
+            cdf[(size_t)r] = acc;
+        }
+        for (double &c : cdf) c /= acc;
+        auto gcd = [](long long a, long long b) { while (b) { long long t = a % b; a = b; b = t; } return a; };
+        while (gcd(A, K) != 1) ++A;
+    }
+    for (long long i = 0; i < n; ++i) {
+        const double u = ptk_rand_unit_f64(o.seed, 1, (uint64_t)i);
+        long long k;
+        if (o.dist == "zipf") {
+            long long r = std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin();
+            if (r >= K) r = K - 1;
+            k = (r * A) % K;
+        } else {
+            k = (long long)(u * (double)o.numKeys);  // [0, numKeys) as rand()/RAND_MAX*numKeys
+        }
+        keys[(size_t)i] = (int32_t)k;
+        vals[(size_t)i] = (int32_t)(uint32_t)ptk_rand_u64(o.seed, 2, (uint64_t)i);
+    }
+}
+
+void Dump(FILE *fp, const void *p, size_t n) {
+    if (fp && n) fwrite(p, 1, n, fp);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    Options o;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto next = [&]() -> const char * {
+            if (i + 1 >= argc) { fprintf(stderr, "missing value for %s\n", a.c_str()); exit(2); }
+            return argv[++i];
+        };
+        if (a == "--workload") o.workload = next();
+        else if (a == "--kernels") o.kernelFile = next();
+        else if (a == "--rows") o.rows = atoll(next());
+        else if (a == "--keys") o.numKeys = atoi(next()) - 1;
+        else if (a == "--num-keys") o.numKeys = atoi(next());
+        else if (a == "--dist") o.dist = next();
+        else if (a == "--zipf-s") o.zipfS = atof(next());
+        else if (a == "--seed") o.seed = strtoull(next(), nullptr, 0);
+        else if (a == "--repeat") o.repeat = atoi(next());
+        else if (a == "--warmup") o.warmup = atoi(next());
+        else if (a == "--dump") o.dump = next();
+        else if (a == "--profile") o.profile = true;
+        else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
+    }
+    if (o.kernelFile.empty() || o.rows < 0 || o.numKeys < 0 || o.rows > 0x7fffffffLL) {
+        fprintf(stderr, "usage: ptk_groupby --kernels <groupby.hsaco|scan.hsaco|hosttasks.so> "
+                        "[--workload groupby|pfxsum|scan2] [--rows N] [--keys K] [--dist zipf|uniform] "
+                        "[--zipf-s S] [--seed S] [--repeat R] [--warmup W] [--dump file] [--profile]\n");
+        return 2;
+    }
+    Runtime::SetExitOnRuntimeFailure(FALSE);
+    if (o.profile) Runtime::SetProfilePSDispatch(TRUE);
+    if (PTFAILED(Runtime::Initialize())) return 1;
+    const bool hip = ptaskutils::SelectAcceleratorClass(o.kernelFile.c_str()) == ACCELERATOR_CLASS_HIP;
+    const int n = (int)o.rows;
+    std::vector<int32_t> keys, vals;
+    MakeInputs(o, keys, vals);
+    const UINT cbRows = (UINT)(sizeof(int32_t) * (size_t)std::max(n, 1));
+    DatablockTemplate *tRows = Runtime::GetDatablockTemplate((char *)"gbRows", sizeof(int32_t), std::max(n, 1), 1, 1);
+    DatablockTemplate *tInt = Runtime::GetDatablockTemplate((char *)"gbInt", sizeof(int), PTPARM_INT);
+    Graph *g = new Graph(o.workload.c_str());
+    UINT uid = 0;
+    std::vector<GraphInputChannel *> keyIn, valIn, nIn, kIn;
+    std::vector<GraphOutputChannel *> outs;
+    std::vector<std::string> outNames;
+    auto K = [&](const char *name) {
+        CompiledKernel *k = Runtime::GetCompiledKernel((char *)o.kernelFile.c_str(), (char *)name);
+        if (!k) {
+            fprintf(stderr, "kernel %s unavailable: %s\n", name, Runtime::GetLastErrorText());
+            exit(1);
+        }
+        return k;
+    };
+    auto geometry = [&](Task *t, int gx, int bx) {
+        if (hip) t->SetBlockAndGridSize(PTASKDIM3(gx, 1, 1), PTASKDIM3(bx, 1, 1));
+        else t->SetComputeGeometry(1, 1, 1);
+    };
+    const long long tableElems = 3ll * ((long long)o.numKeys + 1) + 1;
+    const long long groupElems = 3 + 3ll * ((long long)o.numKeys + 1);
+    if (o.workload == "groupby") {
+        DatablockTemplate *tTable = Runtime::GetDatablockTemplate((char *)"gbTable", sizeof(int64_t), (UINT)tableElems, 1, 1);
+        DatablockTemplate *tOff = Runtime::GetDatablockTemplate((char *)"gbOffsets", sizeof(int64_t), (UINT)(o.numKeys + 1), 1, 1);
+        DatablockTemplate *tGroups = Runtime::GetDatablockTemplate((char *)"gbGroups", sizeof(int64_t), (UINT)groupElems, 1, 1);
+        // GroupbyClear: the keys port only paces one clear per run (no formal index)
+        Port *cIn[2] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"keys(pace)"),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 1)};
+        Port *cOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table", 0)};
+        Task *tc = g->AddTask(K("GroupbyClear"), 2, cIn, 1, cOut, "GroupbyClear");
+        geometry(tc, 256, 256);
+        Port *aIn[5] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"keys", 0),
+                        Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"vals", 1),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", 2),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 3),
+                        Runtime::CreatePort(INPUT_PORT, tTable, uid++, (char *)"table", 4, 0)};
+        Port *aOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table(out)", 4)};
+        Task *ta = g->AddTask(K("GroupbyAccumulate"), 5, aIn, 1, aOut, "GroupbyAccumulate");
+        geometry(ta, 512, 1024);
+        Port *oIn[2] = {Runtime::CreatePort(INPUT_PORT, tTable, uid++, (char *)"table", 0),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 1)};
+        Port *oOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tOff, uid++, (char *)"offsets", 2),
+                         Runtime::CreatePort(OUTPUT_PORT, tGroups, uid++, (char *)"groups", 3)};
+        Task *to = g->AddTask(K("GroupbyOffsets"), 2, oIn, 2, oOut, "GroupbyOffsets");
+        geometry(to, 1, 1024);
+        Port *rIn[6] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"keys", 0),
+                        Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"vals", 1),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", 2),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 3),
+                        Runtime::CreatePort(INPUT_PORT, tOff, uid++, (char *)"offsets", 4),
+                        Runtime::CreatePort(INPUT_PORT, tTable, uid++, (char *)"table", 5, 2)};
+        Port *rOut[3] = {Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outKeys", 6),
+                         Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outVals", 7),
+                         Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table(cursor)", 5)};
+        Task *tr = g->AddTask(K("GroupbyReshuffle"), 6, rIn, 3, rOut, "GroupbyReshuffle");
+        geometry(tr, 2048, 256);
+        keyIn = {g->AddInputChannel(cIn[0], "keys->clear"), g->AddInputChannel(aIn[0], "keys->acc"),
+                 g->AddInputChannel(rIn[0], "keys->shuffle")};
+        valIn = {g->AddInputChannel(aIn[1], "vals->acc"), g->AddInputChannel(rIn[1], "vals->shuffle")};
+        nIn = {g->AddInputChannel(aIn[2], "n->acc"), g->AddInputChannel(rIn[2], "n->shuffle")};
+        kIn = {g->AddInputChannel(cIn[1], "nk->clear"), g->AddInputChannel(aIn[3], "nk->acc"),
+               g->AddInputChannel(oIn[1], "nk->offsets"), g->AddInputChannel(rIn[3], "nk->shuffle")};
+        g->AddInternalChannel(cOut[0], aIn[4], "clear->acc");
+        g->AddInternalChannel(aOut[0], oIn[0], "acc->offsets");
+        g->AddInternalChannel(aOut[0], rIn[5], "acc->shuffle");
+        g->AddInternalChannel(oOut[0], rIn[4], "offsets->shuffle");
+        outs = {g->AddOutputChannel(oOut[1], "groups"), g->AddOutputChannel(rOut[0], "outKeys"),
+                g->AddOutputChannel(rOut[1], "outVals")};
+        outNames = {"groups", "outKeys", "outVals"};
+    } else if (o.workload == "pfxsum") {
+        // cpfxsum.cpp:204-226: pin at 0, pout at 1, N (sticky) at 2
+        Port *in[2] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"pin", 0),
+                       Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"N", 2)};
+        Port *out[1] = {Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"pout", 1)};
+        Task *t = g->AddTask(K("pfxsum"), 2, in, 1, out, "PfxsumTask");
+        geometry(t, 1, 1024);
+        keyIn = {g->AddInputChannel(in[0], "DataInputChannel")};
+        nIn = {g->AddInputChannel(in[1], "ParmsInputChannel")};
+        outs = {g->AddOutputChannel(out[0], "DataOutputChannel")};
+        outNames = {"pout"};
+    } else if (o.workload == "scan2") {
+        const int tiles = std::max(1, (n + 4095) / 4096);
+        DatablockTemplate *tTiles = Runtime::GetDatablockTemplate((char *)"scanTiles", sizeof(int32_t), (UINT)tiles, 1, 1);
+        Port *rIn[2] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"pin", 0),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"N", 1)};
+        Port *rOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tTiles, uid++, (char *)"tileSums", 2)};
+        Task *tr = g->AddTask(K("PfxScanReduce"), 2, rIn, 1, rOut, "PfxScanReduce");
+        geometry(tr, std::min(tiles, 2048), 1024);
+        Port *aIn[3] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"pin", 0),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"N", 1),
+                        Runtime::CreatePort(INPUT_PORT, tTiles, uid++, (char *)"tileSums", 2)};
+        Port *aOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"pout", 3)};
+        Task *ta = g->AddTask(K("PfxScanApply"), 3, aIn, 1, aOut, "PfxScanApply");
+        geometry(ta, std::min(tiles, 2048), 1024);
+        keyIn = {g->AddInputChannel(rIn[0], "pin->reduce"), g->AddInputChannel(aIn[0], "pin->apply")};
+        nIn = {g->AddInputChannel(rIn[1], "N->reduce"), g->AddInputChannel(aIn[1], "N->apply")};
+        g->AddInternalChannel(rOut[0], aIn[2], "tiles");
+        outs = {g->AddOutputChannel(aOut[0], "pout")};
+        outNames = {"pout"};
+    } else {
+        fprintf(stderr, "unknown workload %s\n", o.workload.c_str());
+        return 2;
+    }
+    g->Run();
+    Datablock *bn = Runtime::AllocateDatablock(tInt, (void *)&n, sizeof(int), nullptr);
+    for (auto *c : nIn) c->Push(bn);
+    bn->Release();
+    Datablock *bk = Runtime::AllocateDatablock(tInt, (void *)&o.numKeys, sizeof(int), nullptr);
+    for (auto *c : kIn) c->Push(bk);
+    bk->Release();
+    Datablock *bKeys = Runtime::AllocateDatablock(tRows, keys.data(), cbRows, nullptr);
+    Datablock *bVals = o.workload == "groupby" ? Runtime::AllocateDatablock(tRows, vals.data(), cbRows, nullptr) : nullptr;
+    Accelerator *hipAcc = hip ? Runtime::GetAccelerator(ACCELERATOR_CLASS_HIP, 0) : nullptr;
+    CHighResolutionTimer timer(gran_msec);
+    double timedMs = 0;
+    std::vector<std::vector<unsigned char>> results(outs.size());
+    bool ok = true;
+    const int runs = o.warmup + o.repeat;
+    unsigned long long h2d = 0, d2h = 0;
+    for (int r = 0; r < runs && ok; ++r) {
+        if (r == o.warmup) {
+            g->WaitQuiescent(600000);
+            if (hipAcc) hipAcc->Synchronize();
+            Runtime::ResetTransferStatistics();
+            timer.reset();
+        }
+        for (auto *c : keyIn) c->Push(bKeys);
+        for (auto *c : valIn) c->Push(bVals);
+        const bool last = r == runs - 1;
+        std::vector<Datablock *> got;
+        for (size_t i = 0; i < outs.size(); ++i) {
+            Datablock *b = outs[i]->Pull(600000);
+            if (!b) {
+                fprintf(stderr, "no output on %s\n", outNames[i].c_str());
+                ok = false;
+                break;
+            }
+            got.push_back(b);
+        }
+        if (last && ok) {
+            g->WaitQuiescent(600000);
+            if (hipAcc) hipAcc->Synchronize();
+            timedMs = timer.elapsed(false);
+            Runtime::GetTransferStatistics(&h2d, nullptr, &d2h, nullptr);
+            for (size_t i = 0; i < got.size(); ++i) {
+                got[i]->Lock();
+                const unsigned char *p = (const unsigned char *)got[i]->GetDataPointer(FALSE);
+                results[i].assign(p, p + got[i]->GetDataBufferLogicalSizeBytes());
+                got[i]->Unlock();
+            }
+        }
+        for (Datablock *b : got) b->Release();
+    }
+    std::string taskJson;
+    if (o.profile) {
+        for (UINT t = 0; t < g->GetTaskCount(); ++t) {
+            Task *task = g->GetTaskByIndex(t);
+            char buf[256];
+            snprintf(buf, sizeof(buf), "%s{\"task\": \"%s\", \"dispatches\": %u, \"device_ms\": %.4f}",
+                     taskJson.empty() ? "" : ", ", task->GetTaskName(), task->GetDispatchCount(),
+                     task->GetTotalDispatchDeviceMs());
+            taskJson += buf;
+        }
+    }
+    bKeys->Release();
+    if (bVals) bVals->Release();
+    g->Stop();
+    g->Teardown();
+    Graph::DestroyGraph(g);
+    if (ok && !o.dump.empty()) {
+        FILE *fp = fopen(o.dump.c_str(), "wb");
+        Dump(fp, keys.data(), keys.size() * 4);
+        Dump(fp, vals.data(), vals.size() * 4);
+        for (auto &r : results) Dump(fp, r.data(), r.size());
+        if (fp) fclose(fp);
+    }
+    long long groupsFound = -1;
+    if (o.workload == "groupby" && !results.empty() && results[0].size() >= 8) memcpy(&groupsFound, results[0].data(), 8);
+    printf("{\"ok\": %s, \"workload\": \"%s\", \"backend\": \"%s\", \"rows\": %d, \"num_keys\": %d, "
+           "\"dist\": \"%s\", \"runs\": %d, \"timed_runs\": %d, \"timed_ms\": %.4f, \"ms_per_run\": %.4f, "
+           "\"rows_per_s\": %.1f, \"groups\": %lld, \"h2d_count_timed\": %llu, \"d2h_count_timed\": %llu, "
+           "\"tasks\": [%s], \"outputs\": [",
+           ok ? "true" : "false", o.workload.c_str(), hip ? "hip" : "host", n, o.numKeys, o.dist.c_str(),
+           runs, o.repeat, timedMs, o.repeat ? timedMs / o.repeat : 0.0,
+           timedMs > 0 ? (double)n * o.repeat / (timedMs / 1e3) : 0.0, groupsFound, h2d, d2h,
+           taskJson.c_str());
+    for (size_t i = 0; i < results.size(); ++i)
+        printf("%s{\"name\": \"%s\", \"bytes\": %zu, \"fnv1a\": \"%016llx\"}", i ? ", " : "", outNames[i].c_str(),
+               results[i].size(), (unsigned long long)Fnv1a(results[i].data(), results[i].size()));
+    printf("]}\n");
+    Runtime::Terminate();
+    return ok ? 0 : 1;
+}

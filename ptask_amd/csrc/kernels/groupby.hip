@@ -1,0 +1,191 @@
+// groupby.hip -- the groupby graph's task kernels (code object groupby.hsaco).
+//
+// The reference graph (ptaskUnitTest/graphgroupby.cpp:106-387) counts unique keys, builds
+// per-key counts in a device hash table, prefix-sums them (single block) and reshuffles the
+// input so equal keys are contiguous; its kernels are not in the tree.  This pipeline keeps that
+// shape and adds an int32 value column aggregated to int64 sum and count per key:
+//
+//   GroupbyClear(long long* table, int numKeys)                       -> table zeroed
+//   GroupbyAccumulate(int* keys, int* vals, int n, int numKeys, long long* table)  [table in/out]
+//   GroupbyOffsets(long long* table, int numKeys, long long* offsets, long long* groups)
+//   GroupbyReshuffle(int* keys, int* vals, int n, int numKeys, long long* offsets,
+//                    long long* table, int* outKeys, int* outVals)     [table in/out: cursors]
+//
+// Keys lie in [0, numKeys] as in the reference test (graphgroupby.cpp:428); a key outside is
+// counted in the table's error slot and otherwise ignored, so no access leaves its array.
+// table layout (int64): count[numKeys+1] | sum[numKeys+1] | cursor[numKeys+1] | errors[1].
+// groups layout (int64): header {G, rows, errors} then G triples {key, count, sum} in
+// ascending key order.  offsets[k] = rows with a smaller key (exclusive scan of count).
+//
+// Sums and counts use exact int64 (wrapping) additions, whose result does not depend on the
+// order of the atomics: every output except the order of values inside one group is
+// bit-reproducible.
+#include <hip/hip_runtime.h>
+
+#define GB_LDS_SLOTS 4096   // per-block pre-aggregation table (power of two)
+#define GB_PROBES 8         // linear probes before a row falls back to global atomics
+#define GB_EMPTY 0xFFFFFFFFu
+
+__device__ inline unsigned gb_hash(unsigned k) {
+    k ^= k >> 16;
+    k *= 0x7feb352du;
+    k ^= k >> 15;
+    return k;
+}
+
+__device__ inline unsigned long long *gb_count(long long *t, int numKeys) { return (unsigned long long *)t; }
+__device__ inline unsigned long long *gb_sum(long long *t, int numKeys) { return (unsigned long long *)t + (numKeys + 1); }
+__device__ inline unsigned long long *gb_cursor(long long *t, int numKeys) { return (unsigned long long *)t + 2ll * (numKeys + 1); }
+__device__ inline unsigned long long *gb_errors(long long *t, int numKeys) { return (unsigned long long *)t + 3ll * (numKeys + 1); }
+
+extern "C" __global__ void __launch_bounds__(1024) GroupbyClear(long long *table, int numKeys) {
+    if (numKeys < 0) return;
+    const long long n = 3ll * (numKeys + 1) + 1;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        table[i] = 0;
+}
+
+// Rows are read once (8 B each).  Each block keeps a small open-addressed table in LDS so the
+// frequent keys of a skewed (Zipf) stream are aggregated with LDS atomics; a key that finds no
+// slot within GB_PROBES probes goes straight to the global table.  The block then flushes its
+// LDS table with one global atomic per slot and field.
+extern "C" __global__ void __launch_bounds__(1024)
+GroupbyAccumulate(const int *__restrict__ keys, const int *__restrict__ vals, int n, int numKeys,
+                  long long *table) {
+    __shared__ unsigned sKey[GB_LDS_SLOTS];
+    __shared__ unsigned sCnt[GB_LDS_SLOTS];
+    __shared__ unsigned long long sSum[GB_LDS_SLOTS];
+    if (numKeys < 0 || n <= 0) return;
+    for (int s = threadIdx.x; s < GB_LDS_SLOTS; s += blockDim.x) {
+        sKey[s] = GB_EMPTY;
+        sCnt[s] = 0;
+        sSum[s] = 0;
+    }
+    __syncthreads();
+    unsigned long long *gCnt = gb_count(table, numKeys), *gSum = gb_sum(table, numKeys);
+    // this block's contiguous chunk of rows
+    const long long per = ((long long)n + gridDim.x - 1) / gridDim.x;
+    const long long r0 = per * blockIdx.x, r1 = r0 + per < n ? r0 + per : n;
+    unsigned long long localErr = 0;
+    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const int k = keys[r];
+        const unsigned long long v = (unsigned long long)(long long)vals[r];
+        if (k < 0 || k > numKeys) {
+            ++localErr;
+            continue;
+        }
+        const unsigned uk = (unsigned)k;
+        unsigned h = gb_hash(uk) & (GB_LDS_SLOTS - 1);
+        bool done = false;
+        for (int p = 0; p < GB_PROBES; ++p) {
+            unsigned cur = sKey[h];
+            if (cur == GB_EMPTY) cur = atomicCAS(&sKey[h], GB_EMPTY, uk);
+            if (cur == GB_EMPTY || cur == uk) {
+                atomicAdd(&sCnt[h], 1u);
+                atomicAdd(&sSum[h], v);
+                done = true;
+                break;
+            }
+            h = (h + 1) & (GB_LDS_SLOTS - 1);
+        }
+        if (!done) {
+            atomicAdd(&gCnt[uk], 1ull);
+            atomicAdd(&gSum[uk], v);
+        }
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < GB_LDS_SLOTS; s += blockDim.x) {
+        const unsigned k = sKey[s];
+        if (k == GB_EMPTY) continue;
+        atomicAdd(&gCnt[k], (unsigned long long)sCnt[s]);
+        atomicAdd(&gSum[k], sSum[s]);
+    }
+    if (localErr) atomicAdd(gb_errors(table, numKeys), localErr);
+}
+
+// ---- one-block scans -------------------------------------------------------------------------
+
+__device__ inline unsigned long long wave_incl(unsigned long long v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned long long u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+// Exclusive block scan of one value per thread; blockDim.x a multiple of 64.
+__device__ inline unsigned long long block_excl(unsigned long long v, unsigned long long *total) {
+    __shared__ unsigned long long ws[16];
+    __shared__ unsigned long long tot;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    unsigned long long inc = wave_incl(v);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        unsigned long long s = lane < nw ? ws[lane] : 0ull;
+        unsigned long long si = wave_incl(s);
+        if (lane < nw) ws[lane] = si - s;
+        if (lane == nw - 1) tot = si;
+    }
+    __syncthreads();
+    unsigned long long r = ws[w] + inc - v;
+    *total = tot;
+    __syncthreads();
+    return r;
+}
+
+// One block: offsets = exclusive scan of counts; groups = the non-empty keys in ascending order
+// (their ranks = exclusive scan of count>0), with the header {G, rows, errors}.
+extern "C" __global__ void __launch_bounds__(1024)
+GroupbyOffsets(long long *table, int numKeys, long long *offsets, long long *groups) {
+    if ((blockDim.x & 63) != 0 || blockIdx.x != 0 || numKeys < 0) return;
+    const unsigned long long *cnt = gb_count(table, numKeys), *sum = gb_sum(table, numKeys);
+    const long long K = (long long)numKeys + 1;
+    unsigned long long carryRows = 0, carryGroups = 0;
+    for (long long base = 0; base < K; base += blockDim.x) {
+        const long long k = base + threadIdx.x;
+        const unsigned long long c = k < K ? cnt[k] : 0ull;
+        unsigned long long totRows, totGroups;
+        const unsigned long long exRows = block_excl(c, &totRows);
+        const unsigned long long exGroups = block_excl(c ? 1ull : 0ull, &totGroups);
+        if (k < K) {
+            offsets[k] = (long long)(carryRows + exRows);
+            if (c) {
+                long long *g = groups + 3 + 3 * (long long)(carryGroups + exGroups);
+                g[0] = k;
+                g[1] = (long long)c;
+                g[2] = (long long)sum[k];
+            }
+        }
+        carryRows += totRows;
+        carryGroups += totGroups;
+    }
+    if (threadIdx.x == 0) {
+        groups[0] = (long long)carryGroups;
+        groups[1] = (long long)carryRows;
+        groups[2] = (long long)*gb_errors(table, numKeys);
+    }
+}
+
+// Scatter every row to offsets[key] + (rows of that key placed before it).  The position
+// inside a group comes from an atomic cursor, so equal keys are contiguous and the keys output
+// is exact; the order of values within a group is not fixed.
+extern "C" __global__ void __launch_bounds__(1024)
+GroupbyReshuffle(const int *__restrict__ keys, const int *__restrict__ vals, int n, int numKeys,
+                 const long long *__restrict__ offsets, long long *table, int *__restrict__ outKeys,
+                 int *__restrict__ outVals) {
+    if (numKeys < 0) return;
+    unsigned long long *cursor = gb_cursor(table, numKeys);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        const int k = keys[r];
+        if (k < 0 || k > numKeys) continue;
+        const unsigned long long slot = atomicAdd(&cursor[k], 1ull);
+        const long long pos = offsets[k] + (long long)slot;
+        if (pos < 0 || pos >= n) continue;  // only possible if the table is inconsistent
+        outKeys[pos] = k;
+        outVals[pos] = vals[r];
+    }
+}
