@@ -43,6 +43,7 @@ struct Options {
     int repeat = 1, warmup = 0;
     std::string dump;
     bool profile = false;
+    int gbGrid = 256;  // blocks of the two row-parallel groupby kernels (measured best at 8M rows)
 };
 
 uint64_t Fnv1a(const void *p, size_t n) {
@@ -114,6 +115,7 @@ int main(int argc, char **argv) {
         else if (a == "--warmup") o.warmup = atoi(next());
         else if (a == "--dump") o.dump = next();
         else if (a == "--profile") o.profile = true;
+        else if (a == "--gb-grid") o.gbGrid = atoi(next());
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
     if (o.kernelFile.empty() || o.rows < 0 || o.numKeys < 0 || o.rows > 0x7fffffffLL) {
@@ -168,7 +170,7 @@ int main(int argc, char **argv) {
                         Runtime::CreatePort(INPUT_PORT, tTable, uid++, (char *)"table", 4, 0)};
         Port *aOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table(out)", 4)};
         Task *ta = g->AddTask(K("GroupbyAccumulate"), 5, aIn, 1, aOut, "GroupbyAccumulate");
-        geometry(ta, 512, 1024);
+        geometry(ta, o.gbGrid, 1024);
         Port *oIn[2] = {Runtime::CreatePort(INPUT_PORT, tTable, uid++, (char *)"table", 0),
                         Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 1)};
         Port *oOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tOff, uid++, (char *)"offsets", 2),
@@ -185,7 +187,7 @@ int main(int argc, char **argv) {
                          Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outVals", 7),
                          Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table(cursor)", 5)};
         Task *tr = g->AddTask(K("GroupbyReshuffle"), 6, rIn, 3, rOut, "GroupbyReshuffle");
-        geometry(tr, 2048, 256);
+        geometry(tr, o.gbGrid, 1024);
         keyIn = {g->AddInputChannel(cIn[0], "keys->clear"), g->AddInputChannel(aIn[0], "keys->acc"),
                  g->AddInputChannel(rIn[0], "keys->shuffle")};
         valIn = {g->AddInputChannel(aIn[1], "vals->acc"), g->AddInputChannel(rIn[1], "vals->shuffle")};

@@ -143,47 +143,106 @@ GroupbyOffsets(long long *table, int numKeys, long long *offsets, long long *gro
     if ((blockDim.x & 63) != 0 || blockIdx.x != 0 || numKeys < 0) return;
     const unsigned long long *cnt = gb_count(table, numKeys), *sum = gb_sum(table, numKeys);
     const long long K = (long long)numKeys + 1;
-    unsigned long long carryRows = 0, carryGroups = 0;
-    for (long long base = 0; base < K; base += blockDim.x) {
-        const long long k = base + threadIdx.x;
-        const unsigned long long c = k < K ? cnt[k] : 0ull;
-        unsigned long long totRows, totGroups;
-        const unsigned long long exRows = block_excl(c, &totRows);
-        const unsigned long long exGroups = block_excl(c ? 1ull : 0ull, &totGroups);
-        if (k < K) {
-            offsets[k] = (long long)(carryRows + exRows);
-            if (c) {
-                long long *g = groups + 3 + 3 * (long long)(carryGroups + exGroups);
-                g[0] = k;
-                g[1] = (long long)c;
-                g[2] = (long long)sum[k];
-            }
+    // each thread owns a contiguous run of keys: local totals, one block scan, then a serial
+    // pass over the run
+    const long long per = (K + blockDim.x - 1) / blockDim.x;
+    const long long k0 = per * threadIdx.x, k1 = k0 + per < K ? k0 + per : K;
+    unsigned long long rows = 0, nz = 0;
+    for (long long k = k0; k < k1; ++k) {
+        const unsigned long long c = cnt[k];
+        rows += c;
+        nz += c ? 1ull : 0ull;
+    }
+    unsigned long long totRows, totGroups;
+    unsigned long long r = block_excl(rows, &totRows);
+    unsigned long long gi = block_excl(nz, &totGroups);
+    for (long long k = k0; k < k1; ++k) {
+        const unsigned long long c = cnt[k];
+        offsets[k] = (long long)r;
+        if (c) {
+            long long *g = groups + 3 + 3 * (long long)gi;
+            g[0] = k;
+            g[1] = (long long)c;
+            g[2] = (long long)sum[k];
+            ++gi;
         }
-        carryRows += totRows;
-        carryGroups += totGroups;
+        r += c;
     }
     if (threadIdx.x == 0) {
-        groups[0] = (long long)carryGroups;
-        groups[1] = (long long)carryRows;
+        groups[0] = (long long)totGroups;
+        groups[1] = (long long)totRows;
         groups[2] = (long long)*gb_errors(table, numKeys);
     }
 }
 
-// Scatter every row to offsets[key] + (rows of that key placed before it).  The position
-// inside a group comes from an atomic cursor, so equal keys are contiguous and the keys output
-// is exact; the order of values within a group is not fixed.
+// Scatter every row to offsets[key] + (rows of that key placed before it), so equal keys are
+// contiguous and the keys output is exact; the order of values within a group is not fixed.
+//
+// A block owns a contiguous chunk of rows.  Pass 1 counts the chunk's rows per key in an LDS
+// table (the same insertion rule as GroupbyAccumulate: a key either gets a slot on its first
+// row, and then every later row finds it, or never does).  Each slot then reserves its rows'
+// range with ONE global atomic on the key's cursor; pass 2 re-reads the chunk (from cache) and
+// places rows with an LDS cursor.  Keys without a slot fall back to one global atomic per row --
+// they are the rare keys, so that costs little even under heavy skew.
 extern "C" __global__ void __launch_bounds__(1024)
 GroupbyReshuffle(const int *__restrict__ keys, const int *__restrict__ vals, int n, int numKeys,
                  const long long *__restrict__ offsets, long long *table, int *__restrict__ outKeys,
                  int *__restrict__ outVals) {
-    if (numKeys < 0) return;
+    __shared__ unsigned sKey[GB_LDS_SLOTS];
+    __shared__ unsigned sCnt[GB_LDS_SLOTS];
+    __shared__ unsigned long long sBase[GB_LDS_SLOTS];
+    if (numKeys < 0 || n <= 0) return;
     unsigned long long *cursor = gb_cursor(table, numKeys);
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    for (int s = threadIdx.x; s < GB_LDS_SLOTS; s += blockDim.x) {
+        sKey[s] = GB_EMPTY;
+        sCnt[s] = 0;
+    }
+    __syncthreads();
+    const long long per = ((long long)n + gridDim.x - 1) / gridDim.x;
+    const long long r0 = per * blockIdx.x, r1 = r0 + per < n ? r0 + per : n;
+    // pass 1: per-key row counts of this chunk
+    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         const int k = keys[r];
         if (k < 0 || k > numKeys) continue;
-        const unsigned long long slot = atomicAdd(&cursor[k], 1ull);
-        const long long pos = offsets[k] + (long long)slot;
+        const unsigned uk = (unsigned)k;
+        unsigned h = gb_hash(uk) & (GB_LDS_SLOTS - 1);
+        for (int p = 0; p < GB_PROBES; ++p) {
+            unsigned cur = sKey[h];
+            if (cur == GB_EMPTY) cur = atomicCAS(&sKey[h], GB_EMPTY, uk);
+            if (cur == GB_EMPTY || cur == uk) {
+                atomicAdd(&sCnt[h], 1u);
+                break;
+            }
+            h = (h + 1) & (GB_LDS_SLOTS - 1);
+        }
+    }
+    __syncthreads();
+    // one reservation per key present in the chunk
+    for (int s = threadIdx.x; s < GB_LDS_SLOTS; s += blockDim.x) {
+        const unsigned k = sKey[s];
+        if (k == GB_EMPTY) continue;
+        const unsigned long long first = atomicAdd(&cursor[k], (unsigned long long)sCnt[s]);
+        sBase[s] = (unsigned long long)offsets[k] + first;
+        sCnt[s] = 0;
+    }
+    __syncthreads();
+    // pass 2: place the rows
+    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const int k = keys[r];
+        if (k < 0 || k > numKeys) continue;
+        const unsigned uk = (unsigned)k;
+        unsigned h = gb_hash(uk) & (GB_LDS_SLOTS - 1);
+        long long pos = -1;
+        for (int p = 0; p < GB_PROBES; ++p) {
+            const unsigned cur = sKey[h];
+            if (cur == uk) {
+                pos = (long long)(sBase[h] + atomicAdd(&sCnt[h], 1u));
+                break;
+            }
+            if (cur == GB_EMPTY) break;
+            h = (h + 1) & (GB_LDS_SLOTS - 1);
+        }
+        if (pos < 0) pos = offsets[k] + (long long)atomicAdd(&cursor[k], 1ull);
         if (pos < 0 || pos >= n) continue;  // only possible if the table is inconsistent
         outKeys[pos] = k;
         outVals[pos] = vals[r];
