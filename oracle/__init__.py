@@ -124,6 +124,24 @@ def groupby(keys: np.ndarray, vals: np.ndarray, num_keys: int):
     return groups[3:3 + 3 * G].reshape(G, 3), groups[:3].copy(), offsets, ok, ov
 
 
+def gmm_em(X: np.ndarray, means_init: np.ndarray, iters: int):
+    """Diagonal-covariance EM as CPU_EM drives NR3's Gaumixmod (oracle/gmm.c): the constructor's
+    E+M step, then `iters` more.  Returns (mu[K,D], var[K,D], frac[K], loglike[iters+1])."""
+    L = lib()
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    N, D = X.shape
+    mu = np.array(means_init, dtype=np.float64, copy=True).reshape(-1, D)
+    K = mu.shape[0]
+    var = np.empty((K, D), np.float64)
+    frac = np.empty(K, np.float64)
+    ll = np.empty(iters + 1, np.float64)
+    fn = L.ptk_oracle_gmm_em
+    fn.argtypes = [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4
+    fn(X.ctypes.data, N, D, K, iters, mu.ctypes.data, var.ctypes.data, frac.ctypes.data,
+       ll.ctypes.data)
+    return mu, var, frac, ll
+
+
 def fdtd_update(field: int, own: np.ndarray, a: np.ndarray, b: np.ndarray,
                 params: FdtdParams, dtype=np.float32) -> np.ndarray:
     """One of the six half-step updates (graphfdtd.cpp:185-312) with the reference's argument

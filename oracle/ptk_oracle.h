@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include "ptk_fdtd.h"
+#include "ptk_gmm.h"
 #include "ptk_prng.h"
 
 #ifdef __cplusplus
@@ -42,6 +43,10 @@ void ptk_oracle_pfxsum_i64(const int64_t *in, int64_t *out, long long n);
 void ptk_oracle_groupby(const int32_t *keys, const int32_t *vals, long long n, int numKeys,
                         int64_t *groups, int64_t *offsets, int32_t *outKeys, int32_t *outVals);
 
+/* EM for a diagonal Gaussian mixture (oracle/gmm.c) */
+void ptk_oracle_gmm_em(const double *X, long long N, int D, int K, int iters, double *mu,
+                       double *var, double *frac, double *loglike);
+
 /* Host-task ABI forms (void f(UINT, void**)), loaded by name by the HostAccelerator. */
 void pfxsum(unsigned int n, void **a);
 void GroupbyClear(unsigned int n, void **a);
@@ -57,6 +62,8 @@ void EzComputation(unsigned int n, void **a);
 void FdtdHFields(unsigned int n, void **a);
 void FdtdEFields(unsigned int n, void **a);
 void FdtdStepFused(unsigned int n, void **a);
+void GmmAccumulate(unsigned int n, void **a);
+void GmmFinalize(unsigned int n, void **a);
 
 #ifdef __cplusplus
 }

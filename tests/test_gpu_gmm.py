@@ -1,0 +1,134 @@
+"""Parity of the EM-GMM HIP task kernels (gmm.hsaco) with the binary64 oracle.
+
+Bar: log-likelihood of every E-step within 1e-4 relative of the oracle EM run on the same
+binary32 points; means within 1e-4 of the data scale, variances and mixing fractions within 1e-3
+relative (binary32 arithmetic in the E-step and inside a 128-point tile, binary64 across tiles).
+The errors against the binary64 run are printed.  Results are deterministic: the partial sums
+are reduced in a fixed order, so repeated runs are bitwise equal.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from ptask_amd import gmm, paths
+from test_gmm_graph import run_gmm
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = paths.kernel_path("gmm.hsaco")
+
+
+@pytest.fixture(scope="module")
+def gmod(hip):
+    return hip.Module(KERNELS)
+
+
+def mixture(N, D, K, seed):
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(-10, 10, (K, D))
+    sd = rng.uniform(0.5, 2.0, (K, D))
+    c = rng.integers(0, K, N)
+    return (centers[c] + sd[c] * rng.standard_normal((N, D))).astype(np.float32)
+
+
+def initial_model(X, K, steps):
+    N, D = X.shape
+    o = gmm.model_offsets(D, K)
+    m = np.zeros(gmm.model_elems(D, K, steps))
+    m[1:4] = (N, D, K)
+    mu0 = X[(np.arange(K) * N) // K].astype(np.float64)
+    m[o["mu"]:o["var"]] = mu0.ravel()
+    m[o["var"]:o["frac"]] = 1e-10
+    m[o["frac"]:o["ll"]] = 1.0 / K
+    gmm.fill_table(m, D, K, steps)
+    return m, mu0
+
+
+def em_on_device(hip, gmod, X, K, steps, blocks):
+    """`steps` GmmAccumulate + GmmFinalize launches through the C ABI; returns the final model."""
+    N, D = X.shape
+    assert 1 <= D <= gmm.MAX_D and 1 <= K <= gmm.MAX_K and blocks >= 1
+    dims = gmm.Dims(N, D, K, blocks, steps)
+    model, mu0 = initial_model(X, K, steps)
+    dX = hip.DeviceBuffer(X.nbytes)
+    dX.upload(np.ascontiguousarray(X))
+    dM = hip.DeviceBuffer(model.nbytes)
+    dM.upload(model)
+    dP = hip.DeviceBuffer(8 * blocks * gmm.partial_stride(D, K))
+    fin_grid = K + 1
+    acc, fin = gmod.function("GmmAccumulate"), gmod.function("GmmFinalize")
+    for _ in range(steps):
+        hip.launch(acc, (blocks, 1, 1), (256, 1, 1),
+                   hip.ArgPack().ptr(dX.ptr).raw(bytes(dims)).ptr(dM.ptr).ptr(dP.ptr))
+        hip.launch(fin, (fin_grid, 1, 1), (256, 1, 1),
+                   hip.ArgPack().ptr(dM.ptr).ptr(dP.ptr).raw(bytes(dims)))
+    hip.synchronize()
+    out = np.empty_like(model)
+    dM.download(out)
+    return gmm.parse_model(out, D, K, steps), mu0
+
+
+def check_against_oracle(X, mu0, m, steps, label):
+    mu, var, frac, ll = oracle.gmm_em(X.astype(np.float64), mu0, steps - 1)
+    scale = float(np.abs(X).max())
+    e_ll = float(np.max(np.abs(m.loglike - ll) / np.abs(ll)))
+    e_mu = float(np.max(np.abs(m.mu - mu))) / scale
+    e_var = float(np.max(np.abs(m.var - var) / var))
+    e_frac = float(np.max(np.abs(m.frac - frac)))
+    print(f"{label}: vs binary64 oracle  loglike rel {e_ll:.2e}  mu/scale {e_mu:.2e}  "
+          f"var rel {e_var:.2e}  frac abs {e_frac:.2e}", flush=True)
+    assert m.steps == steps
+    assert e_ll <= 1e-4
+    assert e_mu <= 1e-4
+    assert e_var <= 1e-3
+    assert e_frac <= 1e-3
+
+
+# Seeds are chosen so that no component collapses onto one point (the recurrence then divides
+# zero by zero, in the oracle as in the reference).
+@pytest.mark.parametrize("N,D,K,steps,blocks,seed", [
+    (1000, 1, 1, 3, 1, 1008), (5000, 3, 2, 6, 7, 5023), (4097, 8, 33, 5, 33, 2),
+    (20000, 32, 64, 6, 157, 20288), (129, 2, 5, 4, 2, 148), (50000, 16, 32, 8, 391, 50144),
+    (3000, 31, 17, 5, 24, 3234)])
+def test_kernels_match_oracle(hip, gmod, N, D, K, steps, blocks, seed):
+    print(f"case gmm kernels N={N} D={D} K={K} steps={steps} blocks={blocks}", flush=True)
+    X = mixture(N, D, K, seed=seed)
+    m, mu0 = em_on_device(hip, gmod, X, K, steps, blocks)
+    check_against_oracle(X, mu0, m, steps, f"N={N} D={D} K={K}")
+
+
+def test_kernels_are_deterministic(hip, gmod):
+    print("case gmm determinism", flush=True)
+    X = mixture(30000, 12, 20, seed=3)
+    a, _ = em_on_device(hip, gmod, X, 20, 5, 200)
+    b, _ = em_on_device(hip, gmod, X, 20, 5, 200)
+    for f in ("mu", "var", "frac", "loglike"):
+        assert np.array_equal(getattr(a, f), getattr(b, f))
+
+
+def test_reference_fixture_on_device(hip, gmod):
+    """dist_3 (1-D, 1000 points) for 21 steps, as CPU_EM runs its 20 iterations after the
+    constructor (graphemgmm.cpp:315-317)."""
+    print("case gmm dist_3", flush=True)
+    import os
+    x = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "dist_3.txt"))
+    X = x.astype(np.float32).reshape(-1, 1)
+    m, mu0 = em_on_device(hip, gmod, X, 3, 21, 8)
+    check_against_oracle(X, mu0, m, 21, "dist_3")
+
+
+@pytest.mark.parametrize("mode", ["looped", "unrolled"])
+def test_hip_graph_matches_oracle(tmp_path, mode):
+    js, X, mu0, m = run_gmm(KERNELS, tmp_path, 40000, 8, 16, 7, "--mode", mode, "--blocks", "200",
+                            "--init", "strided")
+    assert js["ok"] and js["backend"] == "hip"
+    check_against_oracle(X, mu0, m, 7, f"graph {mode}")
+
+
+def test_hip_graph_forms_reruns_and_residency(tmp_path):
+    a = run_gmm(KERNELS, tmp_path, 60000, 16, 24, 6, "--mode", "looped", "--repeat", "3",
+                "--warmup", "1")[0]
+    b = run_gmm(KERNELS, tmp_path, 60000, 16, 24, 6, "--mode", "unrolled")[0]
+    assert a["model_fnv1a"] == b["model_fnv1a"]
+    # X stays on the device across runs: only the small model block is uploaded per run
+    assert a["h2d_count_timed"] <= 3 and a["steps_done"] == 6
