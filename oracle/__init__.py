@@ -142,6 +142,23 @@ def gmm_em(X: np.ndarray, means_init: np.ndarray, iters: int):
     return mu, var, frac, ll
 
 
+def pca_gs(X: np.ndarray, K: int, J: int, center: bool = True):
+    """Gram-Schmidt PCA as pca_cpu runs it (oracle/pca.c).  X is M x N (any layout; it is passed
+    column-major).  Returns (T[M,K], P[N,K], R[M,N], L[K])."""
+    L_ = lib()
+    X = np.asfortranarray(X, dtype=np.float64)
+    M, N = X.shape
+    T = np.zeros((M, K), order="F")
+    P = np.zeros((N, K), order="F")
+    R = np.zeros((M, N), order="F")
+    Lv = np.zeros(K)
+    fn = L_.ptk_oracle_pca_gs
+    fn.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4
+    fn(X.ctypes.data, M, N, K, J, int(center), T.ctypes.data, P.ctypes.data, R.ctypes.data,
+       Lv.ctypes.data)
+    return T, P, R, Lv
+
+
 def fdtd_update(field: int, own: np.ndarray, a: np.ndarray, b: np.ndarray,
                 params: FdtdParams, dtype=np.float32) -> np.ndarray:
     """One of the six half-step updates (graphfdtd.cpp:185-312) with the reference's argument

@@ -12,6 +12,7 @@
 
 #include "ptk_fdtd.h"
 #include "ptk_gmm.h"
+#include "ptk_pca.h"
 #include "ptk_prng.h"
 
 #ifdef __cplusplus
@@ -47,6 +48,10 @@ void ptk_oracle_groupby(const int32_t *keys, const int32_t *vals, long long n, i
 void ptk_oracle_gmm_em(const double *X, long long N, int D, int K, int iters, double *mu,
                        double *var, double *frac, double *loglike);
 
+/* Gram-Schmidt PCA (oracle/pca.c); column-major X (M x N), T (M x K), P (N x K), R (M x N) */
+void ptk_oracle_pca_gs(const double *X, int M, int N, int K, int J, int center, double *T, double *P,
+                       double *R, double *L);
+
 /* Host-task ABI forms (void f(UINT, void**)), loaded by name by the HostAccelerator. */
 void pfxsum(unsigned int n, void **a);
 void GroupbyClear(unsigned int n, void **a);
@@ -64,6 +69,15 @@ void FdtdEFields(unsigned int n, void **a);
 void FdtdStepFused(unsigned int n, void **a);
 void GmmAccumulate(unsigned int n, void **a);
 void GmmFinalize(unsigned int n, void **a);
+void PcaLoad(unsigned int n, void **a);
+void PcaStart(unsigned int n, void **a);
+void PcaGram(unsigned int n, void **a);
+void PcaProject(unsigned int n, void **a);
+void PcaDeflate(unsigned int n, void **a);
+void PcaReduce(unsigned int n, void **a);
+void PcaOrthonormalize(unsigned int n, void **a);
+void PcaFinish(unsigned int n, void **a);
+void PcaScaleT(unsigned int n, void **a);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,355 @@
+// pca.hip -- Gram-Schmidt PCA (NIPALS with deflation) as PTask task kernels (pca.hsaco).
+//
+// Semantics follow ptaskUnitTest/pca_cpu.cpp:267-424 (the host reference the unrolled BLAS graph
+// of ptaskUnitTest/graphcupca.cpp is checked against); the decomposition into kernels is the
+// build's own.  The reference runs, per component k and inner iteration, separate BLAS-2 tasks
+// (dgemv R^T t, dgemv for the Gram-Schmidt step, dnrm2, dscal, dgemv R p, dnrm2, dscal) and a
+// rank-1 dger per component.  Here the matrix R lives row-major in HBM and every pass over it
+// does as much as one read allows:
+//   PcaStart    deflation by component k-1 (R -= L t p^T, written back) fused with p = R^T R[:,k]
+//   PcaGram     p = R^T (R p) in one read of R: each 8-row slab's t = R_slab p is reduced inside
+//               the block and immediately used for R_slab^T t (the reference's t = R p, |t|,
+//               t /= |t|, p = R^T t -- the scaling by 1/|t| is undone by the next normalisation)
+//   PcaProject  t = R p and |t|^2 partials (the last inner iteration)
+//   PcaDeflate  R -= L t p^T by the last component (the residual the reference returns)
+// The small steps are single kernels: PcaReduce + PcaOrthonormalize (sum the per-block p
+// partials in a fixed order, p -= P_<k (P_<k^T p), p /= |p|), PcaFinish (L_k = |t|,
+// T[:,k] = t / L_k) and PcaScaleT (T[:,k] *= L_k).  All reductions are fixed-order, so results
+// are bitwise reproducible.  Dot products and norms of length-N and length-M vectors are binary64.
+#include <hip/hip_runtime.h>
+
+#include "ptk_pca.h"
+
+// Tolerance-checked binary32 work: let the compiler fuse multiply-adds (the build default is off).
+#pragma clang fp contract(fast)
+
+enum { PCA_START = 0, PCA_GRAM = 1, PCA_PROJECT = 2, PCA_DEFLATE = 3 };
+
+__device__ inline bool pca_ok(const PCA_DIMS &d) {
+    return d.M >= 1 && d.N >= 1 && d.N <= PTK_PCA_MAX_N && d.K >= 1 && d.K <= PTK_PCA_MAX_K &&
+           d.K <= d.N && d.J >= 1 && d.ld == ptk_pca_ld(d.N) && d.Mp == ptk_pca_mp(d.M) && d.G >= 1 &&
+           d.GR >= 1 && d.GR <= d.G && d.k >= 0 && d.k < d.K;
+}
+
+__device__ inline double *pca_L(float *S, const PCA_DIMS &d) { return (double *)((char *)S + ptk_pca_off_L(&d)); }
+__device__ inline double *pca_tn(float *S, const PCA_DIMS &d) { return (double *)((char *)S + ptk_pca_off_tn(&d)); }
+__device__ inline double *pca_red(float *S, const PCA_DIMS &d) { return (double *)((char *)S + ptk_pca_off_red(&d)); }
+__device__ inline float *pca_P(float *S, const PCA_DIMS &d) { return (float *)((char *)S + ptk_pca_off_P(&d)); }
+__device__ inline float *pca_part(float *S, const PCA_DIMS &d) { return (float *)((char *)S + ptk_pca_off_part(&d)); }
+
+__device__ inline float f4get(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
+__device__ inline float f4dot(const float4 &a, const float4 &b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+
+// ---- PcaLoad: X (column-major M x N) -> R (row-major Mp x ld), rows optionally centred; T and S
+// cleared.  Grid Mp/64 blocks of 256; block b owns rows 64b .. 64b+63.
+extern "C" __global__ void __launch_bounds__(256)
+PcaLoad(const float *__restrict__ X, PCA_DIMS d, float *__restrict__ R, float *__restrict__ T,
+        float *__restrict__ S) {
+    __shared__ double sSum[4][64];
+    __shared__ double sMean[64];
+    __shared__ float tile[64][65];
+    if (!pca_ok(d) || blockDim.x != 256 || (int)gridDim.x != d.Mp / 64) return;
+    const int tid = threadIdx.x, M = d.M, N = d.N, ld = d.ld;
+    const long long r0 = (long long)blockIdx.x * 64;
+    {
+        const int r = tid & 63, grp = tid >> 6;
+        double s = 0;
+        if (d.center && r0 + r < M)
+            for (int n = grp; n < N; n += 4) s += (double)X[(long long)n * M + r0 + r];
+        sSum[grp][r] = s;
+    }
+    __syncthreads();
+    if (tid < 64) sMean[tid] = d.center ? (sSum[0][tid] + sSum[1][tid] + sSum[2][tid] + sSum[3][tid]) / N : 0.0;
+    __syncthreads();
+    for (int n0 = 0; n0 < ld; n0 += 64) {
+        for (int i = tid; i < 64 * 64; i += 256) {
+            const int c = i >> 6, r = i & 63;
+            const int n = n0 + c;
+            const long long m = r0 + r;
+            tile[c][r] = (n < N && m < M) ? (float)((double)X[(long long)n * M + m] - sMean[r]) : 0.f;
+        }
+        __syncthreads();
+        for (int i = tid; i < 64 * 64; i += 256) {
+            const int r = i >> 6, c = i & 63;
+            if (n0 + c < ld) R[(r0 + r) * ld + n0 + c] = tile[c][r];
+        }
+        __syncthreads();
+    }
+    const long long nT = (long long)d.K * d.Mp, nS = ptk_pca_state_bytes(&d) / 4;
+    const long long step = (long long)gridDim.x * 256;
+    for (long long i = (long long)blockIdx.x * 256 + tid; i < nT; i += step) T[i] = 0.f;
+    for (long long i = (long long)blockIdx.x * 256 + tid; i < nS; i += step) S[i] = 0.f;
+}
+
+// ---- the row passes.  Block of 256; thread t owns float4 column chunks t + 256 c (c < NCH); a
+// slab of RB rows is held in registers, its RB row dot products are reduced across the block
+// (wave shuffles, then a double-buffered LDS exchange, one barrier per slab).
+template <int NCH, int RB, int MODE>
+__device__ __forceinline__ void pca_row_pass(float *__restrict__ R, float *__restrict__ T,
+                                             float *__restrict__ S, const PCA_DIMS &d) {
+    __shared__ float sRed[2][4][RB];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int ld = d.ld, ld4 = ld >> 2, k = d.k;
+    const long long Mp = d.Mp;
+    float *P = pca_P(S, d);
+    int cix[NCH];
+    bool cv[NCH];
+    float4 pv[NCH], qv[NCH], acc[NCH];
+    // deflation by component dk: the previous one when starting component k, k itself at the end
+    const bool deflate = (MODE == PCA_START && k > 0) || MODE == PCA_DEFLATE;
+    const int dk = MODE == PCA_START ? k - 1 : k;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        cix[c] = tid + 256 * c;
+        cv[c] = cix[c] < ld4;
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        pv[c] = ((MODE == PCA_GRAM || MODE == PCA_PROJECT) && cv[c]) ? ((const float4 *)(P + (long long)k * ld))[cix[c]] : z;
+        qv[c] = (deflate && cv[c]) ? ((const float4 *)(P + (long long)dk * ld))[cix[c]] : z;
+        acc[c] = z;
+    }
+    const double Lprev = deflate ? pca_L(S, d)[dk] : 0.0;
+    const int ck = k >> 2, ce = k & 3, cowner = ck & 255, creg = ck >> 8;
+    const long long nslab = Mp / RB;
+    int buf = 0;
+    double tsq = 0;
+    for (long long s = blockIdx.x; s < nslab; s += gridDim.x) {
+        const long long r0 = s * RB;
+        float4 x[RB][NCH];
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                x[i][c] = cv[c] ? ((const float4 *)(R + (r0 + i) * ld))[cix[c]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (deflate) {
+            // R -= (L_dk t_dk) p_dk^T  (dger, pca_cpu.cpp:401), t_dk normalised
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                const float a = (float)(Lprev * (double)T[(long long)dk * Mp + r0 + i]);
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) {
+                    if (!cv[c]) continue;
+                    x[i][c].x = fmaf(-a, qv[c].x, x[i][c].x);
+                    x[i][c].y = fmaf(-a, qv[c].y, x[i][c].y);
+                    x[i][c].z = fmaf(-a, qv[c].z, x[i][c].z);
+                    x[i][c].w = fmaf(-a, qv[c].w, x[i][c].w);
+                    ((float4 *)(R + (r0 + i) * ld))[cix[c]] = x[i][c];
+                }
+            }
+        }
+        if (MODE == PCA_DEFLATE) continue;
+        float t[RB];
+        if (MODE == PCA_START) {
+            // t = R[:,k] of the (deflated) residual, from the thread that owns column k
+            if (tid == cowner) {
+#pragma unroll
+                for (int i = 0; i < RB; ++i) {
+                    float v = 0.f;
+#pragma unroll
+                    for (int c = 0; c < NCH; ++c)
+                        if (c == creg) v = f4get(x[i][c], ce);
+                    sRed[buf][0][i] = v;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < RB; ++i) t[i] = sRed[buf][0][i];
+        } else {
+            float v[RB];
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                v[i] = 0.f;
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) v[i] += f4dot(x[i][c], pv[c]);
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) v[i] += __shfl_xor(v[i], off, 64);
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int i = 0; i < RB; ++i) sRed[buf][wv][i] = v[i];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < RB; ++i) t[i] = sRed[buf][0][i] + sRed[buf][1][i] + sRed[buf][2][i] + sRed[buf][3][i];
+        }
+        buf ^= 1;
+        if (MODE == PCA_PROJECT) {
+            if (tid == 0) {
+#pragma unroll
+                for (int i = 0; i < RB; ++i) {
+                    T[(long long)k * Mp + r0 + i] = t[i];
+                    tsq += (double)t[i] * (double)t[i];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+#pragma unroll
+                for (int i = 0; i < RB; ++i) {
+                    acc[c].x = fmaf(x[i][c].x, t[i], acc[c].x);
+                    acc[c].y = fmaf(x[i][c].y, t[i], acc[c].y);
+                    acc[c].z = fmaf(x[i][c].z, t[i], acc[c].z);
+                    acc[c].w = fmaf(x[i][c].w, t[i], acc[c].w);
+                }
+        }
+    }
+    if (MODE == PCA_PROJECT) {
+        if (tid == 0) pca_tn(S, d)[blockIdx.x] = tsq;
+    } else if (MODE != PCA_DEFLATE) {
+        float *part = pca_part(S, d) + (long long)blockIdx.x * ld;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (cv[c]) ((float4 *)part)[cix[c]] = acc[c];
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void pca_row_dispatch(float *R, float *T, float *S, const PCA_DIMS &d) {
+    switch ((d.ld / 4 + 255) / 256) {
+        case 1: pca_row_pass<1, 8, MODE>(R, T, S, d); break;
+        case 2: pca_row_pass<2, 8, MODE>(R, T, S, d); break;
+        case 3: pca_row_pass<3, 4, MODE>(R, T, S, d); break;
+        default: pca_row_pass<4, 4, MODE>(R, T, S, d); break;
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+PcaStart(float *__restrict__ R, float *__restrict__ T, float *__restrict__ S, PCA_DIMS d) {
+    if (!pca_ok(d) || blockDim.x != 256 || (int)gridDim.x != d.G) return;
+    pca_row_dispatch<PCA_START>(R, T, S, d);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+PcaGram(float *__restrict__ R, float *__restrict__ S, PCA_DIMS d) {
+    if (!pca_ok(d) || blockDim.x != 256 || (int)gridDim.x != d.G) return;
+    pca_row_dispatch<PCA_GRAM>(R, nullptr, S, d);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+PcaProject(float *__restrict__ R, float *__restrict__ T, float *__restrict__ S, PCA_DIMS d) {
+    if (!pca_ok(d) || blockDim.x != 256 || (int)gridDim.x != d.G) return;
+    pca_row_dispatch<PCA_PROJECT>(R, T, S, d);
+}
+
+// After the last component: R -= L_k t_k p_k^T for k = d.k (pca_cpu.cpp:401 of the final k).
+extern "C" __global__ void __launch_bounds__(256)
+PcaDeflate(float *__restrict__ R, float *__restrict__ T, float *__restrict__ S, PCA_DIMS d) {
+    if (!pca_ok(d) || blockDim.x != 256 || (int)gridDim.x != d.G) return;
+    pca_row_dispatch<PCA_DEFLATE>(R, T, S, d);
+}
+
+// ---- PcaReduce: red[g][n] = sum of part[b][n] over the g-th of GR contiguous groups of the G
+// rows.  Grid (ceil(ld/64), GR) blocks of 256 (64 columns x 4 interleaved row subsets).
+extern "C" __global__ void __launch_bounds__(256)
+PcaReduce(float *__restrict__ S, PCA_DIMS d) {
+    __shared__ double sh[4][64];
+    if (!pca_ok(d) || blockDim.x != 256 || (int)gridDim.x != (d.ld + 63) / 64 || (int)gridDim.y != d.GR) return;
+    const int tid = threadIdx.x, c = tid & 63, sub = tid >> 6, g = blockIdx.y, ld = d.ld;
+    const int col = blockIdx.x * 64 + c;
+    const int lo = (int)((long long)g * d.G / d.GR), hi = (int)((long long)(g + 1) * d.G / d.GR);
+    const float *part = pca_part(S, d);
+    double s = 0;
+    if (col < ld)
+        for (int b = lo + sub; b < hi; b += 4) s += (double)part[(long long)b * ld + col];
+    sh[sub][c] = s;
+    __syncthreads();
+    if (sub == 0 && col < ld) pca_red(S, d)[(long long)g * ld + col] = sh[0][c] + sh[1][c] + sh[2][c] + sh[3][c];
+}
+
+// ---- PcaOrthonormalize: one block of 1024.  p = sum of the GR rows of red; classical
+// Gram-Schmidt against the components before k (U = P_<k^T p, then p -= P_<k U: pca_cpu.cpp:356-372);
+// p *= 1/|p| (dnrm2 + dscal: pca_cpu.cpp:374-376); P[k] = p.
+extern "C" __global__ void __launch_bounds__(1024)
+PcaOrthonormalize(float *__restrict__ S, PCA_DIMS d) {
+    __shared__ double sW[16][PTK_PCA_MAX_K];
+    __shared__ double sU[PTK_PCA_MAX_K];
+    if (!pca_ok(d) || blockDim.x != 1024 || gridDim.x != 1) return;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, ld = d.ld, k = d.k;
+    const double *red = pca_red(S, d);
+    float *P = pca_P(S, d);
+    double p[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int n = tid + 1024 * i;
+        double s = 0;
+        if (n < ld)
+            for (int g = 0; g < d.GR; ++g) s += red[(long long)g * ld + n];
+        p[i] = s;
+    }
+    if (k > 0) {
+        for (int j = 0; j < k; ++j) {
+            double v = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int n = tid + 1024 * i;
+                if (n < ld) v += (double)P[(long long)j * ld + n] * p[i];
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) sW[wv][j] = v;
+        }
+        __syncthreads();
+        if (tid < k) {
+            double u = 0;
+            for (int w = 0; w < 16; ++w) u += sW[w][tid];
+            sU[tid] = u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int n = tid + 1024 * i;
+            if (n >= ld) continue;
+            double s = 0;
+            for (int j = 0; j < k; ++j) s += (double)P[(long long)j * ld + n] * sU[j];
+            p[i] -= s;
+        }
+        __syncthreads();
+    }
+    double v = p[0] * p[0] + p[1] * p[1] + p[2] * p[2] + p[3] * p[3];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) sW[wv][0] = v;
+    __syncthreads();
+    double nrm2 = 0;
+    for (int w = 0; w < 16; ++w) nrm2 += sW[w][0];
+    const double inv = 1.0 / sqrt(nrm2);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int n = tid + 1024 * i;
+        if (n < ld) P[(long long)k * ld + n] = (float)(p[i] * inv);
+    }
+}
+
+// ---- PcaFinish: L_k = sqrt(sum of tn) (pca_cpu.cpp:385), T[:,k] *= 1/L_k (:386).  Grid
+// ceil(Mp/256) blocks of 256; each block sums tn in the same fixed order.
+extern "C" __global__ void __launch_bounds__(256)
+PcaFinish(float *__restrict__ T, float *__restrict__ S, PCA_DIMS d) {
+    __shared__ double sh[256];
+    if (!pca_ok(d) || blockDim.x != 256 || (int)gridDim.x != (d.Mp + 255) / 256) return;
+    const int tid = threadIdx.x;
+    const double *tn = pca_tn(S, d);
+    double s = 0;
+    for (int b = tid; b < d.G; b += 256) s += tn[b];
+    sh[tid] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) sh[tid] += sh[tid + w];
+        __syncthreads();
+    }
+    const double Lk = sqrt(sh[0]);
+    const long long m = (long long)blockIdx.x * 256 + tid;
+    if (m < d.Mp) {
+        float *t = T + (long long)d.k * d.Mp;
+        t[m] = (float)((double)t[m] * (1.0 / Lk));
+    }
+    if (blockIdx.x == 0 && tid == 0) pca_L(S, d)[d.k] = Lk;
+}
+
+// ---- PcaScaleT: T[:,k] *= L_k for every component (pca_cpu.cpp:404-408).
+extern "C" __global__ void __launch_bounds__(256)
+PcaScaleT(float *__restrict__ T, float *__restrict__ S, PCA_DIMS d) {
+    if (!pca_ok(d) || blockDim.x != 256) return;
+    const long long n = (long long)d.K * d.Mp;
+    const double *L = pca_L(S, d);
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        T[i] = (float)((double)T[i] * L[i / d.Mp]);
+}

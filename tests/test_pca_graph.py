@@ -1,0 +1,91 @@
+"""The Gram-Schmidt PCA graph (ptask_amd/csrc/apps/ptk_pca.cpp) on the HostAccelerator path.
+
+The host-task forms of the PCA task kernels (oracle/pca.c) run through the PTask runtime; the
+singular values, loads, scores and residual must equal the binary64 oracle on the same binary32
+input up to the binary32 storage of R, T and P.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from ptask_amd import paths
+
+APP = paths.bin_path("ptk_pca")
+HOSTLIB = os.path.join(oracle.ORACLE_DIR, "_build", "libptk_oracle.so")
+
+
+def run_pca(kernels, tmp_path, M, N, K, J, *extra, timeout=300):
+    dump = str(tmp_path / f"pca_{M}_{N}_{K}_{J}.bin")
+    cmd = list(map(str, [APP, "--kernels", kernels, "--m", M, "--n", N, "--k", K, "--j", J,
+                         "--dump", dump, *extra]))
+    print("run:", " ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    assert p.returncode == 0, p.stderr + p.stdout
+    js = json.loads(p.stdout.strip().splitlines()[-1])
+    raw = open(dump, "rb").read()
+    o = 0
+
+    def take(n, dt):
+        nonlocal o
+        a = np.frombuffer(raw[o:o + n * np.dtype(dt).itemsize], dtype=dt)
+        o += n * np.dtype(dt).itemsize
+        return a
+
+    X = take(M * N, np.float32).reshape(N, M).T
+    T = take(M * K, np.float32).reshape(K, M).T
+    P = take(N * K, np.float32).reshape(K, N).T
+    R = take(M * N, np.float32).reshape(N, M).T
+    L = take(K, np.float64)
+    assert o == len(raw)
+    return js, X, T, P, R, L
+
+
+def compare(X, T, P, R, L, K, J, center, tol):
+    eT, eP, eR, eL = oracle.pca_gs(X.astype(np.float64), K, J, center)
+    scale = float(np.abs(X).max())  # the residual can vanish (K = rank): measure against X
+    err = {"L": float(np.max(np.abs(L - eL) / eL)),
+           "P": float(np.max(np.abs(P - eP))),
+           "T": float(np.max(np.abs(T - eT)) / float(np.abs(eT).max())),
+           "R": float(np.max(np.abs(R - eR))) / scale}
+    print("errors vs binary64 oracle:", {k: f"{v:.2e}" for k, v in err.items()}, flush=True)
+    for k, v in err.items():
+        assert v <= tol[k], (k, v)
+    return err
+
+
+HOST_TOL = {"L": 1e-6, "P": 1e-5, "T": 1e-5, "R": 1e-5}
+
+
+# K stays below the rank (row centring removes one dimension: K <= N - 1 when centred).
+@pytest.mark.parametrize("M,N,K,J,center", [(300, 70, 3, 4, True), (128, 128, 3, 10, True),
+                                             (97, 41, 5, 2, False), (64, 9, 8, 3, True),
+                                             (40, 12, 12, 3, False)])
+def test_host_graph_equals_oracle(tmp_path, M, N, K, J, center):
+    extra = [] if center else ["--no-center"]
+    js, X, T, P, R, L = run_pca(HOSTLIB, tmp_path, M, N, K, J, *extra, "--dist", "uniform")
+    assert js["ok"] and js["backend"] == "host"
+    assert js["tasks"] == 3 + K * (3 * J + 2)
+    compare(X, T, P, R, L, K, J, center, HOST_TOL)
+
+
+def test_host_graph_reruns_and_data_file(tmp_path):
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((200, 50)).astype(np.float32)
+    f = tmp_path / "x.f32"
+    np.asfortranarray(X).T.tofile(f)   # column-major
+    js, X2, T, P, R, L = run_pca(HOSTLIB, tmp_path, 200, 50, 4, 5, "--data", str(f), "--repeat", "2",
+                                 "--warmup", "1")
+    assert np.array_equal(X2, X)
+    assert js["h2d_count_timed"] == 0
+    compare(X, T, P, R, L, 4, 5, True, HOST_TOL)
+
+
+@pytest.mark.parametrize("args", [["--n", "4097"], ["--k", "65"], ["--k", "0"], ["--j", "0"],
+                                  ["--n", "3", "--k", "4"], ["--dist", "cauchy"]])
+def test_app_rejects_bad_shapes(args):
+    p = subprocess.run([APP, "--kernels", HOSTLIB, *args], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2 and "usage" in p.stderr
