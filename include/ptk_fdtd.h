@@ -37,8 +37,14 @@ typedef struct _fdtdcb_t {
 enum { PTK_FDTD_HX = 0, PTK_FDTD_HY = 1, PTK_FDTD_HZ = 2,
        PTK_FDTD_EX = 3, PTK_FDTD_EY = 4, PTK_FDTD_EZ = 5 };
 
+#if defined(__HIPCC__)
+#define PTK_FDTD_FN static inline __host__ __device__
+#else
+#define PTK_FDTD_FN static inline
+#endif
+
 /* Extents (d1,d2,d3) of field f on an Nx x Ny x Nz grid. */
-static inline void ptk_fdtd_extent(int f, int Nx, int Ny, int Nz, int *d1, int *d2, int *d3) {
+PTK_FDTD_FN void ptk_fdtd_extent(int f, int Nx, int Ny, int Nz, int *d1, int *d2, int *d3) {
     int x = Nx, y = Ny, z = Nz;
     switch (f) {
         case PTK_FDTD_HX: x = Nx + 1; break;
@@ -52,7 +58,7 @@ static inline void ptk_fdtd_extent(int f, int Nx, int Ny, int Nz, int *d1, int *
     *d1 = x; *d2 = y; *d3 = z;
 }
 
-static inline unsigned long long ptk_fdtd_field_elems(int f, int Nx, int Ny, int Nz) {
+PTK_FDTD_FN unsigned long long ptk_fdtd_field_elems(int f, int Nx, int Ny, int Nz) {
     int a, b, c;
     ptk_fdtd_extent(f, Nx, Ny, Nz, &a, &b, &c);
     return (unsigned long long)a * (unsigned long long)b * (unsigned long long)c;

@@ -1,0 +1,192 @@
+// ptk_batch.cpp -- many uneven graph instances through the batched executor (include/ptk_batch.h):
+// FDTD graphs of random cubic edge and groupby graphs of random row count, run to completion
+// once by plain per-task dispatch (the baseline) and once as hipGraph-captured batched
+// iterations, with every instance's result checksums compared between the two.
+//
+// Instance shapes come from the shared generator (seed, stream 30..33): FDTD edge uniform on
+// [fdtd-min, fdtd-max], groupby rows uniform on [gb-min, gb-max]; instance i's input seed is
+// seed ^ (i * golden).  Prints one JSON object: per mode the end-to-end time (admission with
+// device-side input generation, iterations, retirement with checksums) and the execution time
+// of the iterations alone.
+
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ptk_batch.h"
+#include "ptk_hip.h"
+#include "ptk_prng.h"
+
+namespace {
+
+struct Options {
+    std::string kernels = "ptask_amd/lib/kernels", modes = "plain,batched";
+    int fdtd = 2048, fdtdMin = 16, fdtdMax = 96, fdtdSteps = 50;
+    int gb = 512, keys = 65536;
+    long long gbMin = 1000000, gbMax = 8000000;
+    int gmm = 0, gmmN = 100000, gmmD = 16, gmmK = 16, gmmSteps = 10;
+    double zipf = 1.1, arenaGb = 0;
+    unsigned long long seed = 0x5eed;
+    bool autotune = true;
+    int cols = 0, depth = 0, gbRows = 0;
+};
+
+struct Spec {
+    int kind;
+    long long a;
+    int b, c, steps;
+    unsigned long long seed;
+};
+
+double Ms(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    Options o;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto next = [&]() -> const char * {
+            if (i + 1 >= argc) { fprintf(stderr, "missing value for %s\n", a.c_str()); exit(2); }
+            return argv[++i];
+        };
+        if (a == "--kernels") o.kernels = next();
+        else if (a == "--modes") o.modes = next();
+        else if (a == "--fdtd") o.fdtd = atoi(next());
+        else if (a == "--fdtd-min") o.fdtdMin = atoi(next());
+        else if (a == "--fdtd-max") o.fdtdMax = atoi(next());
+        else if (a == "--fdtd-steps") o.fdtdSteps = atoi(next());
+        else if (a == "--groupby") o.gb = atoi(next());
+        else if (a == "--gb-min") o.gbMin = atoll(next());
+        else if (a == "--gb-max") o.gbMax = atoll(next());
+        else if (a == "--keys") o.keys = atoi(next());
+        else if (a == "--zipf") o.zipf = atof(next());
+        else if (a == "--gmm") o.gmm = atoi(next());
+        else if (a == "--gmm-n") o.gmmN = atoi(next());
+        else if (a == "--gmm-d") o.gmmD = atoi(next());
+        else if (a == "--gmm-k") o.gmmK = atoi(next());
+        else if (a == "--gmm-steps") o.gmmSteps = atoi(next());
+        else if (a == "--arena-gb") o.arenaGb = atof(next());
+        else if (a == "--seed") o.seed = strtoull(next(), nullptr, 0);
+        else if (a == "--no-autotune") o.autotune = false;
+        else if (a == "--tiles") {
+            if (sscanf(next(), "%d,%d,%d", &o.cols, &o.depth, &o.gbRows) != 3) { fprintf(stderr, "--tiles C,D,R\n"); return 2; }
+        } else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
+    }
+    if (o.fdtd < 0 || o.gb < 0 || o.gmm < 0 || o.fdtdMin < 1 || o.fdtdMax < o.fdtdMin || o.gbMin < 1 ||
+        o.gbMax < o.gbMin || o.keys < 1 || o.fdtdSteps < 1) {
+        fprintf(stderr, "usage: ptk_batch [--modes plain,batched] [--fdtd N] [--fdtd-min E] [--fdtd-max E] "
+                        "[--fdtd-steps S] [--groupby N] [--gb-min R] [--gb-max R] [--keys K] [--zipf S] "
+                        "[--gmm N ...] [--arena-gb G] [--seed S] [--no-autotune] [--tiles C,D,R]\n");
+        return 2;
+    }
+    std::vector<Spec> specs;
+    unsigned long long need = 0, cells = 0, rows = 0;
+    for (int i = 0; i < o.fdtd; ++i) {
+        const int e = o.fdtdMin + (int)(ptk_rand_unit_f64(o.seed, 30, (uint64_t)i) * (o.fdtdMax - o.fdtdMin + 1));
+        specs.push_back({PTK_INST_FDTD, std::min(e, o.fdtdMax), std::min(e, o.fdtdMax), std::min(e, o.fdtdMax), o.fdtdSteps, 0});
+    }
+    for (int i = 0; i < o.gb; ++i) {
+        const long long r = o.gbMin + (long long)(ptk_rand_unit_f64(o.seed, 31, (uint64_t)i) * (double)(o.gbMax - o.gbMin + 1));
+        specs.push_back({PTK_INST_GROUPBY, std::min(r, o.gbMax), o.keys - 1, 0, 1, 0});
+    }
+    for (int i = 0; i < o.gmm; ++i) specs.push_back({PTK_INST_GMM, o.gmmN, o.gmmD, o.gmmK, o.gmmSteps, 0});
+    for (size_t i = 0; i < specs.size(); ++i) {
+        Spec &s = specs[i];
+        s.seed = o.seed ^ ((unsigned long long)(i + 1) * 0x9E3779B97F4A7C15ull);
+        need += ptk_batch_footprint(s.kind, s.a, s.b, s.c, s.steps);
+        if (s.kind == PTK_INST_FDTD) cells += (unsigned long long)s.a * s.b * s.c * (unsigned long long)s.steps;
+        if (s.kind == PTK_INST_GROUPBY) rows += (unsigned long long)s.a;
+    }
+    unsigned long long arena = o.arenaGb > 0 ? (unsigned long long)(o.arenaGb * 1073741824.0) : need + (need >> 4) + (256ull << 20);
+    size_t freeB = 0, totalB = 0;
+    if (ptk_hip_mem_get_info(&freeB, &totalB) == PTK_HIP_OK && arena > freeB - (freeB >> 3)) arena = freeB - (freeB >> 3);
+
+    std::string json = "{";
+    char buf[512];
+    snprintf(buf, sizeof(buf), "\"instances\": %zu, \"fdtd\": %d, \"groupby\": %d, \"gmm\": %d, \"footprint_bytes\": %llu, "
+             "\"arena_bytes\": %llu, \"fdtd_cell_steps\": %llu, \"groupby_rows\": %llu, \"modes\": {",
+             specs.size(), o.fdtd, o.gb, o.gmm, need, arena, cells, rows);
+    json += buf;
+    std::vector<std::vector<unsigned long long>> sums[2];
+    bool ok = true, first = true;
+    for (int mode : {PTK_BATCH_PLAIN, PTK_BATCH_BATCHED}) {
+        const char *name = mode == PTK_BATCH_PLAIN ? "plain" : "batched";
+        if (o.modes.find(name) == std::string::npos) continue;
+        ptk_batch *B = nullptr;
+        if (ptk_batch_create(o.kernels.c_str(), arena, mode, &B)) {
+            fprintf(stderr, "create: %s\n", ptk_batch_last_error());
+            return 1;
+        }
+        if (o.cols > 0) ptk_batch_set_tiles(B, o.cols, o.depth, o.gbRows);
+        std::vector<int> ids;
+        for (const Spec &s : specs) {
+            int id = -1, r = 0;
+            if (s.kind == PTK_INST_FDTD) r = ptk_batch_submit_fdtd(B, (int)s.a, s.b, s.c, s.steps, s.seed, 0, &id);
+            else if (s.kind == PTK_INST_GROUPBY) r = ptk_batch_submit_groupby(B, s.a, s.b, o.zipf, s.seed, 0, &id);
+            else r = ptk_batch_submit_gmm(B, (int)s.a, s.b, s.c, s.steps, s.seed, 0, &id);
+            if (r) {
+                fprintf(stderr, "submit: %s\n", ptk_batch_last_error());
+                return 1;
+            }
+            ids.push_back(id);
+        }
+        double tuneMs = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        int active = 0, done = 0, r = 0;
+        // first iteration admits everything that fits; the autotuner then sees the resident set
+        r = ptk_batch_tick(B, -1, &active, &done);
+        if (!r && mode == PTK_BATCH_BATCHED && o.autotune) r = ptk_batch_autotune(B, &tuneMs);
+        if (!r) r = ptk_batch_drain(B);
+        auto t1 = std::chrono::steady_clock::now();
+        if (r) {
+            fprintf(stderr, "%s run: %s\n", name, ptk_batch_last_error());
+            ptk_batch_destroy(B);
+            return 1;
+        }
+        unsigned long long st[7];
+        ptk_batch_stats(B, st);
+        int cols, depth, gbr;
+        ptk_batch_get_tiles(B, &cols, &depth, &gbr);
+        sums[mode].clear();
+        unsigned long long gbBad = 0;
+        for (size_t i = 0; i < ids.size(); ++i) {
+            std::vector<unsigned long long> c(8);
+            ptk_batch_checksums(B, ids[i], c.data());
+            if (specs[i].kind == PTK_INST_GROUPBY && c[2] != c[3]) ++gbBad;  // output is a permutation of the input
+            sums[mode].push_back(c);
+        }
+        ok = ok && gbBad == 0;
+        const double ms = Ms(t0, t1);
+        const double ex = st[6] / 1e3;
+        snprintf(buf, sizeof(buf), "%s\"%s\": {\"ms\": %.3f, \"instances_per_s\": %.2f, \"exec_ms\": %.3f, "
+                 "\"exec_instances_per_s\": %.2f, \"ticks\": %llu, \"launches\": %llu, \"graph_launches\": %llu, "
+                 "\"arena_peak\": %llu, \"tiles\": [%d, %d, %d], \"autotune_ms\": %.4f, "
+                 "\"groupby_permutation_failures\": %llu}",
+                 first ? "" : ", ", name, ms, specs.size() / (ms / 1e3), ex, ex > 0 ? specs.size() / (ex / 1e3) : 0.0,
+                 st[0], st[1], st[2], st[4], cols, depth, gbr, tuneMs, gbBad);
+        json += buf;
+        first = false;
+        ptk_batch_destroy(B);
+    }
+    json += "}";
+    if (!sums[0].empty() && !sums[1].empty()) {
+        size_t diff = 0;
+        for (size_t i = 0; i < specs.size(); ++i)
+            if (sums[0][i] != sums[1][i]) ++diff;
+        snprintf(buf, sizeof(buf), ", \"checksums_equal\": %s, \"instances_differing\": %zu", diff ? "false" : "true", diff);
+        json += buf;
+        ok = ok && diff == 0;
+    }
+    snprintf(buf, sizeof(buf), ", \"ok\": %s}", ok ? "true" : "false");
+    json += buf;
+    printf("%s\n", json.c_str());
+    return ok ? 0 : 1;
+}

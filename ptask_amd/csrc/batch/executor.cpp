@@ -1,0 +1,918 @@
+// executor.cpp -- the batched / served executor behind include/ptk_batch.h (libptk_batch.so).
+//
+// Host side of the batching and serving layer: an HBM arena with first-fit sub-allocation
+// (admission by datablock footprint), device descriptor arrays per instance kind, work lists
+// rebuilt on the host whenever the set of resident instances changes, and one iteration ("tick")
+// executed either
+//   * batched: one captured hipGraph of the persistent multi-instance kernels of batch.hsaco,
+//     whose work-list lengths live in device memory so the same graph serves every mix, or
+//   * plain: the baseline, one kernel launch per task per instance (the six Yee tasks of a step,
+//     the four groupby tasks, the two EM tasks) in submission order on one stream.
+// Every HIP call goes through the C ABI of libptk_hip.so.
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ptk_batch.h"
+#include "ptk_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int Fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        int _s = (call);                                                                          \
+        if (_s != PTK_HIP_OK)                                                                     \
+            return Fail(PTK_BATCH_EHIP, std::string(#call) + ": " + ptk_hip_last_error_string()); \
+    } while (0)
+
+constexpr unsigned long long kAlign = 256;
+constexpr int kFdtdCap = 16384, kGbCap = 8192, kGmmCap = 8192;
+
+unsigned long long Up(unsigned long long x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// Host images of the device int4 / int2 work-list entries.
+struct alignas(16) int4 {
+    int x, y, z, w;
+};
+struct alignas(8) int2 {
+    int x, y;
+};
+
+// Kernel arguments packed as the kernels' formal parameters.
+struct Args {
+    std::vector<unsigned char> b;
+    template <class T>
+    Args &add(const T &v) {
+        const size_t a = alignof(T) < 8 ? alignof(T) : 8;
+        while (b.size() % a) b.push_back(0);
+        const unsigned char *p = (const unsigned char *)&v;
+        b.insert(b.end(), p, p + sizeof(T));
+        return *this;
+    }
+    Args &ptr(const void *p) { return add<const void *>(p); }
+};
+
+// First-fit sub-allocation of the arena, coalescing on free.
+class Arena {
+public:
+    void Init(unsigned long long size) {
+        size_ = size;
+        free_.clear();
+        free_[0] = size;
+        used_ = peak_ = 0;
+    }
+    bool Alloc(unsigned long long bytes, unsigned long long *off) {
+        bytes = Up(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const unsigned long long rest = it->second - bytes, at = it->first + bytes;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            used_ += bytes;
+            peak_ = std::max(peak_, used_);
+            return true;
+        }
+        return false;
+    }
+    void Free(unsigned long long off, unsigned long long bytes) {
+        bytes = Up(bytes);
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    unsigned long long size() const { return size_; }
+    unsigned long long used() const { return used_; }
+    unsigned long long peak() const { return peak_; }
+
+private:
+    unsigned long long size_ = 0, used_ = 0, peak_ = 0;
+    std::map<unsigned long long, unsigned long long> free_;
+};
+
+struct Inst {
+    int id = -1, kind = 0, state = PTK_STATE_QUEUED, keep = 0, slot = -1;
+    // FDTD: a=nx, b=ny, c=nz; groupby: rows=a, numKeys=b; GMM: n=a, d=b, k=c
+    long long a = 0;
+    int b = 0, c = 0, steps = 1, stepsDone = 0;
+    double zipf = 0;
+    unsigned long long seed = 0, off = 0, bytes = 0;
+    double tSubmit = -1, tAdmit = -1, tDone = -1;
+    unsigned long long csum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    PTK_FDTD_INST fd{};
+    PTK_GROUPBY_INST gb{};
+    PTK_GMM_INST gm{};
+    int *gmmFirst = nullptr;
+};
+
+// A device work list with its pinned host staging copy.
+template <class T>
+struct List {
+    T *dev = nullptr, *host = nullptr;
+    size_t cap = 0, n = 0;
+};
+
+}  // namespace
+
+struct ptk_batch {
+    int mode = PTK_BATCH_BATCHED;
+    std::chrono::steady_clock::time_point t0;
+    ptk_hip_stream stream = nullptr;
+    ptk_hip_event evStart = nullptr, evStop = nullptr;
+    ptk_hip_module mBatch = nullptr, mFdtd = nullptr, mGb = nullptr, mGmm = nullptr, mUtil = nullptr;
+    ptk_hip_function fH = nullptr, fE = nullptr, fAdv = nullptr, fGbClear = nullptr, fGbAcc = nullptr,
+                     fGbOff = nullptr, fGbRes = nullptr, fGmmAcc = nullptr, fGmmFin = nullptr, fInitFdtd = nullptr,
+                     fInitGb = nullptr, fInitGmm = nullptr, fInitGmmModel = nullptr, fSumWords = nullptr,
+                     fSumPairs = nullptr, fSumGroups = nullptr, fFill = nullptr;
+    ptk_hip_function fSix[6] = {}, fPGbClear = nullptr, fPGbAcc = nullptr, fPGbOff = nullptr, fPGbRes = nullptr,
+                     fPGmmAcc = nullptr, fPGmmFin = nullptr;
+    void *arenaBase = nullptr;
+    Arena arena;
+    PTK_FDTD_INST *dFdtd = nullptr;
+    PTK_GROUPBY_INST *dGb = nullptr;
+    PTK_GMM_INST *dGmm = nullptr;
+    std::vector<int> freeFdtd, freeGb, freeGmm;
+    int *dCounts = nullptr, *hCounts = nullptr;
+    unsigned long long *dSums = nullptr, *hSums = nullptr;  // checksum staging, 8 per retiring instance
+    size_t sumCap = 0;
+    List<int4> tiles;
+    List<int> fdtdActive, gbInsts;
+    List<int2> gbUnits, gmmAcc, gmmFin;
+    double *dCdf = nullptr;
+    int cdfKeys = -1;
+    double cdfS = 0;
+    long long cdfA = 0;
+    ptk_hip_graph graph = nullptr;
+    bool dirty = true;
+    int fdtdCols = 512, fdtdDepth = 16, gbRows = 262144;
+    std::vector<Inst> insts;
+    std::vector<int> queue, active;
+    unsigned long long nTicks = 0, nLaunches = 0, nGraphs = 0;
+    double execMs = 0;  // device time of the iterations' work (admission and retirement excluded)
+
+    double Now() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    int Launch(ptk_hip_function f, unsigned gx, unsigned gy, unsigned gz, unsigned bx, Args &a) {
+        ++nLaunches;
+        HIPCHK(ptk_hip_launch(f, gx, gy, gz, bx, 1, 1, 0, stream, a.b.data(), a.b.size()));
+        return PTK_BATCH_OK;
+    }
+};
+
+namespace {
+
+template <class T>
+int Reserve(ptk_batch *B, List<T> &l, size_t need) {
+    if (need <= l.cap) return PTK_BATCH_OK;
+    size_t cap = std::max<size_t>(need, std::max<size_t>(1024, l.cap * 2));
+    if (l.dev) {
+        HIPCHK(ptk_hip_stream_synchronize(B->stream));
+        ptk_hip_free(l.dev);
+        ptk_hip_host_free(l.host);
+        l.dev = nullptr;
+        l.host = nullptr;
+    }
+    HIPCHK(ptk_hip_malloc((void **)&l.dev, cap * sizeof(T)));
+    HIPCHK(ptk_hip_host_malloc((void **)&l.host, cap * sizeof(T)));
+    l.cap = cap;
+    if (B->graph) {  // the captured graph holds the old list address
+        ptk_hip_graph_destroy(B->graph);
+        B->graph = nullptr;
+    }
+    return PTK_BATCH_OK;
+}
+
+unsigned long long FdtdBytes(int nx, int ny, int nz) {
+    unsigned long long s = 0;
+    for (int f = 0; f < 6; ++f) s += Up(4ull * ptk_fdtd_field_elems(f, nx, ny, nz));
+    return 2 * s;
+}
+
+unsigned long long GbBytes(long long rows, int numKeys) {
+    const unsigned long long K = (unsigned long long)numKeys + 1;
+    return 4 * Up(4ull * rows) + Up(8 * (3 * K + 1)) + Up(8 * K) + Up(8 * (3 + 3 * K));
+}
+
+GMM_DIMS GmmDims(int n, int d, int k, int steps) {
+    GMM_DIMS g{};
+    g.N = n;
+    g.D = d;
+    g.K = k;
+    const long long tiles = ((long long)n + 127) / 128;
+    g.blocks = (int)std::max(1LL, (tiles + 15) / 16);  // 16 tiles (2048 points) per unit
+    g.maxIter = steps;
+    return g;
+}
+
+unsigned long long GmmBytes(int n, int d, int k, int steps) {
+    GMM_DIMS g = GmmDims(n, d, k, steps);
+    return Up(4ull * n * d) + Up(8ull * ptk_gmm_model_elems(&g)) + Up(8ull * g.blocks * ptk_gmm_partial_stride(&g)) +
+           Up(4ull * k);
+}
+
+int Load(ptk_hip_module m, const char *name, ptk_hip_function *f) {
+    HIPCHK(ptk_hip_module_get_function(f, m, name));
+    return PTK_BATCH_OK;
+}
+
+int Submit(ptk_batch *B, Inst &in, int *id) {
+    if (in.bytes > B->arena.size()) return Fail(PTK_BATCH_ENOMEM, "instance larger than the arena");
+    in.id = (int)B->insts.size();
+    in.tSubmit = B->Now();
+    B->insts.push_back(in);
+    B->queue.push_back(in.id);
+    if (id) *id = in.id;
+    return PTK_BATCH_OK;
+}
+
+int PopSlot(std::vector<int> &fr) {
+    if (fr.empty()) return -1;
+    int s = fr.back();
+    fr.pop_back();
+    return s;
+}
+
+// The Zipf inverse-CDF table of the groupby generator (ptk_groupby.cpp MakeInputs).
+int EnsureCdf(ptk_batch *B, int numKeys, double s, long long *A) {
+    const long long K = (long long)numKeys + 1;
+    if (B->cdfKeys == numKeys && B->cdfS == s) {
+        *A = B->cdfA;
+        return PTK_BATCH_OK;
+    }
+    std::vector<double> cdf((size_t)K);
+    double acc = 0;
+    for (long long r = 0; r < K; ++r) {
+        acc += std::pow((double)(r + 1), -s);
+        cdf[(size_t)r] = acc;
+    }
+    for (double &c : cdf) c /= acc;
+    long long a = 40503;
+    auto gcd = [](long long x, long long y) { while (y) { long long t = x % y; x = y; y = t; } return x; };
+    while (gcd(a, K) != 1) ++a;
+    HIPCHK(ptk_hip_stream_synchronize(B->stream));
+    if (B->dCdf) ptk_hip_free(B->dCdf);
+    HIPCHK(ptk_hip_malloc((void **)&B->dCdf, sizeof(double) * (size_t)K));
+    HIPCHK(ptk_hip_memcpy_htod(B->dCdf, cdf.data(), sizeof(double) * (size_t)K));
+    B->cdfKeys = numKeys;
+    B->cdfS = s;
+    B->cdfA = a;
+    *A = a;
+    return PTK_BATCH_OK;
+}
+
+// Places an instance's datablocks in the arena and generates its inputs on the device.
+int Admit(ptk_batch *B, Inst &in) {
+    char *base = (char *)B->arenaBase + in.off;
+    unsigned long long o = 0;
+    auto carve = [&](unsigned long long bytes) {
+        char *p = base + o;
+        o += Up(bytes);
+        return p;
+    };
+    if (in.kind == PTK_INST_FDTD) {
+        in.slot = PopSlot(B->freeFdtd);
+        if (in.slot < 0) return Fail(PTK_BATCH_ENOMEM, "too many resident FDTD instances");
+        PTK_FDTD_INST &d = in.fd;
+        for (int s = 0; s < 2; ++s)
+            for (int f = 0; f < 6; ++f) d.f[s][f] = (float *)carve(4ull * ptk_fdtd_field_elems(f, (int)in.a, in.b, in.c));
+        d.p = ptk_fdtd_make_params((int)in.a, in.b, in.c, 0.5f);
+        d.steps = in.steps;
+        d.done = 0;
+        HIPCHK(ptk_hip_memcpy_htod_async(B->dFdtd + in.slot, &d, sizeof(d), B->stream));
+        Args a;
+        a.ptr(B->dFdtd).add<int>(in.slot).add<unsigned long long>(in.seed);
+        if (int r = B->Launch(B->fInitFdtd, 512, 1, 1, 256, a)) return r;
+    } else if (in.kind == PTK_INST_GROUPBY) {
+        in.slot = PopSlot(B->freeGb);
+        if (in.slot < 0) return Fail(PTK_BATCH_ENOMEM, "too many resident groupby instances");
+        PTK_GROUPBY_INST &d = in.gb;
+        const unsigned long long K = (unsigned long long)in.b + 1;
+        d.n = in.a;
+        d.numKeys = in.b;
+        d.chunkRows = B->gbRows;
+        d.keys = (const int *)carve(4ull * in.a);
+        d.vals = (const int *)carve(4ull * in.a);
+        d.outKeys = (int *)carve(4ull * in.a);
+        d.outVals = (int *)carve(4ull * in.a);
+        d.table = (long long *)carve(8 * (3 * K + 1));
+        d.offsets = (long long *)carve(8 * K);
+        d.groups = (long long *)carve(8 * (3 + 3 * K));
+        HIPCHK(ptk_hip_memcpy_htod_async(B->dGb + in.slot, &d, sizeof(d), B->stream));
+        long long A = 0;
+        const double *cdf = nullptr;
+        if (in.zipf > 0) {
+            if (int r = EnsureCdf(B, in.b, in.zipf, &A)) return r;
+            cdf = B->dCdf;
+        }
+        Args a;
+        a.ptr(d.keys).ptr(d.vals).add<long long>(d.n).ptr(cdf).add<int>(d.numKeys).add<long long>(A).add<unsigned long long>(in.seed);
+        if (int r = B->Launch(B->fInitGb, 1024, 1, 1, 256, a)) return r;
+    } else {
+        in.slot = PopSlot(B->freeGmm);
+        if (in.slot < 0) return Fail(PTK_BATCH_ENOMEM, "too many resident GMM instances");
+        PTK_GMM_INST &d = in.gm;
+        d.d = GmmDims((int)in.a, in.b, in.c, in.steps);
+        d.X = (const float *)carve(4ull * in.a * in.b);
+        d.model = (double *)carve(8ull * ptk_gmm_model_elems(&d.d));
+        d.partials = (double *)carve(8ull * d.d.blocks * ptk_gmm_partial_stride(&d.d));
+        in.gmmFirst = (int *)carve(4ull * in.c);
+        HIPCHK(ptk_hip_memcpy_htod_async(B->dGmm + in.slot, &d, sizeof(d), B->stream));
+        Args f;
+        f.ptr(in.gmmFirst).add<unsigned long long>((unsigned long long)in.c).add<unsigned>(0x7fffffffu);
+        if (int r = B->Launch(B->fFill, 1, 1, 1, 256, f)) return r;
+        Args a;
+        a.ptr(d.X).add<int>(d.d.N).add<int>(d.d.D).add<int>(d.d.K).add<unsigned long long>(in.seed).ptr(in.gmmFirst);
+        if (int r = B->Launch(B->fInitGmm, 1024, 1, 1, 256, a)) return r;
+        Args m;
+        m.ptr(d.model).add(d.d).ptr(d.X).ptr(in.gmmFirst);
+        if (int r = B->Launch(B->fInitGmmModel, 1, 1, 1, 256, m)) return r;
+    }
+    in.state = PTK_STATE_ACTIVE;
+    in.tAdmit = B->Now();
+    in.stepsDone = 0;
+    return PTK_BATCH_OK;
+}
+
+// Work lists of the resident instances for the batched kernels.
+int BuildLists(ptk_batch *B) {
+    size_t nt = 0, nf = 0, ngu = 0, ngi = 0, nga = 0, ngf = 0;
+    for (int id : B->active) {
+        const Inst &in = B->insts[(size_t)id];
+        if (in.kind == PTK_INST_FDTD) {
+            const long long ncol = (in.a + 1) * (long long)(in.b + 1);
+            nt += (size_t)(((ncol + B->fdtdCols - 1) / B->fdtdCols) * ((in.c + 1 + B->fdtdDepth - 1) / B->fdtdDepth));
+            ++nf;
+        } else if (in.kind == PTK_INST_GROUPBY) {
+            ngu += (size_t)((in.a + in.gb.chunkRows - 1) / in.gb.chunkRows);
+            ++ngi;
+        } else {
+            nga += (size_t)in.gm.d.blocks;
+            ngf += (size_t)in.gm.d.K + 1;
+        }
+    }
+    if (int r = Reserve(B, B->tiles, nt)) return r;
+    if (int r = Reserve(B, B->fdtdActive, nf)) return r;
+    if (int r = Reserve(B, B->gbUnits, ngu)) return r;
+    if (int r = Reserve(B, B->gbInsts, ngi)) return r;
+    if (int r = Reserve(B, B->gmmAcc, nga)) return r;
+    if (int r = Reserve(B, B->gmmFin, ngf)) return r;
+    B->tiles.n = B->fdtdActive.n = B->gbUnits.n = B->gbInsts.n = B->gmmAcc.n = B->gmmFin.n = 0;
+    for (int id : B->active) {
+        const Inst &in = B->insts[(size_t)id];
+        if (in.kind == PTK_INST_FDTD) {
+            const int ncol = (int)((in.a + 1) * (long long)(in.b + 1)), depth = in.c + 1;
+            for (int k0 = 0; k0 < depth; k0 += B->fdtdDepth) {
+                const int k1 = std::min(depth, k0 + B->fdtdDepth);
+                for (int c0 = 0; c0 < ncol; c0 += B->fdtdCols)
+                    B->tiles.host[B->tiles.n++] = int4{in.slot, c0, std::min(B->fdtdCols, ncol - c0), k0 | (k1 << 16)};
+            }
+            B->fdtdActive.host[B->fdtdActive.n++] = in.slot;
+        } else if (in.kind == PTK_INST_GROUPBY) {
+            const long long chunks = (in.a + in.gb.chunkRows - 1) / in.gb.chunkRows;
+            for (long long c = 0; c < chunks; ++c) B->gbUnits.host[B->gbUnits.n++] = int2{in.slot, (int)c};
+            B->gbInsts.host[B->gbInsts.n++] = in.slot;
+        } else {
+            for (int b = 0; b < in.gm.d.blocks; ++b) B->gmmAcc.host[B->gmmAcc.n++] = int2{in.slot, b};
+            for (int k = 0; k <= in.gm.d.K; ++k) B->gmmFin.host[B->gmmFin.n++] = int2{in.slot, k};
+        }
+    }
+    B->hCounts[0] = (int)B->tiles.n;
+    B->hCounts[1] = (int)B->fdtdActive.n;
+    B->hCounts[2] = (int)B->gbUnits.n;
+    B->hCounts[3] = (int)B->gbInsts.n;
+    B->hCounts[4] = (int)B->gmmAcc.n;
+    B->hCounts[5] = (int)B->gmmFin.n;
+    auto up = [&](auto &l) -> int {
+        if (l.n) HIPCHK(ptk_hip_memcpy_htod_async(l.dev, l.host, l.n * sizeof(*l.host), B->stream));
+        return PTK_BATCH_OK;
+    };
+    if (int r = up(B->tiles)) return r;
+    if (int r = up(B->fdtdActive)) return r;
+    if (int r = up(B->gbUnits)) return r;
+    if (int r = up(B->gbInsts)) return r;
+    if (int r = up(B->gmmAcc)) return r;
+    if (int r = up(B->gmmFin)) return r;
+    HIPCHK(ptk_hip_memcpy_htod_async(B->dCounts, B->hCounts, 8 * sizeof(int), B->stream));
+    return PTK_BATCH_OK;
+}
+
+// The batched iteration's launches (captured once into a graph).
+int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance) {
+    if (fdtd) {
+        Args h, e;
+        h.ptr(B->tiles.dev).ptr(B->dCounts + 0).ptr(B->dFdtd);
+        e.ptr(B->tiles.dev).ptr(B->dCounts + 0).ptr(B->dFdtd);
+        if (int r = B->Launch(B->fH, 2048, 1, 1, 256, h)) return r;
+        if (int r = B->Launch(B->fE, 2048, 1, 1, 256, e)) return r;
+    }
+    if (advance) {
+        Args a;
+        a.ptr(B->fdtdActive.dev).ptr(B->dCounts + 1).ptr(B->dFdtd);
+        if (int r = B->Launch(B->fAdv, 16, 1, 1, 256, a)) return r;
+    }
+    if (gmm) {
+        Args a, f;
+        a.ptr(B->gmmAcc.dev).ptr(B->dCounts + 4).ptr(B->dGmm);
+        f.ptr(B->gmmFin.dev).ptr(B->dCounts + 5).ptr(B->dGmm);
+        if (int r = B->Launch(B->fGmmAcc, 768, 1, 1, 256, a)) return r;
+        if (int r = B->Launch(B->fGmmFin, 512, 1, 1, 256, f)) return r;
+    }
+    if (gb) {
+        Args c, a, o, s;
+        c.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
+        a.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
+        o.ptr(B->gbInsts.dev).ptr(B->dCounts + 3).ptr(B->dGb);
+        s.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
+        if (int r = B->Launch(B->fGbClear, 512, 1, 1, 1024, c)) return r;
+        if (int r = B->Launch(B->fGbAcc, 512, 1, 1, 1024, a)) return r;
+        if (int r = B->Launch(B->fGbOff, 512, 1, 1, 1024, o)) return r;
+        if (int r = B->Launch(B->fGbRes, 512, 1, 1, 1024, s)) return r;
+    }
+    return PTK_BATCH_OK;
+}
+
+// Plain per-task dispatch of one iteration of one instance.
+int IssuePlain(ptk_batch *B, const Inst &in) {
+    if (in.kind == PTK_INST_FDTD) {
+        const PTK_FDTD_INST &d = in.fd;
+        const int cur = in.stepsDone & 1, nxt = cur ^ 1;
+        // (own, in1, in2) of Hx, Hy, Hz, Ex, Ey, Ez (graphfdtd.cpp:448-560); E reads the new H
+        static const int own[6] = {0, 1, 2, 3, 4, 5}, i1[6] = {4, 3, 3, 1, 0, 0}, i2[6] = {5, 5, 4, 2, 2, 1};
+        for (int t = 0; t < 6; ++t) {
+            const int src = t < 3 ? cur : nxt;
+            int d1, d2, d3;
+            ptk_fdtd_extent(t, d.p.Nx, d.p.Ny, d.p.Nz, &d1, &d2, &d3);
+            Args a;
+            a.ptr(d.f[cur][own[t]]).ptr(d.f[src][i1[t]]).ptr(d.f[src][i2[t]]).add(d.p).ptr(d.f[nxt][own[t]]);
+            const unsigned gx = (unsigned)(((long long)d1 * d2 + 255) / 256), slabs = (unsigned)((d3 + 7) / 8);
+            if (int r = B->Launch(B->fSix[t], gx, slabs, 1, 256, a)) return r;
+        }
+    } else if (in.kind == PTK_INST_GROUPBY) {
+        const PTK_GROUPBY_INST &d = in.gb;
+        const unsigned grid = (unsigned)std::max(1LL, std::min(256LL, (d.n + 32767) / 32768));
+        Args c, a, o, s;
+        c.ptr(d.table).add<int>(d.numKeys);
+        a.ptr(d.keys).ptr(d.vals).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.table);
+        o.ptr(d.table).add<int>(d.numKeys).ptr(d.offsets).ptr(d.groups);
+        s.ptr(d.keys).ptr(d.vals).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.offsets).ptr(d.table).ptr(d.outKeys).ptr(d.outVals);
+        if (int r = B->Launch(B->fPGbClear, 64, 1, 1, 1024, c)) return r;
+        if (int r = B->Launch(B->fPGbAcc, grid, 1, 1, 1024, a)) return r;
+        if (int r = B->Launch(B->fPGbOff, 1, 1, 1, 1024, o)) return r;
+        if (int r = B->Launch(B->fPGbRes, grid, 1, 1, 1024, s)) return r;
+    } else {
+        const PTK_GMM_INST &d = in.gm;
+        Args a, f;
+        a.ptr(d.X).add(d.d).ptr(d.model).ptr(d.partials);
+        f.ptr(d.model).ptr(d.partials).add(d.d);
+        if (int r = B->Launch(B->fPGmmAcc, (unsigned)d.d.blocks, 1, 1, 256, a)) return r;
+        if (int r = B->Launch(B->fPGmmFin, (unsigned)d.d.K + 1, 1, 1, 256, f)) return r;
+    }
+    return PTK_BATCH_OK;
+}
+
+int Checksum(ptk_batch *B, unsigned long long *dst, const void *p, long long words) {
+    Args a;
+    a.ptr(p).add<long long>(words).ptr(dst);
+    const unsigned grid = (unsigned)std::max(1LL, std::min(1024LL, (words + 255) / 256));
+    return B->Launch(B->fSumWords, grid, 1, 1, 256, a);
+}
+
+int ChecksumPairs(ptk_batch *B, unsigned long long *dst, const int *k, const int *v, long long n) {
+    Args a;
+    a.ptr(k).ptr(v).add<long long>(n).ptr(dst);
+    const unsigned grid = (unsigned)std::max(1LL, std::min(1024LL, (n + 255) / 256));
+    return B->Launch(B->fSumPairs, grid, 1, 1, 256, a);
+}
+
+// Device checksums of the completed instances, then their memory goes back to the arena.
+int Retire(ptk_batch *B, const std::vector<int> &done) {
+    if (done.empty()) return PTK_BATCH_OK;
+    if (done.size() * 8 > B->sumCap) {
+        HIPCHK(ptk_hip_stream_synchronize(B->stream));
+        if (B->dSums) ptk_hip_free(B->dSums);
+        if (B->hSums) ptk_hip_host_free(B->hSums);
+        B->sumCap = std::max<size_t>(done.size() * 8, 4096);
+        HIPCHK(ptk_hip_malloc((void **)&B->dSums, B->sumCap * 8));
+        HIPCHK(ptk_hip_host_malloc((void **)&B->hSums, B->sumCap * 8));
+    }
+    std::vector<unsigned long long> zero(done.size() * 8, 0);
+    HIPCHK(ptk_hip_memcpy_htod_async(B->dSums, zero.data(), zero.size() * 8, B->stream));
+    for (size_t i = 0; i < done.size(); ++i) {
+        const Inst &in = B->insts[(size_t)done[i]];
+        unsigned long long *s = B->dSums + 8 * i;
+        int r = PTK_BATCH_OK;
+        if (in.kind == PTK_INST_FDTD) {
+            const int cur = in.stepsDone & 1;
+            for (int f = 0; f < 6 && !r; ++f)
+                r = Checksum(B, s + f, in.fd.f[cur][f], (long long)ptk_fdtd_field_elems(f, (int)in.a, in.b, in.c));
+        } else if (in.kind == PTK_INST_GROUPBY) {
+            const long long K = (long long)in.b + 1;
+            Args g;
+            g.ptr(in.gb.groups).ptr(s + 0);
+            r = B->Launch(B->fSumGroups, (unsigned)std::min<long long>(1024, (2 * (3 + 3 * K) + 255) / 256), 1, 1, 256, g);
+            if (!r) r = Checksum(B, s + 1, in.gb.outKeys, in.a);
+            if (!r) r = ChecksumPairs(B, s + 2, in.gb.outKeys, in.gb.outVals, in.a);
+            if (!r) r = ChecksumPairs(B, s + 3, in.gb.keys, in.gb.vals, in.a);
+        } else {
+            r = Checksum(B, s + 0, in.gm.model, 2 * ptk_gmm_model_elems(&in.gm.d));
+        }
+        if (r) return r;
+    }
+    HIPCHK(ptk_hip_memcpy_dtoh_async(B->hSums, B->dSums, done.size() * 64, B->stream));
+    HIPCHK(ptk_hip_stream_synchronize(B->stream));
+    for (size_t i = 0; i < done.size(); ++i) {
+        Inst &in = B->insts[(size_t)done[i]];
+        memcpy(in.csum, B->hSums + 8 * i, 64);
+        in.state = PTK_STATE_DONE;
+        in.tDone = B->Now();
+        if (in.kind == PTK_INST_FDTD) B->freeFdtd.push_back(in.slot);
+        else if (in.kind == PTK_INST_GROUPBY) B->freeGb.push_back(in.slot);
+        else B->freeGmm.push_back(in.slot);
+        if (!in.keep) {
+            B->arena.Free(in.off, in.bytes);
+            in.state = PTK_STATE_RELEASED;
+        }
+    }
+    return PTK_BATCH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ptk_batch_last_error(void) { return g_err.c_str(); }
+
+unsigned long long ptk_batch_footprint(int kind, long long a, int b_, int c, int steps) {
+    if (kind == PTK_INST_FDTD) return FdtdBytes((int)a, b_, c);
+    if (kind == PTK_INST_GROUPBY) return GbBytes(a, b_);
+    return GmmBytes((int)a, b_, c, steps);
+}
+
+int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int mode, ptk_batch **out) {
+    if (!kernel_dir || !out || arena_bytes == 0 || (mode != PTK_BATCH_PLAIN && mode != PTK_BATCH_BATCHED))
+        return Fail(PTK_BATCH_EINVAL, "bad argument");
+    *out = nullptr;
+    ptk_batch *B = new ptk_batch();
+    B->mode = mode;
+    B->t0 = std::chrono::steady_clock::now();
+    auto fail = [&](int r) {
+        ptk_batch_destroy(B);
+        return r;
+    };
+    const std::string dir(kernel_dir);
+    int r = PTK_BATCH_OK;
+    if (ptk_hip_stream_create(&B->stream) || ptk_hip_event_create(&B->evStart, 1) || ptk_hip_event_create(&B->evStop, 1))
+        return fail(Fail(PTK_BATCH_EHIP, std::string("stream/event: ") + ptk_hip_last_error_string()));
+    const std::pair<ptk_hip_module *, const char *> mods[] = {
+        {&B->mBatch, "batch.hsaco"}, {&B->mFdtd, "fdtd.hsaco"}, {&B->mGb, "groupby.hsaco"},
+        {&B->mGmm, "gmm.hsaco"}, {&B->mUtil, "ptk_util.hsaco"}};
+    for (auto &m : mods)
+        if (ptk_hip_module_load(m.first, (dir + "/" + m.second).c_str()))
+            return fail(Fail(PTK_BATCH_EHIP, std::string("load ") + m.second + ": " + ptk_hip_last_error_string()));
+    const std::pair<ptk_hip_function *, const char *> bf[] = {
+        {&B->fH, "BatchFdtdH"}, {&B->fE, "BatchFdtdE"}, {&B->fAdv, "BatchFdtdAdvance"},
+        {&B->fGbClear, "BatchGroupbyClear"}, {&B->fGbAcc, "BatchGroupbyAccumulate"},
+        {&B->fGbOff, "BatchGroupbyOffsets"}, {&B->fGbRes, "BatchGroupbyReshuffle"},
+        {&B->fGmmAcc, "BatchGmmAccumulate"}, {&B->fGmmFin, "BatchGmmFinalize"},
+        {&B->fInitFdtd, "BatchInitFdtd"}, {&B->fInitGb, "BatchInitGroupby"}, {&B->fInitGmm, "BatchInitGmm"},
+        {&B->fInitGmmModel, "BatchInitGmmModel"}, {&B->fSumWords, "BatchChecksumWords"},
+        {&B->fSumPairs, "BatchChecksumPairs"}, {&B->fSumGroups, "BatchChecksumGroups"}};
+    for (auto &f : bf)
+        if ((r = Load(B->mBatch, f.second, f.first))) return fail(r);
+    static const char *six[6] = {"HxComputation", "HyComputation", "HzComputation",
+                                 "ExComputation", "EyComputation", "EzComputation"};
+    for (int t = 0; t < 6; ++t)
+        if ((r = Load(B->mFdtd, six[t], &B->fSix[t]))) return fail(r);
+    if ((r = Load(B->mGb, "GroupbyClear", &B->fPGbClear)) || (r = Load(B->mGb, "GroupbyAccumulate", &B->fPGbAcc)) ||
+        (r = Load(B->mGb, "GroupbyOffsets", &B->fPGbOff)) || (r = Load(B->mGb, "GroupbyReshuffle", &B->fPGbRes)) ||
+        (r = Load(B->mGmm, "GmmAccumulate", &B->fPGmmAcc)) || (r = Load(B->mGmm, "GmmFinalize", &B->fPGmmFin)) ||
+        (r = Load(B->mUtil, "PtkFillWords", &B->fFill)))
+        return fail(r);
+    if (ptk_hip_malloc(&B->arenaBase, arena_bytes))
+        return fail(Fail(PTK_BATCH_ENOMEM, std::string("arena: ") + ptk_hip_last_error_string()));
+    B->arena.Init(arena_bytes);
+    if (ptk_hip_malloc((void **)&B->dFdtd, sizeof(PTK_FDTD_INST) * kFdtdCap) ||
+        ptk_hip_malloc((void **)&B->dGb, sizeof(PTK_GROUPBY_INST) * kGbCap) ||
+        ptk_hip_malloc((void **)&B->dGmm, sizeof(PTK_GMM_INST) * kGmmCap) ||
+        ptk_hip_malloc((void **)&B->dCounts, 8 * sizeof(int)) || ptk_hip_host_malloc((void **)&B->hCounts, 8 * sizeof(int)))
+        return fail(Fail(PTK_BATCH_EHIP, std::string("tables: ") + ptk_hip_last_error_string()));
+    for (int i = kFdtdCap - 1; i >= 0; --i) B->freeFdtd.push_back(i);
+    for (int i = kGbCap - 1; i >= 0; --i) B->freeGb.push_back(i);
+    for (int i = kGmmCap - 1; i >= 0; --i) B->freeGmm.push_back(i);
+    memset(B->hCounts, 0, 8 * sizeof(int));
+    *out = B;
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_destroy(ptk_batch *B) {
+    if (!B) return PTK_BATCH_OK;
+    if (B->stream) ptk_hip_stream_synchronize(B->stream);
+    if (B->graph) ptk_hip_graph_destroy(B->graph);
+    auto freeList = [](auto &l) {
+        if (l.dev) ptk_hip_free(l.dev);
+        if (l.host) ptk_hip_host_free(l.host);
+    };
+    freeList(B->tiles);
+    freeList(B->fdtdActive);
+    freeList(B->gbInsts);
+    freeList(B->gbUnits);
+    freeList(B->gmmAcc);
+    freeList(B->gmmFin);
+    if (B->dSums) ptk_hip_free(B->dSums);
+    if (B->hSums) ptk_hip_host_free(B->hSums);
+    if (B->dCdf) ptk_hip_free(B->dCdf);
+    if (B->dCounts) ptk_hip_free(B->dCounts);
+    if (B->hCounts) ptk_hip_host_free(B->hCounts);
+    if (B->dFdtd) ptk_hip_free(B->dFdtd);
+    if (B->dGb) ptk_hip_free(B->dGb);
+    if (B->dGmm) ptk_hip_free(B->dGmm);
+    if (B->arenaBase) ptk_hip_free(B->arenaBase);
+    for (ptk_hip_module m : {B->mBatch, B->mFdtd, B->mGb, B->mGmm, B->mUtil})
+        if (m) ptk_hip_module_unload(m);
+    if (B->evStart) ptk_hip_event_destroy(B->evStart);
+    if (B->evStop) ptk_hip_event_destroy(B->evStop);
+    if (B->stream) ptk_hip_stream_destroy(B->stream);
+    delete B;
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_submit_fdtd(ptk_batch *B, int nx, int ny, int nz, int steps, unsigned long long seed, int keep, int *id) {
+    if (!B || nx < 1 || ny < 1 || nz < 1 || nz >= 65535 || steps < 1) return Fail(PTK_BATCH_EINVAL, "bad FDTD shape");
+    for (int f = 0; f < 6; ++f)
+        if (ptk_fdtd_field_elems(f, nx, ny, nz) >= (1ull << 31)) return Fail(PTK_BATCH_EINVAL, "FDTD field too large");
+    Inst in;
+    in.kind = PTK_INST_FDTD;
+    in.a = nx;
+    in.b = ny;
+    in.c = nz;
+    in.steps = steps;
+    in.seed = seed;
+    in.keep = keep;
+    in.bytes = FdtdBytes(nx, ny, nz);
+    return Submit(B, in, id);
+}
+
+int ptk_batch_submit_groupby(ptk_batch *B, long long rows, int num_keys, double zipf_s, unsigned long long seed,
+                             int keep, int *id) {
+    if (!B || rows < 1 || rows > 0x7fffffffLL || num_keys < 0 || num_keys >= 0x7fffffff)
+        return Fail(PTK_BATCH_EINVAL, "bad groupby shape");
+    Inst in;
+    in.kind = PTK_INST_GROUPBY;
+    in.a = rows;
+    in.b = num_keys;
+    in.zipf = zipf_s;
+    in.seed = seed;
+    in.keep = keep;
+    in.bytes = GbBytes(rows, num_keys);
+    return Submit(B, in, id);
+}
+
+int ptk_batch_submit_gmm(ptk_batch *B, int n, int d, int k, int steps, unsigned long long seed, int keep, int *id) {
+    if (!B || n < 1 || d < 1 || d > PTK_GMM_MAX_D || k < 1 || k > PTK_GMM_MAX_K || k > n || steps < 1 ||
+        (long long)n * d > 0x3fffffffLL)
+        return Fail(PTK_BATCH_EINVAL, "bad GMM shape");
+    Inst in;
+    in.kind = PTK_INST_GMM;
+    in.a = n;
+    in.b = d;
+    in.c = k;
+    in.steps = steps;
+    in.seed = seed;
+    in.keep = keep;
+    in.bytes = GmmBytes(n, d, k, steps);
+    return Submit(B, in, id);
+}
+
+int ptk_batch_tick(ptk_batch *B, int max_admit, int *nactive, int *ncompleted) {
+    if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
+    // admission: in submission order while the footprint fits
+    int admitted = 0;
+    size_t qi = 0;
+    while (qi < B->queue.size() && (max_admit < 0 || admitted < max_admit)) {
+        Inst &in = B->insts[(size_t)B->queue[qi]];
+        if (!B->arena.Alloc(in.bytes, &in.off)) break;
+        if (int r = Admit(B, in)) {
+            B->arena.Free(in.off, in.bytes);
+            return r;
+        }
+        B->active.push_back(in.id);
+        ++admitted;
+        ++qi;
+    }
+    B->queue.erase(B->queue.begin(), B->queue.begin() + (long)qi);
+    if (admitted) B->dirty = true;
+    if (!B->active.empty()) {
+        if (admitted) HIPCHK(ptk_hip_stream_synchronize(B->stream));  // admission work is not iteration time
+        const auto te = std::chrono::steady_clock::now();
+        if (B->mode == PTK_BATCH_BATCHED) {
+            if (B->dirty) {
+                if (int r = BuildLists(B)) return r;
+                B->dirty = false;
+            }
+            if (!B->graph) {
+                HIPCHK(ptk_hip_stream_begin_capture(B->stream));
+                int r = IssueBatched(B, true, true, true, true);
+                ptk_hip_graph g = nullptr;
+                int e = ptk_hip_stream_end_capture(B->stream, &g);
+                if (r) return r;
+                if (e) return Fail(PTK_BATCH_EHIP, std::string("capture: ") + ptk_hip_last_error_string());
+                B->graph = g;
+            }
+            HIPCHK(ptk_hip_graph_launch(B->graph, B->stream));
+            ++B->nGraphs;
+        } else {
+            for (int id : B->active)
+                if (int r = IssuePlain(B, B->insts[(size_t)id])) return r;
+        }
+        HIPCHK(ptk_hip_event_record(B->evStop, B->stream));
+        HIPCHK(ptk_hip_event_synchronize(B->evStop));
+        B->execMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - te).count();
+        ++B->nTicks;
+    }
+    std::vector<int> done, keepActive;
+    for (int id : B->active) {
+        Inst &in = B->insts[(size_t)id];
+        if (in.kind == PTK_INST_GROUPBY) {
+            in.stepsDone = 1;
+            done.push_back(id);
+        } else if (++in.stepsDone >= in.steps) {
+            done.push_back(id);
+        } else {
+            keepActive.push_back(id);
+        }
+    }
+    if (!done.empty()) {
+        B->active.swap(keepActive);
+        B->dirty = true;
+        if (int r = Retire(B, done)) return r;
+    }
+    if (nactive) *nactive = (int)B->active.size();
+    if (ncompleted) *ncompleted = (int)done.size();
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_drain(ptk_batch *B) {
+    if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
+    while (!B->queue.empty() || !B->active.empty()) {
+        const size_t q = B->queue.size(), a = B->active.size();
+        if (int r = ptk_batch_tick(B, -1, nullptr, nullptr)) return r;
+        if (a == 0 && B->active.empty() && B->queue.size() == q)
+            return Fail(PTK_BATCH_ENOMEM, "a queued instance does not fit the free arena");
+    }
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_status(ptk_batch *B, int id, int *state, double *ts, double *ta, double *td) {
+    if (!B || id < 0 || id >= (int)B->insts.size()) return Fail(PTK_BATCH_EINVAL, "bad instance id");
+    const Inst &in = B->insts[(size_t)id];
+    if (state) *state = in.state;
+    if (ts) *ts = in.tSubmit;
+    if (ta) *ta = in.tAdmit;
+    if (td) *td = in.tDone;
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_checksums(ptk_batch *B, int id, unsigned long long out[8]) {
+    if (!B || id < 0 || id >= (int)B->insts.size() || !out) return Fail(PTK_BATCH_EINVAL, "bad instance id");
+    const Inst &in = B->insts[(size_t)id];
+    if (in.state < PTK_STATE_DONE) return Fail(PTK_BATCH_ESTATE, "instance not complete");
+    memcpy(out, in.csum, 64);
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_read(ptk_batch *B, int id, int which, void *host, unsigned long long *bytes) {
+    if (!B || id < 0 || id >= (int)B->insts.size()) return Fail(PTK_BATCH_EINVAL, "bad instance id");
+    const Inst &in = B->insts[(size_t)id];
+    if (in.state != PTK_STATE_DONE) return Fail(PTK_BATCH_ESTATE, "instance not complete or released");
+    const void *src = nullptr;
+    unsigned long long n = 0;
+    if (in.kind == PTK_INST_FDTD && which >= 0 && which < 6) {
+        src = in.fd.f[in.stepsDone & 1][which];
+        n = 4 * ptk_fdtd_field_elems(which, (int)in.a, in.b, in.c);
+    } else if (in.kind == PTK_INST_GROUPBY && which >= 0 && which < 6) {
+        const unsigned long long K = (unsigned long long)in.b + 1;
+        const void *p[6] = {in.gb.keys, in.gb.vals, in.gb.groups, in.gb.offsets, in.gb.outKeys, in.gb.outVals};
+        const unsigned long long s[6] = {4ull * in.a, 4ull * in.a, 8 * (3 + 3 * K), 8 * K, 4ull * in.a, 4ull * in.a};
+        src = p[which];
+        n = s[which];
+    } else if (in.kind == PTK_INST_GMM && (which == 0 || which == 1)) {
+        src = which == 0 ? (const void *)in.gm.X : (const void *)in.gm.model;
+        n = which == 0 ? 4ull * in.a * in.b : 8ull * ptk_gmm_model_elems(&in.gm.d);
+    } else {
+        return Fail(PTK_BATCH_EINVAL, "bad result index");
+    }
+    if (bytes) *bytes = n;
+    if (host) {
+        HIPCHK(ptk_hip_stream_synchronize(B->stream));
+        HIPCHK(ptk_hip_memcpy_dtoh(host, src, n));
+    }
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_release(ptk_batch *B, int id) {
+    if (!B || id < 0 || id >= (int)B->insts.size()) return Fail(PTK_BATCH_EINVAL, "bad instance id");
+    Inst &in = B->insts[(size_t)id];
+    if (in.state != PTK_STATE_DONE) return Fail(PTK_BATCH_ESTATE, "instance not complete or already released");
+    HIPCHK(ptk_hip_stream_synchronize(B->stream));
+    B->arena.Free(in.off, in.bytes);
+    in.state = PTK_STATE_RELEASED;
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_set_tiles(ptk_batch *B, int cols, int depth, int gbRows) {
+    if (!B || cols < 1 || depth < 1 || depth > 65535 || gbRows < 1024) return Fail(PTK_BATCH_EINVAL, "bad tile shape");
+    B->fdtdCols = cols;
+    B->fdtdDepth = depth;
+    B->gbRows = gbRows;
+    B->dirty = true;
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_get_tiles(ptk_batch *B, int *cols, int *depth, int *gbRows) {
+    if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
+    if (cols) *cols = B->fdtdCols;
+    if (depth) *depth = B->fdtdDepth;
+    if (gbRows) *gbRows = B->gbRows;
+    return PTK_BATCH_OK;
+}
+
+// Times the FDTD half-step pair over the resident instances for each candidate tile shape (the
+// pair only rewrites the next field set from the current one, so repeating it changes nothing)
+// and keeps the fastest.  Groupby chunk rows are fixed at admission and are not tuned here.
+int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
+    if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
+    bool anyFdtd = false;
+    for (int id : B->active) anyFdtd |= B->insts[(size_t)id].kind == PTK_INST_FDTD;
+    if (!anyFdtd || B->mode != PTK_BATCH_BATCHED) {
+        if (best_ms) *best_ms = 0;
+        return PTK_BATCH_OK;
+    }
+    static const int cand[][2] = {{256, 8}, {512, 8}, {256, 16}, {512, 16}, {1024, 16}, {512, 32}, {1024, 32}, {256, 64}};
+    double best = 1e30;
+    int bc = B->fdtdCols, bd = B->fdtdDepth;
+    for (auto &c : cand) {
+        B->fdtdCols = c[0];
+        B->fdtdDepth = c[1];
+        if (int r = BuildLists(B)) return r;
+        if (int r = IssueBatched(B, true, false, false, false)) return r;  // warm
+        HIPCHK(ptk_hip_event_record(B->evStart, B->stream));
+        for (int rep = 0; rep < 3; ++rep)
+            if (int r = IssueBatched(B, true, false, false, false)) return r;
+        HIPCHK(ptk_hip_event_record(B->evStop, B->stream));
+        HIPCHK(ptk_hip_event_synchronize(B->evStop));
+        float ms = 0;
+        HIPCHK(ptk_hip_event_elapsed_ms(&ms, B->evStart, B->evStop));
+        if (ms / 3 < best) {
+            best = ms / 3;
+            bc = c[0];
+            bd = c[1];
+        }
+    }
+    B->fdtdCols = bc;
+    B->fdtdDepth = bd;
+    if (int r = BuildLists(B)) return r;
+    B->dirty = false;
+    if (best_ms) *best_ms = best;
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_stats(ptk_batch *B, unsigned long long out[7]) {
+    if (!B || !out) return Fail(PTK_BATCH_EINVAL, "bad argument");
+    out[6] = (unsigned long long)(B->execMs * 1000.0);
+    out[0] = B->nTicks;
+    out[1] = B->nLaunches;
+    out[2] = B->nGraphs;
+    out[3] = B->arena.used();
+    out[4] = B->arena.peak();
+    out[5] = B->arena.size();
+    return PTK_BATCH_OK;
+}
+
+}  // extern "C"

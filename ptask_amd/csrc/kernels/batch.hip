@@ -1,0 +1,276 @@
+// batch.hip -- persistent multi-instance task kernels of the batched / served executor
+// (code object batch.hsaco; the executor is ptask_amd/csrc/batch/executor.cpp).
+//
+// Every kernel takes a work list in device memory, the list's length in device memory and the
+// array of instance descriptors (include/ptk_batch.h), and strides over the list with whatever
+// grid it was launched with.  Because lengths live on the device, one captured hipGraph of these
+// launches stays valid while instances are admitted and retired: the host only rewrites the
+// lists between iterations.  The per-cell / per-row / per-tile arithmetic is the single-graph
+// kernels' own (fdtd_halves.h, groupby_body.h, gmm_body.h), so an instance computes bitwise what
+// its graph computes alone.
+//
+//   BatchFdtdH / BatchFdtdE   tiles {inst, col0, ncols, k0 | k1 << 16}: H then E half-step of
+//                             the columns [col0, col0+ncols) and planes [k0, k1) of an instance;
+//                             set (done & 1) is current, the other set receives the new fields
+//   BatchFdtdAdvance          active FDTD instances: done += 1
+//   BatchGroupbyClear/Accumulate/Reshuffle   units {inst, chunk}: table slice / row chunk
+//   BatchGroupbyOffsets       instances: one block each
+//   BatchGmmAccumulate        units {inst, partial row b}
+//   BatchGmmFinalize          units {inst, component k (K = log-likelihood block)}
+//   BatchInit*                device-side input generation at admission
+//   BatchChecksum*            64-bit checksums of results (sums of mixed words: the order in
+//                             which they are added does not matter)
+#include <hip/hip_runtime.h>
+
+#include "fdtd_common.h"
+#include "fdtd_halves.h"
+#include "gmm_body.h"
+#include "groupby_body.h"
+#include "ptk_batch.h"
+#include "ptk_prng.h"
+
+using namespace ptk_fdtd;
+using namespace ptk_groupby;
+
+// ---- FDTD -------------------------------------------------------------------------------------
+
+extern "C" __global__ void __launch_bounds__(256)
+BatchFdtdH(const int4 *__restrict__ tiles, const int *__restrict__ count, const PTK_FDTD_INST *__restrict__ insts) {
+    const int n = *count;
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        const PTK_FDTD_INST &I = insts[tl.x];
+        const int cur = I.done & 1, nxt = cur ^ 1;
+        const HalfH F{I.f[cur][0], I.f[cur][1], I.f[cur][2], I.f[cur][3], I.f[cur][4], I.f[cur][5],
+                      I.f[nxt][0], I.f[nxt][1], I.f[nxt][2]};
+        const Ix X{I.p.Nx, I.p.Ny, I.p.Nz};
+        const Coef c = make_coef(I.p);
+        const int k0 = tl.w & 0xffff, k1 = (int)((unsigned)tl.w >> 16);
+        for (int q = tl.y + (int)threadIdx.x; q < tl.y + tl.z; q += blockDim.x) h_column(F, X, c, (unsigned)q, k0, k1);
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+BatchFdtdE(const int4 *__restrict__ tiles, const int *__restrict__ count, const PTK_FDTD_INST *__restrict__ insts) {
+    const int n = *count;
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        const PTK_FDTD_INST &I = insts[tl.x];
+        const int cur = I.done & 1, nxt = cur ^ 1;
+        const HalfE F{I.f[cur][3], I.f[cur][4], I.f[cur][5], I.f[nxt][0], I.f[nxt][1], I.f[nxt][2],
+                      I.f[nxt][3], I.f[nxt][4], I.f[nxt][5]};
+        const Ix X{I.p.Nx, I.p.Ny, I.p.Nz};
+        const Coef c = make_coef(I.p);
+        const int k0 = tl.w & 0xffff, k1 = (int)((unsigned)tl.w >> 16);
+        for (int q = tl.y + (int)threadIdx.x; q < tl.y + tl.z; q += blockDim.x) e_column(F, X, c, (unsigned)q, k0, k1);
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+BatchFdtdAdvance(const int *__restrict__ list, const int *__restrict__ count, PTK_FDTD_INST *insts) {
+    const int n = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) insts[list[i]].done += 1;
+}
+
+// The six fields of set A of one instance, U[-1,1) as FdtdInitField draws them (stream = field).
+extern "C" __global__ void __launch_bounds__(256)
+BatchInitFdtd(const PTK_FDTD_INST *__restrict__ insts, int inst, unsigned long long seed) {
+    const PTK_FDTD_INST &I = insts[inst];
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (int f = 0; f < 6; ++f) {
+        int d1, d2, d3;
+        ptk_fdtd_extent(f, I.p.Nx, I.p.Ny, I.p.Nz, &d1, &d2, &d3);
+        const unsigned long long n = (unsigned long long)d1 * d2 * d3;
+        float *a = I.f[0][f];
+        for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+            a[i] = ptk_rand_sym_f32(seed, (uint64_t)f, i);
+    }
+}
+
+// ---- groupby ----------------------------------------------------------------------------------
+
+__device__ inline long long gb_chunks(const PTK_GROUPBY_INST &I) {
+    return (I.n + I.chunkRows - 1) / I.chunkRows;
+}
+
+extern "C" __global__ void __launch_bounds__(1024)
+BatchGroupbyClear(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    const int n = *count;
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const int2 w = units[u];
+        const PTK_GROUPBY_INST &I = insts[w.x];
+        const long long total = 3ll * (I.numKeys + 1) + 1, chunks = gb_chunks(I);
+        const long long per = (total + chunks - 1) / chunks;
+        const long long lo = per * w.y, hi = lo + per < total ? lo + per : total;
+        for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) I.table[i] = 0;
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(1024)
+BatchGroupbyAccumulate(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    const int n = *count;
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const int2 w = units[u];
+        const PTK_GROUPBY_INST &I = insts[w.x];
+        const long long r0 = (long long)w.y * I.chunkRows, r1 = r0 + I.chunkRows < I.n ? r0 + I.chunkRows : I.n;
+        gb_accumulate_rows(I.keys, I.vals, r0, r1, I.numKeys, I.table);
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(1024)
+BatchGroupbyOffsets(const int *__restrict__ list, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    const int n = *count;
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const PTK_GROUPBY_INST &I = insts[list[u]];
+        gb_offsets_block(I.table, I.numKeys, I.offsets, I.groups);
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(1024)
+BatchGroupbyReshuffle(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    const int n = *count;
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const int2 w = units[u];
+        const PTK_GROUPBY_INST &I = insts[w.x];
+        const long long r0 = (long long)w.y * I.chunkRows, r1 = r0 + I.chunkRows < I.n ? r0 + I.chunkRows : I.n;
+        gb_reshuffle_rows(I.keys, I.vals, I.n, r0, r1, I.numKeys, I.offsets, I.table, I.outKeys, I.outVals);
+    }
+}
+
+// Keys Zipf(s) by inverse CDF over nk = numKeys + 1 ranks, rank r -> key r*A mod nk (A coprime
+// to nk), or uniform on [0, numKeys) when cdf is null; values are 32 random bits.  The same
+// draws as the groupby graph application's host generator (streams 1 and 2 of the seed).
+extern "C" __global__ void __launch_bounds__(256)
+BatchInitGroupby(int *__restrict__ keys, int *__restrict__ vals, long long n, const double *__restrict__ cdf,
+                 int numKeys, long long A, unsigned long long seed) {
+    const long long nk = (long long)numKeys + 1;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double u = ptk_rand_unit_f64(seed, 1, (uint64_t)i);
+        long long k;
+        if (cdf) {
+            long long lo = 0, hi = nk;  // first rank whose cdf exceeds u (upper bound)
+            while (lo < hi) {
+                const long long mid = (lo + hi) >> 1;
+                if (cdf[mid] <= u) lo = mid + 1;
+                else hi = mid;
+            }
+            const long long r = lo >= nk ? nk - 1 : lo;
+            k = (r * A) % nk;
+        } else {
+            k = (long long)(u * (double)numKeys);
+        }
+        keys[i] = (int)k;
+        vals[i] = (int)(unsigned)ptk_rand_u64(seed, 2, (uint64_t)i);
+    }
+}
+
+// ---- GMM --------------------------------------------------------------------------------------
+
+extern "C" __global__ void __launch_bounds__(256)
+BatchGmmAccumulate(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GMM_INST *__restrict__ insts) {
+    __shared__ __align__(16) GmmShared sh;
+    const int n = *count;
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const int2 w = units[u];
+        const PTK_GMM_INST &I = insts[w.x];
+        gmm_accumulate_unit(I.X, I.d, I.model, I.partials, w.y, sh);
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+BatchGmmFinalize(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GMM_INST *__restrict__ insts) {
+    const int n = *count;
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const int2 w = units[u];
+        const PTK_GMM_INST &I = insts[w.x];
+        gmm_finalize_block(I.model, I.partials, I.d, w.y);
+    }
+}
+
+// Points of a seeded mixture: component c of point p uniform on [0, K) (stream 12); means
+// U[-10,10] (stream 10) and standard deviations U[0.5,2] (stream 11) per (c, m); coordinates
+// mean + sd * z, z by Box-Muller from streams 13 and 14.  first[c] receives the smallest point
+// index of component c (first must hold INT_MAX on entry).
+extern "C" __global__ void __launch_bounds__(256)
+BatchInitGmm(float *__restrict__ X, int N, int D, int K, unsigned long long seed, int *first) {
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (long long)gridDim.x * blockDim.x) {
+        int c = (int)(ptk_rand_unit_f64(seed, 12, (uint64_t)p) * K);
+        if (c >= K) c = K - 1;
+        atomicMin(&first[c], (int)p);
+        for (int m = 0; m < D; ++m) {
+            const uint64_t cm = (uint64_t)c * D + m, i = (uint64_t)p * D + m;
+            const double mean = -10.0 + 20.0 * ptk_rand_unit_f64(seed, 10, cm);
+            const double sd = 0.5 + 1.5 * ptk_rand_unit_f64(seed, 11, cm);
+            const double u1 = 1.0 - ptk_rand_unit_f64(seed, 13, i), u2 = ptk_rand_unit_f64(seed, 14, i);
+            X[i] = (float)(mean + sd * (sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2)));
+        }
+    }
+}
+
+// The initial model of Gaumixmod's constructor (gaumixmod.h:14-27): frac 1/K, var 1e-10, means
+// the first point of each generating component (point c*N/K if a component drew none), and the
+// binary32 E-step table.  One block of 256.
+extern "C" __global__ void __launch_bounds__(256)
+BatchInitGmmModel(double *__restrict__ model, GMM_DIMS d, const float *__restrict__ X, const int *__restrict__ first) {
+    const int D = d.D, K = d.K, Dp = ptk_gmm_dpad(&d), tid = threadIdx.x;
+    const long long elems = ptk_gmm_model_elems(&d);
+    for (long long i = tid; i < elems; i += blockDim.x) model[i] = 0.0;
+    __syncthreads();
+    if (tid == 0) {
+        model[1] = d.N;
+        model[2] = D;
+        model[3] = K;
+    }
+    float *tab = (float *)(model + ptk_gmm_table(&d));
+    for (int i = tid; i < K * D; i += blockDim.x) {
+        const int c = i / D, m = i - c * D;
+        const long long p = first[c] < d.N ? (long long)first[c] : (long long)c * d.N / K;
+        const double mu = (double)X[p * D + m];
+        model[ptk_gmm_mu(&d) + i] = mu;
+        model[ptk_gmm_var(&d) + i] = 1.0e-10;
+        tab[2 * c * Dp + m] = (float)mu;
+        tab[2 * c * Dp + Dp + m] = (float)(1.0 / 1.0e-10);
+    }
+    for (int c = tid; c < K; c += blockDim.x) {
+        model[ptk_gmm_frac(&d) + c] = 1.0 / K;
+        tab[2 * K * Dp + c] = (float)(-0.5 * (D * log(1.0e-10)) + log(1.0 / K));
+    }
+}
+
+// ---- checksums --------------------------------------------------------------------------------
+
+__device__ inline void csum_add(unsigned long long *out, unsigned long long h) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) h += __shfl_xor(h, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, h);
+}
+
+// sum over i of mix(bits(a[i]), i): changes with any bit and any position of the array.
+extern "C" __global__ void __launch_bounds__(256)
+BatchChecksumWords(const unsigned *__restrict__ a, long long n, unsigned long long *out) {
+    unsigned long long h = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        h += ptk_splitmix64(((unsigned long long)a[i] << 32) ^ (unsigned long long)i * 0x9E3779B97F4A7C15ull);
+    csum_add(out, h);
+}
+
+// The groups block of a groupby result: its header and the G = groups[0] triples it holds (the
+// rest of the block is never written and is not part of the result).
+extern "C" __global__ void __launch_bounds__(256)
+BatchChecksumGroups(const long long *__restrict__ groups, unsigned long long *out) {
+    const long long n = 2 * (3 + 3 * groups[0]);
+    const unsigned *a = (const unsigned *)groups;
+    unsigned long long h = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        h += ptk_splitmix64(((unsigned long long)a[i] << 32) ^ (unsigned long long)i * 0x9E3779B97F4A7C15ull);
+    csum_add(out, h);
+}
+
+// sum over rows of mix(key, value): the multiset of (key, value) pairs, independent of order.
+extern "C" __global__ void __launch_bounds__(256)
+BatchChecksumPairs(const int *__restrict__ k, const int *__restrict__ v, long long n, unsigned long long *out) {
+    unsigned long long h = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        h += ptk_splitmix64(((unsigned long long)(unsigned)k[i] << 32) | (unsigned)v[i]);
+    csum_add(out, h);
+}

@@ -20,231 +20,40 @@
 // Sums and counts use exact int64 (wrapping) additions, whose result does not depend on the
 // order of the atomics: every output except the order of values inside one group is
 // bit-reproducible.
-#include <hip/hip_runtime.h>
+#include "groupby_body.h"
 
-#define GB_LDS_SLOTS 4096   // per-block pre-aggregation table (power of two)
-#define GB_PROBES 8         // linear probes before a row falls back to global atomics
-#define GB_EMPTY 0xFFFFFFFFu
-
-__device__ inline unsigned gb_hash(unsigned k) {
-    k ^= k >> 16;
-    k *= 0x7feb352du;
-    k ^= k >> 15;
-    return k;
-}
-
-__device__ inline unsigned long long *gb_count(long long *t, int numKeys) { return (unsigned long long *)t; }
-__device__ inline unsigned long long *gb_sum(long long *t, int numKeys) { return (unsigned long long *)t + (numKeys + 1); }
-__device__ inline unsigned long long *gb_cursor(long long *t, int numKeys) { return (unsigned long long *)t + 2ll * (numKeys + 1); }
-__device__ inline unsigned long long *gb_errors(long long *t, int numKeys) { return (unsigned long long *)t + 3ll * (numKeys + 1); }
+using namespace ptk_groupby;
 
 extern "C" __global__ void __launch_bounds__(1024) GroupbyClear(long long *table, int numKeys) {
     if (numKeys < 0) return;
-    const long long n = 3ll * (numKeys + 1) + 1;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        table[i] = 0;
+    gb_clear(table, numKeys, (long long)blockIdx.x * blockDim.x + threadIdx.x, (long long)gridDim.x * blockDim.x);
 }
 
-// Rows are read once (8 B each).  Each block keeps a small open-addressed table in LDS so the
-// frequent keys of a skewed (Zipf) stream are aggregated with LDS atomics; a key that finds no
-// slot within GB_PROBES probes goes straight to the global table.  The block then flushes its
-// LDS table with one global atomic per slot and field.
+// Each block aggregates a contiguous chunk of the rows (see gb_accumulate_rows).
 extern "C" __global__ void __launch_bounds__(1024)
 GroupbyAccumulate(const int *__restrict__ keys, const int *__restrict__ vals, int n, int numKeys,
                   long long *table) {
-    __shared__ unsigned sKey[GB_LDS_SLOTS];
-    __shared__ unsigned sCnt[GB_LDS_SLOTS];
-    __shared__ unsigned long long sSum[GB_LDS_SLOTS];
     if (numKeys < 0 || n <= 0) return;
-    for (int s = threadIdx.x; s < GB_LDS_SLOTS; s += blockDim.x) {
-        sKey[s] = GB_EMPTY;
-        sCnt[s] = 0;
-        sSum[s] = 0;
-    }
-    __syncthreads();
-    unsigned long long *gCnt = gb_count(table, numKeys), *gSum = gb_sum(table, numKeys);
-    // this block's contiguous chunk of rows
     const long long per = ((long long)n + gridDim.x - 1) / gridDim.x;
     const long long r0 = per * blockIdx.x, r1 = r0 + per < n ? r0 + per : n;
-    unsigned long long localErr = 0;
-    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        const int k = keys[r];
-        const unsigned long long v = (unsigned long long)(long long)vals[r];
-        if (k < 0 || k > numKeys) {
-            ++localErr;
-            continue;
-        }
-        const unsigned uk = (unsigned)k;
-        unsigned h = gb_hash(uk) & (GB_LDS_SLOTS - 1);
-        bool done = false;
-        for (int p = 0; p < GB_PROBES; ++p) {
-            unsigned cur = sKey[h];
-            if (cur == GB_EMPTY) cur = atomicCAS(&sKey[h], GB_EMPTY, uk);
-            if (cur == GB_EMPTY || cur == uk) {
-                atomicAdd(&sCnt[h], 1u);
-                atomicAdd(&sSum[h], v);
-                done = true;
-                break;
-            }
-            h = (h + 1) & (GB_LDS_SLOTS - 1);
-        }
-        if (!done) {
-            atomicAdd(&gCnt[uk], 1ull);
-            atomicAdd(&gSum[uk], v);
-        }
-    }
-    __syncthreads();
-    for (int s = threadIdx.x; s < GB_LDS_SLOTS; s += blockDim.x) {
-        const unsigned k = sKey[s];
-        if (k == GB_EMPTY) continue;
-        atomicAdd(&gCnt[k], (unsigned long long)sCnt[s]);
-        atomicAdd(&gSum[k], sSum[s]);
-    }
-    if (localErr) atomicAdd(gb_errors(table, numKeys), localErr);
+    gb_accumulate_rows(keys, vals, r0, r1, numKeys, table);
 }
 
 // ---- one-block scans -------------------------------------------------------------------------
-
-__device__ inline unsigned long long wave_incl(unsigned long long v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        unsigned long long u = __shfl_up(v, d, 64);
-        if (lane >= d) v += u;
-    }
-    return v;
-}
-
-// Exclusive block scan of one value per thread; blockDim.x a multiple of 64.
-__device__ inline unsigned long long block_excl(unsigned long long v, unsigned long long *total) {
-    __shared__ unsigned long long ws[16];
-    __shared__ unsigned long long tot;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    unsigned long long inc = wave_incl(v);
-    if (lane == 63) ws[w] = inc;
-    __syncthreads();
-    if (w == 0) {
-        unsigned long long s = lane < nw ? ws[lane] : 0ull;
-        unsigned long long si = wave_incl(s);
-        if (lane < nw) ws[lane] = si - s;
-        if (lane == nw - 1) tot = si;
-    }
-    __syncthreads();
-    unsigned long long r = ws[w] + inc - v;
-    *total = tot;
-    __syncthreads();
-    return r;
-}
-
-// One block: offsets = exclusive scan of counts; groups = the non-empty keys in ascending order
-// (their ranks = exclusive scan of count>0), with the header {G, rows, errors}.
+// One block: offsets and the group list (see gb_offsets_block).
 extern "C" __global__ void __launch_bounds__(1024)
 GroupbyOffsets(long long *table, int numKeys, long long *offsets, long long *groups) {
     if ((blockDim.x & 63) != 0 || blockIdx.x != 0 || numKeys < 0) return;
-    const unsigned long long *cnt = gb_count(table, numKeys), *sum = gb_sum(table, numKeys);
-    const long long K = (long long)numKeys + 1;
-    // each thread owns a contiguous run of keys: local totals, one block scan, then a serial
-    // pass over the run
-    const long long per = (K + blockDim.x - 1) / blockDim.x;
-    const long long k0 = per * threadIdx.x, k1 = k0 + per < K ? k0 + per : K;
-    unsigned long long rows = 0, nz = 0;
-    for (long long k = k0; k < k1; ++k) {
-        const unsigned long long c = cnt[k];
-        rows += c;
-        nz += c ? 1ull : 0ull;
-    }
-    unsigned long long totRows, totGroups;
-    unsigned long long r = block_excl(rows, &totRows);
-    unsigned long long gi = block_excl(nz, &totGroups);
-    for (long long k = k0; k < k1; ++k) {
-        const unsigned long long c = cnt[k];
-        offsets[k] = (long long)r;
-        if (c) {
-            long long *g = groups + 3 + 3 * (long long)gi;
-            g[0] = k;
-            g[1] = (long long)c;
-            g[2] = (long long)sum[k];
-            ++gi;
-        }
-        r += c;
-    }
-    if (threadIdx.x == 0) {
-        groups[0] = (long long)totGroups;
-        groups[1] = (long long)totRows;
-        groups[2] = (long long)*gb_errors(table, numKeys);
-    }
+    gb_offsets_block(table, numKeys, offsets, groups);
 }
 
-// Scatter every row to offsets[key] + (rows of that key placed before it), so equal keys are
-// contiguous and the keys output is exact; the order of values within a group is not fixed.
-//
-// A block owns a contiguous chunk of rows.  Pass 1 counts the chunk's rows per key in an LDS
-// table (the same insertion rule as GroupbyAccumulate: a key either gets a slot on its first
-// row, and then every later row finds it, or never does).  Each slot then reserves its rows'
-// range with ONE global atomic on the key's cursor; pass 2 re-reads the chunk (from cache) and
-// places rows with an LDS cursor.  Keys without a slot fall back to one global atomic per row --
-// they are the rare keys, so that costs little even under heavy skew.
+// Each block reshuffles a contiguous chunk of the rows (see gb_reshuffle_rows).
 extern "C" __global__ void __launch_bounds__(1024)
 GroupbyReshuffle(const int *__restrict__ keys, const int *__restrict__ vals, int n, int numKeys,
                  const long long *__restrict__ offsets, long long *table, int *__restrict__ outKeys,
                  int *__restrict__ outVals) {
-    __shared__ unsigned sKey[GB_LDS_SLOTS];
-    __shared__ unsigned sCnt[GB_LDS_SLOTS];
-    __shared__ unsigned long long sBase[GB_LDS_SLOTS];
     if (numKeys < 0 || n <= 0) return;
-    unsigned long long *cursor = gb_cursor(table, numKeys);
-    for (int s = threadIdx.x; s < GB_LDS_SLOTS; s += blockDim.x) {
-        sKey[s] = GB_EMPTY;
-        sCnt[s] = 0;
-    }
-    __syncthreads();
     const long long per = ((long long)n + gridDim.x - 1) / gridDim.x;
     const long long r0 = per * blockIdx.x, r1 = r0 + per < n ? r0 + per : n;
-    // pass 1: per-key row counts of this chunk
-    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        const int k = keys[r];
-        if (k < 0 || k > numKeys) continue;
-        const unsigned uk = (unsigned)k;
-        unsigned h = gb_hash(uk) & (GB_LDS_SLOTS - 1);
-        for (int p = 0; p < GB_PROBES; ++p) {
-            unsigned cur = sKey[h];
-            if (cur == GB_EMPTY) cur = atomicCAS(&sKey[h], GB_EMPTY, uk);
-            if (cur == GB_EMPTY || cur == uk) {
-                atomicAdd(&sCnt[h], 1u);
-                break;
-            }
-            h = (h + 1) & (GB_LDS_SLOTS - 1);
-        }
-    }
-    __syncthreads();
-    // one reservation per key present in the chunk
-    for (int s = threadIdx.x; s < GB_LDS_SLOTS; s += blockDim.x) {
-        const unsigned k = sKey[s];
-        if (k == GB_EMPTY) continue;
-        const unsigned long long first = atomicAdd(&cursor[k], (unsigned long long)sCnt[s]);
-        sBase[s] = (unsigned long long)offsets[k] + first;
-        sCnt[s] = 0;
-    }
-    __syncthreads();
-    // pass 2: place the rows
-    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        const int k = keys[r];
-        if (k < 0 || k > numKeys) continue;
-        const unsigned uk = (unsigned)k;
-        unsigned h = gb_hash(uk) & (GB_LDS_SLOTS - 1);
-        long long pos = -1;
-        for (int p = 0; p < GB_PROBES; ++p) {
-            const unsigned cur = sKey[h];
-            if (cur == uk) {
-                pos = (long long)(sBase[h] + atomicAdd(&sCnt[h], 1u));
-                break;
-            }
-            if (cur == GB_EMPTY) break;
-            h = (h + 1) & (GB_LDS_SLOTS - 1);
-        }
-        if (pos < 0) pos = offsets[k] + (long long)atomicAdd(&cursor[k], 1ull);
-        if (pos < 0 || pos >= n) continue;  // only possible if the table is inconsistent
-        outKeys[pos] = k;
-        outVals[pos] = vals[r];
-    }
+    gb_reshuffle_rows(keys, vals, n, r0, r1, numKeys, offsets, table, outKeys, outVals);
 }
