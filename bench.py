@@ -15,6 +15,15 @@ Graph forms measured (all bit-identical to the six reference updates):
   six    -- the reference's six tasks/step (96 B per cell-step)
   fused  -- 1 task/step, whole Yee step (48 B per cell-step)
 
+The same line also carries, under "workloads", the other graphs of the hot path:
+  batched -- 2048 FDTD graphs (edge U{16..96}, 50 steps) + 512 groupby graphs (1M-8M rows, Zipf 1.1
+             over 65,536 keys) through the batched executor, plain per-task dispatch vs captured
+             batched iterations (instances/s, both on the same instances, result checksums compared)
+  served  -- Poisson arrivals of FDTD / GMM / groupby graph instances at a few rates: p50/p99
+             latency and graphs/s, continuous batching vs plain dispatch, alone-vs-mix identity
+  gmm     -- EM-GMM N=2M, D=32, K=64, 25 EM steps (points x steps per second)
+  pca     -- Gram-Schmidt PCA 65536 x 2048, 16 components, 10 power iterations
+
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build measures one GPU.
 """
 from __future__ import annotations
@@ -43,6 +52,10 @@ def parse():
     ap.add_argument("--forms", default="halves,six,fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--suite", default="batched,served,gmm,pca",
+                    help="extra workloads after the FDTD headline (comma list; empty for none)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-qps", default="20,100,300")
     return ap.parse_args()
 
 
@@ -102,6 +115,100 @@ def kernel_roofline(r: dict, n: int) -> dict:
             "launches_timed": int(dom["dispatches"])}
 
 
+def run_app(name: str, args: list, timeout: int = 600) -> dict:
+    from ptask_amd import paths
+    cmd = [paths.bin_path(name), *map(str, args)]
+    progress("run: " + " ".join(cmd))
+    t0 = time.time()
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd=REPO)
+    if p.returncode != 0:
+        raise RuntimeError(f"{name} failed ({p.returncode}): {p.stderr[-2000:]}{p.stdout[-2000:]}")
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    progress(f"{name}: done in {time.time() - t0:.1f} s")
+    return r
+
+
+def measured_traffic(kernel: str, n: int, form: str):
+    """HBM bytes per launch of `kernel` from the committed PMC summary (two rocprofv3 --pmc runs,
+    FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size), when it covers this workload."""
+    path = os.path.join(REPO, "profiles", "fdtd_384_halves_hbm_traffic.md")
+    if n != 384 or form != "halves" or not os.path.exists(path):
+        return None
+    text = open(path).read()
+    block = text[text.index("```json") + 7:]
+    data = json.loads(block[:block.index("```")])
+    return data.get(kernel, {}).get("traffic_bytes")
+
+
+def leg_batched() -> dict:
+    from ptask_amd import paths
+    r = run_app("ptk_batch", ["--kernels", paths.KERNEL_DIR])
+    pl, ba = r["modes"]["plain"], r["modes"]["batched"]
+    return {"workload": "2048 FDTD graphs (cubic edge U{16..96}, 50 steps) + 512 groupby graphs "
+                        "(rows U[1M,8M], Zipf 1.1 over 65,536 keys), one executor, all resident",
+            "unit": "graph instances/s",
+            "plain_exec": round(pl["exec_instances_per_s"], 1),
+            "batched_exec": round(ba["exec_instances_per_s"], 1),
+            "speedup_exec": round(ba["exec_instances_per_s"] / pl["exec_instances_per_s"], 2),
+            "plain_end_to_end": round(pl["instances_per_s"], 1),
+            "batched_end_to_end": round(ba["instances_per_s"], 1),
+            "speedup_end_to_end": round(ba["instances_per_s"] / pl["instances_per_s"], 2),
+            "plain_exec_ms": pl["exec_ms"], "batched_exec_ms": ba["exec_ms"],
+            "plain_launches": pl["launches"], "batched_graph_launches": ba["graph_launches"],
+            "tiles": ba["tiles"], "checksums_equal": r.get("checksums_equal"),
+            "footprint_bytes": r["footprint_bytes"]}
+
+
+def leg_served(seconds: float, rates: list) -> dict:
+    from ptask_amd import paths
+    runs = []
+    for q in rates:
+        r = run_app("ptk_serve", ["--kernels", paths.KERNEL_DIR, "--qps", q, "--duration", seconds,
+                                  "--identity", 4])
+        runs.append({k: r[k] for k in ("mode", "qps", "requests", "graphs_per_s", "p50_ms", "p99_ms",
+                                       "p50_ms_by_kind", "p99_ms_by_kind", "mean_active",
+                                       "identity_checked", "identity_equal")})
+    q = rates[len(rates) // 2]
+    r = run_app("ptk_serve", ["--kernels", paths.KERNEL_DIR, "--qps", q, "--duration", seconds,
+                              "--identity", 0, "--mode", "plain"])
+    runs.append({k: r[k] for k in ("mode", "qps", "requests", "graphs_per_s", "p50_ms", "p99_ms",
+                                   "p50_ms_by_kind", "p99_ms_by_kind", "mean_active")})
+    return {"workload": "Poisson arrivals; FDTD edge U{32..192} x 20..100 steps, GMM N U[10K,1M] D=16 "
+                        "K U{8..32} 20 EM steps, groupby rows U[1M,16M]; mix 0.5:0.25:0.25",
+            "latency_kinds": ["fdtd", "gmm", "groupby"], "runs": runs}
+
+
+def leg_gmm() -> dict:
+    from ptask_amd import paths
+    r = run_app("ptk_gmm", ["--kernels", paths.kernel_path("gmm.hsaco"), "--n", 2000000, "--d", 32,
+                            "--k", 64, "--steps", 25, "--repeat", 3, "--warmup", 1, "--profile",
+                            "--blocks", 512])
+    acc = next(t for t in r["tasks"] if t["task"] == "GmmAccumulate")
+    avg = acc["device_ms"] / acc["dispatches"]
+    flops = 8 * 2000000 * 32 * 64
+    return {"workload": "EM-GMM N=2,000,000 D=32 K=64 diagonal, 25 EM steps, looped graph",
+            "value": r["point_steps_per_s"], "unit": "point-steps/s", "ms_per_run": r["ms_per_run"],
+            "loglike": r["loglike"], "nonpositive_var": r["nonpositive_var"],
+            "roofline": {"bound": "valu", "kernel": "GmmAccumulate", "kernel_avg_ms": round(avg, 4),
+                         "achieved": round(flops / (avg / 1e3) / 1e12, 2), "peak": 157.3,
+                         "unit": "TFLOP/s", "frac": round(flops / (avg / 1e3) / 1e12 / 157.3, 4),
+                         "flops_per_launch": flops}}
+
+
+def leg_pca() -> dict:
+    from ptask_amd import paths
+    r = run_app("ptk_pca", ["--kernels", paths.kernel_path("pca.hsaco"), "--m", 65536, "--n", 2048,
+                            "--k", 16, "--j", 10, "--repeat", 3, "--warmup", 1, "--profile"])
+    avg = r["gram_ms"] / max(1, r["gram_dispatches"])
+    gbs = r["gram_bytes"] / (avg / 1e3) / 1e9
+    return {"workload": "Gram-Schmidt PCA 65536 x 2048 binary32, 16 components, 10 power iterations",
+            "value": r["runs_per_s"], "unit": "PCA runs/s", "ms_per_run": r["ms_per_run"],
+            "singular_values": r["L"][:4],
+            "roofline": {"bound": "hbm", "kernel": "PcaGram", "kernel_avg_ms": round(avg, 5),
+                         "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": r["gram_bytes"]}}
+
+
 def cpu_baseline(n: int, seconds: float) -> dict:
     """The oracle (oracle/fdtd.c, binary32, one core) timed on the same 384^3 grid for as many
     whole time steps as fit in about `seconds`."""
@@ -140,6 +247,8 @@ def main() -> int:
     results = {f: run_graph(f, n, T, args.steps, args.warmup) for f in forms}
     head = results.get("halves") or results[forms[0]]
     head_form = "halves" if "halves" in results else forms[0]
+    roof = kernel_roofline(head, n)
+    roof["traffic"] = measured_traffic(roof["kernel"], n, head_form)
     line = {
         "metric": METRIC,
         "value": round(head["mcups"], 1),
@@ -158,13 +267,22 @@ def main() -> int:
                                "device-resident datablocks",
                    "grid": [n, n, n], "time_steps": T, "courant": 0.5, "form": head_form,
                    "parallelism": "single GPU"},
-        "roofline": kernel_roofline(head, n),
+        "roofline": roof,
         "forms": {f: {"mcups": round(r["mcups"], 1), "ms_per_run": round(r["ms_per_run"], 3),
                       "tasks_per_step": len(r["tasks"]), "dispatches_per_run": r["dispatches_last_run"],
                       "roofline": kernel_roofline(r, n)} for f, r in results.items()},
         "checksums_equal_across_forms": len({tuple(c["fnv1a"] for c in r["checksums"])
                                              for r in results.values()}) == 1,
     }
+    legs = {"batched": leg_batched, "gmm": leg_gmm, "pca": leg_pca,
+            "served": lambda: leg_served(args.serve_seconds, args.serve_qps.split(","))}
+    line["workloads"] = {}
+    for name in [x for x in args.suite.split(",") if x]:
+        try:
+            line["workloads"][name] = legs[name]()
+        except Exception as e:  # a failed leg is reported, the headline still stands
+            progress(f"{name} failed: {e}")
+            line["workloads"][name] = {"error": str(e)[-500:]}
     if not args.no_cpu:
         progress("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)

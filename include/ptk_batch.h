@@ -59,6 +59,16 @@ typedef struct _ptk_gmm_inst {
     GMM_DIMS d;
 } PTK_GMM_INST;
 
+/* One checksum of a retiring instance's result array (BatchChecksumJobs): mode 0 the words of
+ * a[0..n), mode 1 the (key, value) pairs (a[i], b[i]), mode 2 a groupby groups block (its
+ * length read from its header); the sum is added to *out. */
+typedef struct _ptk_csum_job {
+    const void *a, *b;
+    long long n;
+    unsigned long long *out;
+    int mode, pad;
+} PTK_CSUM_JOB;
+
 typedef struct ptk_batch ptk_batch;
 
 /* kernel_dir holds batch.hsaco, fdtd.hsaco, groupby.hsaco, gmm.hsaco; arena_bytes of HBM are
@@ -82,6 +92,11 @@ unsigned long long ptk_batch_footprint(int kind, long long a, int b_, int c, int
  * < 0 = no limit), run one step of every active instance, retire completed ones.  *active and
  * *completed (either may be NULL) receive the counts after the iteration. */
 int ptk_batch_tick(ptk_batch *b, int max_admit, int *active, int *completed);
+/* As ptk_batch_tick, but runs up to max_iters iterations back to back without waiting between
+ * them: as many as every resident instance still needs before the first of them completes (one
+ * when a groupby instance is resident, since it completes in one).  *iters receives the number
+ * run.  Results are the same as with single iterations. */
+int ptk_batch_run(ptk_batch *b, int max_admit, int max_iters, int *iters, int *active, int *completed);
 /* Ticks until every submitted instance is complete. */
 int ptk_batch_drain(ptk_batch *b);
 
@@ -105,6 +120,12 @@ int ptk_batch_release(ptk_batch *b, int id);
 int ptk_batch_set_tiles(ptk_batch *b, int fdtd_cols, int fdtd_depth, int gb_rows);
 int ptk_batch_get_tiles(ptk_batch *b, int *fdtd_cols, int *fdtd_depth, int *gb_rows);
 int ptk_batch_autotune(ptk_batch *b, double *best_ms);
+
+/* Launch form of batched iterations: 1 (default) one captured hipGraph per mix of instance kinds,
+ * replayed every iteration; 0 the same kernels launched directly with grids cut to the current
+ * work-list lengths.  Results are identical; the cheaper form depends on how much work an
+ * iteration holds (a graph replay costs more than a few direct launches on a nearly idle device). */
+int ptk_batch_set_launch(ptk_batch *b, int graphs);
 
 /* Counters: [0] iterations run, [1] kernel launches issued, [2] graph launches, [3] arena bytes in
  * use, [4] peak, [5] arena size, [6] microseconds spent executing iterations (from the first

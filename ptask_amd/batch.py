@@ -35,12 +35,14 @@ def lib() -> C.CDLL:
             "ptk_batch_submit_gmm": [P, I, I, I, I, U64, I, C.POINTER(I)],
             "ptk_batch_footprint": [I, LL, I, I, I],
             "ptk_batch_tick": [P, I, C.POINTER(I), C.POINTER(I)],
+            "ptk_batch_run": [P, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)],
             "ptk_batch_drain": [P],
             "ptk_batch_status": [P, I, C.POINTER(I), C.POINTER(D), C.POINTER(D), C.POINTER(D)],
             "ptk_batch_checksums": [P, I, C.POINTER(U64)],
             "ptk_batch_read": [P, I, I, P, C.POINTER(U64)],
             "ptk_batch_release": [P, I],
             "ptk_batch_set_tiles": [P, I, I, I],
+            "ptk_batch_set_launch": [P, I],
             "ptk_batch_get_tiles": [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)],
             "ptk_batch_autotune": [P, C.POINTER(D)],
             "ptk_batch_stats": [P, C.POINTER(U64)],
@@ -108,6 +110,11 @@ class Executor:
         _check(lib().ptk_batch_tick(self.h, max_admit, C.byref(a), C.byref(c)), "tick")
         return a.value, c.value
 
+    def run(self, max_iters: int, max_admit: int = -1) -> tuple[int, int, int]:
+        n, a, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().ptk_batch_run(self.h, max_admit, max_iters, C.byref(n), C.byref(a), C.byref(c)), "run")
+        return n.value, a.value, c.value
+
     def drain(self) -> None:
         _check(lib().ptk_batch_drain(self.h), "drain")
 
@@ -133,6 +140,9 @@ class Executor:
 
     def set_tiles(self, cols, depth, gb_rows) -> None:
         _check(lib().ptk_batch_set_tiles(self.h, cols, depth, gb_rows), "set_tiles")
+
+    def set_launch(self, graphs: bool) -> None:
+        _check(lib().ptk_batch_set_launch(self.h, int(graphs)), "set_launch")
 
     def tiles(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

@@ -32,7 +32,7 @@ struct Options {
     int gmm = 0, gmmN = 100000, gmmD = 16, gmmK = 16, gmmSteps = 10;
     double zipf = 1.1, arenaGb = 0;
     unsigned long long seed = 0x5eed;
-    bool autotune = true;
+    bool autotune = true, direct = false;
     int cols = 0, depth = 0, gbRows = 0;
 };
 
@@ -76,6 +76,7 @@ int main(int argc, char **argv) {
         else if (a == "--arena-gb") o.arenaGb = atof(next());
         else if (a == "--seed") o.seed = strtoull(next(), nullptr, 0);
         else if (a == "--no-autotune") o.autotune = false;
+        else if (a == "--direct") o.direct = true;
         else if (a == "--tiles") {
             if (sscanf(next(), "%d,%d,%d", &o.cols, &o.depth, &o.gbRows) != 3) { fprintf(stderr, "--tiles C,D,R\n"); return 2; }
         } else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
@@ -126,6 +127,7 @@ int main(int argc, char **argv) {
             return 1;
         }
         if (o.cols > 0) ptk_batch_set_tiles(B, o.cols, o.depth, o.gbRows);
+        if (o.direct) ptk_batch_set_launch(B, 0);
         std::vector<int> ids;
         for (const Spec &s : specs) {
             int id = -1, r = 0;

@@ -149,7 +149,7 @@ struct ptk_batch {
     ptk_hip_function fH = nullptr, fE = nullptr, fAdv = nullptr, fGbClear = nullptr, fGbAcc = nullptr,
                      fGbOff = nullptr, fGbRes = nullptr, fGmmAcc = nullptr, fGmmFin = nullptr, fInitFdtd = nullptr,
                      fInitGb = nullptr, fInitGmm = nullptr, fInitGmmModel = nullptr, fSumWords = nullptr,
-                     fSumPairs = nullptr, fSumGroups = nullptr, fFill = nullptr;
+                     fSumPairs = nullptr, fSumGroups = nullptr, fSumJobs = nullptr, fFill = nullptr;
     ptk_hip_function fSix[6] = {}, fPGbClear = nullptr, fPGbAcc = nullptr, fPGbOff = nullptr, fPGbRes = nullptr,
                      fPGmmAcc = nullptr, fPGmmFin = nullptr;
     void *arenaBase = nullptr;
@@ -164,13 +164,16 @@ struct ptk_batch {
     List<int4> tiles;
     List<int> fdtdActive, gbInsts;
     List<int2> gbUnits, gmmAcc, gmmFin;
+    List<PTK_CSUM_JOB> csumJobs;
     double *dCdf = nullptr;
     int cdfKeys = -1;
     double cdfS = 0;
     long long cdfA = 0;
-    ptk_hip_graph graph = nullptr;
+    ptk_hip_graph graphs[8] = {};  // captured iteration per mix of kinds (bit 0 FDTD, 1 groupby, 2 GMM)
     bool dirty = true;
-    int fdtdCols = 512, fdtdDepth = 16, gbRows = 262144;
+    bool useGraphs = true;  // batched iterations as captured graphs (else direct, grid-fitted launches)
+    int fdtdCols = 512, fdtdDepth = 16, gbRows = 262144, fdtdGrid = 4096;
+    int curCols = 512, curDepth = 16;  // the shape the current work lists were built with
     std::vector<Inst> insts;
     std::vector<int> queue, active;
     unsigned long long nTicks = 0, nLaunches = 0, nGraphs = 0;
@@ -188,6 +191,14 @@ struct ptk_batch {
 
 namespace {
 
+void DropGraphs(ptk_batch *B) {
+    for (ptk_hip_graph &g : B->graphs)
+        if (g) {
+            ptk_hip_graph_destroy(g);
+            g = nullptr;
+        }
+}
+
 template <class T>
 int Reserve(ptk_batch *B, List<T> &l, size_t need) {
     if (need <= l.cap) return PTK_BATCH_OK;
@@ -202,10 +213,7 @@ int Reserve(ptk_batch *B, List<T> &l, size_t need) {
     HIPCHK(ptk_hip_malloc((void **)&l.dev, cap * sizeof(T)));
     HIPCHK(ptk_hip_host_malloc((void **)&l.host, cap * sizeof(T)));
     l.cap = cap;
-    if (B->graph) {  // the captured graph holds the old list address
-        ptk_hip_graph_destroy(B->graph);
-        B->graph = nullptr;
-    }
+    DropGraphs(B);  // the captured iterations hold the old list address
     return PTK_BATCH_OK;
 }
 
@@ -361,13 +369,33 @@ int Admit(ptk_batch *B, Inst &in) {
 }
 
 // Work lists of the resident instances for the batched kernels.
+// FDTD tiles of the resident instances with the tile shape (cols, depth).
+size_t FdtdTiles(const ptk_batch *B, int cols, int depth) {
+    size_t nt = 0;
+    for (int id : B->active) {
+        const Inst &in = B->insts[(size_t)id];
+        if (in.kind != PTK_INST_FDTD) continue;
+        const long long ncol = (in.a + 1) * (long long)(in.b + 1);
+        nt += (size_t)(((ncol + cols - 1) / cols) * ((in.c + 1 + depth - 1) / depth));
+    }
+    return nt;
+}
+
 int BuildLists(ptk_batch *B) {
-    size_t nt = 0, nf = 0, ngu = 0, ngi = 0, nga = 0, ngf = 0;
+    // The tuned tile shape, made smaller (depth first, then columns) while the resident FDTD
+    // work would give fewer than four tiles per CU: a few small instances must still spread over
+    // the whole device.  Tiling never changes results.
+    int cols = B->fdtdCols, depth = B->fdtdDepth;
+    while (FdtdTiles(B, cols, depth) < 1024 && (depth > 2 || cols > 64)) {
+        if (depth > 2) depth = (depth + 1) / 2;
+        else cols /= 2;
+    }
+    B->curCols = cols;
+    B->curDepth = depth;
+    size_t nt = FdtdTiles(B, cols, depth), nf = 0, ngu = 0, ngi = 0, nga = 0, ngf = 0;
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
         if (in.kind == PTK_INST_FDTD) {
-            const long long ncol = (in.a + 1) * (long long)(in.b + 1);
-            nt += (size_t)(((ncol + B->fdtdCols - 1) / B->fdtdCols) * ((in.c + 1 + B->fdtdDepth - 1) / B->fdtdDepth));
             ++nf;
         } else if (in.kind == PTK_INST_GROUPBY) {
             ngu += (size_t)((in.a + in.gb.chunkRows - 1) / in.gb.chunkRows);
@@ -387,11 +415,11 @@ int BuildLists(ptk_batch *B) {
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
         if (in.kind == PTK_INST_FDTD) {
-            const int ncol = (int)((in.a + 1) * (long long)(in.b + 1)), depth = in.c + 1;
-            for (int k0 = 0; k0 < depth; k0 += B->fdtdDepth) {
-                const int k1 = std::min(depth, k0 + B->fdtdDepth);
-                for (int c0 = 0; c0 < ncol; c0 += B->fdtdCols)
-                    B->tiles.host[B->tiles.n++] = int4{in.slot, c0, std::min(B->fdtdCols, ncol - c0), k0 | (k1 << 16)};
+            const int ncol = (int)((in.a + 1) * (long long)(in.b + 1)), kdepth = in.c + 1;
+            for (int k0 = 0; k0 < kdepth; k0 += depth) {
+                const int k1 = std::min(kdepth, k0 + depth);
+                for (int c0 = 0; c0 < ncol; c0 += cols)
+                    B->tiles.host[B->tiles.n++] = int4{in.slot, c0, std::min(cols, ncol - c0), k0 | (k1 << 16)};
             }
             B->fdtdActive.host[B->fdtdActive.n++] = in.slot;
         } else if (in.kind == PTK_INST_GROUPBY) {
@@ -424,25 +452,32 @@ int BuildLists(ptk_batch *B) {
 }
 
 // The batched iteration's launches (captured once into a graph).
-int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance) {
+// The batched iteration's launches.  Captured into a graph, the grids are the configured
+// persistent sizes (the work-list lengths are read on the device); launched directly (fit), each
+// grid is cut to the current list length, which the host knows.
+int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance, bool fit = false) {
+    auto grid = [&](int configured, int count) {
+        return (unsigned)(fit ? std::max(1, std::min(configured, count)) : configured);
+    };
+    const int *hc = B->hCounts;
     if (fdtd) {
         Args h, e;
         h.ptr(B->tiles.dev).ptr(B->dCounts + 0).ptr(B->dFdtd);
         e.ptr(B->tiles.dev).ptr(B->dCounts + 0).ptr(B->dFdtd);
-        if (int r = B->Launch(B->fH, 2048, 1, 1, 256, h)) return r;
-        if (int r = B->Launch(B->fE, 2048, 1, 1, 256, e)) return r;
+        if (int r = B->Launch(B->fH, grid(B->fdtdGrid, hc[0]), 1, 1, 256, h)) return r;
+        if (int r = B->Launch(B->fE, grid(B->fdtdGrid, hc[0]), 1, 1, 256, e)) return r;
     }
     if (advance) {
         Args a;
         a.ptr(B->fdtdActive.dev).ptr(B->dCounts + 1).ptr(B->dFdtd);
-        if (int r = B->Launch(B->fAdv, 16, 1, 1, 256, a)) return r;
+        if (int r = B->Launch(B->fAdv, grid(16, (hc[1] + 255) / 256), 1, 1, 256, a)) return r;
     }
     if (gmm) {
         Args a, f;
         a.ptr(B->gmmAcc.dev).ptr(B->dCounts + 4).ptr(B->dGmm);
         f.ptr(B->gmmFin.dev).ptr(B->dCounts + 5).ptr(B->dGmm);
-        if (int r = B->Launch(B->fGmmAcc, 768, 1, 1, 256, a)) return r;
-        if (int r = B->Launch(B->fGmmFin, 512, 1, 1, 256, f)) return r;
+        if (int r = B->Launch(B->fGmmAcc, grid(768, hc[4]), 1, 1, 256, a)) return r;
+        if (int r = B->Launch(B->fGmmFin, grid(512, hc[5]), 1, 1, 256, f)) return r;
     }
     if (gb) {
         Args c, a, o, s;
@@ -450,10 +485,10 @@ int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance) {
         a.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
         o.ptr(B->gbInsts.dev).ptr(B->dCounts + 3).ptr(B->dGb);
         s.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
-        if (int r = B->Launch(B->fGbClear, 512, 1, 1, 1024, c)) return r;
-        if (int r = B->Launch(B->fGbAcc, 512, 1, 1, 1024, a)) return r;
-        if (int r = B->Launch(B->fGbOff, 512, 1, 1, 1024, o)) return r;
-        if (int r = B->Launch(B->fGbRes, 512, 1, 1, 1024, s)) return r;
+        if (int r = B->Launch(B->fGbClear, grid(512, hc[2]), 1, 1, 1024, c)) return r;
+        if (int r = B->Launch(B->fGbAcc, grid(512, hc[2]), 1, 1, 1024, a)) return r;
+        if (int r = B->Launch(B->fGbOff, grid(512, hc[3]), 1, 1, 1024, o)) return r;
+        if (int r = B->Launch(B->fGbRes, grid(512, hc[2]), 1, 1, 1024, s)) return r;
     }
     return PTK_BATCH_OK;
 }
@@ -497,20 +532,6 @@ int IssuePlain(ptk_batch *B, const Inst &in) {
     return PTK_BATCH_OK;
 }
 
-int Checksum(ptk_batch *B, unsigned long long *dst, const void *p, long long words) {
-    Args a;
-    a.ptr(p).add<long long>(words).ptr(dst);
-    const unsigned grid = (unsigned)std::max(1LL, std::min(1024LL, (words + 255) / 256));
-    return B->Launch(B->fSumWords, grid, 1, 1, 256, a);
-}
-
-int ChecksumPairs(ptk_batch *B, unsigned long long *dst, const int *k, const int *v, long long n) {
-    Args a;
-    a.ptr(k).ptr(v).add<long long>(n).ptr(dst);
-    const unsigned grid = (unsigned)std::max(1LL, std::min(1024LL, (n + 255) / 256));
-    return B->Launch(B->fSumPairs, grid, 1, 1, 256, a);
-}
-
 // Device checksums of the completed instances, then their memory goes back to the arena.
 int Retire(ptk_batch *B, const std::vector<int> &done) {
     if (done.empty()) return PTK_BATCH_OK;
@@ -524,26 +545,32 @@ int Retire(ptk_batch *B, const std::vector<int> &done) {
     }
     std::vector<unsigned long long> zero(done.size() * 8, 0);
     HIPCHK(ptk_hip_memcpy_htod_async(B->dSums, zero.data(), zero.size() * 8, B->stream));
+    // one job per result array, all summed by one launch per 65535 jobs
+    std::vector<PTK_CSUM_JOB> jobs;
     for (size_t i = 0; i < done.size(); ++i) {
         const Inst &in = B->insts[(size_t)done[i]];
         unsigned long long *s = B->dSums + 8 * i;
-        int r = PTK_BATCH_OK;
         if (in.kind == PTK_INST_FDTD) {
             const int cur = in.stepsDone & 1;
-            for (int f = 0; f < 6 && !r; ++f)
-                r = Checksum(B, s + f, in.fd.f[cur][f], (long long)ptk_fdtd_field_elems(f, (int)in.a, in.b, in.c));
+            for (int f = 0; f < 6; ++f)
+                jobs.push_back({in.fd.f[cur][f], nullptr, (long long)ptk_fdtd_field_elems(f, (int)in.a, in.b, in.c), s + f, 0, 0});
         } else if (in.kind == PTK_INST_GROUPBY) {
-            const long long K = (long long)in.b + 1;
-            Args g;
-            g.ptr(in.gb.groups).ptr(s + 0);
-            r = B->Launch(B->fSumGroups, (unsigned)std::min<long long>(1024, (2 * (3 + 3 * K) + 255) / 256), 1, 1, 256, g);
-            if (!r) r = Checksum(B, s + 1, in.gb.outKeys, in.a);
-            if (!r) r = ChecksumPairs(B, s + 2, in.gb.outKeys, in.gb.outVals, in.a);
-            if (!r) r = ChecksumPairs(B, s + 3, in.gb.keys, in.gb.vals, in.a);
+            jobs.push_back({in.gb.groups, nullptr, 0, s + 0, 2, 0});
+            jobs.push_back({in.gb.outKeys, nullptr, in.a, s + 1, 0, 0});
+            jobs.push_back({in.gb.outKeys, in.gb.outVals, in.a, s + 2, 1, 0});
+            jobs.push_back({in.gb.keys, in.gb.vals, in.a, s + 3, 1, 0});
         } else {
-            r = Checksum(B, s + 0, in.gm.model, 2 * ptk_gmm_model_elems(&in.gm.d));
+            jobs.push_back({in.gm.model, nullptr, 2 * ptk_gmm_model_elems(&in.gm.d), s + 0, 0, 0});
         }
-        if (r) return r;
+    }
+    if (int r = Reserve(B, B->csumJobs, jobs.size())) return r;
+    memcpy(B->csumJobs.host, jobs.data(), jobs.size() * sizeof(PTK_CSUM_JOB));
+    HIPCHK(ptk_hip_memcpy_htod_async(B->csumJobs.dev, B->csumJobs.host, jobs.size() * sizeof(PTK_CSUM_JOB), B->stream));
+    for (size_t j0 = 0; j0 < jobs.size(); j0 += 65535) {
+        Args a;
+        a.ptr(B->csumJobs.dev + j0);
+        const unsigned nj = (unsigned)std::min<size_t>(65535, jobs.size() - j0);
+        if (int r = B->Launch(B->fSumJobs, 64, nj, 1, 256, a)) return r;
     }
     HIPCHK(ptk_hip_memcpy_dtoh_async(B->hSums, B->dSums, done.size() * 64, B->stream));
     HIPCHK(ptk_hip_stream_synchronize(B->stream));
@@ -603,7 +630,8 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
         {&B->fGmmAcc, "BatchGmmAccumulate"}, {&B->fGmmFin, "BatchGmmFinalize"},
         {&B->fInitFdtd, "BatchInitFdtd"}, {&B->fInitGb, "BatchInitGroupby"}, {&B->fInitGmm, "BatchInitGmm"},
         {&B->fInitGmmModel, "BatchInitGmmModel"}, {&B->fSumWords, "BatchChecksumWords"},
-        {&B->fSumPairs, "BatchChecksumPairs"}, {&B->fSumGroups, "BatchChecksumGroups"}};
+        {&B->fSumPairs, "BatchChecksumPairs"}, {&B->fSumGroups, "BatchChecksumGroups"},
+        {&B->fSumJobs, "BatchChecksumJobs"}};
     for (auto &f : bf)
         if ((r = Load(B->mBatch, f.second, f.first))) return fail(r);
     static const char *six[6] = {"HxComputation", "HyComputation", "HzComputation",
@@ -634,7 +662,7 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
 int ptk_batch_destroy(ptk_batch *B) {
     if (!B) return PTK_BATCH_OK;
     if (B->stream) ptk_hip_stream_synchronize(B->stream);
-    if (B->graph) ptk_hip_graph_destroy(B->graph);
+    DropGraphs(B);
     auto freeList = [](auto &l) {
         if (l.dev) ptk_hip_free(l.dev);
         if (l.host) ptk_hip_host_free(l.host);
@@ -645,6 +673,7 @@ int ptk_batch_destroy(ptk_batch *B) {
     freeList(B->gbUnits);
     freeList(B->gmmAcc);
     freeList(B->gmmFin);
+    freeList(B->csumJobs);
     if (B->dSums) ptk_hip_free(B->dSums);
     if (B->hSums) ptk_hip_host_free(B->hSums);
     if (B->dCdf) ptk_hip_free(B->dCdf);
@@ -710,8 +739,8 @@ int ptk_batch_submit_gmm(ptk_batch *B, int n, int d, int k, int steps, unsigned 
     return Submit(B, in, id);
 }
 
-int ptk_batch_tick(ptk_batch *B, int max_admit, int *nactive, int *ncompleted) {
-    if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
+int ptk_batch_run(ptk_batch *B, int max_admit, int max_iters, int *niters, int *nactive, int *ncompleted) {
+    if (!B || max_iters < 1) return Fail(PTK_BATCH_EINVAL, "bad argument");
     // admission: in submission order while the footprint fits
     int admitted = 0;
     size_t qi = 0;
@@ -728,33 +757,52 @@ int ptk_batch_tick(ptk_batch *B, int max_admit, int *nactive, int *ncompleted) {
     }
     B->queue.erase(B->queue.begin(), B->queue.begin() + (long)qi);
     if (admitted) B->dirty = true;
+    // iterations every resident instance still needs before the first completes
+    int iters = 0, mask = 0, need = max_iters;
+    for (int id : B->active) {
+        const Inst &in = B->insts[(size_t)id];
+        mask |= in.kind == PTK_INST_FDTD ? 1 : in.kind == PTK_INST_GROUPBY ? 2 : 4;
+        need = std::min(need, in.kind == PTK_INST_GROUPBY ? 1 : in.steps - in.stepsDone);
+    }
     if (!B->active.empty()) {
-        if (admitted) HIPCHK(ptk_hip_stream_synchronize(B->stream));  // admission work is not iteration time
-        const auto te = std::chrono::steady_clock::now();
-        if (B->mode == PTK_BATCH_BATCHED) {
-            if (B->dirty) {
-                if (int r = BuildLists(B)) return r;
-                B->dirty = false;
-            }
-            if (!B->graph) {
+        iters = std::max(1, need);
+        if (B->mode == PTK_BATCH_BATCHED && B->dirty) {
+            if (int r = BuildLists(B)) return r;
+            B->dirty = false;
+        }
+        // the events bracket only the iterations: admission work queued before them is excluded
+        HIPCHK(ptk_hip_event_record(B->evStart, B->stream));
+        if (B->mode == PTK_BATCH_BATCHED && !B->useGraphs) {
+            for (int it = 0; it < iters; ++it)
+                if (int r = IssueBatched(B, mask & 1, mask & 2, mask & 4, mask & 1, true)) return r;
+        } else if (B->mode == PTK_BATCH_BATCHED) {
+            if (!B->graphs[mask]) {  // one captured iteration per mix of instance kinds
                 HIPCHK(ptk_hip_stream_begin_capture(B->stream));
-                int r = IssueBatched(B, true, true, true, true);
+                int r = IssueBatched(B, mask & 1, mask & 2, mask & 4, mask & 1);
                 ptk_hip_graph g = nullptr;
                 int e = ptk_hip_stream_end_capture(B->stream, &g);
                 if (r) return r;
                 if (e) return Fail(PTK_BATCH_EHIP, std::string("capture: ") + ptk_hip_last_error_string());
-                B->graph = g;
+                B->graphs[mask] = g;
             }
-            HIPCHK(ptk_hip_graph_launch(B->graph, B->stream));
-            ++B->nGraphs;
+            for (int it = 0; it < iters; ++it) {
+                HIPCHK(ptk_hip_graph_launch(B->graphs[mask], B->stream));
+                ++B->nGraphs;
+            }
         } else {
-            for (int id : B->active)
-                if (int r = IssuePlain(B, B->insts[(size_t)id])) return r;
+            for (int it = 0; it < iters; ++it) {
+                for (int id : B->active)
+                    if (int r = IssuePlain(B, B->insts[(size_t)id])) return r;
+                for (int id : B->active) ++B->insts[(size_t)id].stepsDone;  // parity of the next round
+            }
+            for (int id : B->active) B->insts[(size_t)id].stepsDone -= iters;
         }
         HIPCHK(ptk_hip_event_record(B->evStop, B->stream));
         HIPCHK(ptk_hip_event_synchronize(B->evStop));
-        B->execMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - te).count();
-        ++B->nTicks;
+        float ms = 0;
+        HIPCHK(ptk_hip_event_elapsed_ms(&ms, B->evStart, B->evStop));
+        B->execMs += ms;
+        B->nTicks += (unsigned long long)iters;
     }
     std::vector<int> done, keepActive;
     for (int id : B->active) {
@@ -762,7 +810,7 @@ int ptk_batch_tick(ptk_batch *B, int max_admit, int *nactive, int *ncompleted) {
         if (in.kind == PTK_INST_GROUPBY) {
             in.stepsDone = 1;
             done.push_back(id);
-        } else if (++in.stepsDone >= in.steps) {
+        } else if ((in.stepsDone += iters) >= in.steps) {
             done.push_back(id);
         } else {
             keepActive.push_back(id);
@@ -773,16 +821,21 @@ int ptk_batch_tick(ptk_batch *B, int max_admit, int *nactive, int *ncompleted) {
         B->dirty = true;
         if (int r = Retire(B, done)) return r;
     }
+    if (niters) *niters = iters;
     if (nactive) *nactive = (int)B->active.size();
     if (ncompleted) *ncompleted = (int)done.size();
     return PTK_BATCH_OK;
+}
+
+int ptk_batch_tick(ptk_batch *B, int max_admit, int *nactive, int *ncompleted) {
+    return ptk_batch_run(B, max_admit, 1, nullptr, nactive, ncompleted);
 }
 
 int ptk_batch_drain(ptk_batch *B) {
     if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
     while (!B->queue.empty() || !B->active.empty()) {
         const size_t q = B->queue.size(), a = B->active.size();
-        if (int r = ptk_batch_tick(B, -1, nullptr, nullptr)) return r;
+        if (int r = ptk_batch_run(B, -1, 1 << 30, nullptr, nullptr, nullptr)) return r;
         if (a == 0 && B->active.empty() && B->queue.size() == q)
             return Fail(PTK_BATCH_ENOMEM, "a queued instance does not fit the free arena");
     }
@@ -855,6 +908,12 @@ int ptk_batch_set_tiles(ptk_batch *B, int cols, int depth, int gbRows) {
     return PTK_BATCH_OK;
 }
 
+int ptk_batch_set_launch(ptk_batch *B, int graphs) {
+    if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
+    B->useGraphs = graphs != 0;
+    return PTK_BATCH_OK;
+}
+
 int ptk_batch_get_tiles(ptk_batch *B, int *cols, int *depth, int *gbRows) {
     if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
     if (cols) *cols = B->fdtdCols;
@@ -863,9 +922,10 @@ int ptk_batch_get_tiles(ptk_batch *B, int *cols, int *depth, int *gbRows) {
     return PTK_BATCH_OK;
 }
 
-// Times the FDTD half-step pair over the resident instances for each candidate tile shape (the
-// pair only rewrites the next field set from the current one, so repeating it changes nothing)
-// and keeps the fastest.  Groupby chunk rows are fixed at admission and are not tuned here.
+// Times the FDTD half-step pair over the resident instances for each candidate tile shape and
+// persistent grid size (the pair only rewrites the next field set from the current one, so
+// repeating it changes nothing) and keeps the fastest.  Groupby chunk rows are fixed at
+// admission and are not tuned here.
 int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
     if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
     bool anyFdtd = false;
@@ -875,28 +935,35 @@ int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
         return PTK_BATCH_OK;
     }
     static const int cand[][2] = {{256, 8}, {512, 8}, {256, 16}, {512, 16}, {1024, 16}, {512, 32}, {1024, 32}, {256, 64}};
+    static const int grids[] = {2048, 4096, 8192};
     double best = 1e30;
-    int bc = B->fdtdCols, bd = B->fdtdDepth;
+    int bc = B->fdtdCols, bd = B->fdtdDepth, bg = B->fdtdGrid;
     for (auto &c : cand) {
         B->fdtdCols = c[0];
         B->fdtdDepth = c[1];
         if (int r = BuildLists(B)) return r;
-        if (int r = IssueBatched(B, true, false, false, false)) return r;  // warm
-        HIPCHK(ptk_hip_event_record(B->evStart, B->stream));
-        for (int rep = 0; rep < 3; ++rep)
-            if (int r = IssueBatched(B, true, false, false, false)) return r;
-        HIPCHK(ptk_hip_event_record(B->evStop, B->stream));
-        HIPCHK(ptk_hip_event_synchronize(B->evStop));
-        float ms = 0;
-        HIPCHK(ptk_hip_event_elapsed_ms(&ms, B->evStart, B->evStop));
-        if (ms / 3 < best) {
-            best = ms / 3;
-            bc = c[0];
-            bd = c[1];
+        for (int g : grids) {
+            B->fdtdGrid = g;
+            if (int r = IssueBatched(B, true, false, false, false)) return r;  // warm
+            HIPCHK(ptk_hip_event_record(B->evStart, B->stream));
+            for (int rep = 0; rep < 3; ++rep)
+                if (int r = IssueBatched(B, true, false, false, false)) return r;
+            HIPCHK(ptk_hip_event_record(B->evStop, B->stream));
+            HIPCHK(ptk_hip_event_synchronize(B->evStop));
+            float ms = 0;
+            HIPCHK(ptk_hip_event_elapsed_ms(&ms, B->evStart, B->evStop));
+            if (ms / 3 < best) {
+                best = ms / 3;
+                bc = c[0];
+                bd = c[1];
+                bg = g;
+            }
         }
     }
     B->fdtdCols = bc;
     B->fdtdDepth = bd;
+    B->fdtdGrid = bg;
+    DropGraphs(B);  // the captured iterations hold the grid size
     if (int r = BuildLists(B)) return r;
     B->dirty = false;
     if (best_ms) *best_ms = best;

@@ -189,7 +189,7 @@ BatchGmmFinalize(const int2 *__restrict__ units, const int *__restrict__ count, 
 
 // Points of a seeded mixture: component c of point p uniform on [0, K) (stream 12); means
 // U[-10,10] (stream 10) and standard deviations U[0.5,2] (stream 11) per (c, m); coordinates
-// mean + sd * z, z by Box-Muller from streams 13 and 14.  first[c] receives the smallest point
+// mean + sd * z, z by Box-Muller in binary32 from streams 13 and 14 (u1 in (0, 1]).  first[c] receives the smallest point
 // index of component c (first must hold INT_MAX on entry).
 extern "C" __global__ void __launch_bounds__(256)
 BatchInitGmm(float *__restrict__ X, int N, int D, int K, unsigned long long seed, int *first) {
@@ -199,10 +199,10 @@ BatchInitGmm(float *__restrict__ X, int N, int D, int K, unsigned long long seed
         atomicMin(&first[c], (int)p);
         for (int m = 0; m < D; ++m) {
             const uint64_t cm = (uint64_t)c * D + m, i = (uint64_t)p * D + m;
-            const double mean = -10.0 + 20.0 * ptk_rand_unit_f64(seed, 10, cm);
-            const double sd = 0.5 + 1.5 * ptk_rand_unit_f64(seed, 11, cm);
-            const double u1 = 1.0 - ptk_rand_unit_f64(seed, 13, i), u2 = ptk_rand_unit_f64(seed, 14, i);
-            X[i] = (float)(mean + sd * (sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2)));
+            const float mean = -10.0f + 20.0f * ptk_rand_unit_f32(seed, 10, cm);
+            const float sd = 0.5f + 1.5f * ptk_rand_unit_f32(seed, 11, cm);
+            const float u1 = 1.0f - ptk_rand_unit_f32(seed, 13, i), u2 = ptk_rand_unit_f32(seed, 14, i);
+            X[i] = mean + sd * (sqrtf(-2.0f * __logf(u1)) * __cosf(6.2831853f * u2));
         }
     }
 }
@@ -252,6 +252,26 @@ BatchChecksumWords(const unsigned *__restrict__ a, long long n, unsigned long lo
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         h += ptk_splitmix64(((unsigned long long)a[i] << 32) ^ (unsigned long long)i * 0x9E3779B97F4A7C15ull);
     csum_add(out, h);
+}
+
+// All checksums of a set of retiring instances in one launch: grid (stripes, jobs); blocks of
+// job y stride over its elements.  Mode 2 (a groupby groups block) sums the header and the
+// G = groups[0] triples it holds; the rest of the block is never written and is not part of the
+// result.
+extern "C" __global__ void __launch_bounds__(256)
+BatchChecksumJobs(const PTK_CSUM_JOB *__restrict__ jobs) {
+    const PTK_CSUM_JOB J = jobs[blockIdx.y];
+    const long long n = J.mode == 2 ? 2 * (3 + 3 * ((const long long *)J.a)[0]) : J.n;
+    const unsigned *a = (const unsigned *)J.a;
+    const int *k = (const int *)J.a, *v = (const int *)J.b;
+    unsigned long long h = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        if (J.mode == 1)
+            h += ptk_splitmix64(((unsigned long long)(unsigned)k[i] << 32) | (unsigned)v[i]);
+        else
+            h += ptk_splitmix64(((unsigned long long)a[i] << 32) ^ (unsigned long long)i * 0x9E3779B97F4A7C15ull);
+    }
+    csum_add(J.out, h);
 }
 
 // The groups block of a groupby result: its header and the G = groups[0] triples it holds (the

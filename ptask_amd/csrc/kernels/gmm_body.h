@@ -53,12 +53,12 @@ struct GmmShared {
 
 // NC = float4 chunks per point (Dp / 4), a compile-time constant so that the chunk loops unroll
 // without guards and a component's table row arrives in one batch of scalar loads.
-// Tiles [t0, t1) of the points into one row of partial sums.
+// Tiles t0, t0 + tstep, ... (< t1) of the points into one row of partial sums.
 template <int NC>
 __device__ __forceinline__ void gmm_accumulate_tiles(const float *__restrict__ X, const GMM_DIMS &d,
                                                      const double *__restrict__ model,
                                                      double *__restrict__ row, long long t0,
-                                                     long long t1, GmmShared &sh) {
+                                                     long long t1, long long tstep, GmmShared &sh) {
 // Tolerance-checked binary32 work: let the compiler fuse multiply-adds (the build default is off).
 #pragma clang fp contract(fast)
     float *sX = sh.sX, *sR = sh.sR;
@@ -74,15 +74,20 @@ __device__ __forceinline__ void gmm_accumulate_tiles(const float *__restrict__ X
     const int kh = (K + 1) >> 1;
     const int k0 = eh * kh, k1 = min(K, k0 + kh);
 
-    // M-phase ownership: jobs (component, chunk) j = tid and tid + 256
+    // M-phase ownership: jobs (component, chunk).  With more than 128 jobs thread t owns jobs t
+    // and t + 256; with fewer, reps = 256 / jobs groups of threads each own every job and split
+    // the tile's points (group g takes points g, g + reps, ...), their sums combined in a fixed
+    // order at the end of the unit.
     const int jobs = K * nc;
+    const int reps = jobs <= 128 ? 256 / jobs : 1;
+    const int grp = reps > 1 ? tid / jobs : 0;
     int mk[2], mc[2];
     bool mv[2];
     float4 cen[2];
     double aw[2], asu[2][4], asuu[2][4];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        const int j = tid + 256 * q;
+        const int j = reps > 1 ? (q == 0 && tid < reps * jobs ? tid - grp * jobs : jobs) : tid + 256 * q;
         mv[q] = j < jobs;
         mk[q] = mv[q] ? j / nc : 0;
         mc[q] = mv[q] ? j - mk[q] * nc : 0;
@@ -93,7 +98,7 @@ __device__ __forceinline__ void gmm_accumulate_tiles(const float *__restrict__ X
     }
     double ll = 0;
 
-    for (long long t = t0; t < t1; ++t) {
+    for (long long t = t0; t < t1; t += tstep) {
         const long long p0 = t * TILE;
         const int np = (int)min((long long)TILE, (long long)d.N - p0);
         const float *__restrict__ xt = X + p0 * D;
@@ -153,7 +158,7 @@ __device__ __forceinline__ void gmm_accumulate_tiles(const float *__restrict__ X
 #pragma unroll
             for (int e = 0; e < 4; ++e) su[q][e] = suu[q][e] = 0.f;
         }
-        for (int p = 0; p < np; ++p) {
+        auto point = [&](int p) {
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 const float r = sR[p * RS + mk[q]];
@@ -167,6 +172,11 @@ __device__ __forceinline__ void gmm_accumulate_tiles(const float *__restrict__ X
                     suu[q][e] = fmaf(ru, u[e], suu[q][e]);
                 }
             }
+        };
+        if (reps == 1) {  // the common large-K case keeps a unit-stride loop
+            for (int p = 0; p < np; ++p) point(p);
+        } else {
+            for (int p = grp; p < np; p += reps) point(p);
         }
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -180,6 +190,31 @@ __device__ __forceinline__ void gmm_accumulate_tiles(const float *__restrict__ X
         __syncthreads();
     }
 
+    if (reps > 1) {  // combine the groups' sums in group order (LDS of sR is free now)
+        double *acc = (double *)sR;
+        if (mv[0] || tid < reps * jobs) {
+            double *a = acc + (size_t)tid * 9;
+            a[0] = aw[0];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                a[1 + e] = asu[0][e];
+                a[5 + e] = asuu[0][e];
+            }
+        }
+        __syncthreads();
+        if (grp == 0 && mv[0]) {
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                double v = 0;
+                for (int g = 0; g < reps; ++g) v += acc[(size_t)(g * jobs + tid) * 9 + e];
+                if (e == 0) aw[0] = v;
+                else if (e < 5) asu[0][e - 1] = v;
+                else asuu[0][e - 5] = v;
+            }
+        }
+        if (grp != 0) mv[0] = false;
+        __syncthreads();
+    }
     // the row of partial sums
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -204,68 +239,69 @@ __device__ __forceinline__ void gmm_accumulate_tiles(const float *__restrict__ X
     __syncthreads();  // a block may go on to another unit, which reuses sLL
 }
 
-// Row b of the partial sums covers the contiguous tiles [b*tpb, (b+1)*tpb), tpb = ceil(tiles /
-// blocks): a fixed function of (N, blocks), whoever runs it, so every launch form (one graph,
-// a batch of instances) produces bitwise the same partial sums.
-__device__ inline void gmm_unit_tiles(const GMM_DIMS &d, int b, long long *t0, long long *t1) {
-    const long long ntiles = ((long long)d.N + TILE - 1) / TILE;
-    const long long tpb = (ntiles + d.blocks - 1) / d.blocks;
-    *t0 = min(ntiles, (long long)b * tpb);
-    *t1 = min(ntiles, *t0 + tpb);
-}
-
+// Row b of the partial sums covers the tiles b, b + blocks, b + 2*blocks, ...: a fixed function
+// of (N, blocks), whoever runs it, so every launch form (one graph, a batch of instances)
+// produces bitwise the same partial sums; blocks running at the same time read neighbouring
+// tiles.
 __device__ __forceinline__ void gmm_accumulate_unit(const float *__restrict__ X, const GMM_DIMS &d,
                                                     const double *__restrict__ model,
                                                     double *__restrict__ partials, int b, GmmShared &sh) {
-    long long t0, t1;
-    gmm_unit_tiles(d, b, &t0, &t1);
+    const long long t0 = b, t1 = ((long long)d.N + TILE - 1) / TILE, ts = d.blocks;
     double *row = partials + (long long)b * ptk_gmm_partial_stride(&d);
     switch (ptk_gmm_dpad(&d) >> 2) {
-        case 1: gmm_accumulate_tiles<1>(X, d, model, row, t0, t1, sh); break;
-        case 2: gmm_accumulate_tiles<2>(X, d, model, row, t0, t1, sh); break;
-        case 3: gmm_accumulate_tiles<3>(X, d, model, row, t0, t1, sh); break;
-        case 4: gmm_accumulate_tiles<4>(X, d, model, row, t0, t1, sh); break;
-        case 5: gmm_accumulate_tiles<5>(X, d, model, row, t0, t1, sh); break;
-        case 6: gmm_accumulate_tiles<6>(X, d, model, row, t0, t1, sh); break;
-        case 7: gmm_accumulate_tiles<7>(X, d, model, row, t0, t1, sh); break;
-        default: gmm_accumulate_tiles<8>(X, d, model, row, t0, t1, sh); break;
+        case 1: gmm_accumulate_tiles<1>(X, d, model, row, t0, t1, ts, sh); break;
+        case 2: gmm_accumulate_tiles<2>(X, d, model, row, t0, t1, ts, sh); break;
+        case 3: gmm_accumulate_tiles<3>(X, d, model, row, t0, t1, ts, sh); break;
+        case 4: gmm_accumulate_tiles<4>(X, d, model, row, t0, t1, ts, sh); break;
+        case 5: gmm_accumulate_tiles<5>(X, d, model, row, t0, t1, ts, sh); break;
+        case 6: gmm_accumulate_tiles<6>(X, d, model, row, t0, t1, ts, sh); break;
+        case 7: gmm_accumulate_tiles<7>(X, d, model, row, t0, t1, ts, sh); break;
+        default: gmm_accumulate_tiles<8>(X, d, model, row, t0, t1, ts, sh); break;
     }
 }
 
 // Grid K + 1 blocks of 256.  Block k < K sums the 2D + 1 partial columns of component k over the
-// rows (two row groups, combined in a fixed order), applies the M-step and rewrites the
-// component's table entries; block K sums the log-likelihood and advances the step counter.
+// rows (256 / (2D + 1) interleaved row groups, combined in a fixed order), applies the M-step and
+// rewrites the component's table entries; block K sums the log-likelihood and advances the step
+// counter.
 __device__ inline void gmm_finalize_block(double *__restrict__ model, const double *__restrict__ partials,
                                           const GMM_DIMS &d, int k) {
-    __shared__ double sTot[2][128];
+    __shared__ double sTot[256];
     __shared__ double sLn[DMAX];
     __shared__ double sRed[256];
     const int D = d.D, K = d.K, P = d.blocks, Dp = ptk_gmm_dpad(&d);
     const long long S = ptk_gmm_partial_stride(&d);
     const int tid = threadIdx.x;
     if (k < K) {
-        const int c = tid & 127, g = tid >> 7;
+        const int ncol = 2 * D + 1, G = 256 / ncol;
+        const int g = tid / ncol, c = tid - g * ncol;
         double acc = 0;
-        if (c <= 2 * D) {
+        if (g < G) {
             const long long col = c == 0   ? (long long)k
                                   : c <= D ? K + (long long)k * D + (c - 1)
                                            : K + (long long)K * D + (long long)k * D + (c - 1 - D);
-            for (int b = g; b < P; b += 2) acc += partials[(long long)b * S + col];
+            for (int b = g; b < P; b += G) acc += partials[(long long)b * S + col];
         }
-        sTot[g][c] = acc;
+        sTot[tid] = acc;
+        __syncthreads();
+        double t = 0;  // the groups in order; column c's total then kept in sTot[c]
+        if (tid < ncol)
+            for (int q = 0; q < G; ++q) t += sTot[q * ncol + tid];
+        __syncthreads();
+        if (tid < ncol) sTot[tid] = t;
         __syncthreads();
         double *mu = model + ptk_gmm_mu(&d), *var = model + ptk_gmm_var(&d);
         float *tab = (float *)(model + ptk_gmm_table(&d));
-        const double w = sTot[0][0] + sTot[1][0];
+        const double w = sTot[0];
         if (tid < D) {
             const int m = tid;
             const long long i = (long long)k * D + m;
             double mnew = mu[i], vnew = var[i];
             if (w != 0) {
                 const double cen = (double)tab[2 * k * Dp + m];
-                const double dm = (sTot[0][1 + m] + sTot[1][1 + m]) / w;
+                const double dm = sTot[1 + m] / w;
                 mnew = cen + dm;
-                vnew = (sTot[0][1 + D + m] + sTot[1][1 + D + m]) / w - dm * dm;
+                vnew = sTot[1 + D + m] / w - dm * dm;
             }
             mu[i] = mnew;
             var[i] = vnew;

@@ -97,60 +97,86 @@ __device__ inline unsigned long long wave_incl(unsigned long long v) {
     return v;
 }
 
-// Exclusive block scan of one value per thread; blockDim.x a multiple of 64.
-__device__ inline unsigned long long block_excl(unsigned long long v, unsigned long long *total) {
-    __shared__ unsigned long long ws[16];
-    __shared__ unsigned long long tot;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    unsigned long long inc = wave_incl(v);
-    if (lane == 63) ws[w] = inc;
-    __syncthreads();
-    if (w == 0) {
-        unsigned long long s = lane < nw ? ws[lane] : 0ull;
-        unsigned long long si = wave_incl(s);
-        if (lane < nw) ws[lane] = si - s;
-        if (lane == nw - 1) tot = si;
-    }
-    __syncthreads();
-    unsigned long long r = ws[w] + inc - v;
-    *total = tot;
-    __syncthreads();
-    return r;
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
 }
 
-// One block: offsets = exclusive scan of counts; groups = the non-empty keys in ascending order
-// (their ranks = exclusive scan of count>0), with the header {G, rows, errors}.  Each thread owns
-// a contiguous run of keys: local totals, one block scan, then a serial pass over the run.
+// One block (blockDim.x a multiple of 64, at most 1024): offsets = exclusive scan of counts;
+// groups = the non-empty keys in ascending order with their count and sum, after the header
+// {G, rows, errors}.  Wave w owns a contiguous run of keys (a multiple of 64).  Pass 1: each wave
+// totals its run (rows and non-empty keys) with coalesced loads, eight 64-key steps in flight
+// per batch; the wave totals are scanned in LDS.  Pass 2: each wave walks its run again, 64 keys
+// per step: a wave-wide inclusive scan gives the offsets, a ballot gives each non-empty key its
+// rank among the group records.
 __device__ inline void gb_offsets_block(long long *table, int numKeys, long long *offsets, long long *groups) {
+    __shared__ unsigned long long sRows[16], sNz[16], sRowBase[16], sNzBase[16];
     const unsigned long long *cnt = gb_count(table, numKeys), *sum = gb_sum(table, numKeys);
     const long long K = (long long)numKeys + 1;
-    const long long per = (K + blockDim.x - 1) / blockDim.x;
-    const long long k0 = per * threadIdx.x, k1 = k0 + per < K ? k0 + per : K;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const long long per = ((K + nw - 1) / nw + 63) / 64 * 64;
+    const long long lo = min(K, per * w), hi = min(K, lo + per);
     unsigned long long rows = 0, nz = 0;
-    for (long long k = k0; k < k1; ++k) {
-        const unsigned long long c = cnt[k];
-        rows += c;
-        nz += c ? 1ull : 0ull;
-    }
-    unsigned long long totRows, totGroups;
-    unsigned long long r = block_excl(rows, &totRows);
-    unsigned long long gi = block_excl(nz, &totGroups);
-    for (long long k = k0; k < k1; ++k) {
-        const unsigned long long c = cnt[k];
-        offsets[k] = (long long)r;
-        if (c) {
-            long long *g = groups + 3 + 3 * (long long)gi;
-            g[0] = k;
-            g[1] = (long long)c;
-            g[2] = (long long)sum[k];
-            ++gi;
+    for (long long base = lo; base < hi; base += 64 * 8) {
+        unsigned long long c[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const long long k = base + i * 64 + lane;
+            c[i] = k < hi ? cnt[k] : 0ull;
         }
-        r += c;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            rows += c[i];
+            nz += c[i] ? 1ull : 0ull;
+        }
     }
+    rows = wave_sum(rows);
+    nz = wave_sum(nz);
+    if (lane == 0) {
+        sRows[w] = rows;
+        sNz[w] = nz;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        groups[0] = (long long)totGroups;
-        groups[1] = (long long)totRows;
+        unsigned long long r = 0, g = 0;
+        for (int i = 0; i < nw; ++i) {
+            sRowBase[i] = r;
+            sNzBase[i] = g;
+            r += sRows[i];
+            g += sNz[i];
+        }
+        groups[0] = (long long)g;
+        groups[1] = (long long)r;
         groups[2] = (long long)*gb_errors(table, numKeys);
+    }
+    __syncthreads();
+    unsigned long long carry = sRowBase[w], gcarry = sNzBase[w];
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;  // lanes < this lane
+    for (long long base = lo; base < hi; base += 64 * 8) {
+        unsigned long long c[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const long long k = base + i * 64 + lane;
+            c[i] = k < hi ? cnt[k] : 0ull;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const long long k = base + i * 64 + lane;
+            const bool in = k < hi;
+            const unsigned long long incl = wave_incl(c[i]);
+            if (in) offsets[k] = (long long)(carry + incl - c[i]);
+            carry += __shfl(incl, 63, 64);
+            const bool has = in && c[i] != 0;
+            const unsigned long long m = __ballot(has);
+            if (has) {
+                long long *g = groups + 3 + 3 * (long long)(gcarry + __popcll(m & below));
+                g[0] = k;
+                g[1] = (long long)c[i];
+                g[2] = (long long)sum[k];
+            }
+            gcarry += (unsigned long long)__popcll(m);
+        }
     }
     __syncthreads();
 }

@@ -97,8 +97,9 @@ def submit(ex, kind, args, seed, keep=False):
     return ex.submit_gmm(*args, seed, keep)
 
 
-def run_mix(mode, arena, tiles=None, autotune=False):
+def run_mix(mode, arena, tiles=None, autotune=False, graphs=True):
     with batch.Executor(arena, mode) as ex:
+        ex.set_launch(graphs)
         if tiles:
             ex.set_tiles(*tiles)
         ids = [submit(ex, kind, args, 100 + j) for j, (kind, args) in enumerate(MIX)]
@@ -117,6 +118,8 @@ def test_checksums_equal_alone_in_mix_both_modes_and_delayed(hip):
     assert plain == mixed
     tuned, _ = run_mix(batch.BATCHED, 2 * GB, tiles=(128, 4, 16384), autotune=True)
     assert tuned == mixed
+    direct, st = run_mix(batch.BATCHED, 2 * GB, graphs=False)
+    assert direct == mixed and st["graph_launches"] == 0
     # an arena that holds only a few instances at a time: admission by footprint queues the rest
     def fp(kind, a):
         if kind == batch.FDTD:
@@ -148,3 +151,23 @@ def test_staggered_arrivals_reuse_memory_and_keep_results(hip):
             ex.tick()
         ex.drain()
         assert [ex.checksums(i) for i in ids] == alone
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_single_and_multi_iteration_runs_agree(hip, mode):
+    """Iterations launched back to back (no wait between them) give what one-at-a-time ticks give."""
+    print(f"case batch multi-iteration mode={mode}", flush=True)
+    fdtd_only = [(k, a) for k, a in MIX if k != batch.GROUPBY]
+    sums = []
+    for stepwise in (True, False):
+        with batch.Executor(2 * GB, mode) as ex:
+            ids = [submit(ex, kind, args, 300 + j) for j, (kind, args) in enumerate(fdtd_only)]
+            if stepwise:
+                while any(ex.status(i)[0] < batch.DONE for i in ids):
+                    ex.tick()
+            else:
+                n, _, _ = ex.run(1 << 20)
+                assert n >= 3  # the shortest instance needs 3 iterations
+                ex.drain()
+            sums.append([ex.checksums(i) for i in ids])
+    assert sums[0] == sums[1]
