@@ -10,10 +10,12 @@ One benchmark "step" is one 200-time-step run of the graph.  Runs are chained: e
 from the previous run's output datablocks, which never leave HBM, so the timed region moves
 nothing over PCIe (the first, untimed warmup run uploads; a final untimed run downloads).
 
-Graph forms measured (all bit-identical to the six reference updates):
-  halves -- 2 tasks/step: FdtdHFields, FdtdEFields (72 B of HBM traffic per cell-step) [value]
-  six    -- the reference's six tasks/step (96 B per cell-step)
-  fused  -- 1 task/step, whole Yee step (48 B per cell-step)
+Graph forms measured (all bit-identical to the six reference updates); `value` is the fastest:
+  overlap -- 1 task/step, FdtdStepOverlap8: whole Yee step on overlapped 62 x 6 column tiles,
+             one LDS barrier per plane (48 B per cell-step + tile overlap)
+  halves  -- 2 tasks/step: FdtdHFields, FdtdEFields (72 B per cell-step)
+  six     -- the reference's six tasks/step (96 B per cell-step)
+  fused   -- 1 task/step, FdtdStepFused: whole step, halo recomputed by edge lanes (48 B)
 
 The same line also carries, under "workloads", the other graphs of the hot path:
   batched -- 2048 FDTD graphs (edge U{16..96}, 50 steps) + 512 groupby graphs (1M-8M rows, Zipf 1.1
@@ -49,7 +51,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--edge", type=int, default=384)
     ap.add_argument("--fdtd-steps", type=int, default=200)
-    ap.add_argument("--forms", default="halves,six,fused")
+    ap.add_argument("--forms", default="overlap,halves,six,fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--suite", default="batched,served,gmm,pca",
@@ -72,8 +74,10 @@ def algorithmic_bytes(task: str, n: int) -> int:
     """HBM bytes one launch of `task` must move: each input array read once, each output array
     written once (binary32)."""
     E = field_elems(n)
-    reads = {"FdtdHFields": range(6), "FdtdEFields": range(6), "FdtdStepFused": range(6)}
-    writes = {"FdtdHFields": (0, 1, 2), "FdtdEFields": (3, 4, 5), "FdtdStepFused": range(6)}
+    reads = {"FdtdHFields": range(6), "FdtdEFields": range(6), "FdtdStepFused": range(6),
+             "FdtdStepOverlap": range(6), "FdtdStepOverlap8": range(6)}
+    writes = {"FdtdHFields": (0, 1, 2), "FdtdEFields": (3, 4, 5), "FdtdStepFused": range(6),
+              "FdtdStepOverlap": range(6), "FdtdStepOverlap8": range(6)}
     six = {"HxComputation": (0, (0, 4, 5)), "HyComputation": (1, (1, 3, 5)),
            "HzComputation": (2, (2, 3, 4)), "ExComputation": (3, (3, 1, 2)),
            "EyComputation": (4, (4, 0, 2)), "EzComputation": (5, (5, 0, 1))}
@@ -115,6 +119,12 @@ def kernel_roofline(r: dict, n: int) -> dict:
             "launches_timed": int(dom["dispatches"])}
 
 
+def form_roofline(r: dict, n: int, form: str) -> dict:
+    roof = kernel_roofline(r, n)
+    roof["traffic"] = measured_traffic(roof["kernel"], n, form)
+    return roof
+
+
 def run_app(name: str, args: list, timeout: int = 600) -> dict:
     from ptask_amd import paths
     cmd = [paths.bin_path(name), *map(str, args)]
@@ -129,15 +139,16 @@ def run_app(name: str, args: list, timeout: int = 600) -> dict:
 
 
 def measured_traffic(kernel: str, n: int, form: str):
-    """HBM bytes per launch of `kernel` from the committed PMC summary (two rocprofv3 --pmc runs,
-    FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size), when it covers this workload."""
-    path = os.path.join(REPO, "profiles", "fdtd_384_halves_hbm_traffic.md")
-    if n != 384 or form != "halves" or not os.path.exists(path):
+    """HBM bytes per launch of `kernel` from the committed PMC summary of its form (rocprofv3 --pmc
+    runs of FETCH_SIZE and WRITE_SIZE scaled by a copy of known size, cross-checked with the
+    read/write request counts by size), when it covers this workload."""
+    path = os.path.join(REPO, "profiles", f"fdtd_384_{form}_hbm_traffic.md")
+    if n != 384 or not os.path.exists(path):
         return None
     text = open(path).read()
     block = text[text.index("```json") + 7:]
     data = json.loads(block[:block.index("```")])
-    return data.get(kernel, {}).get("traffic_bytes")
+    return data.get(kernel.split("_")[0], {}).get("traffic_bytes")
 
 
 def leg_batched() -> dict:
@@ -245,10 +256,9 @@ def main() -> int:
     n, T = args.edge, args.fdtd_steps
     forms = [f for f in args.forms.split(",") if f]
     results = {f: run_graph(f, n, T, args.steps, args.warmup) for f in forms}
-    head = results.get("halves") or results[forms[0]]
-    head_form = "halves" if "halves" in results else forms[0]
-    roof = kernel_roofline(head, n)
-    roof["traffic"] = measured_traffic(roof["kernel"], n, head_form)
+    head_form = max(results, key=lambda f: results[f]["mcups"])
+    head = results[head_form]
+    roof = form_roofline(head, n, head_form)
     line = {
         "metric": METRIC,
         "value": round(head["mcups"], 1),
@@ -270,7 +280,7 @@ def main() -> int:
         "roofline": roof,
         "forms": {f: {"mcups": round(r["mcups"], 1), "ms_per_run": round(r["ms_per_run"], 3),
                       "tasks_per_step": len(r["tasks"]), "dispatches_per_run": r["dispatches_last_run"],
-                      "roofline": kernel_roofline(r, n)} for f, r in results.items()},
+                      "roofline": form_roofline(r, n, f)} for f, r in results.items()},
         "checksums_equal_across_forms": len({tuple(c["fnv1a"] for c in r["checksums"])
                                              for r in results.values()}) == 1,
     }

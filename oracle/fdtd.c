@@ -189,3 +189,7 @@ void FdtdStepFused(unsigned int n, void **a) {
     ptk_oracle_fdtd_ey_f32((const float *)a[4], (const float *)a[7], (const float *)a[9], p, (float *)a[11]);
     ptk_oracle_fdtd_ez_f32((const float *)a[5], (const float *)a[7], (const float *)a[8], p, (float *)a[12]);
 }
+
+/* The overlapped-tile whole-step kernels compute the same step (same arguments). */
+void FdtdStepOverlap(unsigned int n, void **a) { FdtdStepFused(n, a); }
+void FdtdStepOverlap8(unsigned int n, void **a) { FdtdStepFused(n, a); }

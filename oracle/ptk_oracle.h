@@ -67,6 +67,8 @@ void EzComputation(unsigned int n, void **a);
 void FdtdHFields(unsigned int n, void **a);
 void FdtdEFields(unsigned int n, void **a);
 void FdtdStepFused(unsigned int n, void **a);
+void FdtdStepOverlap(unsigned int n, void **a);
+void FdtdStepOverlap8(unsigned int n, void **a);
 void GmmAccumulate(unsigned int n, void **a);
 void GmmFinalize(unsigned int n, void **a);
 void PcaLoad(unsigned int n, void **a);

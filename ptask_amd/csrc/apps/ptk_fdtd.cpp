@@ -45,6 +45,8 @@ std::vector<TaskSpec> FormTasks(const std::string &form) {
                 {"FdtdEFields", {3, 4, 5, 0, 1, 2}, {3, 4, 5}}};
     if (form == "fused")  // one whole step (48 B per cell-step)
         return {{"FdtdStepFused", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
+    if (form == "overlap")  // one whole step on overlapped tiles (48 B per cell-step + overlap)
+        return {{"FdtdStepOverlap", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
     // the reference's six tasks (graphfdtd.cpp:448-560): (own, in1, in2) -> own
     return {{"HxComputation", {0, 4, 5}, {0}}, {"HyComputation", {1, 3, 5}, {1}},
             {"HzComputation", {2, 3, 4}, {2}}, {"ExComputation", {3, 1, 2}, {3}},
@@ -57,6 +59,7 @@ struct Options {
     std::string form = "six";
     std::string kernelFile;
     std::string geometry = "default";
+    int tileI = 0, tileJ = 0, slab = 0;  // tile shape and k-slab depth (0: the form's default)
     std::string dump;
     unsigned long long seed = 0x5EED;
     float courant = 0.5f;
@@ -90,7 +93,9 @@ bool Parse(int argc, char **argv, Options &o) {
         else if (a == "--form") o.form = next("--form");
         else if (a == "--kernels") o.kernelFile = next("--kernels");
         else if (a == "--geometry") o.geometry = next("--geometry");
-        else if (a == "--dump") o.dump = next("--dump");
+        else if (a == "--tile") {
+            if (sscanf(next("--tile"), "%dx%dx%d", &o.tileI, &o.tileJ, &o.slab) != 3) return false;
+        } else if (a == "--dump") o.dump = next("--dump");
         else if (a == "--seed") o.seed = strtoull(next("--seed"), nullptr, 0);
         else if (a == "--courant") o.courant = (float)atof(next("--courant"));
         else if (a == "--threads") o.threads = atoi(next("--threads"));
@@ -104,6 +109,13 @@ bool Parse(int argc, char **argv, Options &o) {
             fprintf(stderr, "unknown option %s\n", a.c_str());
             return false;
         }
+    }
+    if (o.tileI && (o.tileI < 1 || o.tileJ < 2 || o.slab < 1 || o.tileI * o.tileJ > 1024)) return false;
+    if (o.form == "overlap" && o.tileI && (o.tileI != 64 || o.tileJ < 3 || o.tileJ > 16)) return false;
+    // the overlapped-tile kernel addresses each field with 32-bit buffer offsets
+    if (o.form == "overlap" && (long long)(o.nx + 1) * (o.ny + 1) * (o.nz + 1) >= (1ll << 29)) {
+        fprintf(stderr, "--form overlap needs (nx+1)(ny+1)(nz+1) < 2^29\n");
+        return false;
     }
     return !o.kernelFile.empty() && o.nx > 0 && o.ny > 0 && o.nz > 0 && o.steps > 0;
 }
@@ -137,6 +149,13 @@ struct FdtdGraph {
             t->SetComputeGeometry(1, 1, 1);  // a host task computes its whole arrays per call
             return;
         }
+        if (!strncmp(ts.kernel, "FdtdStepOverlap", 15)) {
+            // 64 x R thread blocks owning 62 x (R-2) columns each, k slabs of `slab` planes
+            const int R = o.tileJ ? o.tileJ : 8, slab = o.slab ? o.slab : 8;
+            t->SetBlockAndGridSize(PTASKDIM3((o.nx + 62) / 62, (o.ny + R - 2) / (R - 2), (o.nz + slab) / slab),
+                                   PTASKDIM3(64, R, 1));
+            return;
+        }
         if (ts.out.size() == 6) {  // whole-step kernel: 64x8 tiles, slabs of 16 planes
             t->SetBlockAndGridSize(PTASKDIM3((o.nx + 64) / 64, (o.ny + 8) / 8, (o.nz + 16) / 16),
                                    PTASKDIM3(64, 8, 1));
@@ -153,6 +172,11 @@ struct FdtdGraph {
             t->SetBlockAndGridSize(PTASKDIM3((d1 + 31) / 32, d2 * d3, 1), PTASKDIM3(32, 1, 1));
         } else if (o.geometry == "compute") {
             t->SetComputeGeometry(d1, d2, d3);
+        } else if (ts.out.size() == 3 && o.tileI) {
+            // half steps with an explicit tileI x tileJ block and k slabs of ~slab planes
+            t->SetBlockAndGridSize(PTASKDIM3((d1 + o.tileI - 1) / o.tileI, (d2 + o.tileJ - 1) / o.tileJ,
+                                             (d3 + o.slab - 1) / o.slab),
+                                   PTASKDIM3(o.tileI, o.tileJ, 1));
         } else {
             // column walk: one lane per (i,j) column, k split in slabs of ~8 planes
             const int bx = ts.out.size() == 3 ? 512 : 256;
@@ -191,6 +215,7 @@ struct FdtdGraph {
     bool Build() {
         hip = ptaskutils::SelectAcceleratorClass(o.kernelFile.c_str()) == ACCELERATOR_CLASS_HIP;
         spec = FormTasks(o.form);
+        if (o.form == "overlap" && o.tileJ <= 8) spec[0].kernel = "FdtdStepOverlap8";  // half the LDS
         for (int f = 0; f < 6; ++f) {
             int d1, d2, d3;
             ptk_fdtd_extent(f, o.nx, o.ny, o.nz, &d1, &d2, &d3);
@@ -295,7 +320,7 @@ int main(int argc, char **argv) {
     if (!Parse(argc, argv, o)) {
         fprintf(stderr, "usage: ptk_fdtd --kernels <fdtd.hsaco|hosttasks.so> [--n N | --nx --ny --nz] "
                         "[--steps S] [--mode looped|unrolled] [--blocks B] [--repeat R] [--seed S] "
-                        "[--form six|halves|fused] [--geometry default|reference|compute] [--dump file] "
+                        "[--form six|halves|fused|overlap] [--geometry default|reference|compute] [--tile IxJxK] [--dump file] "
                         "[--eager] [--profile] [--chain]\n");
         return 2;
     }

@@ -181,3 +181,196 @@ __device__ inline void fused_tile(const Fields &F, const FDTD_PARAMS &p, int i0,
 }
 
 }  // namespace ptk_fdtd
+
+namespace ptk_fdtd {
+
+// ---- overlapped tiles: one whole step, one barrier per plane --------------------------------
+//
+// A block of 64 x R threads (R = blockDim.y, 3..MAXR) covers columns i0-1 .. i0+62 (lane l ->
+// i = i0-1+l) and rows j0-1 .. j0+R-2 (wave r -> j = j0-1+r) and owns the inner 62 x (R-2)
+// cells: it writes new H and E there only.  Every thread computes new H of its cell, so the
+// i-1 / j-1 neighbours that an owned cell's new E needs are in the block; on the low edge these
+// are cells of the neighbouring tile, recomputed bit for bit.  The old E that new H needs at
+// i+1 / j+1 is likewise in the block.  New H of a cell outside a field's extent, or on the
+// block's high edge, is formed from zeros or unrelated values and never reaches an output: an
+// owned cell's new E reads new H only at in-extent cells of the block (graphfdtd.cpp:70-86).
+//
+// Per plane k a thread loads its own old H(k) and old E(k+1) once -- the plane k+1 values are
+// carried to the next plane in registers, and both are prefetched two planes ahead -- so HBM
+// sees each input once per tile (48 B per cell-step plus the overlap).  Neighbour values pass
+// through LDS, double-buffered by plane parity:
+//   H phase   new H(k) from registers and E(k) neighbours in e[k&1];
+//             write new H(k) to h[k&1] and old E(k+1) to e[(k+1)&1];
+//   barrier   (LDS only: loads in flight stay in flight);
+//   E phase   new E(k) from registers (old E(k), new H(k), new H(k-1)) and h[k&1] neighbours.
+// e[(k+1)&1] was last read in the H phase of plane k-1 and h[k&1] in the E phase of plane k-2,
+// both before the barrier of plane k-1, so one barrier per plane orders every access.
+// A slab [k0, k1) starts one plane early (k0-1) to form new H(k0-1) for E(k0).
+#ifndef PTK_OVERLAP_AHEAD
+#define PTK_OVERLAP_AHEAD 2  // planes of loads in flight ahead of the plane being computed (1 or 2)
+#endif
+#ifndef PTK_OVERLAP_STORE_POLICY
+#define PTK_OVERLAP_STORE_POLICY 0  // cache-policy bits of the output stores
+#endif
+
+// LDS planes with one padding row and column on each side, so that the four neighbours of
+// (r, l) are at fixed offsets from its slot: index [r + 1][l + 1].
+template <int MAXR>
+struct OverlapLds {
+    float e[2][3][MAXR + 2][66];  // old Ex, Ey, Ez of a plane
+    float h[2][3][MAXR + 2][66];  // new Hx, Hy, Hz of a plane
+};
+
+// Barrier that orders LDS accesses only; outstanding global loads are not waited for.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Raw buffer access to one field array.  A byte offset at or beyond the array's size reads 0
+// and drops a store (the buffer unit's range check), so a lane outside a field's extent issues
+// the same instruction with bit 31 of its offset set instead of branching around it: every
+// plane then issues a fixed sequence of memory instructions and the loads of later planes stay
+// in flight across the barrier.  Arrays must hold fewer than 2^31 bytes.
+constexpr unsigned kNoCell = 0x80000000u;
+
+struct FieldBuf {
+    __amdgpu_buffer_rsrc_t rs;
+    __device__ float load(unsigned off) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0));
+    }
+    __device__ void store(unsigned off, float v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, (int)off, 0,
+                                              PTK_OVERLAP_STORE_POLICY);
+    }
+};
+
+__device__ inline FieldBuf field_buf(const float *p, long long elems) {
+    return FieldBuf{__builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)(elems * 4), 0x00020000)};
+}
+
+// TRUE when every field array is under 2^31 bytes (the buffer range check of overlap_tile).
+__device__ inline bool fits_buffer(const FDTD_PARAMS &p) {
+    return (long long)(p.Nx + 1) * (p.Ny + 1) * (p.Nz + 1) < (1ll << 29);
+}
+
+template <int MAXR>
+__device__ inline void overlap_tile(const Fields &F, const FDTD_PARAMS &p, int i0, int j0, int k0, int k1,
+                                    OverlapLds<MAXR> &L) {
+    const Ix X{p.Nx, p.Ny, p.Nz};
+    const Coef c = make_coef(p);
+    const int R = (int)blockDim.y;
+    const int l = (int)threadIdx.x, r = (int)threadIdx.y;
+    const int i = i0 - 1 + l, j = j0 - 1 + r;
+    const int Nx = p.Nx, Ny = p.Ny, Nz = p.Nz;
+    const bool dom = i >= 0 && i <= Nx && j >= 0 && j <= Ny;
+    const bool own = dom && l >= 1 && l <= 62 && r >= 1 && r <= R - 2;
+    // byte offset of (i, j, 0) in each field, bit 31 set when the column is outside the field
+    auto col = [](bool ok, int idx) { return ok ? 4u * (unsigned)idx : kNoCell; };
+    const unsigned qHx = col(dom && j < Ny, X.hx(i, j, 0)), qHy = col(dom && i < Nx, X.hy(i, j, 0));
+    const unsigned qHz = col(dom && i < Nx && j < Ny, X.hz(i, j, 0)), qEx = col(dom && i < Nx, X.ex(i, j, 0));
+    const unsigned qEy = col(dom && j < Ny, X.ey(i, j, 0)), qEz = col(dom, X.ez(i, j, 0));
+    const unsigned ownBit = own ? 0u : kNoCell;
+    // byte plane strides
+    const unsigned sHx = 4u * (Nx + 1) * Ny, sHy = 4u * Nx * (Ny + 1), sHz = 4u * Nx * Ny;
+    const unsigned sEx = 4u * Nx * (Ny + 1), sEy = 4u * (Nx + 1) * Ny, sEz = 4u * (Nx + 1) * (Ny + 1);
+    const FieldBuf bHx = field_buf(F.hx, (long long)(Nx + 1) * Ny * Nz), bHy = field_buf(F.hy, (long long)Nx * (Ny + 1) * Nz),
+                   bHz = field_buf(F.hz, (long long)Nx * Ny * (Nz + 1)), bEx = field_buf(F.ex, (long long)Nx * (Ny + 1) * (Nz + 1)),
+                   bEy = field_buf(F.ey, (long long)(Nx + 1) * Ny * (Nz + 1)), bEz = field_buf(F.ez, (long long)(Nx + 1) * (Ny + 1) * Nz);
+    const FieldBuf wHx = field_buf(F.ohx, (long long)(Nx + 1) * Ny * Nz), wHy = field_buf(F.ohy, (long long)Nx * (Ny + 1) * Nz),
+                   wHz = field_buf(F.ohz, (long long)Nx * Ny * (Nz + 1)), wEx = field_buf(F.oex, (long long)Nx * (Ny + 1) * (Nz + 1)),
+                   wEy = field_buf(F.oey, (long long)(Nx + 1) * Ny * (Nz + 1)), wEz = field_buf(F.oez, (long long)(Nx + 1) * (Ny + 1) * Nz);
+    // offset of plane k of a column: the plane test is uniform over the block (a scalar mask)
+    auto at = [](unsigned q, unsigned s, int k, bool ok) { return (q + (unsigned)k * s) | (ok ? 0u : kNoCell); };
+    // per-lane parts of the face tests of graphfdtd.cpp:243-312
+    const bool jIn = j >= 1 && j <= Ny - 1, iIn = i >= 1 && i <= Nx - 1;
+
+    // Register sets S_k = {old E(k+1), old H(k)}.  Plane k reads S_{k-1} (its E part is old
+    // E(k)) and S_k, while the loads of the next PTK_OVERLAP_AHEAD sets are in flight: the sets
+    // rotate through an unrolled loop body, so no set is ever copied.  A set S_k with k >= k1 is
+    // never used and its loads are dropped.
+    struct Set {
+        float ex, ey, ez, hx, hy, hz;
+    };
+    auto load_set = [&](Set &S, int k) {
+        const bool use = k < k1;
+        S.ex = bEx.load(at(qEx, sEx, k + 1, use && k + 1 <= Nz));
+        S.ey = bEy.load(at(qEy, sEy, k + 1, use && k + 1 <= Nz));
+        S.ez = bEz.load(at(qEz, sEz, k + 1, use && k + 1 < Nz));
+        S.hx = bHx.load(at(qHx, sHx, k, use && k < Nz));
+        S.hy = bHy.load(at(qHy, sHy, k, use && k < Nz));
+        S.hz = bHz.load(at(qHz, sHz, k, use && k <= Nz));
+    };
+    const int ks = k0 >= 1 ? k0 - 1 : 0;
+    Set S0, S1, S2, S3;
+    S0.ex = bEx.load(at(qEx, sEx, ks, ks <= Nz));  // old E(ks)
+    S0.ey = bEy.load(at(qEy, sEy, ks, ks <= Nz));
+    S0.ez = bEz.load(at(qEz, sEz, ks, ks < Nz));
+    S0.hx = S0.hy = S0.hz = 0.f;
+    load_set(S1, ks);
+#if PTK_OVERLAP_AHEAD == 2
+    load_set(S2, ks + 1);
+#endif
+    const int y = r + 1, x = l + 1;  // this thread's LDS slot
+    L.e[ks & 1][0][y][x] = S0.ex;
+    L.e[ks & 1][1][y][x] = S0.ey;
+    L.e[ks & 1][2][y][x] = S0.ez;
+    float hxm = 0.f, hym = 0.f;  // new Hx, Hy of plane k-1
+    lds_barrier();
+    // plane k: P = S_{k-1}, S = S_k; loads S_{k+PTK_OVERLAP_AHEAD} into N
+    auto plane = [&](int k, const Set &P, const Set &S, Set &N) {
+        const int b = k & 1;
+        load_set(N, k + PTK_OVERLAP_AHEAD);
+        const float ex = P.ex, ey = P.ey, ez = P.ez;  // old E(k)
+        const float ezj = L.e[b][2][y + 1][x], ezi = L.e[b][2][y][x + 1];
+        const float exj = L.e[b][0][y + 1][x], eyi = L.e[b][1][y][x + 1];
+        // graphfdtd.cpp:185-240, in the reference's operation order
+        const float hx = S.hx + c.cH * ((S.ey - ey) * c.Cz - (ezj - ez) * c.Cy);
+        const float hy = S.hy + c.cH * ((ezi - ez) * c.Cx - (S.ex - ex) * c.Cz);
+        const float hz = S.hz + c.cH * ((exj - ex) * c.Cy - (eyi - ey) * c.Cx);
+        L.h[b][0][y][x] = hx;
+        L.h[b][1][y][x] = hy;
+        L.h[b][2][y][x] = hz;
+        L.e[b ^ 1][0][y][x] = S.ex;
+        L.e[b ^ 1][1][y][x] = S.ey;
+        L.e[b ^ 1][2][y][x] = S.ez;
+        lds_barrier();
+        const float hzj = L.h[b][2][y - 1][x], hzi = L.h[b][2][y][x - 1];
+        const float hyi = L.h[b][1][y][x - 1], hxj = L.h[b][0][y - 1][x];
+        // graphfdtd.cpp:243-312: interior cells update, tangential faces copy through
+        const bool kIn = k >= 1 && k <= Nz - 1;
+        const float wx = ex + c.cE * ((hz - hzj) * c.Cy - (hy - hym) * c.Cz);
+        const float wy = ey + c.cE * ((hx - hxm) * c.Cz - (hz - hzi) * c.Cx);
+        const float wz = ez + c.cE * ((hy - hyi) * c.Cx - (hx - hxj) * c.Cy);
+        const float vx = (jIn && kIn) ? wx : ex;
+        const float vy = (iIn && kIn) ? wy : ey;
+        const float vz = (iIn && jIn) ? wz : ez;
+        const bool out = k >= k0;
+        wHx.store(at(qHx | ownBit, sHx, k, out && k < Nz), hx);
+        wHy.store(at(qHy | ownBit, sHy, k, out && k < Nz), hy);
+        wHz.store(at(qHz | ownBit, sHz, k, out && k <= Nz), hz);
+        wEx.store(at(qEx | ownBit, sEx, k, out && k <= Nz), vx);
+        wEy.store(at(qEy | ownBit, sEy, k, out && k <= Nz), vy);
+        wEz.store(at(qEz | ownBit, sEz, k, out && k < Nz), vz);
+        hxm = hx;
+        hym = hy;
+    };
+#if PTK_OVERLAP_AHEAD == 2
+    for (int k = ks; k < k1; k += 4) {
+        plane(k, S0, S1, S3);
+        if (k + 1 < k1) plane(k + 1, S1, S2, S0);
+        if (k + 2 < k1) plane(k + 2, S2, S3, S1);
+        if (k + 3 < k1) plane(k + 3, S3, S0, S2);
+    }
+#else
+    (void)S3;
+    for (int k = ks; k < k1; k += 3) {
+        plane(k, S0, S1, S2);
+        if (k + 1 < k1) plane(k + 1, S1, S2, S0);
+        if (k + 2 < k1) plane(k + 2, S2, S0, S1);
+    }
+#endif
+}
+
+}  // namespace ptk_fdtd

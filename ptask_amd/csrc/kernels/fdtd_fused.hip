@@ -6,6 +6,7 @@
 // Launch: block exactly (64, TJ, 1); grid (ceil((Nx+1)/64), ceil((Ny+1)/TJ), slabs).  The grid
 // may be smaller than the tile count: blocks stride over tiles and split [0, Nz] into
 // gridDim.z slabs.  Compiled with -ffp-contract=off: bit-identical to the six updates.
+#include "fdtd_common.h"
 #include "fdtd_fused.h"
 
 using namespace ptk_fdtd;
@@ -38,6 +39,54 @@ __device__ inline void fused_grid(const Fields &F, const FDTD_PARAMS &p) {
 FUSED_KERNEL(FdtdStepFused4, 64, 4)
 FUSED_KERNEL(FdtdStepFused, 64, 8)
 FUSED_KERNEL(FdtdStepFused2, 64, 2)
+
+// Overlapped-tile whole step (see overlap_tile in fdtd_fused.h), same ABI as FdtdStepFused.
+// Launch: block (64, R, 1) with 3 <= R <= 16 (FdtdStepOverlap) or 3 <= R <= 8
+// (FdtdStepOverlap8, half the LDS); grid (ceil((Nx+1)/62), ceil((Ny+1)/(R-2)), slabs).
+// Smaller grids stride over the tiles; gridDim.z splits [0, Nz] into slabs.  LDS per block:
+// 2 planes x 6 fields x (MAXR+2) x 66 floats (31.7 KB for the 8-row kernel).
+#ifndef PTK_OVERLAP_XCD
+#define PTK_OVERLAP_XCD 1  // XCD-contiguous tile order
+#endif
+
+template <int MAXR>
+__device__ inline void overlap_grid(const Fields &F, const FDTD_PARAMS &p, OverlapLds<MAXR> &L) {
+    const int R = (int)blockDim.y;
+    if (blockDim.x != 64 || R < 3 || R > MAXR || blockDim.z != 1) return;  // geometry contract
+    if (!fits_buffer(p)) return;  // the launcher rejects such grids before launching
+    const int ti = (p.Nx + 1 + 61) / 62, tj = (p.Ny + 1 + R - 3) / (R - 2);
+    // Tiles that share cache lines (i- and j-neighbours of one slab) go to the same XCD at about
+    // the same time: workgroups are dealt to the 8 XCDs round-robin, so the linear block id is
+    // renumbered to give each XCD a contiguous run of the (x fastest, then y, then z) tile order.
+    const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
+    unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if (PTK_OVERLAP_XCD && nblk >= kNumXcd) {
+        const unsigned xcd = b % kNumXcd, local = b / kNumXcd, per = nblk / kNumXcd, rem = nblk % kNumXcd;
+        b = xcd * per + (xcd < rem ? xcd : rem) + local;
+    }
+    const int gx = (int)(b % gridDim.x), gy = (int)(b / gridDim.x % gridDim.y), gz = (int)(b / (gridDim.x * gridDim.y));
+    const int depth = p.Nz + 1, slabs = (int)gridDim.z;
+    const int k0 = (int)((long long)gz * depth / slabs), k1 = (int)((long long)(gz + 1) * depth / slabs);
+    if (k0 >= k1) return;  // the whole block
+    for (int by = gy; by < tj; by += gridDim.y)
+        for (int bx = gx; bx < ti; bx += gridDim.x) {
+            overlap_tile<MAXR>(F, p, bx * 62, by * (R - 2), k0, k1, L);
+            lds_barrier();  // the next tile reuses LDS
+        }
+}
+
+#define OVERLAP_KERNEL(NAME, MAXR)                                                                   \
+    extern "C" __global__ void __launch_bounds__(64 * MAXR)                                         \
+    NAME(const float *hx, const float *hy, const float *hz, const float *ex, const float *ey,         \
+         const float *ez, FDTD_PARAMS p, float *ohx, float *ohy, float *ohz, float *oex, float *oey, \
+         float *oez) {                                                                                \
+        __shared__ OverlapLds<MAXR> L;                                                                \
+        const Fields F{hx, hy, hz, ex, ey, ez, ohx, ohy, ohz, oex, oey, oez};                         \
+        overlap_grid<MAXR>(F, p, L);                                                                  \
+    }
+
+OVERLAP_KERNEL(FdtdStepOverlap, 16)
+OVERLAP_KERNEL(FdtdStepOverlap8, 8)
 
 // ---- half-step kernels: all three H updates in one streaming pass, then all three E updates ----
 //

@@ -243,3 +243,73 @@ def test_half_step_kernels_bitexact(hip, fmod, shape, geom):
     F = oracle.init_fields(nx, ny, nz, seed=77 + ny)
     got = run_halves(hip, fmod, F, FD.make_params(nx, ny, nz), 4, GEOMS[geom])
     assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 4))
+
+
+def tile_geometry(tj, slab, max_tiles=None):
+    """2-D blocks of 64 x tj over the columns with k slabs of ~slab planes; max_tiles caps the
+    grid."""
+    def geo(d1, d2, d3):
+        ti, tjn = -(-d1 // 64), -(-d2 // tj)
+        if max_tiles:
+            ti, tjn = min(ti, max_tiles), min(tjn, max_tiles)
+        return (ti, tjn, max(1, -(-d3 // slab))), (64, tj, 1)
+    return geo
+
+
+HALF_TILES = {
+    "64x8_slab8": tile_geometry(8, 8),
+    "64x4_slab1": tile_geometry(4, 1),
+    "64x16_one_slab": tile_geometry(16, 1 << 20),
+    "64x2_strided": tile_geometry(2, 3, max_tiles=2),
+    "64x8_strided_single": tile_geometry(8, 5, max_tiles=1),
+}
+
+
+@pytest.mark.parametrize("geom", sorted(HALF_TILES))
+@pytest.mark.parametrize("shape", [(70, 9, 11), (150, 20, 9), (63, 15, 2), (1, 1, 1)])
+def test_half_step_2d_blocks_cover_domain(hip, fmod, shape, geom):
+    """The column walk visits every (column, k) exactly once for 2-D blocks and capped grids."""
+    nx, ny, nz = shape
+    print(f"case halves tiles shape={shape} geom={geom}", flush=True)
+    F = oracle.init_fields(nx, ny, nz, seed=11 + nx)
+    got = run_halves(hip, fmod, F, FD.make_params(nx, ny, nz), 3, HALF_TILES[geom])
+    assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 3))
+
+
+def overlap_geometry(rows, slab, max_tiles=None):
+    """Overlapped tiles: blocks of 64 x rows own 62 x (rows-2) columns; k slabs of ~slab planes;
+    max_tiles caps the grid so blocks take further tiles in grid strides."""
+    def geo(nx, ny, nz):
+        ti, tj = -(-(nx + 1) // 62), -(-(ny + 1) // (rows - 2))
+        if max_tiles:
+            ti, tj = min(ti, max_tiles), min(tj, max_tiles)
+        return (ti, tj, max(1, -(-(nz + 1) // slab))), (64, rows, 1)
+    return geo
+
+
+OVERLAP = {
+    "r16_slab48": (16, 48, None),
+    "r8_slab4": (8, 4, None),
+    "r3_slab1": (3, 1, None),
+    "r4_one_slab": (4, 1 << 20, None),
+    "r16_slab3_strided": (16, 3, 1),
+    "r5_slab2_strided": (5, 2, 2),
+}
+
+
+@pytest.mark.parametrize("name", ["FdtdStepOverlap", "FdtdStepOverlap8"])
+@pytest.mark.parametrize("geom", sorted(OVERLAP))
+@pytest.mark.parametrize("shape", [(8, 8, 8), (61, 13, 4), (62, 14, 3), (124, 9, 7), (130, 29, 6),
+                                   (13, 21, 17), (1, 1, 1), (3, 1, 2), (1, 5, 3)])
+def test_overlap_step_bitexact(hip, fmod, shape, geom, name):
+    """FdtdStepOverlap(8): whole steps on overlapped tiles equal the six reference updates bit for
+    bit, for tile edges at and around the 62-column / (R-2)-row tile sizes and any slab split."""
+    rows, slab, cap = OVERLAP[geom]
+    if name == "FdtdStepOverlap8" and rows > 8:
+        pytest.skip("the 8-row kernel takes blocks of at most 8 rows")
+    nx, ny, nz = shape
+    print(f"case overlap {name} shape={shape} geom={geom}", flush=True)
+    F = oracle.init_fields(nx, ny, nz, seed=211 + nx + ny)
+    geo = overlap_geometry(rows, slab, cap)(nx, ny, nz)
+    got = run_fused(hip, fmod, name, F, FD.make_params(nx, ny, nz), 4, geo)
+    assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 4))

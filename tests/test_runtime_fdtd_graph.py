@@ -50,7 +50,7 @@ def direct(nx, ny, nz, steps, seed):
     return oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), steps)
 
 
-TASKS_PER_STEP = {"six": 6, "halves": 2, "fused": 1}
+TASKS_PER_STEP = {"six": 6, "halves": 2, "fused": 1, "overlap": 1}
 
 
 @pytest.mark.parametrize("form", sorted(TASKS_PER_STEP))
@@ -115,7 +115,7 @@ def test_hip_looped_graph_equals_host_graph(tmp_path, geometry):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", ["halves", "fused"])
+@pytest.mark.parametrize("form", ["halves", "fused", "overlap"])
 @pytest.mark.parametrize("mode", ["looped", "unrolled"])
 def test_hip_fused_forms_equal_host_graph(tmp_path, mode, form):
     nx, ny, nz, steps = 70, 9, 11, 5
@@ -124,6 +124,22 @@ def test_hip_fused_forms_equal_host_graph(tmp_path, mode, form):
             "--mode", mode, "--form", "six", "--seed", 3, dump=dh)
     g = run_app("--kernels", HSACO_FUSED, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
                 "--mode", mode, "--form", form, "--seed", 3, dump=dg)
+    assert g["ok"] and g["dispatches_last_run"] == TASKS_PER_STEP[form] * steps
+    H, G = load_dump(dh, nx, ny, nz), load_dump(dg, nx, ny, nz)
+    for f in range(6):
+        assert np.array_equal(H[f].view(np.uint32), G[f].view(np.uint32)), oracle.FIELDS[f]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form,geo", [("halves", ["--tile", "64x4x3"]), ("halves", ["--tile", "128x2x64"]),
+                                      ("overlap", ["--tile", "64x3x1"]), ("overlap", ["--tile", "64x8x5"])])
+def test_hip_tile_geometries_equal_host_graph(tmp_path, form, geo):
+    nx, ny, nz, steps = 70, 9, 11, 4
+    dh, dg = str(tmp_path / "h.bin"), str(tmp_path / "g.bin")
+    run_app("--kernels", HOSTLIB, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
+            "--form", "six", "--seed", 9, dump=dh)
+    g = run_app("--kernels", HSACO_FUSED, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
+                "--form", form, "--seed", 9, *geo, dump=dg)
     assert g["ok"] and g["dispatches_last_run"] == TASKS_PER_STEP[form] * steps
     H, G = load_dump(dh, nx, ny, nz), load_dump(dg, nx, ny, nz)
     for f in range(6):
