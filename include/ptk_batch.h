@@ -21,6 +21,7 @@
 
 #include "ptk_fdtd.h"
 #include "ptk_gmm.h"
+#include "ptk_groupby.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -50,7 +51,9 @@ typedef struct _ptk_groupby_inst {
     long long n;          /* rows */
     long long *table, *offsets, *groups;
     int *outKeys, *outVals;
-    int numKeys, chunkRows;
+    void *scratch;        /* ptk_gb_scratch_bytes(n, numKeys) (include/ptk_groupby.h) */
+    int numKeys, chunkRows;  /* chunkRows: rows per accumulate / reshuffle work unit, a
+                                multiple of PTK_GB_CHUNK */
 } PTK_GROUPBY_INST;
 
 typedef struct _ptk_gmm_inst {

@@ -22,6 +22,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "ptk_groupby.h"
 #include "ptk_oracle.h"
 
 void ptk_oracle_pfxsum_i32(const int32_t *in, int32_t *out, long long n) {
@@ -108,23 +109,128 @@ void pfxsum(unsigned int n, void **a) {
     ptk_oracle_pfxsum_i32((const int32_t *)a[0], (int32_t *)a[1], IARG(a, 2));
 }
 
-void GroupbyClear(unsigned int n, void **a) {
-    (void)n;
-    gb_clear((int64_t *)a[0], IARG(a, 1));
+/* The chunk / bucket pipeline of the HIP kernels (include/ptk_groupby.h) restated on the host:
+ * the same scratch layout, with each chunk's distinct keys in ascending key order and the merge
+ * taking chunks in order (one of the orders the device may produce). */
+static int cmp_i32(const void *x, const void *y) {
+    const int32_t a = *(const int32_t *)x, b = *(const int32_t *)y;
+    return (a > b) - (a < b);
 }
 
-void GroupbyAccumulate(unsigned int n, void **a) {
-    (void)n;
-    gb_accumulate((const int32_t *)a[0], (const int32_t *)a[1], IARG(a, 2), IARG(a, 3), (int64_t *)a[4]);
+void GroupbyAccumulate(unsigned int n_, void **a) {
+    (void)n_;
+    const int32_t *keys = (const int32_t *)a[0], *vals = (const int32_t *)a[1];
+    const long long n = IARG(a, 2);
+    const int numKeys = IARG(a, 3);
+    if (n <= 0 || numKeys < 0 || (long long)numKeys + 1 > PTK_GB_MAX_KEYS) return;
+    PTK_GB_SCRATCH S = ptk_gb_scratch(a[4], n, numKeys);
+    int32_t *tmp = (int32_t *)malloc(sizeof(int32_t) * PTK_GB_CHUNK);
+    for (long long c = 0; c < S.chunks; ++c) {
+        const long long r0 = c * PTK_GB_CHUNK, r1 = r0 + PTK_GB_CHUNK < n ? r0 + PTK_GB_CHUNK : n;
+        unsigned err = 0, m = 0;
+        for (long long r = r0; r < r1; ++r) {
+            if (keys[r] < 0 || keys[r] > numKeys) ++err;
+            else tmp[m++] = keys[r];
+        }
+        qsort(tmp, m, sizeof(int32_t), cmp_i32);
+        PTK_GB_ENTRY *e = S.entry + c * PTK_GB_CHUNK;
+        unsigned d = 0;
+        for (unsigned i = 0; i < m; ++i)
+            if (i == 0 || tmp[i] != tmp[i - 1]) {
+                e[d].key = tmp[i];
+                e[d].count = 0;
+                e[d].sum = 0;
+                ++d;
+            }
+        for (long long r = r0; r < r1; ++r) {  /* add each row to its entry (binary search) */
+            const int32_t k = keys[r];
+            if (k < 0 || k > numKeys) continue;
+            unsigned lo = 0, hi = d;
+            while (hi - lo > 1) {
+                const unsigned mid = (lo + hi) / 2;
+                if (e[mid].key <= k) lo = mid;
+                else hi = mid;
+            }
+            e[lo].count += 1;
+            e[lo].sum = (long long)((uint64_t)e[lo].sum + (uint64_t)(int64_t)vals[r]);
+        }
+        unsigned *boff = S.boff + c * (S.buckets + 1);
+        unsigned i = 0;
+        for (long long b = 0; b <= S.buckets; ++b) {  /* entries of buckets < b */
+            while (i < d && e[i].key / PTK_GB_BUCKET < b) ++i;
+            boff[b] = i;
+        }
+        S.berr[c] = err;
+    }
+    free(tmp);
+}
+
+void GroupbyMerge(unsigned int n_, void **a) {
+    (void)n_;
+    const long long n = IARG(a, 1);
+    const int numKeys = IARG(a, 2);
+    int64_t *table = (int64_t *)a[3];
+    if (n < 0 || numKeys < 0 || (long long)numKeys + 1 > PTK_GB_MAX_KEYS) return;
+    PTK_GB_SCRATCH S = ptk_gb_scratch(a[0], n, numKeys);
+    uint64_t *cnt = (uint64_t *)table, *sum = cnt + (numKeys + 1), *err = cnt + 3 * ((size_t)numKeys + 1);
+    memset(cnt, 0, sizeof(uint64_t) * 2 * ((size_t)numKeys + 1));
+    uint64_t errors = 0;
+    for (long long c = 0; c < S.chunks; ++c) {
+        const unsigned d = S.boff[c * (S.buckets + 1) + S.buckets];
+        for (unsigned i = 0; i < d; ++i) {
+            const long long at = c * PTK_GB_CHUNK + i;
+            PTK_GB_ENTRY *e = S.entry + at;
+            sum[e->key] += (uint64_t)e->sum;
+            e->sum = (long long)cnt[e->key];  /* the entry's base replaces its sum */
+            cnt[e->key] += e->count;
+        }
+        errors += S.berr[c];
+    }
+    *err = errors;
+    for (long long b = 0; b < S.buckets; ++b) {
+        uint64_t rows = 0, nz = 0;
+        for (long long k = b * PTK_GB_BUCKET; k < (b + 1) * PTK_GB_BUCKET && k <= numKeys; ++k) {
+            rows += cnt[k];
+            nz += cnt[k] ? 1 : 0;
+        }
+        S.btot[2 * b] = (int64_t)rows;
+        S.btot[2 * b + 1] = (int64_t)nz;
+    }
 }
 
 void GroupbyOffsets(unsigned int n, void **a) {
     (void)n;
-    gb_offsets((const int64_t *)a[0], IARG(a, 1), (int64_t *)a[2], (int64_t *)a[3]);
+    gb_offsets((const int64_t *)a[0], IARG(a, 1), (int64_t *)a[3], (int64_t *)a[4]);
 }
 
-void GroupbyReshuffle(unsigned int n, void **a) {
-    (void)n;
-    gb_reshuffle((const int32_t *)a[0], (const int32_t *)a[1], IARG(a, 2), IARG(a, 3),
-                 (const int64_t *)a[4], (int64_t *)a[5], (int32_t *)a[6], (int32_t *)a[7]);
+void GroupbyReshuffle(unsigned int n_, void **a) {
+    (void)n_;
+    const int32_t *keys = (const int32_t *)a[0], *vals = (const int32_t *)a[1];
+    const long long n = IARG(a, 2);
+    const int numKeys = IARG(a, 3);
+    const int64_t *offsets = (const int64_t *)a[4];
+    int32_t *outKeys = (int32_t *)a[6], *outVals = (int32_t *)a[7];
+    if (n <= 0 || numKeys < 0 || (long long)numKeys + 1 > PTK_GB_MAX_KEYS) return;
+    PTK_GB_SCRATCH S = ptk_gb_scratch(a[5], n, numKeys);
+    int64_t *next = (int64_t *)malloc(sizeof(int64_t) * PTK_GB_CHUNK);
+    for (long long c = 0; c < S.chunks; ++c) {
+        const long long r0 = c * PTK_GB_CHUNK, r1 = r0 + PTK_GB_CHUNK < n ? r0 + PTK_GB_CHUNK : n;
+        const PTK_GB_ENTRY *e = S.entry + c * PTK_GB_CHUNK;
+        const unsigned d = S.boff[c * (S.buckets + 1) + S.buckets];
+        for (unsigned i = 0; i < d; ++i) next[i] = offsets[e[i].key] + e[i].sum;  /* sum holds the base */
+        for (long long r = r0; r < r1; ++r) {
+            const int32_t k = keys[r];
+            if (k < 0 || k > numKeys || d == 0) continue;
+            unsigned lo = 0, hi = d;
+            while (hi - lo > 1) {
+                const unsigned mid = (lo + hi) / 2;
+                if (e[mid].key <= k) lo = mid;
+                else hi = mid;
+            }
+            const int64_t pos = next[lo]++;
+            outKeys[pos] = k;
+            outVals[pos] = vals[r];
+        }
+    }
+    free(next);
 }

@@ -54,9 +54,9 @@ void ptk_oracle_pca_gs(const double *X, int M, int N, int K, int J, int center, 
 
 /* Host-task ABI forms (void f(UINT, void**)), loaded by name by the HostAccelerator. */
 void pfxsum(unsigned int n, void **a);
-void GroupbyClear(unsigned int n, void **a);
 void GroupbyAccumulate(unsigned int n, void **a);
 void GroupbyOffsets(unsigned int n, void **a);
+void GroupbyMerge(unsigned int n, void **a);
 void GroupbyReshuffle(unsigned int n, void **a);
 void HxComputation(unsigned int n, void **a);
 void HyComputation(unsigned int n, void **a);

@@ -2,11 +2,11 @@
 // ptaskUnitTest/graphgroupby.cpp and ptaskUnitTest/cpfxsum.cpp), against ptaskapi.h only.
 //
 // --workload groupby: four tasks per run (graphgroupby.cpp builds the same stages:
-//   allocate/count/prefix-sum/reshuffle, :106-387):
-//     GroupbyClear      (pacing: keys; numKeys)                -> table
-//     GroupbyAccumulate (keys, vals, n, numKeys, table in/out) -> table
-//     GroupbyOffsets    (table, numKeys)                       -> offsets, groups
-//     GroupbyReshuffle  (keys, vals, n, numKeys, offsets, table in/out) -> outKeys, outVals
+//   allocate/count/prefix-sum/reshuffle, :106-387; the layouts are in include/ptk_groupby.h):
+//     GroupbyAccumulate (keys, vals, n, numKeys)                         -> scratch
+//     GroupbyMerge      (scratch in/out, n, numKeys)                     -> scratch, table
+//     GroupbyOffsets    (table, numKeys, scratch)                        -> offsets, groups
+//     GroupbyReshuffle  (keys, vals, n, numKeys, offsets, scratch)       -> outKeys, outVals
 // --workload pfxsum: one task pfxsum(pin, pout, N) wired as cpfxsum.cpp:204-226 wires it.
 // --workload scan2: the two-task large scan PfxScanReduce -> PfxScanApply.
 //
@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "ptaskapi.h"
+#include "ptk_groupby.h"
 #include "ptk_prng.h"
 
 using namespace PTask;
@@ -43,7 +44,7 @@ struct Options {
     int repeat = 1, warmup = 0;
     std::string dump;
     bool profile = false;
-    int gbGrid = 256;  // blocks of the two row-parallel groupby kernels (measured best at 8M rows)
+    int gbGrid = 2048;  // at most this many blocks for the two chunk-parallel groupby kernels
 };
 
 uint64_t Fnv1a(const void *p, size_t n) {
@@ -118,7 +119,8 @@ int main(int argc, char **argv) {
         else if (a == "--gb-grid") o.gbGrid = atoi(next());
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
-    if (o.kernelFile.empty() || o.rows < 0 || o.numKeys < 0 || o.rows > 0x7fffffffLL) {
+    if (o.kernelFile.empty() || o.rows < 0 || o.numKeys < 0 || o.rows > 0x7fffffffLL ||
+        (o.workload == "groupby" && (long long)o.numKeys + 1 > PTK_GB_MAX_KEYS)) {
         fprintf(stderr, "usage: ptk_groupby --kernels <groupby.hsaco|scan.hsaco|hosttasks.so> "
                         "[--workload groupby|pfxsum|scan2] [--rows N] [--keys K] [--dist zipf|uniform] "
                         "[--zipf-s S] [--seed S] [--repeat R] [--warmup W] [--dump file] [--profile]\n");
@@ -157,46 +159,54 @@ int main(int argc, char **argv) {
         DatablockTemplate *tTable = Runtime::GetDatablockTemplate((char *)"gbTable", sizeof(int64_t), (UINT)tableElems, 1, 1);
         DatablockTemplate *tOff = Runtime::GetDatablockTemplate((char *)"gbOffsets", sizeof(int64_t), (UINT)(o.numKeys + 1), 1, 1);
         DatablockTemplate *tGroups = Runtime::GetDatablockTemplate((char *)"gbGroups", sizeof(int64_t), (UINT)groupElems, 1, 1);
-        // GroupbyClear: the keys port only paces one clear per run (no formal index)
-        Port *cIn[2] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"keys(pace)"),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 1)};
-        Port *cOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table", 0)};
-        Task *tc = g->AddTask(K("GroupbyClear"), 2, cIn, 1, cOut, "GroupbyClear");
-        geometry(tc, 256, 256);
-        Port *aIn[5] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"keys", 0),
+        const long long scratchBytes = ptk_gb_scratch_bytes(n, o.numKeys);
+        DatablockTemplate *tScratch = Runtime::GetDatablockTemplate((char *)"gbScratch", 16, (UINT)(scratchBytes / 16), 1, 1);
+        const int chunkGrid = (int)std::max(1LL, std::min((long long)o.gbGrid, ptk_gb_chunks(n)));
+        const int buckets = (int)ptk_gb_buckets(o.numKeys);
+        // GroupbyAccumulate: per-chunk LDS aggregation into the scratch block
+        Port *aIn[4] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"keys", 0),
                         Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"vals", 1),
                         Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", 2),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 3),
-                        Runtime::CreatePort(INPUT_PORT, tTable, uid++, (char *)"table", 4, 0)};
-        Port *aOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table(out)", 4)};
-        Task *ta = g->AddTask(K("GroupbyAccumulate"), 5, aIn, 1, aOut, "GroupbyAccumulate");
-        geometry(ta, o.gbGrid, 1024);
-        Port *oIn[2] = {Runtime::CreatePort(INPUT_PORT, tTable, uid++, (char *)"table", 0),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 1)};
-        Port *oOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tOff, uid++, (char *)"offsets", 2),
-                         Runtime::CreatePort(OUTPUT_PORT, tGroups, uid++, (char *)"groups", 3)};
-        Task *to = g->AddTask(K("GroupbyOffsets"), 2, oIn, 2, oOut, "GroupbyOffsets");
-        geometry(to, 1, 1024);
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 3)};
+        Port *aOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch", 4)};
+        Task *ta = g->AddTask(K("GroupbyAccumulate"), 4, aIn, 1, aOut, "GroupbyAccumulate");
+        geometry(ta, chunkGrid, 512);
+        // GroupbyMerge: one block per key bucket; scratch in/out, table out (every count, sum and
+        // the error count are written)
+        Port *mIn[3] = {Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 0, 0),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", 1),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 2)};
+        Port *mOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch(out)", 0),
+                         Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table", 3)};
+        Task *tm = g->AddTask(K("GroupbyMerge"), 3, mIn, 2, mOut, "GroupbyMerge");
+        geometry(tm, buckets, 1024);
+        Port *oIn[3] = {Runtime::CreatePort(INPUT_PORT, tTable, uid++, (char *)"table", 0),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 1),
+                        Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 2)};
+        Port *oOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tOff, uid++, (char *)"offsets", 3),
+                         Runtime::CreatePort(OUTPUT_PORT, tGroups, uid++, (char *)"groups", 4)};
+        Task *to = g->AddTask(K("GroupbyOffsets"), 3, oIn, 2, oOut, "GroupbyOffsets");
+        geometry(to, buckets, PTK_GB_BUCKET);
         Port *rIn[6] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"keys", 0),
                         Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"vals", 1),
                         Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", 2),
                         Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 3),
                         Runtime::CreatePort(INPUT_PORT, tOff, uid++, (char *)"offsets", 4),
-                        Runtime::CreatePort(INPUT_PORT, tTable, uid++, (char *)"table", 5, 2)};
-        Port *rOut[3] = {Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outKeys", 6),
-                         Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outVals", 7),
-                         Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table(cursor)", 5)};
-        Task *tr = g->AddTask(K("GroupbyReshuffle"), 6, rIn, 3, rOut, "GroupbyReshuffle");
-        geometry(tr, o.gbGrid, 1024);
-        keyIn = {g->AddInputChannel(cIn[0], "keys->clear"), g->AddInputChannel(aIn[0], "keys->acc"),
-                 g->AddInputChannel(rIn[0], "keys->shuffle")};
+                        Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 5)};
+        Port *rOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outKeys", 6),
+                         Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outVals", 7)};
+        Task *tr = g->AddTask(K("GroupbyReshuffle"), 6, rIn, 2, rOut, "GroupbyReshuffle");
+        geometry(tr, chunkGrid, 512);
+        keyIn = {g->AddInputChannel(aIn[0], "keys->acc"), g->AddInputChannel(rIn[0], "keys->shuffle")};
         valIn = {g->AddInputChannel(aIn[1], "vals->acc"), g->AddInputChannel(rIn[1], "vals->shuffle")};
-        nIn = {g->AddInputChannel(aIn[2], "n->acc"), g->AddInputChannel(rIn[2], "n->shuffle")};
-        kIn = {g->AddInputChannel(cIn[1], "nk->clear"), g->AddInputChannel(aIn[3], "nk->acc"),
+        nIn = {g->AddInputChannel(aIn[2], "n->acc"), g->AddInputChannel(mIn[1], "n->merge"),
+               g->AddInputChannel(rIn[2], "n->shuffle")};
+        kIn = {g->AddInputChannel(aIn[3], "nk->acc"), g->AddInputChannel(mIn[2], "nk->merge"),
                g->AddInputChannel(oIn[1], "nk->offsets"), g->AddInputChannel(rIn[3], "nk->shuffle")};
-        g->AddInternalChannel(cOut[0], aIn[4], "clear->acc");
-        g->AddInternalChannel(aOut[0], oIn[0], "acc->offsets");
-        g->AddInternalChannel(aOut[0], rIn[5], "acc->shuffle");
+        g->AddInternalChannel(aOut[0], mIn[0], "acc->merge");
+        g->AddInternalChannel(mOut[1], oIn[0], "merge->offsets(table)");
+        g->AddInternalChannel(mOut[0], oIn[2], "merge->offsets(scratch)");
+        g->AddInternalChannel(mOut[0], rIn[5], "merge->shuffle(scratch)");
         g->AddInternalChannel(oOut[0], rIn[4], "offsets->shuffle");
         outs = {g->AddOutputChannel(oOut[1], "groups"), g->AddOutputChannel(rOut[0], "outKeys"),
                 g->AddOutputChannel(rOut[1], "outVals")};

@@ -146,11 +146,12 @@ struct ptk_batch {
     ptk_hip_stream stream = nullptr;
     ptk_hip_event evStart = nullptr, evStop = nullptr;
     ptk_hip_module mBatch = nullptr, mFdtd = nullptr, mGb = nullptr, mGmm = nullptr, mUtil = nullptr;
-    ptk_hip_function fH = nullptr, fE = nullptr, fAdv = nullptr, fGbClear = nullptr, fGbAcc = nullptr,
-                     fGbOff = nullptr, fGbRes = nullptr, fGmmAcc = nullptr, fGmmFin = nullptr, fInitFdtd = nullptr,
+    ptk_hip_function fH = nullptr, fE = nullptr, fAdv = nullptr, fGbAcc = nullptr,
+                     fGbMerge = nullptr, fGbOff = nullptr, fGbRes = nullptr, fGmmAcc = nullptr, fGmmFin = nullptr, fInitFdtd = nullptr,
                      fInitGb = nullptr, fInitGmm = nullptr, fInitGmmModel = nullptr, fSumWords = nullptr,
                      fSumPairs = nullptr, fSumGroups = nullptr, fSumJobs = nullptr, fFill = nullptr;
-    ptk_hip_function fSix[6] = {}, fPGbClear = nullptr, fPGbAcc = nullptr, fPGbOff = nullptr, fPGbRes = nullptr,
+    ptk_hip_function fSix[6] = {}, fPGbAcc = nullptr, fPGbMerge = nullptr, fPGbOff = nullptr,
+                     fPGbRes = nullptr,
                      fPGmmAcc = nullptr, fPGmmFin = nullptr;
     void *arenaBase = nullptr;
     Arena arena;
@@ -162,8 +163,8 @@ struct ptk_batch {
     unsigned long long *dSums = nullptr, *hSums = nullptr;  // checksum staging, 8 per retiring instance
     size_t sumCap = 0;
     List<int4> tiles;
-    List<int> fdtdActive, gbInsts;
-    List<int2> gbUnits, gmmAcc, gmmFin;
+    List<int> fdtdActive;
+    List<int2> gbUnits, gbBuckets, gmmAcc, gmmFin;
     List<PTK_CSUM_JOB> csumJobs;
     double *dCdf = nullptr;
     int cdfKeys = -1;
@@ -225,7 +226,8 @@ unsigned long long FdtdBytes(int nx, int ny, int nz) {
 
 unsigned long long GbBytes(long long rows, int numKeys) {
     const unsigned long long K = (unsigned long long)numKeys + 1;
-    return 4 * Up(4ull * rows) + Up(8 * (3 * K + 1)) + Up(8 * K) + Up(8 * (3 + 3 * K));
+    return 4 * Up(4ull * rows) + Up(8 * (3 * K + 1)) + Up(8 * K) + Up(8 * (3 + 3 * K)) +
+           Up((unsigned long long)ptk_gb_scratch_bytes(rows, numKeys));
 }
 
 GMM_DIMS GmmDims(int n, int d, int k, int steps) {
@@ -332,6 +334,7 @@ int Admit(ptk_batch *B, Inst &in) {
         d.table = (long long *)carve(8 * (3 * K + 1));
         d.offsets = (long long *)carve(8 * K);
         d.groups = (long long *)carve(8 * (3 + 3 * K));
+        d.scratch = carve((unsigned long long)ptk_gb_scratch_bytes(in.a, in.b));
         HIPCHK(ptk_hip_memcpy_htod_async(B->dGb + in.slot, &d, sizeof(d), B->stream));
         long long A = 0;
         const double *cdf = nullptr;
@@ -399,7 +402,7 @@ int BuildLists(ptk_batch *B) {
             ++nf;
         } else if (in.kind == PTK_INST_GROUPBY) {
             ngu += (size_t)((in.a + in.gb.chunkRows - 1) / in.gb.chunkRows);
-            ++ngi;
+            ngi += (size_t)ptk_gb_buckets(in.b);
         } else {
             nga += (size_t)in.gm.d.blocks;
             ngf += (size_t)in.gm.d.K + 1;
@@ -408,10 +411,10 @@ int BuildLists(ptk_batch *B) {
     if (int r = Reserve(B, B->tiles, nt)) return r;
     if (int r = Reserve(B, B->fdtdActive, nf)) return r;
     if (int r = Reserve(B, B->gbUnits, ngu)) return r;
-    if (int r = Reserve(B, B->gbInsts, ngi)) return r;
+    if (int r = Reserve(B, B->gbBuckets, ngi)) return r;
     if (int r = Reserve(B, B->gmmAcc, nga)) return r;
     if (int r = Reserve(B, B->gmmFin, ngf)) return r;
-    B->tiles.n = B->fdtdActive.n = B->gbUnits.n = B->gbInsts.n = B->gmmAcc.n = B->gmmFin.n = 0;
+    B->tiles.n = B->fdtdActive.n = B->gbUnits.n = B->gbBuckets.n = B->gmmAcc.n = B->gmmFin.n = 0;
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
         if (in.kind == PTK_INST_FDTD) {
@@ -425,7 +428,7 @@ int BuildLists(ptk_batch *B) {
         } else if (in.kind == PTK_INST_GROUPBY) {
             const long long chunks = (in.a + in.gb.chunkRows - 1) / in.gb.chunkRows;
             for (long long c = 0; c < chunks; ++c) B->gbUnits.host[B->gbUnits.n++] = int2{in.slot, (int)c};
-            B->gbInsts.host[B->gbInsts.n++] = in.slot;
+            for (long long q = 0; q < ptk_gb_buckets(in.b); ++q) B->gbBuckets.host[B->gbBuckets.n++] = int2{in.slot, (int)q};
         } else {
             for (int b = 0; b < in.gm.d.blocks; ++b) B->gmmAcc.host[B->gmmAcc.n++] = int2{in.slot, b};
             for (int k = 0; k <= in.gm.d.K; ++k) B->gmmFin.host[B->gmmFin.n++] = int2{in.slot, k};
@@ -434,7 +437,7 @@ int BuildLists(ptk_batch *B) {
     B->hCounts[0] = (int)B->tiles.n;
     B->hCounts[1] = (int)B->fdtdActive.n;
     B->hCounts[2] = (int)B->gbUnits.n;
-    B->hCounts[3] = (int)B->gbInsts.n;
+    B->hCounts[3] = (int)B->gbBuckets.n;
     B->hCounts[4] = (int)B->gmmAcc.n;
     B->hCounts[5] = (int)B->gmmFin.n;
     auto up = [&](auto &l) -> int {
@@ -444,7 +447,7 @@ int BuildLists(ptk_batch *B) {
     if (int r = up(B->tiles)) return r;
     if (int r = up(B->fdtdActive)) return r;
     if (int r = up(B->gbUnits)) return r;
-    if (int r = up(B->gbInsts)) return r;
+    if (int r = up(B->gbBuckets)) return r;
     if (int r = up(B->gmmAcc)) return r;
     if (int r = up(B->gmmFin)) return r;
     HIPCHK(ptk_hip_memcpy_htod_async(B->dCounts, B->hCounts, 8 * sizeof(int), B->stream));
@@ -480,15 +483,15 @@ int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance, bool 
         if (int r = B->Launch(B->fGmmFin, grid(512, hc[5]), 1, 1, 256, f)) return r;
     }
     if (gb) {
-        Args c, a, o, s;
-        c.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
+        Args a, m, o, s;
         a.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
-        o.ptr(B->gbInsts.dev).ptr(B->dCounts + 3).ptr(B->dGb);
+        m.ptr(B->gbBuckets.dev).ptr(B->dCounts + 3).ptr(B->dGb);
+        o.ptr(B->gbBuckets.dev).ptr(B->dCounts + 3).ptr(B->dGb);
         s.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
-        if (int r = B->Launch(B->fGbClear, grid(512, hc[2]), 1, 1, 1024, c)) return r;
-        if (int r = B->Launch(B->fGbAcc, grid(512, hc[2]), 1, 1, 1024, a)) return r;
-        if (int r = B->Launch(B->fGbOff, grid(512, hc[3]), 1, 1, 1024, o)) return r;
-        if (int r = B->Launch(B->fGbRes, grid(512, hc[2]), 1, 1, 1024, s)) return r;
+        if (int r = B->Launch(B->fGbAcc, grid(1024, hc[2]), 1, 1, 512, a)) return r;
+        if (int r = B->Launch(B->fGbMerge, grid(2048, hc[3]), 1, 1, 1024, m)) return r;
+        if (int r = B->Launch(B->fGbOff, grid(2048, hc[3]), 1, 1, PTK_GB_BUCKET, o)) return r;
+        if (int r = B->Launch(B->fGbRes, grid(1024, hc[2]), 1, 1, 512, s)) return r;
     }
     return PTK_BATCH_OK;
 }
@@ -511,16 +514,17 @@ int IssuePlain(ptk_batch *B, const Inst &in) {
         }
     } else if (in.kind == PTK_INST_GROUPBY) {
         const PTK_GROUPBY_INST &d = in.gb;
-        const unsigned grid = (unsigned)std::max(1LL, std::min(256LL, (d.n + 32767) / 32768));
-        Args c, a, o, s;
-        c.ptr(d.table).add<int>(d.numKeys);
-        a.ptr(d.keys).ptr(d.vals).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.table);
-        o.ptr(d.table).add<int>(d.numKeys).ptr(d.offsets).ptr(d.groups);
-        s.ptr(d.keys).ptr(d.vals).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.offsets).ptr(d.table).ptr(d.outKeys).ptr(d.outVals);
-        if (int r = B->Launch(B->fPGbClear, 64, 1, 1, 1024, c)) return r;
-        if (int r = B->Launch(B->fPGbAcc, grid, 1, 1, 1024, a)) return r;
-        if (int r = B->Launch(B->fPGbOff, 1, 1, 1, 1024, o)) return r;
-        if (int r = B->Launch(B->fPGbRes, grid, 1, 1, 1024, s)) return r;
+        const unsigned grid = (unsigned)std::max(1LL, std::min(2048LL, ptk_gb_chunks(d.n)));
+        const unsigned buckets = (unsigned)ptk_gb_buckets(d.numKeys);
+        Args a, m, o, s;
+        a.ptr(d.keys).ptr(d.vals).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.scratch);
+        m.ptr(d.scratch).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.table);
+        o.ptr(d.table).add<int>(d.numKeys).ptr(d.scratch).ptr(d.offsets).ptr(d.groups);
+        s.ptr(d.keys).ptr(d.vals).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.offsets).ptr(d.scratch).ptr(d.outKeys).ptr(d.outVals);
+        if (int r = B->Launch(B->fPGbAcc, grid, 1, 1, 512, a)) return r;
+        if (int r = B->Launch(B->fPGbMerge, buckets, 1, 1, 1024, m)) return r;
+        if (int r = B->Launch(B->fPGbOff, buckets, 1, 1, PTK_GB_BUCKET, o)) return r;
+        if (int r = B->Launch(B->fPGbRes, grid, 1, 1, 512, s)) return r;
     } else {
         const PTK_GMM_INST &d = in.gm;
         Args a, f;
@@ -625,7 +629,7 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
             return fail(Fail(PTK_BATCH_EHIP, std::string("load ") + m.second + ": " + ptk_hip_last_error_string()));
     const std::pair<ptk_hip_function *, const char *> bf[] = {
         {&B->fH, "BatchFdtdH"}, {&B->fE, "BatchFdtdE"}, {&B->fAdv, "BatchFdtdAdvance"},
-        {&B->fGbClear, "BatchGroupbyClear"}, {&B->fGbAcc, "BatchGroupbyAccumulate"},
+        {&B->fGbAcc, "BatchGroupbyAccumulate"}, {&B->fGbMerge, "BatchGroupbyMerge"},
         {&B->fGbOff, "BatchGroupbyOffsets"}, {&B->fGbRes, "BatchGroupbyReshuffle"},
         {&B->fGmmAcc, "BatchGmmAccumulate"}, {&B->fGmmFin, "BatchGmmFinalize"},
         {&B->fInitFdtd, "BatchInitFdtd"}, {&B->fInitGb, "BatchInitGroupby"}, {&B->fInitGmm, "BatchInitGmm"},
@@ -638,7 +642,8 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
                                  "ExComputation", "EyComputation", "EzComputation"};
     for (int t = 0; t < 6; ++t)
         if ((r = Load(B->mFdtd, six[t], &B->fSix[t]))) return fail(r);
-    if ((r = Load(B->mGb, "GroupbyClear", &B->fPGbClear)) || (r = Load(B->mGb, "GroupbyAccumulate", &B->fPGbAcc)) ||
+    if ((r = Load(B->mGb, "GroupbyAccumulate", &B->fPGbAcc)) ||
+        (r = Load(B->mGb, "GroupbyMerge", &B->fPGbMerge)) ||
         (r = Load(B->mGb, "GroupbyOffsets", &B->fPGbOff)) || (r = Load(B->mGb, "GroupbyReshuffle", &B->fPGbRes)) ||
         (r = Load(B->mGmm, "GmmAccumulate", &B->fPGmmAcc)) || (r = Load(B->mGmm, "GmmFinalize", &B->fPGmmFin)) ||
         (r = Load(B->mUtil, "PtkFillWords", &B->fFill)))
@@ -669,7 +674,7 @@ int ptk_batch_destroy(ptk_batch *B) {
     };
     freeList(B->tiles);
     freeList(B->fdtdActive);
-    freeList(B->gbInsts);
+    freeList(B->gbBuckets);
     freeList(B->gbUnits);
     freeList(B->gmmAcc);
     freeList(B->gmmFin);
@@ -710,7 +715,7 @@ int ptk_batch_submit_fdtd(ptk_batch *B, int nx, int ny, int nz, int steps, unsig
 
 int ptk_batch_submit_groupby(ptk_batch *B, long long rows, int num_keys, double zipf_s, unsigned long long seed,
                              int keep, int *id) {
-    if (!B || rows < 1 || rows > 0x7fffffffLL || num_keys < 0 || num_keys >= 0x7fffffff)
+    if (!B || rows < 1 || rows > 0x7fffffffLL || num_keys < 0 || (long long)num_keys + 1 > PTK_GB_MAX_KEYS)
         return Fail(PTK_BATCH_EINVAL, "bad groupby shape");
     Inst in;
     in.kind = PTK_INST_GROUPBY;
@@ -900,7 +905,8 @@ int ptk_batch_release(ptk_batch *B, int id) {
 }
 
 int ptk_batch_set_tiles(ptk_batch *B, int cols, int depth, int gbRows) {
-    if (!B || cols < 1 || depth < 1 || depth > 65535 || gbRows < 1024) return Fail(PTK_BATCH_EINVAL, "bad tile shape");
+    if (!B || cols < 1 || depth < 1 || depth > 65535 || gbRows < PTK_GB_CHUNK || gbRows % PTK_GB_CHUNK)
+        return Fail(PTK_BATCH_EINVAL, "bad tile shape");
     B->fdtdCols = cols;
     B->fdtdDepth = depth;
     B->gbRows = gbRows;

@@ -13,8 +13,8 @@
 //                             the columns [col0, col0+ncols) and planes [k0, k1) of an instance;
 //                             set (done & 1) is current, the other set receives the new fields
 //   BatchFdtdAdvance          active FDTD instances: done += 1
-//   BatchGroupbyClear/Accumulate/Reshuffle   units {inst, chunk}: table slice / row chunk
-//   BatchGroupbyOffsets       instances: one block each
+//   BatchGroupbyAccumulate/Reshuffle          units {inst, unit}: chunkRows rows
+//   BatchGroupbyMerge/Offsets                 units {inst, key bucket}
 //   BatchGmmAccumulate        units {inst, partial row b}
 //   BatchGmmFinalize          units {inst, component k (K = log-likelihood block)}
 //   BatchInit*                device-side input generation at admission
@@ -93,47 +93,62 @@ __device__ inline long long gb_chunks(const PTK_GROUPBY_INST &I) {
     return (I.n + I.chunkRows - 1) / I.chunkRows;
 }
 
-extern "C" __global__ void __launch_bounds__(1024)
-BatchGroupbyClear(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
-    const int n = *count;
-    for (int u = blockIdx.x; u < n; u += gridDim.x) {
-        const int2 w = units[u];
-        const PTK_GROUPBY_INST &I = insts[w.x];
-        const long long total = 3ll * (I.numKeys + 1) + 1, chunks = gb_chunks(I);
-        const long long per = (total + chunks - 1) / chunks;
-        const long long lo = per * w.y, hi = lo + per < total ? lo + per : total;
-        for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) I.table[i] = 0;
-    }
-}
-
+// A unit {inst, u} covers the chunks of rows [u * chunkRows, (u + 1) * chunkRows).
 extern "C" __global__ void __launch_bounds__(1024)
 BatchGroupbyAccumulate(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ GbChunkLds L;
     const int n = *count;
     for (int u = blockIdx.x; u < n; u += gridDim.x) {
         const int2 w = units[u];
         const PTK_GROUPBY_INST &I = insts[w.x];
-        const long long r0 = (long long)w.y * I.chunkRows, r1 = r0 + I.chunkRows < I.n ? r0 + I.chunkRows : I.n;
-        gb_accumulate_rows(I.keys, I.vals, r0, r1, I.numKeys, I.table);
+        if (!gb_keys_ok(I.numKeys) || I.n <= 0) continue;
+        const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
+        const long long per = I.chunkRows / PTK_GB_CHUNK, c0 = (long long)w.y * per;
+        for (long long c = c0; c < c0 + per && c < S.chunks; ++c) gb_accumulate_chunk(I.keys, I.vals, I.n, I.numKeys, c, S, L);
     }
 }
 
+// Units {inst, bucket}, taken in an XCD-contiguous order (neighbouring buckets share lines).
 extern "C" __global__ void __launch_bounds__(1024)
-BatchGroupbyOffsets(const int *__restrict__ list, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+BatchGroupbyMerge(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ GbMergeLds L;
+    const int n = *count;
+    for (int u = (int)gb_xcd_block(blockIdx.x, gridDim.x); u < n; u += gridDim.x) {
+        const int2 w = units[u];
+        const PTK_GROUPBY_INST &I = insts[w.x];
+        if (!gb_keys_ok(I.numKeys)) continue;
+        const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
+        if (w.y < S.buckets) gb_merge_bucket(S, I.numKeys, w.y, I.table, L);
+    }
+}
+
+// Units {inst, bucket}; blocks of PTK_GB_BUCKET threads.
+extern "C" __global__ void __launch_bounds__(PTK_GB_BUCKET)
+BatchGroupbyOffsets(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ GbOffsetsLds L;
+    if (blockDim.x != PTK_GB_BUCKET) return;
     const int n = *count;
     for (int u = blockIdx.x; u < n; u += gridDim.x) {
-        const PTK_GROUPBY_INST &I = insts[list[u]];
-        gb_offsets_block(I.table, I.numKeys, I.offsets, I.groups);
+        const int2 w = units[u];
+        const PTK_GROUPBY_INST &I = insts[w.x];
+        if (!gb_keys_ok(I.numKeys)) continue;
+        const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
+        if (w.y < S.buckets) gb_offsets_bucket(I.table, I.numKeys, S, w.y, I.offsets, I.groups, L);
     }
 }
 
 extern "C" __global__ void __launch_bounds__(1024)
 BatchGroupbyReshuffle(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ GbPlaceLds L;
     const int n = *count;
     for (int u = blockIdx.x; u < n; u += gridDim.x) {
         const int2 w = units[u];
         const PTK_GROUPBY_INST &I = insts[w.x];
-        const long long r0 = (long long)w.y * I.chunkRows, r1 = r0 + I.chunkRows < I.n ? r0 + I.chunkRows : I.n;
-        gb_reshuffle_rows(I.keys, I.vals, I.n, r0, r1, I.numKeys, I.offsets, I.table, I.outKeys, I.outVals);
+        if (!gb_keys_ok(I.numKeys) || I.n <= 0) continue;
+        const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
+        const long long per = I.chunkRows / PTK_GB_CHUNK, c0 = (long long)w.y * per;
+        for (long long c = c0; c < c0 + per && c < S.chunks; ++c)
+            gb_place_chunk(I.keys, I.vals, I.n, I.numKeys, c, I.offsets, S, I.outKeys, I.outVals, L);
     }
 }
 

@@ -2,7 +2,7 @@
 include/ptk_batch.h declares, and computes instance footprints (no device calls)."""
 import numpy as np
 
-from ptask_amd import batch, fdtd, gmm
+from ptask_amd import batch, fdtd, gmm, groupby
 
 
 def test_library_exports_every_declared_symbol():
@@ -20,7 +20,9 @@ def test_footprints():
     assert batch.footprint(batch.FDTD, nx, ny, nz) == want
     rows, nk = 1000, 99
     K = nk + 1
-    assert batch.footprint(batch.GROUPBY, rows, nk) == 4 * up(4 * rows) + up(8 * (3 * K + 1)) + up(8 * K) + up(8 * (3 + 3 * K))
+    assert groupby.scratch_bytes(rows, nk) == 32816  # btot 16, berr 16, boff 16, entries 32768
+    assert batch.footprint(batch.GROUPBY, rows, nk) == (4 * up(4 * rows) + up(8 * (3 * K + 1)) + up(8 * K) +
+                                                        up(8 * (3 + 3 * K)) + up(groupby.scratch_bytes(rows, nk)))
     n, d, k, steps = 5000, 7, 5, 9
     blocks = max(1, (-(-n // 128) + 15) // 16)
     model = gmm.model_elems(d, k, steps)

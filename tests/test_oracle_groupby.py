@@ -134,3 +134,42 @@ def test_host_pfxsum_graph_equals_oracle(tmp_path):
         raw = np.fromfile(d, dtype=np.int32)
         keys, out = raw[:n], raw[2 * n:3 * n]
         assert np.array_equal(out, oracle.pfxsum(keys)), n
+
+
+@pytest.mark.parametrize("n,nk", [(1, 0), (2049, 513), (10_000, 1000), (6000, 255)])
+def test_host_task_forms_pipeline(n, nk):
+    """The host-task forms of the four groupby tasks (the chunk/bucket scratch layout of
+    include/ptk_groupby.h restated on the host) chained as the graph chains them, with invalid
+    keys, equal the oracle's pipeline; with chunks merged in order the values come out in input
+    order, so every output is exact."""
+    import ctypes as C
+
+    from ptask_amd import groupby as GB
+    L = C.CDLL(HOSTLIB)
+    rng = np.random.default_rng(n)
+    K = nk + 1
+    keys = rng.integers(0, K, n).astype(np.int32)
+    bad = rng.random(n) < 0.05
+    keys[bad] = rng.choice([-1, K, 2**31 - 1], bad.sum()).astype(np.int32)
+    vals = rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32)
+    table = np.full(3 * K + 1, 0x5A5A5A5A5A5A5A5A, np.int64)  # merge writes what it owns
+    scratch = np.zeros(GB.scratch_bytes(n, nk) // 8 + 2, np.int64)
+    offsets = np.zeros(K, np.int64)
+    groups = np.zeros(3 + 3 * K, np.int64)
+    ok, ov = np.zeros(n, np.int32), np.zeros(n, np.int32)
+
+    def call(name, *args):
+        arr = (C.c_void_p * len(args))(*[C.c_void_p(a) for a in args])
+        getattr(L, name)(C.c_uint(len(args)), arr)
+
+    p = lambda a: a.ctypes.data
+    call("GroupbyAccumulate", p(keys), p(vals), n, nk, p(scratch))
+    call("GroupbyMerge", p(scratch), n, nk, p(table))
+    call("GroupbyOffsets", p(table), nk, p(scratch), p(offsets), p(groups))
+    call("GroupbyReshuffle", p(keys), p(vals), n, nk, p(offsets), p(scratch), p(ok), p(ov))
+    g, head, eoff, eok, eov = oracle.groupby(keys, vals, nk)
+    assert np.array_equal(groups[:3], head) and head[2] == bad.sum()
+    assert np.array_equal(groups[3:3 + g.size], g.reshape(-1))
+    assert np.array_equal(offsets, eoff)
+    rows = int(head[1])
+    assert np.array_equal(ok[:rows], eok[:rows]) and np.array_equal(ov[:rows], eov[:rows])
