@@ -238,7 +238,34 @@ def cpu_baseline(n: int, seconds: float) -> dict:
     del cur, F
     return {"value": round(rate, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
             "sample": f"{n}^3 grid, {steps} time step(s) of the binary32 oracle on one core "
-                      f"({el:.1f} s)"}
+                      f"({el:.1f} s)",
+            "host_graph_64": host_graph_baseline()}
+
+
+def host_graph_baseline(runs: int = 5) -> dict:
+    """The plain CPU method: the looped 64^3 FDTD graph, 10 time steps, on the HostAccelerator
+    with the oracle's host-task forms of the six Yee tasks; median of `runs` runs with one host
+    task at a time and with as many as there are cores (at most 16, the box's CPU share)."""
+    import statistics
+
+    import oracle
+    from ptask_amd import paths
+    lib = os.path.join(oracle.ORACLE_DIR, "_build", "libptk_oracle.so")
+    cores = max(1, min(os.cpu_count() or 1, 16))
+    out = {"workload": "looped FDTD graph 64^3, 10 steps, six host tasks per step", "runs": runs,
+           "unit": "Mcell-updates/s"}
+    for label, conc in (("one_core", 1), ("all_cores", cores)):
+        cmd = [paths.bin_path("ptk_fdtd"), "--kernels", lib, "--n", "64", "--steps", "10",
+               "--repeat", str(runs), "--host-concurrency", str(conc)]
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        if p.returncode != 0:
+            out[label] = {"error": p.stderr[-300:]}
+            continue
+        r = json.loads(p.stdout.strip().splitlines()[-1])
+        med = statistics.median(r["run_ms"])
+        out[label] = {"host_concurrency": conc, "median_ms": round(med, 3),
+                      "value": round(64 ** 3 * 10 / (med / 1e3) / 1e6, 2)}
+    return out
 
 
 def main() -> int:

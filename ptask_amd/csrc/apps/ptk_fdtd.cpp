@@ -64,6 +64,7 @@ struct Options {
     unsigned long long seed = 0x5EED;
     float courant = 0.5f;
     int threads = 0;
+    int hostConcurrency = 0;  // host tasks running at once (0: the runtime default)
     bool singleThreaded = false;
     bool eager = false;
     bool profile = false;
@@ -99,6 +100,7 @@ bool Parse(int argc, char **argv, Options &o) {
         else if (a == "--seed") o.seed = strtoull(next("--seed"), nullptr, 0);
         else if (a == "--courant") o.courant = (float)atof(next("--courant"));
         else if (a == "--threads") o.threads = atoi(next("--threads"));
+        else if (a == "--host-concurrency") o.hostConcurrency = atoi(next("--host-concurrency"));
         else if (a == "--single-threaded") o.singleThreaded = true;
         else if (a == "--eager") o.eager = true;
         else if (a == "--profile") o.profile = true;
@@ -326,6 +328,7 @@ int main(int argc, char **argv) {
     }
     Runtime::SetExitOnRuntimeFailure(FALSE);
     if (o.threads) Runtime::SetTaskThreadPoolSize((UINT)o.threads);
+    if (o.hostConcurrency > 0) Runtime::SetMaximumHostConcurrency(o.hostConcurrency);
     if (o.profile) Runtime::SetProfilePSDispatch(TRUE);
     Runtime::SetTaskStreamPolicy(o.streams == "shared" ? PTK_STREAMS_SHARED : PTK_STREAMS_PER_TASK);
     if (o.mode == "unrolled") Runtime::SetDefaultChannelCapacity(std::max(4, o.blocks));
@@ -484,6 +487,8 @@ int main(int argc, char **argv) {
         printf("%s{\"field\": \"%s\", \"fnv1a\": \"%016llx\", \"sum\": %.17g}", f ? ", " : "", kFieldNames[f],
                (unsigned long long)Fnv1a(final[f].data(), final[f].size() * sizeof(float)), s);
     }
+    printf("], \"run_ms\": [");
+    for (size_t r = 0; r < runMs.size(); ++r) printf("%s%.4f", r ? ", " : "", runMs[r]);
     printf("]}\n");
     fflush(stdout);
     Runtime::Terminate();
