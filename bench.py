@@ -18,6 +18,8 @@ Graph forms measured (all bit-identical to the six reference updates); `value` i
   fused   -- 1 task/step, FdtdStepFused: whole step, halo recomputed by edge lanes (48 B)
 
 The same line also carries, under "workloads", the other graphs of the hot path:
+  captured -- the same 384^3 graph as one instance of the batched executor: plain per-task
+             dispatch of the six reference tasks vs one hipGraph-captured iteration per step
   batched -- 2048 FDTD graphs (edge U{16..96}, 50 steps) + 512 groupby graphs (1M-8M rows, Zipf 1.1
              over 65,536 keys) through the batched executor, plain per-task dispatch vs captured
              batched iterations (instances/s, both on the same instances, result checksums compared)
@@ -54,7 +56,7 @@ def parse():
     ap.add_argument("--forms", default="overlap,halves,six,fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--suite", default="batched,served,gmm,pca",
+    ap.add_argument("--suite", default="captured,batched,served,gmm,pca",
                     help="extra workloads after the FDTD headline (comma list; empty for none)")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--serve-qps", default="20,100,300")
@@ -189,6 +191,34 @@ def leg_served(seconds: float, rates: list) -> dict:
             "latency_kinds": ["fdtd", "gmm", "groupby"], "runs": runs}
 
 
+def leg_captured(n: int, steps: int) -> dict:
+    """The 384^3 FDTD graph as one instance of the batched executor: plain per-task dispatch
+    (the six reference Yee tasks launched one by one, every step) against hipGraph-captured
+    iterations (one replayed graph per step, the whole-step kernel on overlapped tiles).  Device
+    time of the iterations from the executor's events; wall time includes admission (field
+    generation on the device) and retirement (result checksums)."""
+    from ptask_amd import batch
+    out = {"workload": f"FDTD {n}^3 binary32, {steps} time steps, one executor instance",
+           "unit": "Mcell-updates/s"}
+    sums = {}
+    for name, mode in (("plain", batch.PLAIN), ("captured", batch.BATCHED)):
+        with batch.Executor(8 << 30, mode) as ex:
+            t0 = time.perf_counter()
+            i = ex.submit_fdtd(n, n, n, steps, 1, keep=True)
+            ex.drain()
+            wall = time.perf_counter() - t0
+            st = ex.stats()
+            sums[name] = ex.checksums(i)
+            ex_s = st["exec_us"] / 1e6
+            out[name] = {"exec_ms": round(ex_s * 1e3, 3), "value": round(n ** 3 * steps / ex_s / 1e6, 1),
+                         "wall_ms": round(wall * 1e3, 3), "launches": st["launches"],
+                         "graph_launches": st["graph_launches"]}
+        progress(f"captured leg {name}: {out[name]['value']} Mcell-updates/s")
+    out["speedup"] = round(out["captured"]["value"] / out["plain"]["value"], 2)
+    out["checksums_equal"] = sums["plain"] == sums["captured"]
+    return out
+
+
 def leg_gmm() -> dict:
     from ptask_amd import paths
     r = run_app("ptk_gmm", ["--kernels", paths.kernel_path("gmm.hsaco"), "--n", 2000000, "--d", 32,
@@ -312,6 +342,7 @@ def main() -> int:
                                              for r in results.values()}) == 1,
     }
     legs = {"batched": leg_batched, "gmm": leg_gmm, "pca": leg_pca,
+            "captured": lambda: leg_captured(n, T),
             "served": lambda: leg_served(args.serve_seconds, args.serve_qps.split(","))}
     line["workloads"] = {}
     for name in [x for x in args.suite.split(",") if x]:

@@ -118,8 +118,11 @@ int ptk_batch_read(ptk_batch *b, int id, int which, void *host, unsigned long lo
 int ptk_batch_release(ptk_batch *b, int id);
 
 /* Tile shapes of the batched kernels: FDTD columns per tile and k planes per tile, groupby
- * rows per work unit.  ptk_batch_autotune times candidates on the active instances (the timed
- * launches are idempotent, so results are unchanged) and keeps the fastest. */
+ * rows per work unit (a multiple of PTK_GB_CHUNK).  fdtd_cols == 0 (the default, with 8 planes)
+ * selects the whole-step kernel on overlapped 62 x 6-column tiles instead of the two half-step
+ * kernels; instances with (nx+1)(ny+1)(nz+1) >= 2^29 use the half steps.  ptk_batch_autotune
+ * times candidates of both forms on the active instances (the timed launches are idempotent, so
+ * results are unchanged) and keeps the fastest. */
 int ptk_batch_set_tiles(ptk_batch *b, int fdtd_cols, int fdtd_depth, int gb_rows);
 int ptk_batch_get_tiles(ptk_batch *b, int *fdtd_cols, int *fdtd_depth, int *gb_rows);
 int ptk_batch_autotune(ptk_batch *b, double *best_ms);

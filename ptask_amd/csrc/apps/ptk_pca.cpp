@@ -137,7 +137,7 @@ int main(int argc, char **argv) {
     const int rb = (d.ld / 4 + 255) / 256 <= 2 ? 8 : 4;  // rows per slab of the row passes
     const int slabs = d.Mp / rb;
     d.G = o.blocks > 0 ? o.blocks : std::min(slabs, 512);
-    d.GR = o.gr > 0 ? std::min(o.gr, d.G) : std::min(d.G, 32);
+    d.GR = o.gr > 0 ? std::min(o.gr, d.G) : std::min(d.G, 8);  // 8: measured best at 65536 x 2048
 
     std::vector<float> X((size_t)o.m * o.n);
     if (!o.data.empty()) {

@@ -146,7 +146,7 @@ struct ptk_batch {
     ptk_hip_stream stream = nullptr;
     ptk_hip_event evStart = nullptr, evStop = nullptr;
     ptk_hip_module mBatch = nullptr, mFdtd = nullptr, mGb = nullptr, mGmm = nullptr, mUtil = nullptr;
-    ptk_hip_function fH = nullptr, fE = nullptr, fAdv = nullptr, fGbAcc = nullptr,
+    ptk_hip_function fH = nullptr, fE = nullptr, fStep = nullptr, fAdv = nullptr, fGbAcc = nullptr,
                      fGbMerge = nullptr, fGbOff = nullptr, fGbRes = nullptr, fGmmAcc = nullptr, fGmmFin = nullptr, fInitFdtd = nullptr,
                      fInitGb = nullptr, fInitGmm = nullptr, fInitGmmModel = nullptr, fSumWords = nullptr,
                      fSumPairs = nullptr, fSumGroups = nullptr, fSumJobs = nullptr, fFill = nullptr;
@@ -173,8 +173,9 @@ struct ptk_batch {
     ptk_hip_graph graphs[8] = {};  // captured iteration per mix of kinds (bit 0 FDTD, 1 groupby, 2 GMM)
     bool dirty = true;
     bool useGraphs = true;  // batched iterations as captured graphs (else direct, grid-fitted launches)
-    int fdtdCols = 512, fdtdDepth = 16, gbRows = 262144, fdtdGrid = 4096;
+    int fdtdCols = 0, fdtdDepth = 8, gbRows = 262144, fdtdGrid = 4096;
     int curCols = 512, curDepth = 16;  // the shape the current work lists were built with
+    bool curStep = false;              // the current FDTD tiles are whole-step (overlapped) tiles
     std::vector<Inst> insts;
     std::vector<int> queue, active;
     unsigned long long nTicks = 0, nLaunches = 0, nGraphs = 0;
@@ -372,14 +373,32 @@ int Admit(ptk_batch *B, Inst &in) {
 }
 
 // Work lists of the resident instances for the batched kernels.
-// FDTD tiles of the resident instances with the tile shape (cols, depth).
+// Whole-step tiles own 62 x 6 columns (blocks of 64 x 8 threads overlapping by one cell).
+constexpr int kStepTileI = 62, kStepTileJ = 6;
+
+// TRUE when every resident FDTD instance can take the whole-step kernel (32-bit buffer offsets).
+bool StepFits(const ptk_batch *B) {
+    for (int id : B->active) {
+        const Inst &in = B->insts[(size_t)id];
+        if (in.kind == PTK_INST_FDTD && (in.a + 1) * (long long)(in.b + 1) * (in.c + 1) >= (1ll << 29)) return false;
+    }
+    return true;
+}
+
+// FDTD tiles of the resident instances: half-step column tiles (cols, depth), or whole-step
+// tiles of `depth` planes when cols == 0.
 size_t FdtdTiles(const ptk_batch *B, int cols, int depth) {
     size_t nt = 0;
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
         if (in.kind != PTK_INST_FDTD) continue;
-        const long long ncol = (in.a + 1) * (long long)(in.b + 1);
-        nt += (size_t)(((ncol + cols - 1) / cols) * ((in.c + 1 + depth - 1) / depth));
+        const long long slabs = (in.c + 1 + depth - 1) / depth;
+        if (cols == 0) {
+            nt += (size_t)(((in.a + 1 + kStepTileI - 1) / kStepTileI) * ((in.b + 1 + kStepTileJ - 1) / kStepTileJ) * slabs);
+        } else {
+            const long long ncol = (in.a + 1) * (long long)(in.b + 1);
+            nt += (size_t)(((ncol + cols - 1) / cols) * slabs);
+        }
     }
     return nt;
 }
@@ -389,12 +408,16 @@ int BuildLists(ptk_batch *B) {
     // work would give fewer than four tiles per CU: a few small instances must still spread over
     // the whole device.  Tiling never changes results.
     int cols = B->fdtdCols, depth = B->fdtdDepth;
+    if (cols == 0 && !StepFits(B)) cols = 512, depth = 16;  // an instance too large for the whole-step form
     while (FdtdTiles(B, cols, depth) < 1024 && (depth > 2 || cols > 64)) {
         if (depth > 2) depth = (depth + 1) / 2;
+        else if (cols == 0) break;
         else cols /= 2;
     }
+    if (B->curStep != (cols == 0)) DropGraphs(B);  // the captured iterations hold the FDTD kernels
     B->curCols = cols;
     B->curDepth = depth;
+    B->curStep = cols == 0;
     size_t nt = FdtdTiles(B, cols, depth), nf = 0, ngu = 0, ngi = 0, nga = 0, ngf = 0;
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
@@ -419,10 +442,16 @@ int BuildLists(ptk_batch *B) {
         const Inst &in = B->insts[(size_t)id];
         if (in.kind == PTK_INST_FDTD) {
             const int ncol = (int)((in.a + 1) * (long long)(in.b + 1)), kdepth = in.c + 1;
+            const int ti = (int)((in.a + kStepTileI) / kStepTileI), tj = (in.b + kStepTileJ) / kStepTileJ;
             for (int k0 = 0; k0 < kdepth; k0 += depth) {
                 const int k1 = std::min(kdepth, k0 + depth);
-                for (int c0 = 0; c0 < ncol; c0 += cols)
-                    B->tiles.host[B->tiles.n++] = int4{in.slot, c0, std::min(cols, ncol - c0), k0 | (k1 << 16)};
+                if (cols == 0) {
+                    for (int y = 0; y < tj; ++y)
+                        for (int x = 0; x < ti; ++x) B->tiles.host[B->tiles.n++] = int4{in.slot, x, y, k0 | (k1 << 16)};
+                } else {
+                    for (int c0 = 0; c0 < ncol; c0 += cols)
+                        B->tiles.host[B->tiles.n++] = int4{in.slot, c0, std::min(cols, ncol - c0), k0 | (k1 << 16)};
+                }
             }
             B->fdtdActive.host[B->fdtdActive.n++] = in.slot;
         } else if (in.kind == PTK_INST_GROUPBY) {
@@ -463,7 +492,12 @@ int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance, bool 
         return (unsigned)(fit ? std::max(1, std::min(configured, count)) : configured);
     };
     const int *hc = B->hCounts;
-    if (fdtd) {
+    if (fdtd && B->curStep) {
+        Args t;
+        t.ptr(B->tiles.dev).ptr(B->dCounts + 0).ptr(B->dFdtd);
+        ++B->nLaunches;
+        HIPCHK(ptk_hip_launch(B->fStep, grid(B->fdtdGrid, hc[0]), 1, 1, 64, 8, 1, 0, B->stream, t.b.data(), t.b.size()));
+    } else if (fdtd) {
         Args h, e;
         h.ptr(B->tiles.dev).ptr(B->dCounts + 0).ptr(B->dFdtd);
         e.ptr(B->tiles.dev).ptr(B->dCounts + 0).ptr(B->dFdtd);
@@ -628,7 +662,7 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
         if (ptk_hip_module_load(m.first, (dir + "/" + m.second).c_str()))
             return fail(Fail(PTK_BATCH_EHIP, std::string("load ") + m.second + ": " + ptk_hip_last_error_string()));
     const std::pair<ptk_hip_function *, const char *> bf[] = {
-        {&B->fH, "BatchFdtdH"}, {&B->fE, "BatchFdtdE"}, {&B->fAdv, "BatchFdtdAdvance"},
+        {&B->fH, "BatchFdtdH"}, {&B->fE, "BatchFdtdE"}, {&B->fStep, "BatchFdtdStep"}, {&B->fAdv, "BatchFdtdAdvance"},
         {&B->fGbAcc, "BatchGroupbyAccumulate"}, {&B->fGbMerge, "BatchGroupbyMerge"},
         {&B->fGbOff, "BatchGroupbyOffsets"}, {&B->fGbRes, "BatchGroupbyReshuffle"},
         {&B->fGmmAcc, "BatchGmmAccumulate"}, {&B->fGmmFin, "BatchGmmFinalize"},
@@ -905,7 +939,7 @@ int ptk_batch_release(ptk_batch *B, int id) {
 }
 
 int ptk_batch_set_tiles(ptk_batch *B, int cols, int depth, int gbRows) {
-    if (!B || cols < 1 || depth < 1 || depth > 65535 || gbRows < PTK_GB_CHUNK || gbRows % PTK_GB_CHUNK)
+    if (!B || cols < 0 || depth < 1 || depth > 65535 || gbRows < PTK_GB_CHUNK || gbRows % PTK_GB_CHUNK)
         return Fail(PTK_BATCH_EINVAL, "bad tile shape");
     B->fdtdCols = cols;
     B->fdtdDepth = depth;
@@ -928,9 +962,9 @@ int ptk_batch_get_tiles(ptk_batch *B, int *cols, int *depth, int *gbRows) {
     return PTK_BATCH_OK;
 }
 
-// Times the FDTD half-step pair over the resident instances for each candidate tile shape and
-// persistent grid size (the pair only rewrites the next field set from the current one, so
-// repeating it changes nothing) and keeps the fastest.  Groupby chunk rows are fixed at
+// Times one FDTD iteration (the half-step pair or the whole step) over the resident instances
+// for each candidate tile shape and persistent grid size (an iteration only rewrites the next
+// field set from the current one, so repeating it changes nothing) and keeps the fastest.  Groupby chunk rows are fixed at
 // admission and are not tuned here.
 int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
     if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
@@ -940,7 +974,8 @@ int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
         if (best_ms) *best_ms = 0;
         return PTK_BATCH_OK;
     }
-    static const int cand[][2] = {{256, 8}, {512, 8}, {256, 16}, {512, 16}, {1024, 16}, {512, 32}, {1024, 32}, {256, 64}};
+    // {columns, planes} of half-step tiles; {0, planes}: whole-step overlapped tiles
+    static const int cand[][2] = {{0, 4}, {0, 8}, {0, 16}, {256, 8}, {512, 16}, {1024, 32}};
     static const int grids[] = {2048, 4096, 8192};
     double best = 1e30;
     int bc = B->fdtdCols, bd = B->fdtdDepth, bg = B->fdtdGrid;
@@ -952,14 +987,14 @@ int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
             B->fdtdGrid = g;
             if (int r = IssueBatched(B, true, false, false, false)) return r;  // warm
             HIPCHK(ptk_hip_event_record(B->evStart, B->stream));
-            for (int rep = 0; rep < 3; ++rep)
+            for (int rep = 0; rep < 2; ++rep)
                 if (int r = IssueBatched(B, true, false, false, false)) return r;
             HIPCHK(ptk_hip_event_record(B->evStop, B->stream));
             HIPCHK(ptk_hip_event_synchronize(B->evStop));
             float ms = 0;
             HIPCHK(ptk_hip_event_elapsed_ms(&ms, B->evStart, B->evStop));
-            if (ms / 3 < best) {
-                best = ms / 3;
+            if (ms / 2 < best) {
+                best = ms / 2;
                 bc = c[0];
                 bd = c[1];
                 bg = g;

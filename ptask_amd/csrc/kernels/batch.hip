@@ -13,6 +13,8 @@
 //                             the columns [col0, col0+ncols) and planes [k0, k1) of an instance;
 //                             set (done & 1) is current, the other set receives the new fields
 //   BatchFdtdAdvance          active FDTD instances: done += 1
+//   BatchFdtdStep             tiles {inst, tile x, tile y, k0 | k1 << 16}: a whole step on
+//                             overlapped 62 x 6-column tiles (the form when fdtd_cols == 0)
 //   BatchGroupbyAccumulate/Reshuffle          units {inst, unit}: chunkRows rows
 //   BatchGroupbyMerge/Offsets                 units {inst, key bucket}
 //   BatchGmmAccumulate        units {inst, partial row b}
@@ -63,6 +65,28 @@ BatchFdtdE(const int4 *__restrict__ tiles, const int *__restrict__ count, const 
         const Coef c = make_coef(I.p);
         const int k0 = tl.w & 0xffff, k1 = (int)((unsigned)tl.w >> 16);
         for (int q = tl.y + (int)threadIdx.x; q < tl.y + tl.z; q += blockDim.x) e_column(F, X, c, (unsigned)q, k0, k1);
+    }
+}
+
+// One whole Yee step on overlapped tiles (overlap_tile, fdtd_fused.h): tiles {inst, tile x, tile
+// y, k0 | k1 << 16} of 62 x 6 owned columns; blocks of 64 x 8 threads.  Tiles are taken in an
+// XCD-contiguous order so neighbouring tiles share their XCD's L2.  Instances must satisfy
+// fits_buffer (the executor checks before it builds these tiles).
+extern "C" __global__ void __launch_bounds__(512)
+BatchFdtdStep(const int4 *__restrict__ tiles, const int *__restrict__ count, const PTK_FDTD_INST *__restrict__ insts) {
+    __shared__ OverlapLds<8> L;
+    if (blockDim.x != 64 || blockDim.y != 8 || blockDim.z != 1) return;  // geometry contract
+    const int n = *count;
+    for (int t = (int)gb_xcd_block(blockIdx.x, gridDim.x); t < n; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        const PTK_FDTD_INST &I = insts[tl.x];
+        if (!fits_buffer(I.p)) continue;
+        const int cur = I.done & 1, nxt = cur ^ 1;
+        const Fields F{I.f[cur][0], I.f[cur][1], I.f[cur][2], I.f[cur][3], I.f[cur][4], I.f[cur][5],
+                       I.f[nxt][0], I.f[nxt][1], I.f[nxt][2], I.f[nxt][3], I.f[nxt][4], I.f[nxt][5]};
+        const int k0 = tl.w & 0xffff, k1 = (int)((unsigned)tl.w >> 16);
+        overlap_tile<8>(F, I.p, tl.y * 62, tl.z * 6, k0, k1, L);
+        lds_barrier();  // the next tile reuses LDS
     }
 }
 

@@ -276,16 +276,31 @@ PcaOrthonormalize(float *__restrict__ S, PCA_DIMS d) {
         p[i] = s;
     }
     if (k > 0) {
-        for (int j = 0; j < k; ++j) {
-            double v = 0;
+        // the k dot products eight at a time: the loads of P[j] are independent across j, and the
+        // eight wave reductions run together
+        constexpr int kGroup = 8;
+        for (int j0 = 0; j0 < k; j0 += kGroup) {
+            double v[kGroup];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int n = tid + 1024 * i;
-                if (n < ld) v += (double)P[(long long)j * ld + n] * p[i];
+            for (int jj = 0; jj < kGroup; ++jj) {
+                const int j = j0 + jj;
+                v[jj] = 0;
+                if (j < k) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int n = tid + 1024 * i;
+                        if (n < ld) v[jj] += (double)P[(long long)j * ld + n] * p[i];
+                    }
+                }
             }
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (lane == 0) sW[wv][j] = v;
+            for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+                for (int jj = 0; jj < kGroup; ++jj) v[jj] += __shfl_xor(v[jj], off, 64);
+            if (lane == 0)
+#pragma unroll
+                for (int jj = 0; jj < kGroup; ++jj)
+                    if (j0 + jj < k) sW[wv][j0 + jj] = v[jj];
         }
         __syncthreads();
         if (tid < k) {

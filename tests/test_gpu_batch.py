@@ -21,12 +21,15 @@ GB = 1 << 30
 MODES = [batch.PLAIN, batch.BATCHED]
 
 
+@pytest.mark.parametrize("tiles", [(64, 4), (0, 3), (0, 1), (0, 64)])
 @pytest.mark.parametrize("mode", MODES)
-def test_fdtd_instances_bit_exact(hip, mode):
-    print(f"case batch fdtd mode={mode}", flush=True)
-    shapes = [(12, 10, 8, 5, 11), (16, 16, 16, 4, 12), (20, 17, 9, 3, 13), (5, 33, 7, 6, 14)]
+def test_fdtd_instances_bit_exact(hip, mode, tiles):
+    """Half-step column tiles (cols > 0) and whole-step overlapped tiles (cols == 0) of several
+    depths; small tiles put several per instance."""
+    print(f"case batch fdtd mode={mode} tiles={tiles}", flush=True)
+    shapes = [(12, 10, 8, 5, 11), (16, 16, 16, 4, 12), (20, 17, 9, 3, 13), (5, 33, 7, 6, 14), (70, 13, 6, 2, 15)]
     with batch.Executor(1 * GB, mode) as ex:
-        ex.set_tiles(64, 4, 65536)   # small tiles: several per instance
+        ex.set_tiles(*tiles, 65536)
         ids = [ex.submit_fdtd(nx, ny, nz, s, seed, keep=True) for nx, ny, nz, s, seed in shapes]
         ex.drain()
         for i, (nx, ny, nz, s, seed) in zip(ids, shapes):
@@ -118,6 +121,8 @@ def test_checksums_equal_alone_in_mix_both_modes_and_delayed(hip):
     assert plain == mixed
     tuned, _ = run_mix(batch.BATCHED, 2 * GB, tiles=(128, 4, 16384), autotune=True)
     assert tuned == mixed
+    halves, _ = run_mix(batch.BATCHED, 2 * GB, tiles=(256, 8, 262144))
+    assert halves == mixed  # the half-step and whole-step FDTD forms agree bit for bit
     direct, st = run_mix(batch.BATCHED, 2 * GB, graphs=False)
     assert direct == mixed and st["graph_launches"] == 0
     # an arena that holds only a few instances at a time: admission by footprint queues the rest
