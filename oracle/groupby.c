@@ -234,3 +234,46 @@ void GroupbyReshuffle(unsigned int n_, void **a) {
     }
     free(next);
 }
+
+/* CoalesceGroups (HostTasks/HostTasks.cpp:131-197) with the ABI of its coalesce_int_float entry
+ * (:164-197): (float* values, int n, int* groupIDs, int numGroups, int* counts, void* out,
+ * int* meta).  The values arrive grouped; group i is written as its key (groupIDs at the group's
+ * first element) followed by its counts[i] values, groups one after another, and meta[i] =
+ * counts[i].  The reference's loop never advances its output cursor past a group (every group
+ * is written at the start of the output); this restatement writes the layout the function
+ * documents, groups consecutive. */
+void coalesce_int_float(unsigned int nArgs, void **a) {
+    (void)nArgs;
+    const float *in = (const float *)a[0];
+    const int n = IARG(a, 1);
+    const int32_t *ids = (const int32_t *)a[2];
+    const int groups = IARG(a, 3);
+    const int32_t *counts = (const int32_t *)a[4];
+    unsigned char *out = (unsigned char *)a[5];
+    int32_t *meta = (int32_t *)a[6];
+    int pos = 0;
+    for (int g = 0; g < groups && pos < n; ++g) {
+        const int32_t c = counts[g];
+        memcpy(out, &ids[pos], sizeof(int32_t));
+        out += sizeof(int32_t);
+        for (int j = 0; j < c && pos < n; ++j, ++pos) {
+            memcpy(out, &in[pos], sizeof(float));
+            out += sizeof(float);
+        }
+        meta[g] = c;
+    }
+}
+
+/* GroupbyCoalesce: the same layout over this pipeline's outputs, with the HIP kernel's ABI
+ * (int* outVals, int* outKeys, long long* groups, long long* offsets, int* out, int* meta):
+ * G and the valid rows come from the groups header, the counts from its records. */
+void GroupbyCoalesce(unsigned int nArgs, void **a) {
+    (void)nArgs;
+    const int64_t *groups = (const int64_t *)a[2];
+    const int G = (int)groups[0], rows = (int)groups[1];
+    int32_t *counts = (int32_t *)malloc(sizeof(int32_t) * (size_t)(G > 0 ? G : 1));
+    for (int g = 0; g < G; ++g) counts[g] = (int32_t)groups[3 + 3 * (size_t)g + 1];
+    void *args[7] = {a[0], (void *)(intptr_t)rows, a[1], (void *)(intptr_t)G, counts, a[4], a[5]};
+    coalesce_int_float(7, args);
+    free(counts);
+}

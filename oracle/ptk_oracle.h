@@ -58,6 +58,8 @@ void GroupbyAccumulate(unsigned int n, void **a);
 void GroupbyOffsets(unsigned int n, void **a);
 void GroupbyMerge(unsigned int n, void **a);
 void GroupbyReshuffle(unsigned int n, void **a);
+void coalesce_int_float(unsigned int n, void **a);
+void GroupbyCoalesce(unsigned int n, void **a);
 void HxComputation(unsigned int n, void **a);
 void HyComputation(unsigned int n, void **a);
 void HzComputation(unsigned int n, void **a);

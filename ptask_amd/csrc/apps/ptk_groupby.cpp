@@ -7,6 +7,8 @@
 //     GroupbyMerge      (scratch in/out, n, numKeys)                     -> scratch, table
 //     GroupbyOffsets    (table, numKeys, scratch)                        -> offsets, groups
 //     GroupbyReshuffle  (keys, vals, n, numKeys, offsets, scratch)       -> outKeys, outVals
+//   and with --coalesce the reference's CoalesceGroups layout (HostTasks.cpp:131-197):
+//     GroupbyCoalesce   (outVals, outKeys, groups, offsets)              -> coalesced, groupCounts
 // --workload pfxsum: one task pfxsum(pin, pout, N) wired as cpfxsum.cpp:204-226 wires it.
 // --workload scan2: the two-task large scan PfxScanReduce -> PfxScanApply.
 //
@@ -45,6 +47,7 @@ struct Options {
     std::string dump;
     bool profile = false;
     int gbGrid = 2048;  // at most this many blocks for the two chunk-parallel groupby kernels
+    bool coalesce = false;  // append GroupbyCoalesce (the reference's CoalesceGroups layout)
 };
 
 uint64_t Fnv1a(const void *p, size_t n) {
@@ -117,13 +120,14 @@ int main(int argc, char **argv) {
         else if (a == "--dump") o.dump = next();
         else if (a == "--profile") o.profile = true;
         else if (a == "--gb-grid") o.gbGrid = atoi(next());
+        else if (a == "--coalesce") o.coalesce = true;
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
     if (o.kernelFile.empty() || o.rows < 0 || o.numKeys < 0 || o.rows > 0x7fffffffLL ||
         (o.workload == "groupby" && (long long)o.numKeys + 1 > PTK_GB_MAX_KEYS)) {
         fprintf(stderr, "usage: ptk_groupby --kernels <groupby.hsaco|scan.hsaco|hosttasks.so> "
                         "[--workload groupby|pfxsum|scan2] [--rows N] [--keys K] [--dist zipf|uniform] "
-                        "[--zipf-s S] [--seed S] [--repeat R] [--warmup W] [--dump file] [--profile]\n");
+                        "[--zipf-s S] [--seed S] [--repeat R] [--warmup W] [--dump file] [--profile] [--coalesce]\n");
         return 2;
     }
     Runtime::SetExitOnRuntimeFailure(FALSE);
@@ -211,6 +215,30 @@ int main(int argc, char **argv) {
         outs = {g->AddOutputChannel(oOut[1], "groups"), g->AddOutputChannel(rOut[0], "outKeys"),
                 g->AddOutputChannel(rOut[1], "outVals")};
         outNames = {"groups", "outKeys", "outVals"};
+        if (o.coalesce) {
+            // GroupbyCoalesce: per group its key then its values (n + numKeys + 1 words at most),
+            // and the per-group counts (HostTasks.cpp:131-197)
+            DatablockTemplate *tCo = Runtime::GetDatablockTemplate((char *)"gbCoalesced", sizeof(int32_t),
+                                                                   (UINT)(n + o.numKeys + 1), 1, 1);
+            DatablockTemplate *tMeta = Runtime::GetDatablockTemplate((char *)"gbGroupCounts", sizeof(int32_t),
+                                                                     (UINT)(o.numKeys + 1), 1, 1);
+            Port *kIn4[4] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"outVals", 0),
+                             Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"outKeys", 1),
+                             Runtime::CreatePort(INPUT_PORT, tGroups, uid++, (char *)"groups", 2),
+                             Runtime::CreatePort(INPUT_PORT, tOff, uid++, (char *)"offsets", 3)};
+            Port *kOut2[2] = {Runtime::CreatePort(OUTPUT_PORT, tCo, uid++, (char *)"coalesced", 4),
+                              Runtime::CreatePort(OUTPUT_PORT, tMeta, uid++, (char *)"groupCounts", 5)};
+            Task *tk = g->AddTask(K("GroupbyCoalesce"), 4, kIn4, 2, kOut2, "GroupbyCoalesce");
+            geometry(tk, chunkGrid, 512);
+            g->AddInternalChannel(rOut[1], kIn4[0], "shuffle->coalesce(vals)");
+            g->AddInternalChannel(rOut[0], kIn4[1], "shuffle->coalesce(keys)");
+            g->AddInternalChannel(oOut[1], kIn4[2], "offsets->coalesce(groups)");
+            g->AddInternalChannel(oOut[0], kIn4[3], "offsets->coalesce(offsets)");
+            outs.push_back(g->AddOutputChannel(kOut2[0], "coalesced"));
+            outs.push_back(g->AddOutputChannel(kOut2[1], "groupCounts"));
+            outNames.push_back("coalesced");
+            outNames.push_back("groupCounts");
+        }
     } else if (o.workload == "pfxsum") {
         // cpfxsum.cpp:204-226: pin at 0, pout at 1, N (sticky) at 2
         Port *in[2] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"pin", 0),

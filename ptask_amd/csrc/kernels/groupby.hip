@@ -60,3 +60,19 @@ GroupbyReshuffle(const int *__restrict__ keys, const int *__restrict__ vals, int
     for (long long c = blockIdx.x; c < S.chunks; c += gridDim.x)
         gb_place_chunk(keys, vals, n, numKeys, c, offsets, S, outKeys, outVals, L);
 }
+
+// The reference's CoalesceGroups host task (HostTasks/HostTasks.cpp:131-197) over this
+// pipeline's outputs: primary output, per group in key order, the key followed by the group's
+// values (rows + G words); meta output, the count of each group.  Values are copied as 4-byte
+// words (int32 here; the reference instantiates float).  Any grid of 64k-thread blocks; blocks
+// stride over chunks of PTK_GB_CHUNK grouped rows.
+//   GroupbyCoalesce(int* outVals, int* outKeys, long long* groups, long long* offsets,
+//                   int* coalesced, int* groupCounts)
+extern "C" __global__ void __launch_bounds__(1024)
+GroupbyCoalesce(const int *__restrict__ vals, const int *__restrict__ keys, const long long *__restrict__ groups,
+                const long long *__restrict__ offsets, int *__restrict__ out, int *__restrict__ meta) {
+    __shared__ GbCoalesceLds L;
+    if ((blockDim.x & 63) != 0) return;
+    const long long chunks = (groups[1] + PTK_GB_CHUNK - 1) / PTK_GB_CHUNK;
+    for (long long c = blockIdx.x; c < chunks; c += gridDim.x) gb_coalesce_chunk(vals, keys, groups, offsets, c, out, meta, L);
+}

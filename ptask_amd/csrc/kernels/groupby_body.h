@@ -390,4 +390,54 @@ __device__ inline void gb_place_chunk(const int *__restrict__ keys, const int *_
     __syncthreads();
 }
 
+struct GbCoalesceLds {
+    long long g0;
+    unsigned long long wtot[16];
+};
+
+// The coalesced layout of HostTasks/HostTasks.cpp:131-197 for rows [i0, i0 + PTK_GB_CHUNK) of a
+// grouped output: group g (in key order) becomes the words key, then its rows' values, so row i
+// of group g lands at word i + g + 1 and the key of g at start(g) + g; meta[g] = count of g.
+// The block finds the group of its first row by binary search over the group starts
+// (offsets[key of g], ascending in g) and the rest from key changes (a block scan of
+// boundaries).  blockDim.x a multiple of 64, at most 1024.
+__device__ inline void gb_coalesce_chunk(const int *__restrict__ vals, const int *__restrict__ keys,
+                                         const long long *__restrict__ groups, const long long *__restrict__ offsets,
+                                         long long c, int *__restrict__ out, int *__restrict__ meta, GbCoalesceLds &L) {
+    const long long G = groups[0], rows = groups[1];
+    const long long i0 = c * PTK_GB_CHUNK, i1 = i0 + PTK_GB_CHUNK < rows ? i0 + PTK_GB_CHUNK : rows;
+    if (i0 >= i1 || G <= 0) return;  // uniform over the block
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (threadIdx.x == 0) {  // the last group whose start is <= i0
+        long long lo = 0, hi = G;
+        while (hi - lo > 1) {
+            const long long mid = (lo + hi) / 2;
+            if (offsets[groups[3 + 3 * mid]] <= i0) lo = mid;
+            else hi = mid;
+        }
+        L.g0 = lo;
+    }
+    // each thread takes PTK_GB_CHUNK / blockDim.x consecutive rows
+    const int per = (PTK_GB_CHUNK + blockDim.x - 1) / blockDim.x;
+    const long long a = i0 + (long long)threadIdx.x * per, b = a + per < i1 ? a + per : i1;
+    unsigned long long mine = 0;
+    for (long long i = a; i < b; ++i) mine += (i > i0 && keys[i] != keys[i - 1]) ? 1 : 0;
+    const unsigned long long incl = wave_incl(mine);
+    if (lane == 63) L.wtot[w] = incl;
+    __syncthreads();
+    long long g = L.g0 + (long long)(incl - mine);
+    for (int q = 0; q < w && q < nw; ++q) g += (long long)L.wtot[q];
+    for (long long i = a; i < b; ++i) {
+        const bool starts = i > i0 ? keys[i] != keys[i - 1] : offsets[groups[3 + 3 * g]] == i0;
+        if (i > i0 && starts) ++g;
+        if (g >= G) break;  // only for an inconsistent input
+        if (starts) {
+            out[i + g] = keys[i];
+            meta[g] = (int)groups[3 + 3 * g + 1];
+        }
+        out[i + g + 1] = vals[i];
+    }
+    __syncthreads();
+}
+
 }  // namespace ptk_groupby

@@ -123,3 +123,27 @@ def test_hip_scan_graphs_equal_oracle(tmp_path, workload):
                 "--dump", d)
         raw = np.fromfile(d, dtype=np.int32)
         assert np.array_equal(raw[2 * n:3 * n], oracle.pfxsum(raw[:n])), (workload, n)
+
+
+@pytest.mark.parametrize("dist,n,nk", [("zipf", 300_001, 65535), ("uniform", 5000, 99), ("zipf", 1, 0)])
+def test_hip_coalesce_equals_reference_layout(tmp_path, dist, n, nk):
+    """GroupbyCoalesce (the reference's CoalesceGroups layout, HostTasks.cpp:131-197) after the
+    HIP groupby: key words at their group starts and the per-group counts exact; the values of
+    each group equal the oracle's as a multiset (their order inside a group is not fixed)."""
+    from test_oracle_groupby import coalesce_brute, load_coalesce_dump, load_groupby_dump_prefix
+    d = str(tmp_path / "c.bin")
+    r = run_app("--kernels", paths.kernel_path("groupby.hsaco"), "--rows", n, "--num-keys", nk,
+                "--dist", dist, "--coalesce", "--dump", d)
+    assert r["ok"] and r["backend"] == "hip"
+    keys, vals, groups, ok, ov = load_groupby_dump_prefix(d, n, nk)
+    g, head, offsets, eok, eov = oracle.groupby(keys, vals, nk)
+    _, co, meta = load_coalesce_dump(d, n, nk)
+    want, wmeta = coalesce_brute(g, eov)
+    G = g.shape[0]
+    assert np.array_equal(meta[:G], wmeta)
+    pos = 0
+    for key, cnt, _ in g:
+        assert co[pos] == key
+        assert np.array_equal(np.sort(co[pos + 1:pos + 1 + cnt]), np.sort(want[pos + 1:pos + 1 + cnt]))
+        pos += 1 + cnt
+    assert pos == want.size

@@ -173,3 +173,66 @@ def test_host_task_forms_pipeline(n, nk):
     assert np.array_equal(offsets, eoff)
     rows = int(head[1])
     assert np.array_equal(ok[:rows], eok[:rows]) and np.array_equal(ov[:rows], eov[:rows])
+
+
+def coalesce_brute(g, eov):
+    """The coalesced layout built directly: per group its key, then its values."""
+    words, pos = [], 0
+    for key, cnt, _ in g:
+        words.append(np.int32(key))
+        words.extend(eov[pos:pos + cnt].view(np.int32))
+        pos += cnt
+    return np.array(words, dtype=np.int32), g[:, 1].astype(np.int32)
+
+
+def load_coalesce_dump(path, n, nk):
+    raw = np.fromfile(path, dtype=np.uint8)
+    off = 4 * n * 2 + 8 * (3 + 3 * (nk + 1)) + 4 * n * 2
+    head = raw[:off]
+    co = raw[off:off + 4 * (n + nk + 1)].view(np.int32)
+    meta = raw[off + 4 * (n + nk + 1):].view(np.int32)
+    return head, co, meta
+
+
+def test_coalesce_int_float_restates_the_reference_layout():
+    """coalesce_int_float (HostTasks.cpp:164-197 ABI) writes key, values per group, groups
+    consecutive, and the per-group counts; pinned against the layout built directly."""
+    import ctypes as C
+    L = C.CDLL(HOSTLIB)
+    ids = np.array([4, 4, 4, 7, 9, 9], np.int32)
+    vals = np.array([1.5, -2.0, 3.25, 7.0, 0.5, 1e30], np.float32)
+    counts = np.array([3, 1, 2], np.int32)
+    out = np.full(9, -1, np.int32)
+    meta = np.zeros(3, np.int32)
+    args = (C.c_void_p * 7)(vals.ctypes.data, 6, ids.ctypes.data, 3, counts.ctypes.data, out.ctypes.data,
+                            meta.ctypes.data)
+    L.coalesce_int_float(C.c_uint(7), args)
+    want = np.concatenate([[4], vals[:3].view(np.int32), [7], vals[3:4].view(np.int32), [9], vals[4:].view(np.int32)])
+    assert np.array_equal(out, want.astype(np.int32)) and np.array_equal(meta, counts)
+
+
+def test_host_groupby_graph_coalesce(tmp_path):
+    n, nk = 20000, 511
+    d = str(tmp_path / "c.bin")
+    r = run_app("--kernels", HOSTLIB, "--rows", n, "--num-keys", nk, "--coalesce", "--dump", d)
+    assert r["ok"]
+    keys, vals, groups, ok, ov = load_groupby_dump_prefix(d, n, nk)
+    g, head, offsets, eok, eov = oracle.groupby(keys, vals, nk)
+    _, co, meta = load_coalesce_dump(d, n, nk)
+    want, wmeta = coalesce_brute(g, eov)
+    G = g.shape[0]
+    assert np.array_equal(co[:want.size], want) and np.array_equal(meta[:G], wmeta)
+
+
+def load_groupby_dump_prefix(path, n, nk):
+    raw = np.fromfile(path, dtype=np.uint8)
+    off = 0
+
+    def take(dtype, count):
+        nonlocal off
+        a = raw[off:off + count * np.dtype(dtype).itemsize].view(dtype)
+        off += a.nbytes
+        return a
+    keys, vals = take(np.int32, n), take(np.int32, n)
+    groups = take(np.int64, 3 + 3 * (nk + 1))
+    return keys, vals, groups, take(np.int32, n), take(np.int32, n)
