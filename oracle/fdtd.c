@@ -20,6 +20,7 @@
  * the curl from the Yee stencil on tiny grids through an independent numpy formulation; the
  * graph form (six host tasks) equals this direct form.
  */
+#include <stdlib.h>
 #include <stddef.h>
 #include <string.h>
 
@@ -193,3 +194,25 @@ void FdtdStepFused(unsigned int n, void **a) {
 /* The overlapped-tile whole-step kernels compute the same step (same arguments). */
 void FdtdStepOverlap(unsigned int n, void **a) { FdtdStepFused(n, a); }
 void FdtdStepOverlap8(unsigned int n, void **a) { FdtdStepFused(n, a); }
+
+/* FdtdTwoStepOverlap(Hx..Ez,&p,oHx..oEz): two whole steps; the intermediate fields live in a
+ * temporary copy. */
+void FdtdTwoStepOverlap(unsigned int n, void **a) {
+    const FDTD_PARAMS *p = (const FDTD_PARAMS *)a[6];
+    const long long X = p->Nx, Y = p->Ny, Z = p->Nz;
+    const long long sz[6] = {(X + 1) * Y * Z, X * (Y + 1) * Z, X * Y * (Z + 1),
+                             X * (Y + 1) * (Z + 1), (X + 1) * Y * (Z + 1), (X + 1) * (Y + 1) * Z};
+    float *mid[6];
+    void *s1[13], *s2[13];
+    for (int f = 0; f < 6; ++f) {
+        mid[f] = (float *)malloc((size_t)(sz[f] > 0 ? sz[f] : 1) * sizeof(float));
+        s1[f] = a[f];
+        s1[7 + f] = mid[f];
+        s2[f] = mid[f];
+        s2[7 + f] = a[7 + f];
+    }
+    s1[6] = s2[6] = a[6];
+    FdtdStepFused(n, s1);
+    FdtdStepFused(n, s2);
+    for (int f = 0; f < 6; ++f) free(mid[f]);
+}

@@ -71,6 +71,7 @@ void FdtdEFields(unsigned int n, void **a);
 void FdtdStepFused(unsigned int n, void **a);
 void FdtdStepOverlap(unsigned int n, void **a);
 void FdtdStepOverlap8(unsigned int n, void **a);
+void FdtdTwoStepOverlap(unsigned int n, void **a);
 void GmmAccumulate(unsigned int n, void **a);
 void GmmFinalize(unsigned int n, void **a);
 void PcaLoad(unsigned int n, void **a);

@@ -47,11 +47,16 @@ std::vector<TaskSpec> FormTasks(const std::string &form) {
         return {{"FdtdStepFused", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
     if (form == "overlap")  // one whole step on overlapped tiles (48 B per cell-step + overlap)
         return {{"FdtdStepOverlap", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
+    if (form == "overlap2")  // two whole steps per task (24 B per cell-step + overlap)
+        return {{"FdtdTwoStepOverlap", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
     // the reference's six tasks (graphfdtd.cpp:448-560): (own, in1, in2) -> own
     return {{"HxComputation", {0, 4, 5}, {0}}, {"HyComputation", {1, 3, 5}, {1}},
             {"HzComputation", {2, 3, 4}, {2}}, {"ExComputation", {3, 1, 2}, {3}},
             {"EyComputation", {4, 0, 2}, {4}}, {"EzComputation", {5, 0, 1}, {5}}};
 }
+
+// Time steps one task of the form advances the fields by.
+int StepsPerTask(const std::string &form) { return form == "overlap2" ? 2 : 1; }
 
 struct Options {
     int nx = 32, ny = 32, nz = 32, steps = 10, blocks = 1, repeat = 1;
@@ -114,9 +119,13 @@ bool Parse(int argc, char **argv, Options &o) {
     }
     if (o.tileI && (o.tileI < 1 || o.tileJ < 2 || o.slab < 1 || o.tileI * o.tileJ > 1024)) return false;
     if (o.form == "overlap" && o.tileI && (o.tileI != 64 || o.tileJ < 3 || o.tileJ > 16)) return false;
+    if (o.form == "overlap2" && (o.steps % 2 || (o.tileI && (o.tileI != 64 || o.tileJ < 5 || o.tileJ > 16)))) {
+        fprintf(stderr, "--form overlap2 advances two steps per task: --steps must be even\n");
+        return false;
+    }
     // the overlapped-tile kernel addresses each field with 32-bit buffer offsets
-    if (o.form == "overlap" && (long long)(o.nx + 1) * (o.ny + 1) * (o.nz + 1) >= (1ll << 29)) {
-        fprintf(stderr, "--form overlap needs (nx+1)(ny+1)(nz+1) < 2^29\n");
+    if ((o.form == "overlap" || o.form == "overlap2") && (long long)(o.nx + 1) * (o.ny + 1) * (o.nz + 1) >= (1ll << 29)) {
+        fprintf(stderr, "--form overlap / overlap2 need (nx+1)(ny+1)(nz+1) < 2^29\n");
         return false;
     }
     return !o.kernelFile.empty() && o.nx > 0 && o.ny > 0 && o.nz > 0 && o.steps > 0;
@@ -149,6 +158,13 @@ struct FdtdGraph {
     void Geometry(Task *t, const TaskSpec &ts) {
         if (!hip) {
             t->SetComputeGeometry(1, 1, 1);  // a host task computes its whole arrays per call
+            return;
+        }
+        if (!strcmp(ts.kernel, "FdtdTwoStepOverlap")) {
+            // 64 x R thread blocks owning 60 x (R-4) columns each, k slabs of `slab` planes
+            const int R = o.tileJ ? o.tileJ : 16, slab = o.slab ? o.slab : 16;
+            t->SetBlockAndGridSize(PTASKDIM3((o.nx + 60) / 60, (o.ny + R - 4) / (R - 4), (o.nz + slab) / slab),
+                                   PTASKDIM3(64, R, 1));
             return;
         }
         if (!strncmp(ts.kernel, "FdtdStepOverlap", 15)) {
@@ -286,7 +302,7 @@ struct FdtdGraph {
         int prodTask[6], prodSlot[6];
         Producers(prodTask, prodSlot);
         std::vector<std::vector<Port *>> prevOuts;
-        for (int s = 0; s < o.steps; ++s) {
+        for (int s = 0; s < o.steps / StepsPerTask(o.form); ++s) {
             std::vector<std::vector<Port *>> ins(T), outs(T);
             Port *meta = nullptr;
             for (size_t t = 0; t < T; ++t)
@@ -322,7 +338,7 @@ int main(int argc, char **argv) {
     if (!Parse(argc, argv, o)) {
         fprintf(stderr, "usage: ptk_fdtd --kernels <fdtd.hsaco|hosttasks.so> [--n N | --nx --ny --nz] "
                         "[--steps S] [--mode looped|unrolled] [--blocks B] [--repeat R] [--seed S] "
-                        "[--form six|halves|fused|overlap] [--geometry default|reference|compute] [--tile IxJxK] [--dump file] "
+                        "[--form six|halves|fused|overlap|overlap2] [--geometry default|reference|compute] [--tile IxJxK] [--dump file] "
                         "[--eager] [--profile] [--chain]\n");
         return 2;
     }
@@ -399,7 +415,8 @@ int main(int argc, char **argv) {
             }
             carried.clear();
             if (o.mode == "looped") {
-                Datablock *it = Runtime::AllocateDatablock(G.iters, &o.steps, sizeof(int), nullptr);
+                int iters = o.steps / StepsPerTask(o.form);  // graph iterations
+                Datablock *it = Runtime::AllocateDatablock(G.iters, &iters, sizeof(int), nullptr);
                 for (GraphInputChannel *c : G.iterInputs) c->Push(it);
                 it->Release();
             }

@@ -373,4 +373,199 @@ __device__ inline void overlap_tile(const Fields &F, const FDTD_PARAMS &p, int i
 #endif
 }
 
+// ---- two time steps per pass on overlapped tiles -----------------------------------------------
+//
+// A block of 64 x R threads (R = blockDim.y, 5..MAXR) covers columns i0-2 .. i0+61 (lane l ->
+// i = i0-2+l) and rows j0-2 .. j0+R-3 (wave r -> j = j0-2+r) and owns the inner 60 x (R-4)
+// cells, for which it writes H and E after two time steps.  Each half step loses one cell of
+// valid data on one side (new H needs E at i+1 / j+1, new E needs H at i-1 / j-1), so after
+// two steps the block minus two cells on each side is exact; the lost cells are recomputed by
+// the neighbouring tiles, bit for bit.  HBM sees each input read and each output written once
+// per two steps (24 B per cell-step plus the overlap).
+//
+// Planes stream upwards with the second step two planes behind the first.  Iteration m:
+//   S1  H1(m) from E0(m), E0(m+1), H0(m) and the E0(m) plane;  H2(m-2) from E1(m-2), E1(m-1),
+//       H1(m-2) and the E1(m-2) plane; write H1(m), H2(m-2) and E0(m+1) to LDS
+//   barrier
+//   S2  E1(m) from E0(m), H1(m), H1(m-1) and the H1(m) plane;  E2(m-2) from E1(m-2), H2(m-2),
+//       H2(m-3) and the H2(m-2) plane; write E1(m) to LDS; store H2(m-2), E2(m-2) if owned
+//   barrier
+// E0 and E1 planes are double-buffered by plane parity (E0(m) is written in S1(m-1) and read in
+// S1(m); E1(m-2) is read in S1(m) and its buffer rewritten with E1(m) in S2(m)); H1 and H2 are
+// written in S1 and read in S2 of the same iteration.  A slab [k0, k1) starts two planes early
+// (k0-2) and runs two planes late (k1+1) for the lag.  Every expression is the reference's, in
+// its order (graphfdtd.cpp:185-312), so the result is bitwise that of two sequential steps.
+template <int MAXR>
+struct Overlap2Lds {
+    float e0[2][3][MAXR + 2][66];
+    float e1[2][3][MAXR + 2][66];
+    float h1[3][MAXR + 2][66];
+    float h2[3][MAXR + 2][66];
+};
+
+// One field array for the two-step tiles: the column's byte offset travels in a VGPR and the
+// plane's in an SGPR, so a plane costs no vector address arithmetic.  A lane outside the field's
+// columns carries bit 31 in its column offset, so its loads read 0 and its stores are dropped.
+// Plane offsets are always those of planes inside the array, so a valid cell's column plus plane
+// offset is below the array's size and no sum exceeds 2^32: the range check drops exactly the
+// invalid lanes whether or not it counts the plane offset.
+struct PlaneBuf {
+    __amdgpu_buffer_rsrc_t rs;
+    __device__ float load(unsigned col, unsigned plane) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)col, (int)plane, 0));
+    }
+    __device__ void store(unsigned col, unsigned plane, float v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, (int)col, (int)plane,
+                                              PTK_OVERLAP_STORE_POLICY);
+    }
+};
+
+__device__ inline PlaneBuf plane_buf(const float *p, unsigned bytes) {
+    return PlaneBuf{__builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000)};
+}
+
+template <int MAXR>
+__device__ inline void overlap2_tile(const Fields &F, const FDTD_PARAMS &p, int i0, int j0, int k0, int k1,
+                                     Overlap2Lds<MAXR> &L) {
+    const Ix X{p.Nx, p.Ny, p.Nz};
+    const Coef c = make_coef(p);
+    const int R = (int)blockDim.y;
+    const int l = (int)threadIdx.x, r = (int)threadIdx.y;
+    const int i = i0 - 2 + l, j = j0 - 2 + r;
+    const int Nx = p.Nx, Ny = p.Ny, Nz = p.Nz;
+    const bool dom = i >= 0 && i <= Nx && j >= 0 && j <= Ny;
+    const bool own = dom && l >= 2 && l <= 61 && r >= 2 && r <= R - 3;
+    auto col = [](bool ok, int idx) { return ok ? 4u * (unsigned)idx : kNoCell; };
+    const unsigned qHx = col(dom && j < Ny, X.hx(i, j, 0)), qHy = col(dom && i < Nx, X.hy(i, j, 0));
+    const unsigned qHz = col(dom && i < Nx && j < Ny, X.hz(i, j, 0)), qEx = col(dom && i < Nx, X.ex(i, j, 0));
+    const unsigned qEy = col(dom && j < Ny, X.ey(i, j, 0)), qEz = col(dom, X.ez(i, j, 0));
+    const unsigned ownBit = own ? 0u : kNoCell;
+    // store column offsets: bit 31 also where the block does not own the column
+    const unsigned wqHx = qHx | ownBit, wqHy = qHy | ownBit, wqHz = qHz | ownBit;
+    const unsigned wqEx = qEx | ownBit, wqEy = qEy | ownBit, wqEz = qEz | ownBit;
+    const unsigned sHx = 4u * (Nx + 1) * Ny, sHy = 4u * Nx * (Ny + 1), sHz = 4u * Nx * Ny;
+    const unsigned sEx = 4u * Nx * (Ny + 1), sEy = 4u * (Nx + 1) * Ny, sEz = 4u * (Nx + 1) * (Ny + 1);
+    const PlaneBuf rHx = plane_buf(F.hx, sHx * Nz), rHy = plane_buf(F.hy, sHy * Nz), rHz = plane_buf(F.hz, sHz * (Nz + 1)),
+                   rEx = plane_buf(F.ex, sEx * (Nz + 1)), rEy = plane_buf(F.ey, sEy * (Nz + 1)), rEz = plane_buf(F.ez, sEz * Nz);
+    const PlaneBuf wHx = plane_buf(F.ohx, sHx * Nz), wHy = plane_buf(F.ohy, sHy * Nz), wHz = plane_buf(F.ohz, sHz * (Nz + 1)),
+                   wEx = plane_buf(F.oex, sEx * (Nz + 1)), wEy = plane_buf(F.oey, sEy * (Nz + 1)), wEz = plane_buf(F.oez, sEz * Nz);
+    const bool jIn = j >= 1 && j <= Ny - 1, iIn = i >= 1 && i <= Nx - 1;
+
+    struct Set {
+        float ex, ey, ez, hx, hy, hz;
+    };
+    // S_m = {E0(m+1), H0(m)}.  Only m <= k1 feed owned cells; a later set reloads S_k1's planes
+    // (cached) and a plane past a field's last is read as its last: those values reach no
+    // stored cell, as the planes of the first step that are formed from them do not.
+    auto load_set = [&](Set &S, int m) {
+        const int k = m < k1 ? m : k1;
+        const int ke = k + 1 <= Nz ? k + 1 : Nz, kez = k + 1 <= Nz - 1 ? k + 1 : Nz - 1;
+        const int kh = k <= Nz - 1 ? k : Nz - 1, khz = k <= Nz ? k : Nz;
+        S.ex = rEx.load(qEx, (unsigned)ke * sEx);
+        S.ey = rEy.load(qEy, (unsigned)ke * sEy);
+        S.ez = rEz.load(qEz, (unsigned)kez * sEz);
+        S.hx = rHx.load(qHx, (unsigned)kh * sHx);
+        S.hy = rHy.load(qHy, (unsigned)kh * sHy);
+        S.hz = rHz.load(qHz, (unsigned)khz * sHz);
+    };
+    // six dropped stores: the prologue then leaves the memory counter as a loop iteration does,
+    // so the waits the compiler places in the loop stay two planes behind the loads
+    auto pad_stores = [&]() {
+        wHx.store(kNoCell, 0u, 0.f);
+        wHy.store(kNoCell, 0u, 0.f);
+        wHz.store(kNoCell, 0u, 0.f);
+        wEx.store(kNoCell, 0u, 0.f);
+        wEy.store(kNoCell, 0u, 0.f);
+        wEz.store(kNoCell, 0u, 0.f);
+    };
+    const int ms = k0 >= 2 ? k0 - 2 : 0, me = k1 + 2;  // iterations m in [ms, me)
+    Set S0, S1, S2, S3;
+    {
+        const int ke = ms, kez = ms <= Nz - 1 ? ms : Nz - 1;  // ms <= k0 < k1 <= Nz + 1
+        S0.ex = rEx.load(qEx, (unsigned)ke * sEx);  // E0(ms)
+        S0.ey = rEy.load(qEy, (unsigned)ke * sEy);
+        S0.ez = rEz.load(qEz, (unsigned)kez * sEz);
+        S0.hx = S0.hy = S0.hz = 0.f;
+    }
+    load_set(S1, ms);
+    pad_stores();
+    load_set(S2, ms + 1);
+    pad_stores();
+    const int y = r + 1, x = l + 1;
+    L.e0[ms & 1][0][y][x] = S0.ex;
+    L.e0[ms & 1][1][y][x] = S0.ey;
+    L.e0[ms & 1][2][y][x] = S0.ez;
+    // carried own values (planes relative to the iteration m)
+    float h1x1 = 0.f, h1y1 = 0.f, h1z1 = 0.f;  // H1(m-1)
+    float h1x2 = 0.f, h1y2 = 0.f, h1z2 = 0.f;  // H1(m-2)
+    float e1x1 = 0.f, e1y1 = 0.f, e1z1 = 0.f;  // E1(m-1)
+    float e1x2 = 0.f, e1y2 = 0.f, e1z2 = 0.f;  // E1(m-2)
+    float h2x3 = 0.f, h2y3 = 0.f;              // H2(m-3): Hx, Hy
+    lds_barrier();
+    auto plane = [&](int m, const Set &P, const Set &S, Set &N) {
+        const int b = m & 1, q = m - 2;
+        load_set(N, m + 2);
+        // ---- S1: H1(m) and H2(m-2)
+        const float ex0 = P.ex, ey0 = P.ey, ez0 = P.ez;  // E0(m)
+        const float hx1 = S.hx + c.cH * ((S.ey - ey0) * c.Cz - (L.e0[b][2][y + 1][x] - ez0) * c.Cy);
+        const float hy1 = S.hy + c.cH * ((L.e0[b][2][y][x + 1] - ez0) * c.Cx - (S.ex - ex0) * c.Cz);
+        const float hz1 = S.hz + c.cH * ((L.e0[b][0][y + 1][x] - ex0) * c.Cy - (L.e0[b][1][y][x + 1] - ey0) * c.Cx);
+        const float hx2 = h1x2 + c.cH * ((e1y1 - e1y2) * c.Cz - (L.e1[b][2][y + 1][x] - e1z2) * c.Cy);
+        const float hy2 = h1y2 + c.cH * ((L.e1[b][2][y][x + 1] - e1z2) * c.Cx - (e1x1 - e1x2) * c.Cz);
+        const float hz2 = h1z2 + c.cH * ((L.e1[b][0][y + 1][x] - e1x2) * c.Cy - (L.e1[b][1][y][x + 1] - e1y2) * c.Cx);
+        L.h1[0][y][x] = hx1;
+        L.h1[1][y][x] = hy1;
+        L.h1[2][y][x] = hz1;
+        L.h2[0][y][x] = hx2;
+        L.h2[1][y][x] = hy2;
+        L.h2[2][y][x] = hz2;
+        L.e0[b ^ 1][0][y][x] = S.ex;
+        L.e0[b ^ 1][1][y][x] = S.ey;
+        L.e0[b ^ 1][2][y][x] = S.ez;
+        lds_barrier();
+        // ---- S2: E1(m) and E2(m-2) (graphfdtd.cpp:243-312: interior cells update, faces copy)
+        const bool kIn1 = m >= 1 && m <= Nz - 1, kIn2 = q >= 1 && q <= Nz - 1;
+        const float wx1 = ex0 + c.cE * ((hz1 - L.h1[2][y - 1][x]) * c.Cy - (hy1 - h1y1) * c.Cz);
+        const float wy1 = ey0 + c.cE * ((hx1 - h1x1) * c.Cz - (hz1 - L.h1[2][y][x - 1]) * c.Cx);
+        const float wz1 = ez0 + c.cE * ((hy1 - L.h1[1][y][x - 1]) * c.Cx - (hx1 - L.h1[0][y - 1][x]) * c.Cy);
+        const float ex1 = (jIn && kIn1) ? wx1 : ex0;
+        const float ey1 = (iIn && kIn1) ? wy1 : ey0;
+        const float ez1 = (iIn && jIn) ? wz1 : ez0;
+        const float wx2 = e1x2 + c.cE * ((hz2 - L.h2[2][y - 1][x]) * c.Cy - (hy2 - h2y3) * c.Cz);
+        const float wy2 = e1y2 + c.cE * ((hx2 - h2x3) * c.Cz - (hz2 - L.h2[2][y][x - 1]) * c.Cx);
+        const float wz2 = e1z2 + c.cE * ((hy2 - L.h2[1][y][x - 1]) * c.Cx - (hx2 - L.h2[0][y - 1][x]) * c.Cy);
+        const float ex2 = (jIn && kIn2) ? wx2 : e1x2;
+        const float ey2 = (iIn && kIn2) ? wy2 : e1y2;
+        const float ez2 = (iIn && jIn) ? wz2 : e1z2;
+        L.e1[b][0][y][x] = ex1;  // E1(m-2) in this buffer was last read in S1
+        L.e1[b][1][y][x] = ey1;
+        L.e1[b][2][y][x] = ez1;
+        // plane q of each output if it is in the slab and the field, else bit 31 and plane 0
+        const bool out = q >= k0 && q < k1;
+        const bool o1 = out && q < Nz, o2 = out && q <= Nz;
+        const unsigned m1 = o1 ? 0u : kNoCell, m2 = o2 ? 0u : kNoCell;
+        const unsigned p1 = o1 ? (unsigned)q : 0u, p2 = o2 ? (unsigned)q : 0u;
+        wHx.store(wqHx | m1, p1 * sHx, hx2);
+        wHy.store(wqHy | m1, p1 * sHy, hy2);
+        wHz.store(wqHz | m2, p2 * sHz, hz2);
+        wEx.store(wqEx | m2, p2 * sEx, ex2);
+        wEy.store(wqEy | m2, p2 * sEy, ey2);
+        wEz.store(wqEz | m1, p1 * sEz, ez2);
+        h1x2 = h1x1, h1y2 = h1y1, h1z2 = h1z1;
+        h1x1 = hx1, h1y1 = hy1, h1z1 = hz1;
+        e1x2 = e1x1, e1y2 = e1y1, e1z2 = e1z1;
+        e1x1 = ex1, e1y1 = ey1, e1z1 = ez1;
+        h2x3 = hx2, h2y3 = hy2;
+        lds_barrier();
+    };
+    // The sets rotate through four planes, and the loop runs whole groups of four with no branch
+    // inside: up to three planes past me - 1 are formed and discarded (q >= k1: nothing stored).
+    for (int m = ms; m < me; m += 4) {
+        plane(m, S0, S1, S3);
+        plane(m + 1, S1, S2, S0);
+        plane(m + 2, S2, S3, S1);
+        plane(m + 3, S3, S0, S2);
+    }
+}
+
 }  // namespace ptk_fdtd

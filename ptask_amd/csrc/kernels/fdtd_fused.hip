@@ -88,6 +88,43 @@ __device__ inline void overlap_grid(const Fields &F, const FDTD_PARAMS &p, Overl
 OVERLAP_KERNEL(FdtdStepOverlap, 16)
 OVERLAP_KERNEL(FdtdStepOverlap8, 8)
 
+// Two time steps per launch on overlapped tiles (see overlap2_tile in fdtd_fused.h), same ABI as
+// FdtdStepFused: the outputs hold the fields two steps after the inputs.  Launch: block
+// (64, R, 1) with 5 <= R <= 16; grid (ceil((Nx+1)/60), ceil((Ny+1)/(R-4)), slabs).  Smaller
+// grids stride over the tiles; gridDim.z splits [0, Nz] into slabs.  Tiles are taken in the
+// XCD-contiguous order of FdtdStepOverlap.
+__device__ inline void overlap2_grid(const Fields &F, const FDTD_PARAMS &p, Overlap2Lds<16> &L) {
+    const int R = (int)blockDim.y;
+    if (blockDim.x != 64 || R < 5 || R > 16 || blockDim.z != 1) return;  // geometry contract
+    // the launcher rejects such grids before launching; planes are addressed from 0 to Nz
+    if (!fits_buffer(p) || p.Nx < 1 || p.Ny < 1 || p.Nz < 1) return;
+    const int ti = (p.Nx + 1 + 59) / 60, tj = (p.Ny + 1 + R - 5) / (R - 4);
+    const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
+    unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if (PTK_OVERLAP_XCD && nblk >= kNumXcd) {
+        const unsigned xcd = b % kNumXcd, local = b / kNumXcd, per = nblk / kNumXcd, rem = nblk % kNumXcd;
+        b = xcd * per + (xcd < rem ? xcd : rem) + local;
+    }
+    const int gx = (int)(b % gridDim.x), gy = (int)(b / gridDim.x % gridDim.y), gz = (int)(b / (gridDim.x * gridDim.y));
+    const int depth = p.Nz + 1, slabs = (int)gridDim.z;
+    const int k0 = (int)((long long)gz * depth / slabs), k1 = (int)((long long)(gz + 1) * depth / slabs);
+    if (k0 >= k1) return;  // the whole block
+    for (int by = gy; by < tj; by += gridDim.y)
+        for (int bx = gx; bx < ti; bx += gridDim.x) {
+            overlap2_tile<16>(F, p, bx * 60, by * (R - 4), k0, k1, L);
+            lds_barrier();  // the next tile reuses LDS
+        }
+}
+
+extern "C" __global__ void __launch_bounds__(1024)
+FdtdTwoStepOverlap(const float *hx, const float *hy, const float *hz, const float *ex, const float *ey,
+                   const float *ez, FDTD_PARAMS p, float *ohx, float *ohy, float *ohz, float *oex, float *oey,
+                   float *oez) {
+    __shared__ Overlap2Lds<16> L;
+    const Fields F{hx, hy, hz, ex, ey, ez, ohx, ohy, ohz, oex, oey, oez};
+    overlap2_grid(F, p, L);
+}
+
 // ---- half-step kernels: all three H updates in one streaming pass, then all three E updates ----
 //
 // FdtdHFields(Hx, Hy, Hz, Ex, Ey, Ez, FDTD_PARAMS p, oHx, oHy, oHz)

@@ -313,3 +313,42 @@ def test_overlap_step_bitexact(hip, fmod, shape, geom, name):
     geo = overlap_geometry(rows, slab, cap)(nx, ny, nz)
     got = run_fused(hip, fmod, name, F, FD.make_params(nx, ny, nz), 4, geo)
     assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 4))
+
+
+def overlap2_geometry(rows, slab, max_tiles=None):
+    """Two-step overlapped tiles: blocks of 64 x rows own 60 x (rows-4) columns; k slabs of ~slab
+    planes; max_tiles caps the grid so blocks take further tiles in grid strides."""
+    def geo(nx, ny, nz):
+        ti, tj = -(-(nx + 1) // 60), -(-(ny + 1) // (rows - 4))
+        if max_tiles:
+            ti, tj = min(ti, max_tiles), min(tj, max_tiles)
+        return (ti, tj, max(1, -(-(nz + 1) // slab))), (64, rows, 1)
+    return geo
+
+
+OVERLAP2 = {
+    "r16_slab16": (16, 16, None),
+    "r8_slab4": (8, 4, None),
+    "r5_slab1": (5, 1, None),
+    "r6_slab2": (6, 2, None),
+    "r12_one_slab": (12, 1 << 20, None),
+    "r16_slab3_strided": (16, 3, 1),
+    "r7_slab5_strided": (7, 5, 2),
+}
+
+
+@pytest.mark.parametrize("geom", sorted(OVERLAP2))
+@pytest.mark.parametrize("shape", [(8, 8, 8), (59, 11, 4), (60, 12, 3), (119, 9, 7), (130, 29, 6),
+                                   (13, 21, 17), (1, 1, 1), (3, 1, 2), (1, 5, 3), (2, 2, 9)])
+def test_two_step_overlap_bitexact(hip, fmod, shape, geom):
+    """FdtdTwoStepOverlap: each launch advances two whole steps on overlapped tiles and equals
+    two rounds of the six reference updates bit for bit, for tile edges at and around the
+    60-column / (R-4)-row tile sizes, slabs of one plane up to the whole depth, and grids that
+    stride over tiles."""
+    rows, slab, cap = OVERLAP2[geom]
+    nx, ny, nz = shape
+    print(f"case overlap2 shape={shape} geom={geom}", flush=True)
+    F = oracle.init_fields(nx, ny, nz, seed=307 + nx + ny)
+    geo = overlap2_geometry(rows, slab, cap)(nx, ny, nz)
+    got = run_fused(hip, fmod, "FdtdTwoStepOverlap", F, FD.make_params(nx, ny, nz), 3, geo)
+    assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 6))

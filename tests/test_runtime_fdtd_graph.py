@@ -50,7 +50,7 @@ def direct(nx, ny, nz, steps, seed):
     return oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), steps)
 
 
-TASKS_PER_STEP = {"six": 6, "halves": 2, "fused": 1, "overlap": 1}
+TASKS_PER_STEP = {"six": 6, "halves": 2, "fused": 1, "overlap": 1, "overlap2": 0.5}
 
 
 @pytest.mark.parametrize("form", sorted(TASKS_PER_STEP))
@@ -59,12 +59,12 @@ TASKS_PER_STEP = {"six": 6, "halves": 2, "fused": 1, "overlap": 1}
 def test_host_graph_equals_direct_oracle(tmp_path, mode, shape, form):
     nx, ny, nz = shape
     d = str(tmp_path / "f.bin")
-    r = run_app("--kernels", HOSTLIB, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", 5,
+    r = run_app("--kernels", HOSTLIB, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", 6,
                 "--mode", mode, "--form", form, "--seed", 99, dump=d)
     assert r["ok"] and r["backend"] == "host"
-    assert r["dispatches_last_run"] == TASKS_PER_STEP[form] * 5
+    assert r["dispatches_last_run"] == TASKS_PER_STEP[form] * 6
     got = load_dump(d, nx, ny, nz)
-    ref = direct(nx, ny, nz, 5, 99)
+    ref = direct(nx, ny, nz, 6, 99)
     for f in range(6):
         assert np.array_equal(got[f].view(np.uint32), ref[f].view(np.uint32)), oracle.FIELDS[f]
 
@@ -99,6 +99,12 @@ def test_missing_kernel_file_fails_loudly():
         or "accelerator" in p.stderr
 
 
+def test_two_step_form_rejects_odd_step_counts():
+    p = subprocess.run([APP, "--kernels", HOSTLIB, "--n", "4", "--steps", "3", "--form", "overlap2"],
+                       capture_output=True, text=True, timeout=60)
+    assert p.returncode != 0 and "even" in p.stderr
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("geometry", ["default", "reference", "compute"])
 def test_hip_looped_graph_equals_host_graph(tmp_path, geometry):
@@ -115,10 +121,10 @@ def test_hip_looped_graph_equals_host_graph(tmp_path, geometry):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", ["halves", "fused", "overlap"])
+@pytest.mark.parametrize("form", ["halves", "fused", "overlap", "overlap2"])
 @pytest.mark.parametrize("mode", ["looped", "unrolled"])
 def test_hip_fused_forms_equal_host_graph(tmp_path, mode, form):
-    nx, ny, nz, steps = 70, 9, 11, 5
+    nx, ny, nz, steps = 70, 9, 11, 6
     dh, dg = str(tmp_path / "h.bin"), str(tmp_path / "g.bin")
     run_app("--kernels", HOSTLIB, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
             "--mode", mode, "--form", "six", "--seed", 3, dump=dh)
@@ -132,7 +138,8 @@ def test_hip_fused_forms_equal_host_graph(tmp_path, mode, form):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("form,geo", [("halves", ["--tile", "64x4x3"]), ("halves", ["--tile", "128x2x64"]),
-                                      ("overlap", ["--tile", "64x3x1"]), ("overlap", ["--tile", "64x8x5"])])
+                                      ("overlap", ["--tile", "64x3x1"]), ("overlap", ["--tile", "64x8x5"]),
+                                      ("overlap2", ["--tile", "64x5x1"]), ("overlap2", ["--tile", "64x12x7"])])
 def test_hip_tile_geometries_equal_host_graph(tmp_path, form, geo):
     nx, ny, nz, steps = 70, 9, 11, 4
     dh, dg = str(tmp_path / "h.bin"), str(tmp_path / "g.bin")
