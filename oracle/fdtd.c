@@ -195,24 +195,28 @@ void FdtdStepFused(unsigned int n, void **a) {
 void FdtdStepOverlap(unsigned int n, void **a) { FdtdStepFused(n, a); }
 void FdtdStepOverlap8(unsigned int n, void **a) { FdtdStepFused(n, a); }
 
-/* FdtdTwoStepOverlap(Hx..Ez,&p,oHx..oEz): two whole steps; the intermediate fields live in a
- * temporary copy. */
-void FdtdTwoStepOverlap(unsigned int n, void **a) {
+/* FdtdTwoStepOverlap / FdtdThreeStepOverlap(Hx..Ez,&p,oHx..oEz): T whole steps, the
+ * intermediate fields in temporary arrays. */
+static void fdtd_steps(unsigned int n, void **a, int T) {
     const FDTD_PARAMS *p = (const FDTD_PARAMS *)a[6];
     const long long X = p->Nx, Y = p->Ny, Z = p->Nz;
     const long long sz[6] = {(X + 1) * Y * Z, X * (Y + 1) * Z, X * Y * (Z + 1),
                              X * (Y + 1) * (Z + 1), (X + 1) * Y * (Z + 1), (X + 1) * (Y + 1) * Z};
-    float *mid[6];
-    void *s1[13], *s2[13];
-    for (int f = 0; f < 6; ++f) {
-        mid[f] = (float *)malloc((size_t)(sz[f] > 0 ? sz[f] : 1) * sizeof(float));
-        s1[f] = a[f];
-        s1[7 + f] = mid[f];
-        s2[f] = mid[f];
-        s2[7 + f] = a[7 + f];
+    float *buf[2][6];
+    for (int f = 0; f < 6; ++f)
+        for (int k = 0; k < 2; ++k) buf[k][f] = (float *)malloc((size_t)(sz[f] > 0 ? sz[f] : 1) * sizeof(float));
+    for (int t = 0; t < T; ++t) {
+        void *s[13];
+        s[6] = a[6];
+        for (int f = 0; f < 6; ++f) {
+            s[f] = t == 0 ? a[f] : (void *)buf[(t - 1) & 1][f];
+            s[7 + f] = t == T - 1 ? a[7 + f] : (void *)buf[t & 1][f];
+        }
+        FdtdStepFused(n, s);
     }
-    s1[6] = s2[6] = a[6];
-    FdtdStepFused(n, s1);
-    FdtdStepFused(n, s2);
-    for (int f = 0; f < 6; ++f) free(mid[f]);
+    for (int f = 0; f < 6; ++f)
+        for (int k = 0; k < 2; ++k) free(buf[k][f]);
 }
+
+void FdtdTwoStepOverlap(unsigned int n, void **a) { fdtd_steps(n, a, 2); }
+void FdtdThreeStepOverlap(unsigned int n, void **a) { fdtd_steps(n, a, 3); }

@@ -72,6 +72,7 @@ void FdtdStepFused(unsigned int n, void **a);
 void FdtdStepOverlap(unsigned int n, void **a);
 void FdtdStepOverlap8(unsigned int n, void **a);
 void FdtdTwoStepOverlap(unsigned int n, void **a);
+void FdtdThreeStepOverlap(unsigned int n, void **a);
 void GmmAccumulate(unsigned int n, void **a);
 void GmmFinalize(unsigned int n, void **a);
 void PcaLoad(unsigned int n, void **a);

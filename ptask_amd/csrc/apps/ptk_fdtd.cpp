@@ -49,6 +49,8 @@ std::vector<TaskSpec> FormTasks(const std::string &form) {
         return {{"FdtdStepOverlap", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
     if (form == "overlap2")  // two whole steps per task (24 B per cell-step + overlap)
         return {{"FdtdTwoStepOverlap", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
+    if (form == "overlap3")  // three whole steps per task (16 B per cell-step + overlap)
+        return {{"FdtdThreeStepOverlap", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
     // the reference's six tasks (graphfdtd.cpp:448-560): (own, in1, in2) -> own
     return {{"HxComputation", {0, 4, 5}, {0}}, {"HyComputation", {1, 3, 5}, {1}},
             {"HzComputation", {2, 3, 4}, {2}}, {"ExComputation", {3, 1, 2}, {3}},
@@ -56,7 +58,7 @@ std::vector<TaskSpec> FormTasks(const std::string &form) {
 }
 
 // Time steps one task of the form advances the fields by.
-int StepsPerTask(const std::string &form) { return form == "overlap2" ? 2 : 1; }
+int StepsPerTask(const std::string &form) { return form == "overlap2" ? 2 : form == "overlap3" ? 3 : 1; }
 
 struct Options {
     int nx = 32, ny = 32, nz = 32, steps = 10, blocks = 1, repeat = 1;
@@ -119,13 +121,18 @@ bool Parse(int argc, char **argv, Options &o) {
     }
     if (o.tileI && (o.tileI < 1 || o.tileJ < 2 || o.slab < 1 || o.tileI * o.tileJ > 1024)) return false;
     if (o.form == "overlap" && o.tileI && (o.tileI != 64 || o.tileJ < 3 || o.tileJ > 16)) return false;
-    if (o.form == "overlap2" && (o.steps % 2 || (o.tileI && (o.tileI != 64 || o.tileJ < 5 || o.tileJ > 16)))) {
-        fprintf(stderr, "--form overlap2 advances two steps per task: --steps must be even\n");
-        return false;
+    if (o.form == "overlap2" || o.form == "overlap3") {
+        const int T = StepsPerTask(o.form);
+        if (o.steps < T) {  // the remainder of steps % T runs in one task after the T-step tasks
+            fprintf(stderr, "--form %s advances %d steps per task: --steps must be at least %d\n",
+                    o.form.c_str(), T, T);
+            return false;
+        }
+        if (o.tileI && (o.tileI != 64 || o.tileJ < 2 * T + 1 || o.tileJ > 16)) return false;
     }
     // the overlapped-tile kernel addresses each field with 32-bit buffer offsets
-    if ((o.form == "overlap" || o.form == "overlap2") && (long long)(o.nx + 1) * (o.ny + 1) * (o.nz + 1) >= (1ll << 29)) {
-        fprintf(stderr, "--form overlap / overlap2 need (nx+1)(ny+1)(nz+1) < 2^29\n");
+    if (o.form.compare(0, 7, "overlap") == 0 && (long long)(o.nx + 1) * (o.ny + 1) * (o.nz + 1) >= (1ll << 29)) {
+        fprintf(stderr, "the overlap forms need (nx+1)(ny+1)(nz+1) < 2^29\n");
         return false;
     }
     return !o.kernelFile.empty() && o.nx > 0 && o.ny > 0 && o.nz > 0 && o.steps > 0;
@@ -160,16 +167,20 @@ struct FdtdGraph {
             t->SetComputeGeometry(1, 1, 1);  // a host task computes its whole arrays per call
             return;
         }
-        if (!strcmp(ts.kernel, "FdtdTwoStepOverlap")) {
-            // 64 x R thread blocks owning 60 x (R-4) columns each, k slabs of `slab` planes
-            const int R = o.tileJ ? o.tileJ : 16, slab = o.slab ? o.slab : 16;
-            t->SetBlockAndGridSize(PTASKDIM3((o.nx + 60) / 60, (o.ny + R - 4) / (R - 4), (o.nz + slab) / slab),
+        // --tile applies to the form's own task; a remainder task keeps its kernel's defaults
+        const bool tiled = o.tileI && !strcmp(ts.kernel, spec[0].kernel);
+        if (!strcmp(ts.kernel, "FdtdTwoStepOverlap") || !strcmp(ts.kernel, "FdtdThreeStepOverlap")) {
+            // 64 x R thread blocks owning (64-2T) x (R-2T) columns each, k slabs of `slab` planes
+            // (deeper slabs for three steps, whose pipeline runs 4 planes past each slab end)
+            const int T = !strcmp(ts.kernel, "FdtdThreeStepOverlap") ? 3 : 2, ow = 64 - 2 * T;
+            const int R = tiled ? o.tileJ : 16, slab = tiled ? o.slab : T == 3 ? 64 : 16, oh = R - 2 * T;
+            t->SetBlockAndGridSize(PTASKDIM3((o.nx + ow) / ow, (o.ny + oh) / oh, (o.nz + slab) / slab),
                                    PTASKDIM3(64, R, 1));
             return;
         }
         if (!strncmp(ts.kernel, "FdtdStepOverlap", 15)) {
             // 64 x R thread blocks owning 62 x (R-2) columns each, k slabs of `slab` planes
-            const int R = o.tileJ ? o.tileJ : 8, slab = o.slab ? o.slab : 8;
+            const int R = tiled ? o.tileJ : 8, slab = tiled ? o.slab : 8;
             t->SetBlockAndGridSize(PTASKDIM3((o.nx + 62) / 62, (o.ny + R - 2) / (R - 2), (o.nz + slab) / slab),
                                    PTASKDIM3(64, R, 1));
             return;
@@ -224,7 +235,7 @@ struct FdtdGraph {
         for (size_t b = 0; b < ts.out.size(); ++b)
             outs.push_back(Runtime::CreatePort(OUTPUT_PORT, field[ts.out[b]], uid++,
                                                (char *)kFieldNames[ts.out[b]], n + 1 + (UINT)b));
-        std::string name = std::string(ts.kernel) + (step >= 0 ? "_" + std::to_string(step) : "");
+        std::string name = std::string(ts.kernel) + (step >= 0 ? "_" + std::to_string(step) : step == -2 ? "_tail" : "");
         Task *task = g->AddTask(k, (UINT)ins.size(), ins.data(), (UINT)outs.size(), outs.data(), name.c_str());
         if (task) Geometry(task, ts);
         return task;
@@ -255,12 +266,44 @@ struct FdtdGraph {
             }
     }
 
+    // The remainder of the steps when the form's task advances several (steps % T): one task
+    // after the loop, or after the unrolled tasks, advancing the last one or two steps.
+    bool TailSpec(TaskSpec &ts) const {
+        const int r = o.steps % StepsPerTask(o.form);
+        if (!r) return false;
+        ts = {r == 2 ? "FdtdTwoStepOverlap" : "FdtdStepOverlap", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}};
+        return true;
+    }
+
+    // A remainder task fed by `src[f]`, the producers of the fields after the last full task; its
+    // outputs are the graph's.  `endOfLoop`: the feeding channels open on the loop's last
+    // iteration only.
+    bool AddTail(const TaskSpec &ts, Port *const src[6], bool endOfLoop) {
+        std::vector<Port *> ins, outs;
+        Port *meta = nullptr;
+        if (!AddTask(ts, false, ins, outs, meta, -2)) return false;
+        for (size_t a = 0; a < ts.in.size(); ++a) {
+            const int f = ts.in[a];
+            InternalChannel *c = g->AddInternalChannel(src[f], ins[a], (std::string(kFieldNames[f]) + "->tail").c_str());
+            if (endOfLoop) c->SetPredicationType(CE_SRC, CGATEFN_OPEN_ON_ENDITERATION);
+        }
+        paramInputs.push_back(g->AddInputChannel(ins[ts.in.size()], "tail.cfg"));
+        for (size_t b = 0; b < ts.out.size(); ++b) {
+            const int f = ts.out[b];
+            outputs[f] = g->AddOutputChannel(outs[b], (std::string(kFieldNames[f]) + ".out").c_str());
+            if (o.eager) outputs[f]->SetViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY_EAGER);
+        }
+        return true;
+    }
+
     // Looped: every task carries a MF_GENERAL_ITERATOR meta port.  A field produced earlier in
     // the step arrives on an always-open channel; a field produced by the same or a later task
     // is loop-carried: the application's switch channel feeds the first iteration, a
     // back-channel closed on ENDITERATION feeds the rest (iteration.cpp:887-1060).
     bool BuildLooped() {
         const size_t T = spec.size();
+        TaskSpec tailSpec;
+        const bool tail = TailSpec(tailSpec);
         std::vector<std::vector<Port *>> ins(T), outs(T);
         std::vector<Port *> metas(T);
         for (size_t t = 0; t < T; ++t)
@@ -286,11 +329,17 @@ struct FdtdGraph {
             iterInputs.push_back(g->AddInputChannel(metas[t], (tn + ".iters").c_str()));
             for (size_t b = 0; b < spec[t].out.size(); ++b) {
                 g->BindControlPropagationPort(metas[t], outs[t][b]);
+                if (tail) continue;  // the remainder task takes the loop's last outputs
                 const int f = spec[t].out[b];
                 outputs[f] = g->AddOutputChannel(outs[t][b], (std::string(kFieldNames[f]) + ".out").c_str());
                 outputs[f]->SetPredicationType(CE_SRC, CGATEFN_OPEN_ON_ENDITERATION);
                 if (o.eager) outputs[f]->SetViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY_EAGER);
             }
+        }
+        if (tail) {
+            Port *src[6];
+            for (int f = 0; f < 6; ++f) src[f] = outs[prodTask[f]][prodSlot[f]];
+            return AddTail(tailSpec, src, true);
         }
         return true;
     }
@@ -322,9 +371,16 @@ struct FdtdGraph {
             }
             prevOuts = outs;
         }
-        for (int f = 0; f < 6; ++f) {
-            outputs[f] = g->AddOutputChannel(prevOuts[prodTask[f]][prodSlot[f]], (std::string(kFieldNames[f]) + ".out").c_str());
-            if (o.eager) outputs[f]->SetViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY_EAGER);
+        TaskSpec tailSpec;
+        if (TailSpec(tailSpec)) {
+            Port *src[6];
+            for (int f = 0; f < 6; ++f) src[f] = prevOuts[prodTask[f]][prodSlot[f]];
+            if (!AddTail(tailSpec, src, false)) return false;
+        } else {
+            for (int f = 0; f < 6; ++f) {
+                outputs[f] = g->AddOutputChannel(prevOuts[prodTask[f]][prodSlot[f]], (std::string(kFieldNames[f]) + ".out").c_str());
+                if (o.eager) outputs[f]->SetViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY_EAGER);
+            }
         }
         for (GraphInputChannel *c : paramInputs) c->SetNoDraw();
         return true;
@@ -338,7 +394,7 @@ int main(int argc, char **argv) {
     if (!Parse(argc, argv, o)) {
         fprintf(stderr, "usage: ptk_fdtd --kernels <fdtd.hsaco|hosttasks.so> [--n N | --nx --ny --nz] "
                         "[--steps S] [--mode looped|unrolled] [--blocks B] [--repeat R] [--seed S] "
-                        "[--form six|halves|fused|overlap|overlap2] [--geometry default|reference|compute] [--tile IxJxK] [--dump file] "
+                        "[--form six|halves|fused|overlap|overlap2|overlap3] [--geometry default|reference|compute] [--tile IxJxK] [--dump file] "
                         "[--eager] [--profile] [--chain]\n");
         return 2;
     }

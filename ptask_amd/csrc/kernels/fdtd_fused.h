@@ -373,37 +373,40 @@ __device__ inline void overlap_tile(const Fields &F, const FDTD_PARAMS &p, int i
 #endif
 }
 
-// ---- two time steps per pass on overlapped tiles -----------------------------------------------
+// ---- several time steps per pass on overlapped tiles -------------------------------------------
 //
-// A block of 64 x R threads (R = blockDim.y, 5..MAXR) covers columns i0-2 .. i0+61 (lane l ->
-// i = i0-2+l) and rows j0-2 .. j0+R-3 (wave r -> j = j0-2+r) and owns the inner 60 x (R-4)
-// cells, for which it writes H and E after two time steps.  Each half step loses one cell of
-// valid data on one side (new H needs E at i+1 / j+1, new E needs H at i-1 / j-1), so after
-// two steps the block minus two cells on each side is exact; the lost cells are recomputed by
-// the neighbouring tiles, bit for bit.  HBM sees each input read and each output written once
-// per two steps (24 B per cell-step plus the overlap).
+// A block of 64 x R threads (R = blockDim.y, 2T+1..MAXR) advances T time steps in one pass.  It
+// covers columns i0-T .. i0+63-T (lane l -> i = i0-T+l) and rows j0-T .. j0+R-1-T (wave r ->
+// j = j0-T+r) and owns the inner (64-2T) x (R-2T) cells, for which it writes H and E after the T
+// steps.  Each half step loses one cell of valid data on one side (new H needs E at i+1 / j+1,
+// new E needs H at i-1 / j-1), so after T steps the block minus T cells on each side is exact;
+// the lost cells are recomputed by the neighbouring tiles, bit for bit.  HBM sees each input read
+// and each output written once per T steps (48/T B per cell-step plus the overlap).
 //
-// Planes stream upwards with the second step two planes behind the first.  Iteration m:
-//   S1  H1(m) from E0(m), E0(m+1), H0(m) and the E0(m) plane;  H2(m-2) from E1(m-2), E1(m-1),
-//       H1(m-2) and the E1(m-2) plane; write H1(m), H2(m-2) and E0(m+1) to LDS
+// Planes stream upwards; step s (1..T) runs 2(s-1) planes behind the first, on plane
+// p_s = m - 2(s-1) in iteration m:
+//   A  for each s: H_s(p_s) from E_{s-1}(p_s), E_{s-1}(p_s+1), H_{s-1}(p_s) and the E_{s-1}(p_s)
+//      plane; write H_s(p_s) and E_0(m+1) to LDS
 //   barrier
-//   S2  E1(m) from E0(m), H1(m), H1(m-1) and the H1(m) plane;  E2(m-2) from E1(m-2), H2(m-2),
-//       H2(m-3) and the H2(m-2) plane; write E1(m) to LDS; store H2(m-2), E2(m-2) if owned
+//   B  for each s: E_s(p_s) from E_{s-1}(p_s), H_s(p_s), H_s(p_s-1) and the H_s(p_s) plane; write
+//      E_s(p_s) to LDS (s < T); store H_T, E_T of plane p_T if owned
 //   barrier
-// E0 and E1 planes are double-buffered by plane parity (E0(m) is written in S1(m-1) and read in
-// S1(m); E1(m-2) is read in S1(m) and its buffer rewritten with E1(m) in S2(m)); H1 and H2 are
-// written in S1 and read in S2 of the same iteration.  A slab [k0, k1) starts two planes early
-// (k0-2) and runs two planes late (k1+1) for the lag.  Every expression is the reference's, in
-// its order (graphfdtd.cpp:185-312), so the result is bitwise that of two sequential steps.
-template <int MAXR>
-struct Overlap2Lds {
-    float e0[2][3][MAXR + 2][66];
-    float e1[2][3][MAXR + 2][66];
-    float h1[3][MAXR + 2][66];
-    float h2[3][MAXR + 2][66];
+// Step 0 is the input: E_0(m), E_0(m+1) and H_0(m) are loaded two planes ahead.  Step s-1's
+// values two and one planes behind its own (those of planes p_s and p_s+1) are carried in
+// registers (for T >= 3, the E values are read back from the own slots of the LDS planes).  E_s
+// planes are double-buffered by iteration parity (E_s(p_s) is written in B of iteration m and
+// read as a plane in A of iteration m+2); H_s planes are written in A and read in B of the same
+// iteration (HB = 1, two barriers per plane), or by parity (HB = 2, one barrier per plane).  A
+// slab [k0, k1) starts T planes early and runs 2(T-1) planes late for the lag.  Every expression
+// is the reference's, in its order (graphfdtd.cpp:185-312), so the result is bitwise that of T
+// sequential steps.
+template <int T, int MAXR, int HB>
+struct OverlapNLds {
+    float e[T][2][3][MAXR + 2][66];  // E_0 .. E_{T-1} planes by iteration parity
+    float h[HB][T][3][MAXR + 2][66]; // H_1 .. H_T planes of the iteration (by parity if HB = 2)
 };
 
-// One field array for the two-step tiles: the column's byte offset travels in a VGPR and the
+// One field array for the multi-step tiles: the column's byte offset travels in a VGPR and the
 // plane's in an SGPR, so a plane costs no vector address arithmetic.  A lane outside the field's
 // columns carries bit 31 in its column offset, so its loads read 0 and its stores are dropped.
 // Plane offsets are always those of planes inside the array, so a valid cell's column plus plane
@@ -424,25 +427,27 @@ __device__ inline PlaneBuf plane_buf(const float *p, unsigned bytes) {
     return PlaneBuf{__builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000)};
 }
 
-template <int MAXR>
-__device__ inline void overlap2_tile(const Fields &F, const FDTD_PARAMS &p, int i0, int j0, int k0, int k1,
-                                     Overlap2Lds<MAXR> &L) {
+struct F3 {
+    float x, y, z;
+};
+
+template <int T, int MAXR, int HB>
+__device__ inline void overlapn_tile(const Fields &F, const FDTD_PARAMS &p, int i0, int j0, int k0, int k1,
+                                     OverlapNLds<T, MAXR, HB> &L) {
+    static_assert(T >= 1 && 2 * T < 64, "steps per pass");
     const Ix X{p.Nx, p.Ny, p.Nz};
     const Coef c = make_coef(p);
     const int R = (int)blockDim.y;
     const int l = (int)threadIdx.x, r = (int)threadIdx.y;
-    const int i = i0 - 2 + l, j = j0 - 2 + r;
+    const int i = i0 - T + l, j = j0 - T + r;
     const int Nx = p.Nx, Ny = p.Ny, Nz = p.Nz;
     const bool dom = i >= 0 && i <= Nx && j >= 0 && j <= Ny;
-    const bool own = dom && l >= 2 && l <= 61 && r >= 2 && r <= R - 3;
+    const bool own = dom && l >= T && l <= 63 - T && r >= T && r <= R - 1 - T;
     auto col = [](bool ok, int idx) { return ok ? 4u * (unsigned)idx : kNoCell; };
     const unsigned qHx = col(dom && j < Ny, X.hx(i, j, 0)), qHy = col(dom && i < Nx, X.hy(i, j, 0));
     const unsigned qHz = col(dom && i < Nx && j < Ny, X.hz(i, j, 0)), qEx = col(dom && i < Nx, X.ex(i, j, 0));
     const unsigned qEy = col(dom && j < Ny, X.ey(i, j, 0)), qEz = col(dom, X.ez(i, j, 0));
-    const unsigned ownBit = own ? 0u : kNoCell;
-    // store column offsets: bit 31 also where the block does not own the column
-    const unsigned wqHx = qHx | ownBit, wqHy = qHy | ownBit, wqHz = qHz | ownBit;
-    const unsigned wqEx = qEx | ownBit, wqEy = qEy | ownBit, wqEz = qEz | ownBit;
+    const unsigned ownBit = own ? 0u : kNoCell;  // stores also carry bit 31 where not owned
     const unsigned sHx = 4u * (Nx + 1) * Ny, sHy = 4u * Nx * (Ny + 1), sHz = 4u * Nx * Ny;
     const unsigned sEx = 4u * Nx * (Ny + 1), sEy = 4u * (Nx + 1) * Ny, sEz = 4u * (Nx + 1) * (Ny + 1);
     const PlaneBuf rHx = plane_buf(F.hx, sHx * Nz), rHy = plane_buf(F.hy, sHy * Nz), rHz = plane_buf(F.hz, sHz * (Nz + 1)),
@@ -452,21 +457,22 @@ __device__ inline void overlap2_tile(const Fields &F, const FDTD_PARAMS &p, int 
     const bool jIn = j >= 1 && j <= Ny - 1, iIn = i >= 1 && i <= Nx - 1;
 
     struct Set {
-        float ex, ey, ez, hx, hy, hz;
+        F3 e, h;
     };
-    // S_m = {E0(m+1), H0(m)}.  Only m <= k1 feed owned cells; a later set reloads S_k1's planes
-    // (cached) and a plane past a field's last is read as its last: those values reach no
-    // stored cell, as the planes of the first step that are formed from them do not.
+    // S_m = {E_0(m+1), H_0(m)}.  The owned planes below k1 depend on input planes up to T above
+    // them (each step reaches one plane up), so only m <= k1 + T - 2 feed owned cells; a later
+    // set reloads that set's planes (cached) and a plane past a field's last is read as its
+    // last: those values reach no stored cell, as the planes formed from them do not.
     auto load_set = [&](Set &S, int m) {
-        const int k = m < k1 ? m : k1;
+        const int k = m < k1 + T - 2 ? m : k1 + T - 2;
         const int ke = k + 1 <= Nz ? k + 1 : Nz, kez = k + 1 <= Nz - 1 ? k + 1 : Nz - 1;
         const int kh = k <= Nz - 1 ? k : Nz - 1, khz = k <= Nz ? k : Nz;
-        S.ex = rEx.load(qEx, (unsigned)ke * sEx);
-        S.ey = rEy.load(qEy, (unsigned)ke * sEy);
-        S.ez = rEz.load(qEz, (unsigned)kez * sEz);
-        S.hx = rHx.load(qHx, (unsigned)kh * sHx);
-        S.hy = rHy.load(qHy, (unsigned)kh * sHy);
-        S.hz = rHz.load(qHz, (unsigned)khz * sHz);
+        S.e.x = rEx.load(qEx, (unsigned)ke * sEx);
+        S.e.y = rEy.load(qEy, (unsigned)ke * sEy);
+        S.e.z = rEz.load(qEz, (unsigned)kez * sEz);
+        S.h.x = rHx.load(qHx, (unsigned)kh * sHx);
+        S.h.y = rHy.load(qHy, (unsigned)kh * sHy);
+        S.h.z = rHz.load(qHz, (unsigned)khz * sHz);
     };
     // six dropped stores: the prologue then leaves the memory counter as a loop iteration does,
     // so the waits the compiler places in the loop stay two planes behind the loads
@@ -478,85 +484,108 @@ __device__ inline void overlap2_tile(const Fields &F, const FDTD_PARAMS &p, int 
         wEy.store(kNoCell, 0u, 0.f);
         wEz.store(kNoCell, 0u, 0.f);
     };
-    const int ms = k0 >= 2 ? k0 - 2 : 0, me = k1 + 2;  // iterations m in [ms, me)
+    constexpr int kLag = 2 * (T - 1);
+    constexpr bool kEFromLds = T >= 3;  // registers: 128 per lane at 16 waves per CU
+    // Owned planes from k0 need step s from plane k0 - (T - s) (E) or one below (H), i.e. the
+    // first step from k0 - T; the last owned plane k1 - 1 leaves step T at iteration k1 - 1 +
+    // 2(T-1).  Iterations m in [ms, me).
+    const int ms = k0 >= T ? k0 - T : 0, me = k1 + kLag;
     Set S0, S1, S2, S3;
     {
         const int ke = ms, kez = ms <= Nz - 1 ? ms : Nz - 1;  // ms <= k0 < k1 <= Nz + 1
-        S0.ex = rEx.load(qEx, (unsigned)ke * sEx);  // E0(ms)
-        S0.ey = rEy.load(qEy, (unsigned)ke * sEy);
-        S0.ez = rEz.load(qEz, (unsigned)kez * sEz);
-        S0.hx = S0.hy = S0.hz = 0.f;
+        S0.e.x = rEx.load(qEx, (unsigned)ke * sEx);  // E_0(ms)
+        S0.e.y = rEy.load(qEy, (unsigned)ke * sEy);
+        S0.e.z = rEz.load(qEz, (unsigned)kez * sEz);
+        S0.h = F3{0.f, 0.f, 0.f};
     }
     load_set(S1, ms);
     pad_stores();
     load_set(S2, ms + 1);
     pad_stores();
     const int y = r + 1, x = l + 1;
-    L.e0[ms & 1][0][y][x] = S0.ex;
-    L.e0[ms & 1][1][y][x] = S0.ey;
-    L.e0[ms & 1][2][y][x] = S0.ez;
-    // carried own values (planes relative to the iteration m)
-    float h1x1 = 0.f, h1y1 = 0.f, h1z1 = 0.f;  // H1(m-1)
-    float h1x2 = 0.f, h1y2 = 0.f, h1z2 = 0.f;  // H1(m-2)
-    float e1x1 = 0.f, e1y1 = 0.f, e1z1 = 0.f;  // E1(m-1)
-    float e1x2 = 0.f, e1y2 = 0.f, e1z2 = 0.f;  // E1(m-2)
-    float h2x3 = 0.f, h2y3 = 0.f;              // H2(m-3): Hx, Hy
+    L.e[0][ms & 1][0][y][x] = S0.e.x;
+    L.e[0][ms & 1][1][y][x] = S0.e.y;
+    L.e[0][ms & 1][2][y][x] = S0.e.z;
+    // carried own values of steps 1..T: planes one and two behind the step's own (index 0 unused)
+    F3 h1[T + 1], h2[T + 1], e1[T + 1], e2[T + 1];
+#pragma unroll
+    for (int s = 0; s <= T; ++s) h1[s] = h2[s] = e1[s] = e2[s] = F3{0.f, 0.f, 0.f};
+
     lds_barrier();
     auto plane = [&](int m, const Set &P, const Set &S, Set &N) {
-        const int b = m & 1, q = m - 2;
+        const int b = m & 1, hb = HB == 2 ? b : 0;
         load_set(N, m + 2);
-        // ---- S1: H1(m) and H2(m-2)
-        const float ex0 = P.ex, ey0 = P.ey, ez0 = P.ez;  // E0(m)
-        const float hx1 = S.hx + c.cH * ((S.ey - ey0) * c.Cz - (L.e0[b][2][y + 1][x] - ez0) * c.Cy);
-        const float hy1 = S.hy + c.cH * ((L.e0[b][2][y][x + 1] - ez0) * c.Cx - (S.ex - ex0) * c.Cz);
-        const float hz1 = S.hz + c.cH * ((L.e0[b][0][y + 1][x] - ex0) * c.Cy - (L.e0[b][1][y][x + 1] - ey0) * c.Cx);
-        const float hx2 = h1x2 + c.cH * ((e1y1 - e1y2) * c.Cz - (L.e1[b][2][y + 1][x] - e1z2) * c.Cy);
-        const float hy2 = h1y2 + c.cH * ((L.e1[b][2][y][x + 1] - e1z2) * c.Cx - (e1x1 - e1x2) * c.Cz);
-        const float hz2 = h1z2 + c.cH * ((L.e1[b][0][y + 1][x] - e1x2) * c.Cy - (L.e1[b][1][y][x + 1] - e1y2) * c.Cx);
-        L.h1[0][y][x] = hx1;
-        L.h1[1][y][x] = hy1;
-        L.h1[2][y][x] = hz1;
-        L.h2[0][y][x] = hx2;
-        L.h2[1][y][x] = hy2;
-        L.h2[2][y][x] = hz2;
-        L.e0[b ^ 1][0][y][x] = S.ex;
-        L.e0[b ^ 1][1][y][x] = S.ey;
-        L.e0[b ^ 1][2][y][x] = S.ez;
+        // issue the loads of the plane two ahead before this plane's LDS traffic and its waits
+        __builtin_amdgcn_sched_barrier(0);
+        // ---- A: H_s(p_s) for every step
+        F3 hn[T + 1], eo[T + 1];
+#pragma unroll
+        for (int s = 1; s <= T; ++s) {
+            // E_{s-1}(p_s) and E_{s-1}(p_s + 1): with three or more steps, the own slots of the
+            // E_{s-1} planes in LDS rather than registers carried across planes
+            const auto &E = L.e[s - 1][b];
+            const auto &Eu = L.e[s - 1][b ^ 1];
+            const F3 ep = s == 1 ? P.e : kEFromLds ? F3{E[0][y][x], E[1][y][x], E[2][y][x]} : e2[s - 1];
+            const F3 eu = s == 1 ? S.e : kEFromLds ? F3{Eu[0][y][x], Eu[1][y][x], Eu[2][y][x]} : e1[s - 1];
+            const F3 ho = s == 1 ? S.h : h2[s - 1];  // H_{s-1}(p_s)
+            eo[s] = ep;
+            hn[s].x = ho.x + c.cH * ((eu.y - ep.y) * c.Cz - (E[2][y + 1][x] - ep.z) * c.Cy);
+            hn[s].y = ho.y + c.cH * ((E[2][y][x + 1] - ep.z) * c.Cx - (eu.x - ep.x) * c.Cz);
+            hn[s].z = ho.z + c.cH * ((E[0][y + 1][x] - ep.x) * c.Cy - (E[1][y][x + 1] - ep.y) * c.Cx);
+            L.h[hb][s - 1][0][y][x] = hn[s].x;
+            L.h[hb][s - 1][1][y][x] = hn[s].y;
+            L.h[hb][s - 1][2][y][x] = hn[s].z;
+        }
+        L.e[0][b ^ 1][0][y][x] = S.e.x;
+        L.e[0][b ^ 1][1][y][x] = S.e.y;
+        L.e[0][b ^ 1][2][y][x] = S.e.z;
         lds_barrier();
-        // ---- S2: E1(m) and E2(m-2) (graphfdtd.cpp:243-312: interior cells update, faces copy)
-        const bool kIn1 = m >= 1 && m <= Nz - 1, kIn2 = q >= 1 && q <= Nz - 1;
-        const float wx1 = ex0 + c.cE * ((hz1 - L.h1[2][y - 1][x]) * c.Cy - (hy1 - h1y1) * c.Cz);
-        const float wy1 = ey0 + c.cE * ((hx1 - h1x1) * c.Cz - (hz1 - L.h1[2][y][x - 1]) * c.Cx);
-        const float wz1 = ez0 + c.cE * ((hy1 - L.h1[1][y][x - 1]) * c.Cx - (hx1 - L.h1[0][y - 1][x]) * c.Cy);
-        const float ex1 = (jIn && kIn1) ? wx1 : ex0;
-        const float ey1 = (iIn && kIn1) ? wy1 : ey0;
-        const float ez1 = (iIn && jIn) ? wz1 : ez0;
-        const float wx2 = e1x2 + c.cE * ((hz2 - L.h2[2][y - 1][x]) * c.Cy - (hy2 - h2y3) * c.Cz);
-        const float wy2 = e1y2 + c.cE * ((hx2 - h2x3) * c.Cz - (hz2 - L.h2[2][y][x - 1]) * c.Cx);
-        const float wz2 = e1z2 + c.cE * ((hy2 - L.h2[1][y][x - 1]) * c.Cx - (hx2 - L.h2[0][y - 1][x]) * c.Cy);
-        const float ex2 = (jIn && kIn2) ? wx2 : e1x2;
-        const float ey2 = (iIn && kIn2) ? wy2 : e1y2;
-        const float ez2 = (iIn && jIn) ? wz2 : e1z2;
-        L.e1[b][0][y][x] = ex1;  // E1(m-2) in this buffer was last read in S1
-        L.e1[b][1][y][x] = ey1;
-        L.e1[b][2][y][x] = ez1;
+        // ---- B: E_s(p_s) for every step (graphfdtd.cpp:243-312: interior cells update, faces
+        // copy through)
+        F3 en[T + 1];
+#pragma unroll
+        for (int s = 1; s <= T; ++s) {
+            const int ps = m - 2 * (s - 1);
+            const bool kIn = ps >= 1 && ps <= Nz - 1;
+            const F3 e = eo[s];    // E_{s-1}(p_s)
+            const F3 hp = h1[s];  // H_s(p_s - 1)
+            const auto &H = L.h[hb][s - 1];
+            const float wx = e.x + c.cE * ((hn[s].z - H[2][y - 1][x]) * c.Cy - (hn[s].y - hp.y) * c.Cz);
+            const float wy = e.y + c.cE * ((hn[s].x - hp.x) * c.Cz - (hn[s].z - H[2][y][x - 1]) * c.Cx);
+            const float wz = e.z + c.cE * ((hn[s].y - H[1][y][x - 1]) * c.Cx - (hn[s].x - H[0][y - 1][x]) * c.Cy);
+            en[s].x = (jIn && kIn) ? wx : e.x;
+            en[s].y = (iIn && kIn) ? wy : e.y;
+            en[s].z = (iIn && jIn) ? wz : e.z;
+            if (s < T) {  // E_s(p_s) in this parity's buffer was last read in A
+                L.e[s][b][0][y][x] = en[s].x;
+                L.e[s][b][1][y][x] = en[s].y;
+                L.e[s][b][2][y][x] = en[s].z;
+            }
+        }
         // plane q of each output if it is in the slab and the field, else bit 31 and plane 0
+        const int q = m - kLag;
         const bool out = q >= k0 && q < k1;
         const bool o1 = out && q < Nz, o2 = out && q <= Nz;
-        const unsigned m1 = o1 ? 0u : kNoCell, m2 = o2 ? 0u : kNoCell;
+        const unsigned m1 = ownBit | (o1 ? 0u : kNoCell), m2 = ownBit | (o2 ? 0u : kNoCell);
         const unsigned p1 = o1 ? (unsigned)q : 0u, p2 = o2 ? (unsigned)q : 0u;
-        wHx.store(wqHx | m1, p1 * sHx, hx2);
-        wHy.store(wqHy | m1, p1 * sHy, hy2);
-        wHz.store(wqHz | m2, p2 * sHz, hz2);
-        wEx.store(wqEx | m2, p2 * sEx, ex2);
-        wEy.store(wqEy | m2, p2 * sEy, ey2);
-        wEz.store(wqEz | m1, p1 * sEz, ez2);
-        h1x2 = h1x1, h1y2 = h1y1, h1z2 = h1z1;
-        h1x1 = hx1, h1y1 = hy1, h1z1 = hz1;
-        e1x2 = e1x1, e1y2 = e1y1, e1z2 = e1z1;
-        e1x1 = ex1, e1y1 = ey1, e1z1 = ez1;
-        h2x3 = hx2, h2y3 = hy2;
-        lds_barrier();
+        wHx.store(qHx | m1, p1 * sHx, hn[T].x);
+        wHy.store(qHy | m1, p1 * sHy, hn[T].y);
+        wHz.store(qHz | m2, p2 * sHz, hn[T].z);
+        wEx.store(qEx | m2, p2 * sEx, en[T].x);
+        wEy.store(qEy | m2, p2 * sEy, en[T].y);
+        wEz.store(qEz | m1, p1 * sEz, en[T].z);
+#pragma unroll
+        for (int s = 1; s <= T; ++s) {
+            h2[s] = h1[s];
+            h1[s] = hn[s];
+            e2[s] = e1[s];
+            e1[s] = en[s];
+        }
+        // With one H buffer, A of the next plane must not overwrite it before every wave has
+        // read it here.  With two, the next A writes the other buffer, and every plane B writes
+        // here is read two planes on, after the next barrier; the E_0 plane A writes next (the
+        // other parity of e[0]) was last read in this plane's A, before the barrier above.
+        if (HB == 1) lds_barrier();
     };
     // The sets rotate through four planes, and the loop runs whole groups of four with no branch
     // inside: up to three planes past me - 1 are formed and discarded (q >= k1: nothing stored).

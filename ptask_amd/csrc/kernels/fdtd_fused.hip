@@ -88,17 +88,21 @@ __device__ inline void overlap_grid(const Fields &F, const FDTD_PARAMS &p, Overl
 OVERLAP_KERNEL(FdtdStepOverlap, 16)
 OVERLAP_KERNEL(FdtdStepOverlap8, 8)
 
-// Two time steps per launch on overlapped tiles (see overlap2_tile in fdtd_fused.h), same ABI as
-// FdtdStepFused: the outputs hold the fields two steps after the inputs.  Launch: block
-// (64, R, 1) with 5 <= R <= 16; grid (ceil((Nx+1)/60), ceil((Ny+1)/(R-4)), slabs).  Smaller
-// grids stride over the tiles; gridDim.z splits [0, Nz] into slabs.  Tiles are taken in the
+// T time steps per launch on overlapped tiles (overlapn_tile in fdtd_fused.h), same ABI as
+// FdtdStepFused: the outputs hold the fields T steps after the inputs.  Launch: block (64, R, 1)
+// with 2T+1 <= R <= 16; grid (ceil((Nx+1)/(64-2T)), ceil((Ny+1)/(R-2T)), slabs).  Smaller grids
+// stride over the tiles; gridDim.z splits [0, Nz] into slabs.  Tiles are taken in the
 // XCD-contiguous order of FdtdStepOverlap.
-__device__ inline void overlap2_grid(const Fields &F, const FDTD_PARAMS &p, Overlap2Lds<16> &L) {
+//   FdtdTwoStepOverlap    T = 2: 60 x (R-4) owned columns per block, 85.5 KB of LDS
+//   FdtdThreeStepOverlap  T = 3: 58 x (R-6) owned columns per block, 125.3 KB of LDS
+template <int T, int HB>
+__device__ inline void overlapn_grid(const Fields &F, const FDTD_PARAMS &p, OverlapNLds<T, 16, HB> &L) {
     const int R = (int)blockDim.y;
-    if (blockDim.x != 64 || R < 5 || R > 16 || blockDim.z != 1) return;  // geometry contract
+    if (blockDim.x != 64 || R < 2 * T + 1 || R > 16 || blockDim.z != 1) return;  // geometry contract
     // the launcher rejects such grids before launching; planes are addressed from 0 to Nz
     if (!fits_buffer(p) || p.Nx < 1 || p.Ny < 1 || p.Nz < 1) return;
-    const int ti = (p.Nx + 1 + 59) / 60, tj = (p.Ny + 1 + R - 5) / (R - 4);
+    const int ow = 64 - 2 * T, oh = R - 2 * T;  // owned columns and rows of a tile
+    const int ti = (p.Nx + ow) / ow, tj = (p.Ny + oh) / oh;
     const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
     unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     if (PTK_OVERLAP_XCD && nblk >= kNumXcd) {
@@ -111,19 +115,25 @@ __device__ inline void overlap2_grid(const Fields &F, const FDTD_PARAMS &p, Over
     if (k0 >= k1) return;  // the whole block
     for (int by = gy; by < tj; by += gridDim.y)
         for (int bx = gx; bx < ti; bx += gridDim.x) {
-            overlap2_tile<16>(F, p, bx * 60, by * (R - 4), k0, k1, L);
+            overlapn_tile<T, 16>(F, p, bx * ow, by * oh, k0, k1, L);
             lds_barrier();  // the next tile reuses LDS
         }
 }
 
-extern "C" __global__ void __launch_bounds__(1024)
-FdtdTwoStepOverlap(const float *hx, const float *hy, const float *hz, const float *ex, const float *ey,
-                   const float *ez, FDTD_PARAMS p, float *ohx, float *ohy, float *ohz, float *oex, float *oey,
-                   float *oez) {
-    __shared__ Overlap2Lds<16> L;
-    const Fields F{hx, hy, hz, ex, ey, ez, ohx, ohy, ohz, oex, oey, oez};
-    overlap2_grid(F, p, L);
-}
+#define OVERLAPN_KERNEL(NAME, T, HB)                                                                  \
+    extern "C" __global__ void __launch_bounds__(1024)                                                \
+    NAME(const float *hx, const float *hy, const float *hz, const float *ex, const float *ey,         \
+         const float *ez, FDTD_PARAMS p, float *ohx, float *ohy, float *ohz, float *oex, float *oey, \
+         float *oez) {                                                                                \
+        __shared__ OverlapNLds<T, 16, HB> L;                                                          \
+        const Fields F{hx, hy, hz, ex, ey, ez, ohx, ohy, ohz, oex, oey, oez};                         \
+        overlapn_grid<T, HB>(F, p, L);                                                                \
+    }
+
+// One H buffer and two barriers per plane: measured faster than H planes by parity with one
+// barrier (the 16 waves then issue each plane's loads together).
+OVERLAPN_KERNEL(FdtdTwoStepOverlap, 2, 1)
+OVERLAPN_KERNEL(FdtdThreeStepOverlap, 3, 1)
 
 // ---- half-step kernels: all three H updates in one streaming pass, then all three E updates ----
 //

@@ -315,40 +315,46 @@ def test_overlap_step_bitexact(hip, fmod, shape, geom, name):
     assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 4))
 
 
-def overlap2_geometry(rows, slab, max_tiles=None):
-    """Two-step overlapped tiles: blocks of 64 x rows own 60 x (rows-4) columns; k slabs of ~slab
-    planes; max_tiles caps the grid so blocks take further tiles in grid strides."""
+def overlapn_geometry(T, rows, slab, max_tiles=None):
+    """T-step overlapped tiles: blocks of 64 x rows own (64-2T) x (rows-2T) columns; k slabs of
+    ~slab planes; max_tiles caps the grid so blocks take further tiles in grid strides."""
     def geo(nx, ny, nz):
-        ti, tj = -(-(nx + 1) // 60), -(-(ny + 1) // (rows - 4))
+        ti, tj = -(-(nx + 1) // (64 - 2 * T)), -(-(ny + 1) // (rows - 2 * T))
         if max_tiles:
             ti, tj = min(ti, max_tiles), min(tj, max_tiles)
         return (ti, tj, max(1, -(-(nz + 1) // slab))), (64, rows, 1)
     return geo
 
 
-OVERLAP2 = {
+MULTI_STEP = {"FdtdTwoStepOverlap": 2, "FdtdThreeStepOverlap": 3}
+OVERLAPN = {
     "r16_slab16": (16, 16, None),
     "r8_slab4": (8, 4, None),
-    "r5_slab1": (5, 1, None),
-    "r6_slab2": (6, 2, None),
+    "rmin_slab1": (0, 1, None),
+    "rmin1_slab2": (1, 2, None),
     "r12_one_slab": (12, 1 << 20, None),
     "r16_slab3_strided": (16, 3, 1),
-    "r7_slab5_strided": (7, 5, 2),
+    "rmin2_slab5_strided": (2, 5, 2),
 }
 
 
-@pytest.mark.parametrize("geom", sorted(OVERLAP2))
-@pytest.mark.parametrize("shape", [(8, 8, 8), (59, 11, 4), (60, 12, 3), (119, 9, 7), (130, 29, 6),
-                                   (13, 21, 17), (1, 1, 1), (3, 1, 2), (1, 5, 3), (2, 2, 9)])
-def test_two_step_overlap_bitexact(hip, fmod, shape, geom):
-    """FdtdTwoStepOverlap: each launch advances two whole steps on overlapped tiles and equals
-    two rounds of the six reference updates bit for bit, for tile edges at and around the
-    60-column / (R-4)-row tile sizes, slabs of one plane up to the whole depth, and grids that
-    stride over tiles."""
-    rows, slab, cap = OVERLAP2[geom]
+@pytest.mark.parametrize("name", sorted(MULTI_STEP))
+@pytest.mark.parametrize("geom", sorted(OVERLAPN))
+@pytest.mark.parametrize("shape", [(8, 8, 8), (57, 9, 4), (58, 10, 3), (59, 11, 4), (60, 12, 3),
+                                   (119, 9, 7), (130, 29, 6), (13, 21, 17), (1, 1, 1), (3, 1, 2),
+                                   (1, 5, 3), (2, 2, 9)])
+def test_multi_step_overlap_bitexact(hip, fmod, shape, geom, name):
+    """FdtdTwoStepOverlap / FdtdThreeStepOverlap: each launch advances T whole steps on
+    overlapped tiles and equals T rounds of the six reference updates bit for bit, for tile edges
+    at and around the (64-2T)-column / (R-2T)-row tile sizes, the smallest blocks (R = 2T+1),
+    slabs of one plane up to the whole depth, and grids that stride over tiles."""
+    T = MULTI_STEP[name]
+    rows, slab, cap = OVERLAPN[geom]
+    if rows < 2 * T + 1:  # "rmin*": the smallest block height for T, plus rows
+        rows += 2 * T + 1
     nx, ny, nz = shape
-    print(f"case overlap2 shape={shape} geom={geom}", flush=True)
+    print(f"case {name} shape={shape} geom={geom}", flush=True)
     F = oracle.init_fields(nx, ny, nz, seed=307 + nx + ny)
-    geo = overlap2_geometry(rows, slab, cap)(nx, ny, nz)
-    got = run_fused(hip, fmod, "FdtdTwoStepOverlap", F, FD.make_params(nx, ny, nz), 3, geo)
-    assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 6))
+    geo = overlapn_geometry(T, rows, slab, cap)(nx, ny, nz)
+    got = run_fused(hip, fmod, name, F, FD.make_params(nx, ny, nz), 2, geo)
+    assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 2 * T))

@@ -50,7 +50,7 @@ def direct(nx, ny, nz, steps, seed):
     return oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), steps)
 
 
-TASKS_PER_STEP = {"six": 6, "halves": 2, "fused": 1, "overlap": 1, "overlap2": 0.5}
+TASKS_PER_STEP = {"six": 6, "halves": 2, "fused": 1, "overlap": 1, "overlap2": 1 / 2, "overlap3": 1 / 3}
 
 
 @pytest.mark.parametrize("form", sorted(TASKS_PER_STEP))
@@ -62,7 +62,7 @@ def test_host_graph_equals_direct_oracle(tmp_path, mode, shape, form):
     r = run_app("--kernels", HOSTLIB, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", 6,
                 "--mode", mode, "--form", form, "--seed", 99, dump=d)
     assert r["ok"] and r["backend"] == "host"
-    assert r["dispatches_last_run"] == TASKS_PER_STEP[form] * 6
+    assert r["dispatches_last_run"] == round(TASKS_PER_STEP[form] * 6)
     got = load_dump(d, nx, ny, nz)
     ref = direct(nx, ny, nz, 6, 99)
     for f in range(6):
@@ -99,10 +99,43 @@ def test_missing_kernel_file_fails_loudly():
         or "accelerator" in p.stderr
 
 
-def test_two_step_form_rejects_odd_step_counts():
-    p = subprocess.run([APP, "--kernels", HOSTLIB, "--n", "4", "--steps", "3", "--form", "overlap2"],
+MULTI = {"overlap2": 2, "overlap3": 3}
+
+
+def multi_dispatches(form, steps):
+    """T-step tasks, then one task for the remaining steps % T."""
+    T = MULTI[form]
+    return steps // T + (1 if steps % T else 0)
+
+
+@pytest.mark.parametrize("mode", ["looped", "unrolled"])
+@pytest.mark.parametrize("form,steps", [("overlap2", 5), ("overlap3", 7), ("overlap3", 8), ("overlap3", 3)])
+def test_multi_step_forms_run_the_remainder_after_the_loop(tmp_path, mode, form, steps):
+    """A step count that is not a multiple of the form's T: the looped graph runs steps // T
+    iterations and then one remainder task (one or two steps) fed on the loop's last iteration;
+    the result is that of `steps` reference steps."""
+    nx, ny, nz = 7, 6, 5
+    d = str(tmp_path / "f.bin")
+    r = run_app("--kernels", HOSTLIB, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
+                "--mode", mode, "--form", form, "--seed", 41, dump=d)
+    assert r["ok"] and r["dispatches_last_run"] == multi_dispatches(form, steps)
+    got, ref = load_dump(d, nx, ny, nz), direct(nx, ny, nz, steps, 41)
+    for f in range(6):
+        assert np.array_equal(got[f].view(np.uint32), ref[f].view(np.uint32)), oracle.FIELDS[f]
+
+
+def test_multi_step_chained_runs_with_remainder_equal_one_long_run():
+    c = run_app("--kernels", HOSTLIB, "--form", "overlap3", "--n", 6, "--steps", 5, "--repeat", 2,
+                "--warmup", 1, "--chain")
+    one = run_app("--kernels", HOSTLIB, "--n", 6, "--steps", 20)
+    assert [x["fnv1a"] for x in c["checksums"]] == [x["fnv1a"] for x in one["checksums"]]
+
+
+@pytest.mark.parametrize("form,steps", [("overlap2", 1), ("overlap3", 2)])
+def test_multi_step_forms_need_one_whole_task(form, steps):
+    p = subprocess.run([APP, "--kernels", HOSTLIB, "--n", "4", "--steps", str(steps), "--form", form],
                        capture_output=True, text=True, timeout=60)
-    assert p.returncode != 0 and "even" in p.stderr
+    assert p.returncode != 0 and "at least" in p.stderr
 
 
 @pytest.mark.gpu
@@ -121,7 +154,7 @@ def test_hip_looped_graph_equals_host_graph(tmp_path, geometry):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", ["halves", "fused", "overlap", "overlap2"])
+@pytest.mark.parametrize("form", ["halves", "fused", "overlap", "overlap2", "overlap3"])
 @pytest.mark.parametrize("mode", ["looped", "unrolled"])
 def test_hip_fused_forms_equal_host_graph(tmp_path, mode, form):
     nx, ny, nz, steps = 70, 9, 11, 6
@@ -130,7 +163,7 @@ def test_hip_fused_forms_equal_host_graph(tmp_path, mode, form):
             "--mode", mode, "--form", "six", "--seed", 3, dump=dh)
     g = run_app("--kernels", HSACO_FUSED, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
                 "--mode", mode, "--form", form, "--seed", 3, dump=dg)
-    assert g["ok"] and g["dispatches_last_run"] == TASKS_PER_STEP[form] * steps
+    assert g["ok"] and g["dispatches_last_run"] == round(TASKS_PER_STEP[form] * steps)
     H, G = load_dump(dh, nx, ny, nz), load_dump(dg, nx, ny, nz)
     for f in range(6):
         assert np.array_equal(H[f].view(np.uint32), G[f].view(np.uint32)), oracle.FIELDS[f]
@@ -139,15 +172,32 @@ def test_hip_fused_forms_equal_host_graph(tmp_path, mode, form):
 @pytest.mark.gpu
 @pytest.mark.parametrize("form,geo", [("halves", ["--tile", "64x4x3"]), ("halves", ["--tile", "128x2x64"]),
                                       ("overlap", ["--tile", "64x3x1"]), ("overlap", ["--tile", "64x8x5"]),
-                                      ("overlap2", ["--tile", "64x5x1"]), ("overlap2", ["--tile", "64x12x7"])])
+                                      ("overlap2", ["--tile", "64x5x1"]), ("overlap2", ["--tile", "64x12x7"]),
+                                      ("overlap3", ["--tile", "64x7x1"]), ("overlap3", ["--tile", "64x13x5"])])
 def test_hip_tile_geometries_equal_host_graph(tmp_path, form, geo):
-    nx, ny, nz, steps = 70, 9, 11, 4
+    nx, ny, nz, steps = 70, 9, 11, 6
     dh, dg = str(tmp_path / "h.bin"), str(tmp_path / "g.bin")
     run_app("--kernels", HOSTLIB, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
             "--form", "six", "--seed", 9, dump=dh)
     g = run_app("--kernels", HSACO_FUSED, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
                 "--form", form, "--seed", 9, *geo, dump=dg)
-    assert g["ok"] and g["dispatches_last_run"] == TASKS_PER_STEP[form] * steps
+    assert g["ok"] and g["dispatches_last_run"] == round(TASKS_PER_STEP[form] * steps)
+    H, G = load_dump(dh, nx, ny, nz), load_dump(dg, nx, ny, nz)
+    for f in range(6):
+        assert np.array_equal(H[f].view(np.uint32), G[f].view(np.uint32)), oracle.FIELDS[f]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["looped", "unrolled"])
+@pytest.mark.parametrize("form,steps", [("overlap2", 7), ("overlap3", 7), ("overlap3", 8)])
+def test_hip_multi_step_remainder_equals_host_graph(tmp_path, mode, form, steps):
+    nx, ny, nz = 70, 9, 11
+    dh, dg = str(tmp_path / "h.bin"), str(tmp_path / "g.bin")
+    run_app("--kernels", HOSTLIB, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
+            "--form", "six", "--seed", 13, dump=dh)
+    g = run_app("--kernels", HSACO_FUSED, "--nx", nx, "--ny", ny, "--nz", nz, "--steps", steps,
+                "--mode", mode, "--form", form, "--seed", 13, dump=dg)
+    assert g["ok"] and g["dispatches_last_run"] == multi_dispatches(form, steps)
     H, G = load_dump(dh, nx, ny, nz), load_dump(dg, nx, ny, nz)
     for f in range(6):
         assert np.array_equal(H[f].view(np.uint32), G[f].view(np.uint32)), oracle.FIELDS[f]
