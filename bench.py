@@ -206,7 +206,8 @@ def leg_served(seconds: float, rates: list) -> dict:
 def leg_captured(n: int, steps: int) -> dict:
     """The 384^3 FDTD graph as one instance of the batched executor: plain per-task dispatch
     (the six reference Yee tasks launched one by one, every step) against hipGraph-captured
-    iterations (one replayed graph per step, the whole-step kernel on overlapped tiles).  Device
+    iterations (one replayed graph per three steps, the three-step kernel on overlapped tiles,
+    then one-step graphs for the remaining steps % 3).  Device
     time of the iterations from the executor's events; wall time includes admission (field
     generation on the device) and retirement (result checksums)."""
     from ptask_amd import batch
@@ -224,7 +225,8 @@ def leg_captured(n: int, steps: int) -> dict:
             ex_s = st["exec_us"] / 1e6
             out[name] = {"exec_ms": round(ex_s * 1e3, 3), "value": round(n ** 3 * steps / ex_s / 1e6, 1),
                          "wall_ms": round(wall * 1e3, 3), "launches": st["launches"],
-                         "graph_launches": st["graph_launches"]}
+                         "graph_launches": st["graph_launches"],
+                         "fdtd_steps_per_launch": ex.fdtd_steps()[0] if mode == batch.BATCHED else 1}
         progress(f"captured leg {name}: {out[name]['value']} Mcell-updates/s")
     out["speedup"] = round(out["captured"]["value"] / out["plain"]["value"], 2)
     out["checksums_equal"] = sums["plain"] == sums["captured"]

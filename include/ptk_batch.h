@@ -43,7 +43,9 @@ enum { PTK_STATE_QUEUED = 0, PTK_STATE_ACTIVE = 1, PTK_STATE_DONE = 2, PTK_STATE
 typedef struct _ptk_fdtd_inst {
     float *f[2][6];       /* field sets A and B: Hx, Hy, Hz, Ex, Ey, Ez */
     FDTD_PARAMS p;
-    int steps, done;      /* target steps; steps completed (advanced on the device) */
+    int steps, done;      /* target steps; launches completed (advanced on the device): each
+                             launch, of one or three steps, writes the other set, so the current
+                             set is done & 1 */
 } PTK_FDTD_INST;
 
 typedef struct _ptk_groupby_inst {
@@ -126,6 +128,16 @@ int ptk_batch_release(ptk_batch *b, int id);
 int ptk_batch_set_tiles(ptk_batch *b, int fdtd_cols, int fdtd_depth, int gb_rows);
 int ptk_batch_get_tiles(ptk_batch *b, int *fdtd_cols, int *fdtd_depth, int *gb_rows);
 int ptk_batch_autotune(ptk_batch *b, double *best_ms);
+
+/* FDTD time steps per batched launch with the whole-step form (fdtd_cols == 0): 3 (the default)
+ * advances every resident FDTD instance three steps per launch on overlapped 58 x 10-column tiles
+ * (BatchFdtdSteps3) while each has at least three steps left, then one step per launch for the
+ * rest; 1 keeps one step per launch.  Results are bitwise those of single steps.  No
+ * counterpart in the reference, whose graphs run one task per step (graphfdtd.cpp:735-739).
+ * ptk_batch_autotune also compares both, per step, and *depth receives the three-step tiles' k
+ * planes. */
+int ptk_batch_set_fdtd_steps(ptk_batch *b, int steps_per_launch);
+int ptk_batch_get_fdtd_steps(ptk_batch *b, int *steps_per_launch, int *depth);
 
 /* Launch form of batched iterations: 1 (default) one captured hipGraph per mix of instance kinds,
  * replayed every iteration; 0 the same kernels launched directly with grids cut to the current

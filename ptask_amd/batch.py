@@ -43,6 +43,8 @@ def lib() -> C.CDLL:
             "ptk_batch_release": [P, I],
             "ptk_batch_set_tiles": [P, I, I, I],
             "ptk_batch_set_launch": [P, I],
+            "ptk_batch_set_fdtd_steps": [P, I],
+            "ptk_batch_get_fdtd_steps": [P, C.POINTER(I), C.POINTER(I)],
             "ptk_batch_get_tiles": [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)],
             "ptk_batch_autotune": [P, C.POINTER(D)],
             "ptk_batch_stats": [P, C.POINTER(U64)],
@@ -140,6 +142,15 @@ class Executor:
 
     def set_tiles(self, cols, depth, gb_rows) -> None:
         _check(lib().ptk_batch_set_tiles(self.h, cols, depth, gb_rows), "set_tiles")
+
+    def set_fdtd_steps(self, steps: int) -> None:
+        """FDTD steps per batched launch with the whole-step form: 3 (default) or 1."""
+        _check(lib().ptk_batch_set_fdtd_steps(self.h, steps), "set_fdtd_steps")
+
+    def fdtd_steps(self):
+        a, b = C.c_int(), C.c_int()
+        _check(lib().ptk_batch_get_fdtd_steps(self.h, C.byref(a), C.byref(b)), "get_fdtd_steps")
+        return a.value, b.value
 
     def set_launch(self, graphs: bool) -> None:
         _check(lib().ptk_batch_set_launch(self.h, int(graphs)), "set_launch")

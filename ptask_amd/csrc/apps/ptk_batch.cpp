@@ -33,7 +33,7 @@ struct Options {
     double zipf = 1.1, arenaGb = 0;
     unsigned long long seed = 0x5eed;
     bool autotune = true, direct = false;
-    int cols = 0, depth = 0, gbRows = 0;
+    int cols = 0, depth = 0, gbRows = 0, launchSteps = 0;  // launchSteps 0: the executor's default
 };
 
 struct Spec {
@@ -77,6 +77,7 @@ int main(int argc, char **argv) {
         else if (a == "--seed") o.seed = strtoull(next(), nullptr, 0);
         else if (a == "--no-autotune") o.autotune = false;
         else if (a == "--direct") o.direct = true;
+        else if (a == "--launch-steps") o.launchSteps = atoi(next());
         else if (a == "--tiles") {
             if (sscanf(next(), "%d,%d,%d", &o.cols, &o.depth, &o.gbRows) != 3) { fprintf(stderr, "--tiles C,D,R\n"); return 2; }
         } else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
@@ -85,7 +86,8 @@ int main(int argc, char **argv) {
         o.gbMax < o.gbMin || o.keys < 1 || o.fdtdSteps < 1) {
         fprintf(stderr, "usage: ptk_batch [--modes plain,batched] [--fdtd N] [--fdtd-min E] [--fdtd-max E] "
                         "[--fdtd-steps S] [--groupby N] [--gb-min R] [--gb-max R] [--keys K] [--zipf S] "
-                        "[--gmm N ...] [--arena-gb G] [--seed S] [--no-autotune] [--tiles C,D,R]\n");
+                        "[--gmm N ...] [--arena-gb G] [--seed S] [--no-autotune] [--tiles C,D,R] [--direct] "
+                        "[--launch-steps 1|3]\n");
         return 2;
     }
     std::vector<Spec> specs;
@@ -128,6 +130,10 @@ int main(int argc, char **argv) {
         }
         if (o.cols > 0) ptk_batch_set_tiles(B, o.cols, o.depth, o.gbRows);
         if (o.direct) ptk_batch_set_launch(B, 0);
+        if (o.launchSteps && ptk_batch_set_fdtd_steps(B, o.launchSteps)) {
+            fprintf(stderr, "--launch-steps: %s\n", ptk_batch_last_error());
+            return 2;
+        }
         std::vector<int> ids;
         for (const Spec &s : specs) {
             int id = -1, r = 0;
@@ -155,8 +161,9 @@ int main(int argc, char **argv) {
         }
         unsigned long long st[7];
         ptk_batch_stats(B, st);
-        int cols, depth, gbr;
+        int cols, depth, gbr, lsteps, depth3;
         ptk_batch_get_tiles(B, &cols, &depth, &gbr);
+        ptk_batch_get_fdtd_steps(B, &lsteps, &depth3);
         sums[mode].clear();
         unsigned long long gbBad = 0;
         for (size_t i = 0; i < ids.size(); ++i) {
@@ -170,10 +177,10 @@ int main(int argc, char **argv) {
         const double ex = st[6] / 1e3;
         snprintf(buf, sizeof(buf), "%s\"%s\": {\"ms\": %.3f, \"instances_per_s\": %.2f, \"exec_ms\": %.3f, "
                  "\"exec_instances_per_s\": %.2f, \"ticks\": %llu, \"launches\": %llu, \"graph_launches\": %llu, "
-                 "\"arena_peak\": %llu, \"tiles\": [%d, %d, %d], \"autotune_ms\": %.4f, "
-                 "\"groupby_permutation_failures\": %llu}",
+                 "\"arena_peak\": %llu, \"tiles\": [%d, %d, %d], \"fdtd_steps_per_launch\": %d, "
+                 "\"fdtd_depth3\": %d, \"autotune_ms\": %.4f, \"groupby_permutation_failures\": %llu}",
                  first ? "" : ", ", name, ms, specs.size() / (ms / 1e3), ex, ex > 0 ? specs.size() / (ex / 1e3) : 0.0,
-                 st[0], st[1], st[2], st[4], cols, depth, gbr, tuneMs, gbBad);
+                 st[0], st[1], st[2], st[4], cols, depth, gbr, lsteps, depth3, tuneMs, gbBad);
         json += buf;
         first = false;
         ptk_batch_destroy(B);

@@ -146,7 +146,7 @@ struct ptk_batch {
     ptk_hip_stream stream = nullptr;
     ptk_hip_event evStart = nullptr, evStop = nullptr;
     ptk_hip_module mBatch = nullptr, mFdtd = nullptr, mGb = nullptr, mGmm = nullptr, mUtil = nullptr;
-    ptk_hip_function fH = nullptr, fE = nullptr, fStep = nullptr, fAdv = nullptr, fGbAcc = nullptr,
+    ptk_hip_function fH = nullptr, fE = nullptr, fStep = nullptr, fStep3 = nullptr, fAdv = nullptr, fGbAcc = nullptr,
                      fGbMerge = nullptr, fGbOff = nullptr, fGbRes = nullptr, fGmmAcc = nullptr, fGmmFin = nullptr, fInitFdtd = nullptr,
                      fInitGb = nullptr, fInitGmm = nullptr, fInitGmmModel = nullptr, fSumWords = nullptr,
                      fSumPairs = nullptr, fSumGroups = nullptr, fSumJobs = nullptr, fFill = nullptr;
@@ -162,7 +162,7 @@ struct ptk_batch {
     int *dCounts = nullptr, *hCounts = nullptr;
     unsigned long long *dSums = nullptr, *hSums = nullptr;  // checksum staging, 8 per retiring instance
     size_t sumCap = 0;
-    List<int4> tiles;
+    List<int4> tiles, tiles3;  // FDTD tiles of the current form; three-step tiles (when cur3)
     List<int> fdtdActive;
     List<int2> gbUnits, gbBuckets, gmmAcc, gmmFin;
     List<PTK_CSUM_JOB> csumJobs;
@@ -170,12 +170,15 @@ struct ptk_batch {
     int cdfKeys = -1;
     double cdfS = 0;
     long long cdfA = 0;
-    ptk_hip_graph graphs[8] = {};  // captured iteration per mix of kinds (bit 0 FDTD, 1 groupby, 2 GMM)
+    // captured iteration per mix of kinds (bit 0 FDTD, 1 groupby, 2 GMM; bit 3 three FDTD steps)
+    ptk_hip_graph graphs[16] = {};
     bool dirty = true;
     bool useGraphs = true;  // batched iterations as captured graphs (else direct, grid-fitted launches)
     int fdtdCols = 0, fdtdDepth = 8, gbRows = 262144, fdtdGrid = 4096;
+    int fdtdSteps = 3, depth3 = 64, grid3 = 2048;  // FDTD steps per launch (1 or 3); three-step slabs, grid
     int curCols = 512, curDepth = 16;  // the shape the current work lists were built with
     bool curStep = false;              // the current FDTD tiles are whole-step (overlapped) tiles
+    bool cur3 = false;                 // tiles3 holds the three-step tiles of the resident set
     std::vector<Inst> insts;
     std::vector<int> queue, active;
     unsigned long long nTicks = 0, nLaunches = 0, nGraphs = 0;
@@ -373,8 +376,9 @@ int Admit(ptk_batch *B, Inst &in) {
 }
 
 // Work lists of the resident instances for the batched kernels.
-// Whole-step tiles own 62 x 6 columns (blocks of 64 x 8 threads overlapping by one cell).
-constexpr int kStepTileI = 62, kStepTileJ = 6;
+// Whole-step tiles own 62 x 6 columns (blocks of 64 x 8 threads overlapping by one cell);
+// three-step tiles own 58 x 10 (blocks of 64 x 16 threads overlapping by three cells).
+constexpr int kStepTileI = 62, kStepTileJ = 6, kStep3TileI = 58, kStep3TileJ = 10;
 
 // TRUE when every resident FDTD instance can take the whole-step kernel (32-bit buffer offsets).
 bool StepFits(const ptk_batch *B) {
@@ -403,6 +407,18 @@ size_t FdtdTiles(const ptk_batch *B, int cols, int depth) {
     return nt;
 }
 
+// Three-step tiles of the resident FDTD instances with slabs of `depth` planes.
+size_t Fdtd3Tiles(const ptk_batch *B, int depth) {
+    size_t nt = 0;
+    for (int id : B->active) {
+        const Inst &in = B->insts[(size_t)id];
+        if (in.kind != PTK_INST_FDTD) continue;
+        nt += (size_t)(((in.a + kStep3TileI) / kStep3TileI) * ((in.b + kStep3TileJ) / kStep3TileJ) *
+                       ((in.c + depth) / depth));
+    }
+    return nt;
+}
+
 int BuildLists(ptk_batch *B) {
     // The tuned tile shape, made smaller (depth first, then columns) while the resident FDTD
     // work would give fewer than four tiles per CU: a few small instances must still spread over
@@ -418,6 +434,13 @@ int BuildLists(ptk_batch *B) {
     B->curCols = cols;
     B->curDepth = depth;
     B->curStep = cols == 0;
+    // Three-step tiles beside them when the form is the whole step and asks for three steps per
+    // launch; slabs halve while the work would give fewer than two tiles per CU.
+    const bool three = cols == 0 && B->fdtdSteps == 3;
+    int depth3 = B->depth3;
+    while (three && Fdtd3Tiles(B, depth3) < 512 && depth3 > 8) depth3 = (depth3 + 1) / 2;
+    const size_t nt3 = three ? Fdtd3Tiles(B, depth3) : 0;
+    B->cur3 = three;
     size_t nt = FdtdTiles(B, cols, depth), nf = 0, ngu = 0, ngi = 0, nga = 0, ngf = 0;
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
@@ -432,25 +455,38 @@ int BuildLists(ptk_batch *B) {
         }
     }
     if (int r = Reserve(B, B->tiles, nt)) return r;
+    if (int r = Reserve(B, B->tiles3, nt3)) return r;
     if (int r = Reserve(B, B->fdtdActive, nf)) return r;
     if (int r = Reserve(B, B->gbUnits, ngu)) return r;
     if (int r = Reserve(B, B->gbBuckets, ngi)) return r;
     if (int r = Reserve(B, B->gmmAcc, nga)) return r;
     if (int r = Reserve(B, B->gmmFin, ngf)) return r;
-    B->tiles.n = B->fdtdActive.n = B->gbUnits.n = B->gbBuckets.n = B->gmmAcc.n = B->gmmFin.n = 0;
+    B->tiles.n = B->tiles3.n = B->fdtdActive.n = B->gbUnits.n = B->gbBuckets.n = B->gmmAcc.n = B->gmmFin.n = 0;
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
         if (in.kind == PTK_INST_FDTD) {
             const int ncol = (int)((in.a + 1) * (long long)(in.b + 1)), kdepth = in.c + 1;
             const int ti = (int)((in.a + kStepTileI) / kStepTileI), tj = (in.b + kStepTileJ) / kStepTileJ;
-            for (int k0 = 0; k0 < kdepth; k0 += depth) {
-                const int k1 = std::min(kdepth, k0 + depth);
+            // ceil(kdepth / depth) slabs of equal depth (within one plane): a short last slab
+            // would leave the device's last round of tiles uneven
+            const int slabs = (kdepth + depth - 1) / depth;
+            for (int sl = 0; sl < slabs; ++sl) {
+                const int k0 = (int)((long long)sl * kdepth / slabs), k1 = (int)((long long)(sl + 1) * kdepth / slabs);
                 if (cols == 0) {
                     for (int y = 0; y < tj; ++y)
                         for (int x = 0; x < ti; ++x) B->tiles.host[B->tiles.n++] = int4{in.slot, x, y, k0 | (k1 << 16)};
                 } else {
                     for (int c0 = 0; c0 < ncol; c0 += cols)
                         B->tiles.host[B->tiles.n++] = int4{in.slot, c0, std::min(cols, ncol - c0), k0 | (k1 << 16)};
+                }
+            }
+            if (three) {
+                const int ti3 = (int)((in.a + kStep3TileI) / kStep3TileI), tj3 = (in.b + kStep3TileJ) / kStep3TileJ;
+                const int slabs3 = (kdepth + depth3 - 1) / depth3;
+                for (int sl = 0; sl < slabs3; ++sl) {
+                    const int k0 = (int)((long long)sl * kdepth / slabs3), k1 = (int)((long long)(sl + 1) * kdepth / slabs3);
+                    for (int y = 0; y < tj3; ++y)
+                        for (int x = 0; x < ti3; ++x) B->tiles3.host[B->tiles3.n++] = int4{in.slot, x, y, k0 | (k1 << 16)};
                 }
             }
             B->fdtdActive.host[B->fdtdActive.n++] = in.slot;
@@ -469,11 +505,13 @@ int BuildLists(ptk_batch *B) {
     B->hCounts[3] = (int)B->gbBuckets.n;
     B->hCounts[4] = (int)B->gmmAcc.n;
     B->hCounts[5] = (int)B->gmmFin.n;
+    B->hCounts[6] = (int)B->tiles3.n;
     auto up = [&](auto &l) -> int {
         if (l.n) HIPCHK(ptk_hip_memcpy_htod_async(l.dev, l.host, l.n * sizeof(*l.host), B->stream));
         return PTK_BATCH_OK;
     };
     if (int r = up(B->tiles)) return r;
+    if (int r = up(B->tiles3)) return r;
     if (int r = up(B->fdtdActive)) return r;
     if (int r = up(B->gbUnits)) return r;
     if (int r = up(B->gbBuckets)) return r;
@@ -487,12 +525,17 @@ int BuildLists(ptk_batch *B) {
 // The batched iteration's launches.  Captured into a graph, the grids are the configured
 // persistent sizes (the work-list lengths are read on the device); launched directly (fit), each
 // grid is cut to the current list length, which the host knows.
-int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance, bool fit = false) {
+int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance, bool fit = false, int steps = 1) {
     auto grid = [&](int configured, int count) {
         return (unsigned)(fit ? std::max(1, std::min(configured, count)) : configured);
     };
     const int *hc = B->hCounts;
-    if (fdtd && B->curStep) {
+    if (fdtd && steps == 3) {  // three whole steps per launch (tiles3)
+        Args t;
+        t.ptr(B->tiles3.dev).ptr(B->dCounts + 6).ptr(B->dFdtd);
+        ++B->nLaunches;
+        HIPCHK(ptk_hip_launch(B->fStep3, grid(B->grid3, hc[6]), 1, 1, 64, 16, 1, 0, B->stream, t.b.data(), t.b.size()));
+    } else if (fdtd && B->curStep) {
         Args t;
         t.ptr(B->tiles.dev).ptr(B->dCounts + 0).ptr(B->dFdtd);
         ++B->nLaunches;
@@ -662,7 +705,8 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
         if (ptk_hip_module_load(m.first, (dir + "/" + m.second).c_str()))
             return fail(Fail(PTK_BATCH_EHIP, std::string("load ") + m.second + ": " + ptk_hip_last_error_string()));
     const std::pair<ptk_hip_function *, const char *> bf[] = {
-        {&B->fH, "BatchFdtdH"}, {&B->fE, "BatchFdtdE"}, {&B->fStep, "BatchFdtdStep"}, {&B->fAdv, "BatchFdtdAdvance"},
+        {&B->fH, "BatchFdtdH"}, {&B->fE, "BatchFdtdE"}, {&B->fStep, "BatchFdtdStep"},
+        {&B->fStep3, "BatchFdtdSteps3"}, {&B->fAdv, "BatchFdtdAdvance"},
         {&B->fGbAcc, "BatchGroupbyAccumulate"}, {&B->fGbMerge, "BatchGroupbyMerge"},
         {&B->fGbOff, "BatchGroupbyOffsets"}, {&B->fGbRes, "BatchGroupbyReshuffle"},
         {&B->fGmmAcc, "BatchGmmAccumulate"}, {&B->fGmmFin, "BatchGmmFinalize"},
@@ -707,6 +751,7 @@ int ptk_batch_destroy(ptk_batch *B) {
         if (l.host) ptk_hip_host_free(l.host);
     };
     freeList(B->tiles);
+    freeList(B->tiles3);
     freeList(B->fdtdActive);
     freeList(B->gbBuckets);
     freeList(B->gbUnits);
@@ -796,36 +841,46 @@ int ptk_batch_run(ptk_batch *B, int max_admit, int max_iters, int *niters, int *
     }
     B->queue.erase(B->queue.begin(), B->queue.begin() + (long)qi);
     if (admitted) B->dirty = true;
+    if (!B->active.empty() && B->mode == PTK_BATCH_BATCHED && B->dirty) {
+        if (int r = BuildLists(B)) return r;
+        B->dirty = false;
+    }
+    // FDTD steps per batched launch this round: three while every resident FDTD instance has at
+    // least three left (three is odd, so the current field set stays stepsDone & 1), else one
+    int T = B->mode == PTK_BATCH_BATCHED && B->cur3 ? 3 : 1;
+    for (int id : B->active) {
+        const Inst &in = B->insts[(size_t)id];
+        if (in.kind == PTK_INST_FDTD && in.steps - in.stepsDone < 3) T = 1;
+    }
     // iterations every resident instance still needs before the first completes
     int iters = 0, mask = 0, need = max_iters;
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
         mask |= in.kind == PTK_INST_FDTD ? 1 : in.kind == PTK_INST_GROUPBY ? 2 : 4;
-        need = std::min(need, in.kind == PTK_INST_GROUPBY ? 1 : in.steps - in.stepsDone);
+        need = std::min(need, in.kind == PTK_INST_GROUPBY ? 1
+                              : in.kind == PTK_INST_FDTD  ? (in.steps - in.stepsDone) / T
+                                                          : in.steps - in.stepsDone);
     }
     if (!B->active.empty()) {
         iters = std::max(1, need);
-        if (B->mode == PTK_BATCH_BATCHED && B->dirty) {
-            if (int r = BuildLists(B)) return r;
-            B->dirty = false;
-        }
         // the events bracket only the iterations: admission work queued before them is excluded
         HIPCHK(ptk_hip_event_record(B->evStart, B->stream));
         if (B->mode == PTK_BATCH_BATCHED && !B->useGraphs) {
             for (int it = 0; it < iters; ++it)
-                if (int r = IssueBatched(B, mask & 1, mask & 2, mask & 4, mask & 1, true)) return r;
+                if (int r = IssueBatched(B, mask & 1, mask & 2, mask & 4, mask & 1, true, T)) return r;
         } else if (B->mode == PTK_BATCH_BATCHED) {
-            if (!B->graphs[mask]) {  // one captured iteration per mix of instance kinds
+            const int key = mask | (T == 3 ? 8 : 0);
+            if (!B->graphs[key]) {  // one captured iteration per mix of instance kinds and FDTD form
                 HIPCHK(ptk_hip_stream_begin_capture(B->stream));
-                int r = IssueBatched(B, mask & 1, mask & 2, mask & 4, mask & 1);
+                int r = IssueBatched(B, mask & 1, mask & 2, mask & 4, mask & 1, false, T);
                 ptk_hip_graph g = nullptr;
                 int e = ptk_hip_stream_end_capture(B->stream, &g);
                 if (r) return r;
                 if (e) return Fail(PTK_BATCH_EHIP, std::string("capture: ") + ptk_hip_last_error_string());
-                B->graphs[mask] = g;
+                B->graphs[key] = g;
             }
             for (int it = 0; it < iters; ++it) {
-                HIPCHK(ptk_hip_graph_launch(B->graphs[mask], B->stream));
+                HIPCHK(ptk_hip_graph_launch(B->graphs[key], B->stream));
                 ++B->nGraphs;
             }
         } else {
@@ -849,7 +904,7 @@ int ptk_batch_run(ptk_batch *B, int max_admit, int max_iters, int *niters, int *
         if (in.kind == PTK_INST_GROUPBY) {
             in.stepsDone = 1;
             done.push_back(id);
-        } else if ((in.stepsDone += iters) >= in.steps) {
+        } else if ((in.stepsDone += in.kind == PTK_INST_FDTD ? iters * T : iters) >= in.steps) {
             done.push_back(id);
         } else {
             keepActive.push_back(id);
@@ -948,6 +1003,20 @@ int ptk_batch_set_tiles(ptk_batch *B, int cols, int depth, int gbRows) {
     return PTK_BATCH_OK;
 }
 
+int ptk_batch_set_fdtd_steps(ptk_batch *B, int steps) {
+    if (!B || (steps != 1 && steps != 3)) return Fail(PTK_BATCH_EINVAL, "FDTD steps per launch must be 1 or 3");
+    B->fdtdSteps = steps;
+    B->dirty = true;
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_get_fdtd_steps(ptk_batch *B, int *steps, int *depth) {
+    if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
+    if (steps) *steps = B->fdtdSteps;
+    if (depth) *depth = B->depth3;
+    return PTK_BATCH_OK;
+}
+
 int ptk_batch_set_launch(ptk_batch *B, int graphs) {
     if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
     B->useGraphs = graphs != 0;
@@ -962,9 +1031,10 @@ int ptk_batch_get_tiles(ptk_batch *B, int *cols, int *depth, int *gbRows) {
     return PTK_BATCH_OK;
 }
 
-// Times one FDTD iteration (the half-step pair or the whole step) over the resident instances
-// for each candidate tile shape and persistent grid size (an iteration only rewrites the next
-// field set from the current one, so repeating it changes nothing) and keeps the fastest.  Groupby chunk rows are fixed at
+// Times one FDTD iteration (the half-step pair, the whole step or three whole steps) over the
+// resident instances for each candidate tile shape and persistent grid size (an iteration only
+// rewrites the next field set from the current one, so repeating it changes nothing) and keeps
+// the fastest per time step.  Groupby chunk rows are fixed at
 // admission and are not tuned here.
 int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
     if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
@@ -974,33 +1044,53 @@ int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
         if (best_ms) *best_ms = 0;
         return PTK_BATCH_OK;
     }
-    // {columns, planes} of half-step tiles; {0, planes}: whole-step overlapped tiles
-    static const int cand[][2] = {{0, 4}, {0, 8}, {0, 16}, {256, 8}, {512, 16}, {1024, 32}};
-    static const int grids[] = {2048, 4096, 8192};
+    // {columns, planes, steps per launch} of half-step tiles; {0, planes, 1}: whole-step
+    // overlapped tiles; {0, planes, 3}: three steps per launch on overlapped tiles (compared per
+    // step; the one-step whole-step tiles it falls back to near the end keep their shape)
+    static const int cand[][3] = {{0, 4, 1}, {0, 8, 1}, {0, 16, 1}, {256, 8, 1}, {512, 16, 1}, {1024, 32, 1},
+                                  {0, 32, 3}, {0, 64, 3}, {0, 128, 3}};
+    static const int grids[] = {2048, 4096, 8192}, grids3[] = {512, 1024, 2048};
     double best = 1e30;
-    int bc = B->fdtdCols, bd = B->fdtdDepth, bg = B->fdtdGrid;
+    int bc = B->fdtdCols, bd = B->fdtdDepth, bg = B->fdtdGrid, bs = B->fdtdSteps, bd3 = B->depth3, bg3 = B->grid3;
+    const int oneCols = bs == 3 ? 0 : bc, oneDepth = bd;  // the one-step shape kept beside three steps
     for (auto &c : cand) {
-        B->fdtdCols = c[0];
-        B->fdtdDepth = c[1];
+        const int T = c[2];
+        B->fdtdSteps = T;
+        B->fdtdCols = T == 3 ? 0 : c[0];
+        B->fdtdDepth = T == 3 ? (oneCols == 0 ? oneDepth : 8) : c[1];
+        if (T == 3) B->depth3 = c[1];
         if (int r = BuildLists(B)) return r;
-        for (int g : grids) {
-            B->fdtdGrid = g;
-            if (int r = IssueBatched(B, true, false, false, false)) return r;  // warm
+        if (T == 3 && !B->cur3) continue;  // an instance too large for the whole-step forms
+        for (int g : T == 3 ? grids3 : grids) {
+            (T == 3 ? B->grid3 : B->fdtdGrid) = g;
+            if (int r = IssueBatched(B, true, false, false, false, false, T)) return r;  // warm
             HIPCHK(ptk_hip_event_record(B->evStart, B->stream));
             for (int rep = 0; rep < 2; ++rep)
-                if (int r = IssueBatched(B, true, false, false, false)) return r;
+                if (int r = IssueBatched(B, true, false, false, false, false, T)) return r;
             HIPCHK(ptk_hip_event_record(B->evStop, B->stream));
             HIPCHK(ptk_hip_event_synchronize(B->evStop));
             float ms = 0;
             HIPCHK(ptk_hip_event_elapsed_ms(&ms, B->evStart, B->evStop));
-            if (ms / 2 < best) {
-                best = ms / 2;
-                bc = c[0];
-                bd = c[1];
-                bg = g;
+            const double perStep = ms / 2 / T;
+            if (perStep < best) {
+                best = perStep;
+                bs = T;
+                if (T == 3) {
+                    bc = 0;
+                    bd = B->fdtdDepth;
+                    bd3 = c[1];
+                    bg3 = g;
+                } else {
+                    bc = c[0];
+                    bd = c[1];
+                    bg = g;
+                }
             }
         }
     }
+    B->fdtdSteps = bs;
+    B->depth3 = bd3;
+    B->grid3 = bg3;
     B->fdtdCols = bc;
     B->fdtdDepth = bd;
     B->fdtdGrid = bg;

@@ -21,15 +21,19 @@ GB = 1 << 30
 MODES = [batch.PLAIN, batch.BATCHED]
 
 
-@pytest.mark.parametrize("tiles", [(64, 4), (0, 3), (0, 1), (0, 64)])
+@pytest.mark.parametrize("tiles,launch_steps", [((64, 4), 1), ((0, 3), 1), ((0, 1), 1), ((0, 64), 1),
+                                                ((0, 3), 3), ((0, 8), 3)])
 @pytest.mark.parametrize("mode", MODES)
-def test_fdtd_instances_bit_exact(hip, mode, tiles):
+def test_fdtd_instances_bit_exact(hip, mode, tiles, launch_steps):
     """Half-step column tiles (cols > 0) and whole-step overlapped tiles (cols == 0) of several
-    depths; small tiles put several per instance."""
-    print(f"case batch fdtd mode={mode} tiles={tiles}", flush=True)
-    shapes = [(12, 10, 8, 5, 11), (16, 16, 16, 4, 12), (20, 17, 9, 3, 13), (5, 33, 7, 6, 14), (70, 13, 6, 2, 15)]
+    depths, one or three steps per launch (three while every instance has three left, then single
+    steps; step counts that are not multiples of three); small tiles put several per instance."""
+    print(f"case batch fdtd mode={mode} tiles={tiles} launch_steps={launch_steps}", flush=True)
+    shapes = [(12, 10, 8, 5, 11), (16, 16, 16, 4, 12), (20, 17, 9, 3, 13), (5, 33, 7, 6, 14), (70, 13, 6, 2, 15),
+              (61, 23, 19, 10, 16), (1, 1, 1, 7, 17)]
     with batch.Executor(1 * GB, mode) as ex:
         ex.set_tiles(*tiles, 65536)
+        ex.set_fdtd_steps(launch_steps)
         ids = [ex.submit_fdtd(nx, ny, nz, s, seed, keep=True) for nx, ny, nz, s, seed in shapes]
         ex.drain()
         for i, (nx, ny, nz, s, seed) in zip(ids, shapes):
@@ -100,9 +104,10 @@ def submit(ex, kind, args, seed, keep=False):
     return ex.submit_gmm(*args, seed, keep)
 
 
-def run_mix(mode, arena, tiles=None, autotune=False, graphs=True):
+def run_mix(mode, arena, tiles=None, autotune=False, graphs=True, launch_steps=3):
     with batch.Executor(arena, mode) as ex:
         ex.set_launch(graphs)
+        ex.set_fdtd_steps(launch_steps)
         if tiles:
             ex.set_tiles(*tiles)
         ids = [submit(ex, kind, args, 100 + j) for j, (kind, args) in enumerate(MIX)]
@@ -123,6 +128,8 @@ def test_checksums_equal_alone_in_mix_both_modes_and_delayed(hip):
     assert tuned == mixed
     halves, _ = run_mix(batch.BATCHED, 2 * GB, tiles=(256, 8, 262144))
     assert halves == mixed  # the half-step and whole-step FDTD forms agree bit for bit
+    single, _ = run_mix(batch.BATCHED, 2 * GB, launch_steps=1)
+    assert single == mixed  # one step per launch and three steps per launch agree bit for bit
     direct, st = run_mix(batch.BATCHED, 2 * GB, graphs=False)
     assert direct == mixed and st["graph_launches"] == 0
     # an arena that holds only a few instances at a time: admission by footprint queues the rest
@@ -172,7 +179,8 @@ def test_single_and_multi_iteration_runs_agree(hip, mode):
                     ex.tick()
             else:
                 n, _, _ = ex.run(1 << 20)
-                assert n >= 3  # the shortest instance needs 3 iterations
+                # the shortest instance needs 3 steps: 3 iterations, or 1 of three steps
+                assert n == (1 if mode == batch.BATCHED else 3)
                 ex.drain()
             sums.append([ex.checksums(i) for i in ids])
     assert sums[0] == sums[1]
