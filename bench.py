@@ -11,6 +11,8 @@ from the previous run's output datablocks, which never leave HBM, so the timed r
 nothing over PCIe (the first, untimed warmup run uploads; a final untimed run downloads).
 
 Graph forms measured (all bit-identical to the six reference updates); `value` is the fastest:
+  overlap3p -- overlap3 on a persistent grid of one block per CU: each block streams one
+             whole-depth tile and an equal run of the remaining tiles' planes
   overlap3 -- 1 task per 3 steps, FdtdThreeStepOverlap: three whole Yee steps per pass on
              overlapped 58 x 10 column tiles, each step two planes behind the previous (16 B per
              cell-step + tile and slab overlap); 200 = 66 x 3 + 2: the last two steps run in one
@@ -60,7 +62,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--edge", type=int, default=384)
     ap.add_argument("--fdtd-steps", type=int, default=200)
-    ap.add_argument("--forms", default="overlap3,overlap2,overlap,halves,six,fused")
+    ap.add_argument("--forms", default="overlap3p,overlap3,overlap2,overlap,halves,six,fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--suite", default="captured,batched,served,gmm,pca",
@@ -84,7 +86,7 @@ def algorithmic_bytes(task: str, n: int) -> int:
     written once (binary32)."""
     E = field_elems(n)
     whole = ("FdtdStepFused", "FdtdStepOverlap", "FdtdStepOverlap8", "FdtdTwoStepOverlap",
-             "FdtdThreeStepOverlap")
+             "FdtdThreeStepOverlap", "FdtdThreeStepPersistent")
     reads = {"FdtdHFields": range(6), "FdtdEFields": range(6), **{k: range(6) for k in whole}}
     writes = {"FdtdHFields": (0, 1, 2), "FdtdEFields": (3, 4, 5), **{k: range(6) for k in whole}}
     six = {"HxComputation": (0, (0, 4, 5)), "HyComputation": (1, (1, 3, 5)),
@@ -98,7 +100,7 @@ def algorithmic_bytes(task: str, n: int) -> int:
 
 
 # Time steps one launch of a task kernel advances the fields by (1 unless listed).
-STEPS_PER_LAUNCH = {"FdtdTwoStepOverlap": 2, "FdtdThreeStepOverlap": 3}
+STEPS_PER_LAUNCH = {"FdtdTwoStepOverlap": 2, "FdtdThreeStepOverlap": 3, "FdtdThreeStepPersistent": 3}
 
 
 def run_graph(form: str, n: int, steps: int, repeat: int, warmup: int) -> dict:

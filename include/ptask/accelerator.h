@@ -116,6 +116,7 @@ public:
     virtual BOOL CopyDeviceToHost(void *pDst, PBuffer *pSrc, size_t cb) = 0;
     virtual BOOL CopyDeviceToDevice(PBuffer *pDst, PBuffer *pSrc, size_t cb, AsyncContext *pCtx) = 0;
     virtual size_t GetMemorySize() { return 0; }
+    virtual UINT GetCoreCount() { return 0; }  // compute units of a device (0: not known)
 
 protected:
     ACCELERATOR_CLASS m_eClass;

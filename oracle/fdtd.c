@@ -195,7 +195,8 @@ void FdtdStepFused(unsigned int n, void **a) {
 void FdtdStepOverlap(unsigned int n, void **a) { FdtdStepFused(n, a); }
 void FdtdStepOverlap8(unsigned int n, void **a) { FdtdStepFused(n, a); }
 
-/* FdtdTwoStepOverlap / FdtdThreeStepOverlap(Hx..Ez,&p,oHx..oEz): T whole steps, the
+/* FdtdTwoStepOverlap / FdtdThreeStepOverlap / FdtdThreeStepPersistent(Hx..Ez,&p,oHx..oEz): T
+ * whole steps, the
  * intermediate fields in temporary arrays. */
 static void fdtd_steps(unsigned int n, void **a, int T) {
     const FDTD_PARAMS *p = (const FDTD_PARAMS *)a[6];
@@ -220,3 +221,4 @@ static void fdtd_steps(unsigned int n, void **a, int T) {
 
 void FdtdTwoStepOverlap(unsigned int n, void **a) { fdtd_steps(n, a, 2); }
 void FdtdThreeStepOverlap(unsigned int n, void **a) { fdtd_steps(n, a, 3); }
+void FdtdThreeStepPersistent(unsigned int n, void **a) { fdtd_steps(n, a, 3); }

@@ -73,6 +73,7 @@ void FdtdStepOverlap(unsigned int n, void **a);
 void FdtdStepOverlap8(unsigned int n, void **a);
 void FdtdTwoStepOverlap(unsigned int n, void **a);
 void FdtdThreeStepOverlap(unsigned int n, void **a);
+void FdtdThreeStepPersistent(unsigned int n, void **a);
 void GmmAccumulate(unsigned int n, void **a);
 void GmmFinalize(unsigned int n, void **a);
 void PcaLoad(unsigned int n, void **a);

@@ -51,6 +51,8 @@ std::vector<TaskSpec> FormTasks(const std::string &form) {
         return {{"FdtdTwoStepOverlap", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
     if (form == "overlap3")  // three whole steps per task (16 B per cell-step + overlap)
         return {{"FdtdThreeStepOverlap", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
+    if (form == "overlap3p")  // the same on a persistent grid of one block per CU
+        return {{"FdtdThreeStepPersistent", {0, 1, 2, 3, 4, 5}, {0, 1, 2, 3, 4, 5}}};
     // the reference's six tasks (graphfdtd.cpp:448-560): (own, in1, in2) -> own
     return {{"HxComputation", {0, 4, 5}, {0}}, {"HyComputation", {1, 3, 5}, {1}},
             {"HzComputation", {2, 3, 4}, {2}}, {"ExComputation", {3, 1, 2}, {3}},
@@ -58,7 +60,9 @@ std::vector<TaskSpec> FormTasks(const std::string &form) {
 }
 
 // Time steps one task of the form advances the fields by.
-int StepsPerTask(const std::string &form) { return form == "overlap2" ? 2 : form == "overlap3" ? 3 : 1; }
+int StepsPerTask(const std::string &form) {
+    return form == "overlap2" ? 2 : form == "overlap3" || form == "overlap3p" ? 3 : 1;
+}
 
 struct Options {
     int nx = 32, ny = 32, nz = 32, steps = 10, blocks = 1, repeat = 1;
@@ -121,7 +125,7 @@ bool Parse(int argc, char **argv, Options &o) {
     }
     if (o.tileI && (o.tileI < 1 || o.tileJ < 2 || o.slab < 1 || o.tileI * o.tileJ > 1024)) return false;
     if (o.form == "overlap" && o.tileI && (o.tileI != 64 || o.tileJ < 3 || o.tileJ > 16)) return false;
-    if (o.form == "overlap2" || o.form == "overlap3") {
+    if (o.form == "overlap2" || o.form == "overlap3" || o.form == "overlap3p") {
         const int T = StepsPerTask(o.form);
         if (o.steps < T) {  // the remainder of steps % T runs in one task after the T-step tasks
             fprintf(stderr, "--form %s advances %d steps per task: --steps must be at least %d\n",
@@ -162,6 +166,15 @@ struct FdtdGraph {
     GraphOutputChannel *outputs[6] = {};
     UINT uid = 0;
 
+    // Compute units of the first HIP accelerator (256 on an MI355X if the runtime cannot say).
+    static int DeviceCores() {
+        Accelerator *a = Runtime::GetAccelerator(ACCELERATOR_CLASS_HIP, 0);
+        if (!a) return 256;
+        ACCELERATOR_DESCRIPTOR d;
+        a->GetDescriptor(&d);
+        return d.nCoreCount ? (int)d.nCoreCount : 256;
+    }
+
     void Geometry(Task *t, const TaskSpec &ts) {
         if (!hip) {
             t->SetComputeGeometry(1, 1, 1);  // a host task computes its whole arrays per call
@@ -169,6 +182,13 @@ struct FdtdGraph {
         }
         // --tile applies to the form's own task; a remainder task keeps its kernel's defaults
         const bool tiled = o.tileI && !strcmp(ts.kernel, spec[0].kernel);
+        if (!strcmp(ts.kernel, "FdtdThreeStepPersistent")) {
+            // 64 x R thread blocks on a persistent grid of G blocks (--tile 64xRxG; default one
+            // per compute unit of the device, whose LDS holds one such block)
+            const int R = tiled ? o.tileJ : 16, G = tiled ? o.slab : DeviceCores();
+            t->SetBlockAndGridSize(PTASKDIM3(G, 1, 1), PTASKDIM3(64, R, 1));
+            return;
+        }
         if (!strcmp(ts.kernel, "FdtdTwoStepOverlap") || !strcmp(ts.kernel, "FdtdThreeStepOverlap")) {
             // 64 x R thread blocks owning (64-2T) x (R-2T) columns each, k slabs of `slab` planes
             // (deeper slabs for three steps, whose pipeline runs 4 planes past each slab end)
@@ -394,7 +414,7 @@ int main(int argc, char **argv) {
     if (!Parse(argc, argv, o)) {
         fprintf(stderr, "usage: ptk_fdtd --kernels <fdtd.hsaco|hosttasks.so> [--n N | --nx --ny --nz] "
                         "[--steps S] [--mode looped|unrolled] [--blocks B] [--repeat R] [--seed S] "
-                        "[--form six|halves|fused|overlap|overlap2|overlap3] [--geometry default|reference|compute] [--tile IxJxK] [--dump file] "
+                        "[--form six|halves|fused|overlap|overlap2|overlap3|overlap3p] [--geometry default|reference|compute] [--tile IxJxK] [--dump file] "
                         "[--eager] [--profile] [--chain]\n");
         return 2;
     }

@@ -135,6 +135,57 @@ __device__ inline void overlapn_grid(const Fields &F, const FDTD_PARAMS &p, Over
 OVERLAPN_KERNEL(FdtdTwoStepOverlap, 2, 1)
 OVERLAPN_KERNEL(FdtdThreeStepOverlap, 3, 1)
 
+// The same T steps with a persistent 1-D grid of G blocks (one per CU): the N tiles split into
+// q = N / G whole-depth tiles per block, taken by all blocks together from plane 0 upwards (the
+// tiles live at the same time share their i/j overlap in L2), and the planes of the remaining
+// N - qG tiles cut into G equal runs in (tile, plane) order, one per block.  Every block then
+// streams q(Nz+1) + (N - qG)(Nz+1)/G planes plus the pipeline's lag once or twice, instead of
+// whole slabs in rounds of G blocks.  Any G covers every tile and plane exactly once.  Launch:
+// block (64, R, 1), grid (G, 1, 1).
+template <int T, int HB>
+__device__ inline void overlapn_persistent(const Fields &F, const FDTD_PARAMS &p, OverlapNLds<T, 16, HB> &L) {
+    const int R = (int)blockDim.y;
+    if (blockDim.x != 64 || R < 2 * T + 1 || R > 16 || blockDim.z != 1) return;  // geometry contract
+    if (gridDim.y != 1 || gridDim.z != 1) return;
+    if (!fits_buffer(p) || p.Nx < 1 || p.Ny < 1 || p.Nz < 1) return;
+    const int ow = 64 - 2 * T, oh = R - 2 * T;
+    const int ti = (p.Nx + ow) / ow, tj = (p.Ny + oh) / oh, N = ti * tj, D = p.Nz + 1;
+    const int G = (int)gridDim.x;
+    int b = (int)blockIdx.x;
+    if (PTK_OVERLAP_XCD && G >= kNumXcd) {  // XCD x takes blocks [x G/8, (x+1) G/8): neighbouring tiles
+        const int xcd = b % kNumXcd, local = b / kNumXcd, per = G / kNumXcd, rem = G % kNumXcd;
+        b = xcd * per + (xcd < rem ? xcd : rem) + local;
+    }
+    const int q = N / G;
+    const long long W = (long long)(N - q * G) * D;  // tile-planes left after the whole tiles
+    long long w = W * b / G;
+    const long long hi = W * (b + 1) / G;
+    // one call site for both kinds of segment (the tile body is large)
+    for (int r = 0;; ++r) {
+        int t, k0, k1;
+        if (r < q) {  // whole-depth tile
+            t = r * G + b, k0 = 0, k1 = D;
+        } else if (w < hi) {  // a run of the remaining tiles' planes, within one tile
+            t = q * G + (int)(w / D), k0 = (int)(w % D);
+            k1 = (int)(k0 + (hi - w) < D ? k0 + (hi - w) : D);
+            w += k1 - k0;
+        } else {
+            break;
+        }
+        overlapn_tile<T, 16>(F, p, (t % ti) * ow, (t / ti) * oh, k0, k1, L);
+        lds_barrier();  // the next segment reuses LDS
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(1024)
+FdtdThreeStepPersistent(const float *hx, const float *hy, const float *hz, const float *ex, const float *ey,
+                        const float *ez, FDTD_PARAMS p, float *ohx, float *ohy, float *ohz, float *oex,
+                        float *oey, float *oez) {
+    __shared__ OverlapNLds<3, 16, 1> L;
+    const Fields F{hx, hy, hz, ex, ey, ez, ohx, ohy, ohz, oex, oey, oez};
+    overlapn_persistent<3, 1>(F, p, L);
+}
+
 // ---- half-step kernels: all three H updates in one streaming pass, then all three E updates ----
 //
 // FdtdHFields(Hx, Hy, Hz, Ex, Ey, Ez, FDTD_PARAMS p, oHx, oHy, oHz)

@@ -23,6 +23,7 @@ void Accelerator::GetDescriptor(ACCELERATOR_DESCRIPTOR *d) {
     d->accClass = m_eClass;
     d->bEnabled = m_bEnabled;
     d->nMemorySize = (UINT)(GetMemorySize() >> 20);
+    d->nCoreCount = GetCoreCount();
     snprintf(d->szDescription, sizeof(d->szDescription), "%s", m_strName.c_str());
     d->pAccelerator = this;
 }

@@ -133,6 +133,7 @@ public:
     BOOL CopyDeviceToHost(void *pDst, PBuffer *pSrc, size_t cb) override;
     BOOL CopyDeviceToDevice(PBuffer *pDst, PBuffer *pSrc, size_t cb, AsyncContext *pCtx) override;
     size_t GetMemorySize() override { return m_props.total_global_mem; }
+    UINT GetCoreCount() override { return (UINT)m_props.compute_units; }
     int GetDeviceOrdinal() { return m_nDevice; }
     // The one stream every task shares when Runtime::SetTaskStreamPolicy(PTK_STREAMS_SHARED),
     // else NULL (each task then gets its own stream).

@@ -358,3 +358,17 @@ def test_multi_step_overlap_bitexact(hip, fmod, shape, geom, name):
     geo = overlapn_geometry(T, rows, slab, cap)(nx, ny, nz)
     got = run_fused(hip, fmod, name, F, FD.make_params(nx, ny, nz), 2, geo)
     assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 2 * T))
+
+
+@pytest.mark.parametrize("grid,rows", [(1, 16), (3, 7), (8, 16), (13, 11), (256, 16), (600, 16)])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (57, 9, 4), (130, 29, 6), (13, 21, 17), (1, 1, 1),
+                                   (3, 1, 2), (200, 40, 3)])
+def test_persistent_three_step_bitexact(hip, fmod, shape, grid, rows):
+    """FdtdThreeStepPersistent: a 1-D grid of any size covers the whole-depth tiles and the even
+    runs of the remaining tiles' planes exactly once; two launches equal six reference steps."""
+    nx, ny, nz = shape
+    print(f"case FdtdThreeStepPersistent shape={shape} grid={grid} rows={rows}", flush=True)
+    F = oracle.init_fields(nx, ny, nz, seed=401 + nx + nz)
+    got = run_fused(hip, fmod, "FdtdThreeStepPersistent", F, FD.make_params(nx, ny, nz), 2,
+                    ((grid, 1, 1), (64, rows, 1)))
+    assert_parity(got, oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), 6))

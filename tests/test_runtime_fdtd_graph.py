@@ -50,7 +50,8 @@ def direct(nx, ny, nz, steps, seed):
     return oracle.fdtd_run(F, oracle.make_params(nx, ny, nz, 0.5), steps)
 
 
-TASKS_PER_STEP = {"six": 6, "halves": 2, "fused": 1, "overlap": 1, "overlap2": 1 / 2, "overlap3": 1 / 3}
+TASKS_PER_STEP = {"six": 6, "halves": 2, "fused": 1, "overlap": 1, "overlap2": 1 / 2, "overlap3": 1 / 3,
+                  "overlap3p": 1 / 3}
 
 
 @pytest.mark.parametrize("form", sorted(TASKS_PER_STEP))
@@ -99,7 +100,7 @@ def test_missing_kernel_file_fails_loudly():
         or "accelerator" in p.stderr
 
 
-MULTI = {"overlap2": 2, "overlap3": 3}
+MULTI = {"overlap2": 2, "overlap3": 3, "overlap3p": 3}
 
 
 def multi_dispatches(form, steps):
@@ -154,7 +155,7 @@ def test_hip_looped_graph_equals_host_graph(tmp_path, geometry):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", ["halves", "fused", "overlap", "overlap2", "overlap3"])
+@pytest.mark.parametrize("form", ["halves", "fused", "overlap", "overlap2", "overlap3", "overlap3p"])
 @pytest.mark.parametrize("mode", ["looped", "unrolled"])
 def test_hip_fused_forms_equal_host_graph(tmp_path, mode, form):
     nx, ny, nz, steps = 70, 9, 11, 6
@@ -173,7 +174,8 @@ def test_hip_fused_forms_equal_host_graph(tmp_path, mode, form):
 @pytest.mark.parametrize("form,geo", [("halves", ["--tile", "64x4x3"]), ("halves", ["--tile", "128x2x64"]),
                                       ("overlap", ["--tile", "64x3x1"]), ("overlap", ["--tile", "64x8x5"]),
                                       ("overlap2", ["--tile", "64x5x1"]), ("overlap2", ["--tile", "64x12x7"]),
-                                      ("overlap3", ["--tile", "64x7x1"]), ("overlap3", ["--tile", "64x13x5"])])
+                                      ("overlap3", ["--tile", "64x7x1"]), ("overlap3", ["--tile", "64x13x5"]),
+                                      ("overlap3p", ["--tile", "64x7x3"]), ("overlap3p", ["--tile", "64x16x256"])])
 def test_hip_tile_geometries_equal_host_graph(tmp_path, form, geo):
     nx, ny, nz, steps = 70, 9, 11, 6
     dh, dg = str(tmp_path / "h.bin"), str(tmp_path / "g.bin")
@@ -189,7 +191,7 @@ def test_hip_tile_geometries_equal_host_graph(tmp_path, form, geo):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["looped", "unrolled"])
-@pytest.mark.parametrize("form,steps", [("overlap2", 7), ("overlap3", 7), ("overlap3", 8)])
+@pytest.mark.parametrize("form,steps", [("overlap2", 7), ("overlap3", 7), ("overlap3", 8), ("overlap3p", 8)])
 def test_hip_multi_step_remainder_equals_host_graph(tmp_path, mode, form, steps):
     nx, ny, nz = 70, 9, 11
     dh, dg = str(tmp_path / "h.bin"), str(tmp_path / "g.bin")
