@@ -134,10 +134,11 @@ int ptk_batch_autotune(ptk_batch *b, double *best_ms);
  * (BatchFdtdSteps3) while each has at least three steps left, then one step per launch for the
  * rest; 1 keeps one step per launch.  Results are bitwise those of single steps.  No
  * counterpart in the reference, whose graphs run one task per step (graphfdtd.cpp:735-739).
- * ptk_batch_autotune also compares both, per step, and *depth receives the three-step tiles' k
- * planes. */
+ * The three-step launches follow a persistent schedule over *blocks virtual blocks (by default
+ * one per compute unit): each takes whole-depth tiles round-robin, then an equal run of the
+ * remaining tiles' planes.  ptk_batch_autotune also compares both, per step. */
 int ptk_batch_set_fdtd_steps(ptk_batch *b, int steps_per_launch);
-int ptk_batch_get_fdtd_steps(ptk_batch *b, int *steps_per_launch, int *depth);
+int ptk_batch_get_fdtd_steps(ptk_batch *b, int *steps_per_launch, int *blocks);
 
 /* Launch form of batched iterations: 1 (default) one captured hipGraph per mix of instance kinds,
  * replayed every iteration; 0 the same kernels launched directly with grids cut to the current

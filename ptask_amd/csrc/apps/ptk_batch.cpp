@@ -161,9 +161,9 @@ int main(int argc, char **argv) {
         }
         unsigned long long st[7];
         ptk_batch_stats(B, st);
-        int cols, depth, gbr, lsteps, depth3;
+        int cols, depth, gbr, lsteps, blocks3;
         ptk_batch_get_tiles(B, &cols, &depth, &gbr);
-        ptk_batch_get_fdtd_steps(B, &lsteps, &depth3);
+        ptk_batch_get_fdtd_steps(B, &lsteps, &blocks3);
         sums[mode].clear();
         unsigned long long gbBad = 0;
         for (size_t i = 0; i < ids.size(); ++i) {
@@ -178,9 +178,9 @@ int main(int argc, char **argv) {
         snprintf(buf, sizeof(buf), "%s\"%s\": {\"ms\": %.3f, \"instances_per_s\": %.2f, \"exec_ms\": %.3f, "
                  "\"exec_instances_per_s\": %.2f, \"ticks\": %llu, \"launches\": %llu, \"graph_launches\": %llu, "
                  "\"arena_peak\": %llu, \"tiles\": [%d, %d, %d], \"fdtd_steps_per_launch\": %d, "
-                 "\"fdtd_depth3\": %d, \"autotune_ms\": %.4f, \"groupby_permutation_failures\": %llu}",
+                 "\"fdtd_blocks3\": %d, \"autotune_ms\": %.4f, \"groupby_permutation_failures\": %llu}",
                  first ? "" : ", ", name, ms, specs.size() / (ms / 1e3), ex, ex > 0 ? specs.size() / (ex / 1e3) : 0.0,
-                 st[0], st[1], st[2], st[4], cols, depth, gbr, lsteps, depth3, tuneMs, gbBad);
+                 st[0], st[1], st[2], st[4], cols, depth, gbr, lsteps, blocks3, tuneMs, gbBad);
         json += buf;
         first = false;
         ptk_batch_destroy(B);

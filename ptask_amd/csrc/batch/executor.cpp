@@ -162,7 +162,8 @@ struct ptk_batch {
     int *dCounts = nullptr, *hCounts = nullptr;
     unsigned long long *dSums = nullptr, *hSums = nullptr;  // checksum staging, 8 per retiring instance
     size_t sumCap = 0;
-    List<int4> tiles, tiles3;  // FDTD tiles of the current form; three-step tiles (when cur3)
+    List<int4> tiles, tiles3;  // FDTD tiles of the current form; three-step items (when cur3)
+    List<int> t3start;         // three-step items of virtual block b: [t3start[b], t3start[b+1])
     List<int> fdtdActive;
     List<int2> gbUnits, gbBuckets, gmmAcc, gmmFin;
     List<PTK_CSUM_JOB> csumJobs;
@@ -175,7 +176,8 @@ struct ptk_batch {
     bool dirty = true;
     bool useGraphs = true;  // batched iterations as captured graphs (else direct, grid-fitted launches)
     int fdtdCols = 0, fdtdDepth = 8, gbRows = 262144, fdtdGrid = 4096;
-    int fdtdSteps = 3, depth3 = 64, grid3 = 2048;  // FDTD steps per launch (1 or 3); three-step slabs, grid
+    int cus = 256;                   // compute units of the device
+    int fdtdSteps = 3, grid3 = 256;  // FDTD steps per launch (1 or 3); three-step virtual blocks (one per CU)
     int curCols = 512, curDepth = 16;  // the shape the current work lists were built with
     bool curStep = false;              // the current FDTD tiles are whole-step (overlapped) tiles
     bool cur3 = false;                 // tiles3 holds the three-step tiles of the resident set
@@ -407,16 +409,38 @@ size_t FdtdTiles(const ptk_batch *B, int cols, int depth) {
     return nt;
 }
 
-// Three-step tiles of the resident FDTD instances with slabs of `depth` planes.
-size_t Fdtd3Tiles(const ptk_batch *B, int depth) {
-    size_t nt = 0;
-    for (int id : B->active) {
-        const Inst &in = B->insts[(size_t)id];
-        if (in.kind != PTK_INST_FDTD) continue;
-        nt += (size_t)(((in.a + kStep3TileI) / kStep3TileI) * ((in.b + kStep3TileJ) / kStep3TileJ) *
-                       ((in.c + depth) / depth));
+// The three-step persistent schedule over G virtual blocks (one per CU by default): the FDTD
+// tiles of the resident instances in order, q = N / G whole-depth tiles per block taken
+// round-robin (block b: tiles b, b + G, ...; at any time the blocks stream neighbouring tiles at
+// about the same planes, so their overlap is shared in L2), then the planes of the remaining
+// tiles cut into G equal runs in (tile, plane) order, one per block.  Items are grouped by
+// block; at most N + G of them.
+void BuildSchedule3(ptk_batch *B, const std::vector<int4> &t3, int G) {
+    const size_t N = t3.size(), q = N / (size_t)G;
+    long long W = 0;
+    for (size_t i = q * (size_t)G; i < N; ++i) W += t3[i].w;
+    size_t idx = q * (size_t)G;  // first remaining tile whose planes reach past `pos`
+    long long pos = 0;          // planes of the remaining tiles before tile idx
+    B->tiles3.n = 0;
+    for (int b = 0; b < G; ++b) {
+        B->t3start.host[b] = (int)B->tiles3.n;
+        for (size_t r = 0; r < q; ++r) {
+            const int4 &t = t3[r * (size_t)G + (size_t)b];
+            B->tiles3.host[B->tiles3.n++] = int4{t.x, t.y, t.z, 0 | (t.w << 16)};
+        }
+        const long long lo = W * b / G, hi = W * (b + 1) / G;
+        while (idx < N && pos + t3[idx].w <= lo) pos += t3[idx++].w;
+        size_t j = idx;
+        long long pj = pos, w = lo;
+        while (w < hi) {
+            const int4 &t = t3[j];
+            const int k0 = (int)(w - pj), k1 = (int)std::min<long long>(t.w, hi - pj);
+            B->tiles3.host[B->tiles3.n++] = int4{t.x, t.y, t.z, k0 | (k1 << 16)};
+            w = pj + k1;
+            if (k1 == t.w) pj += t3[j++].w;
+        }
     }
-    return nt;
+    B->t3start.host[G] = (int)B->tiles3.n;
 }
 
 int BuildLists(ptk_batch *B) {
@@ -434,12 +458,19 @@ int BuildLists(ptk_batch *B) {
     B->curCols = cols;
     B->curDepth = depth;
     B->curStep = cols == 0;
-    // Three-step tiles beside them when the form is the whole step and asks for three steps per
-    // launch; slabs halve while the work would give fewer than two tiles per CU.
+    // The three-step schedule beside them when the form is the whole step and asks for three
+    // steps per launch: tiles {slot, x, y, depth} of every resident FDTD instance.
     const bool three = cols == 0 && B->fdtdSteps == 3;
-    int depth3 = B->depth3;
-    while (three && Fdtd3Tiles(B, depth3) < 512 && depth3 > 8) depth3 = (depth3 + 1) / 2;
-    const size_t nt3 = three ? Fdtd3Tiles(B, depth3) : 0;
+    std::vector<int4> t3;
+    if (three)
+        for (int id : B->active) {
+            const Inst &in = B->insts[(size_t)id];
+            if (in.kind != PTK_INST_FDTD) continue;
+            const int ti3 = (int)((in.a + kStep3TileI) / kStep3TileI), tj3 = (in.b + kStep3TileJ) / kStep3TileJ;
+            for (int y = 0; y < tj3; ++y)
+                for (int x = 0; x < ti3; ++x) t3.push_back(int4{in.slot, x, y, in.c + 1});
+        }
+    const size_t nt3 = three ? t3.size() + (size_t)B->grid3 : 0;
     B->cur3 = three;
     size_t nt = FdtdTiles(B, cols, depth), nf = 0, ngu = 0, ngi = 0, nga = 0, ngf = 0;
     for (int id : B->active) {
@@ -456,12 +487,13 @@ int BuildLists(ptk_batch *B) {
     }
     if (int r = Reserve(B, B->tiles, nt)) return r;
     if (int r = Reserve(B, B->tiles3, nt3)) return r;
+    if (int r = Reserve(B, B->t3start, three ? (size_t)B->grid3 + 1 : 0)) return r;
     if (int r = Reserve(B, B->fdtdActive, nf)) return r;
     if (int r = Reserve(B, B->gbUnits, ngu)) return r;
     if (int r = Reserve(B, B->gbBuckets, ngi)) return r;
     if (int r = Reserve(B, B->gmmAcc, nga)) return r;
     if (int r = Reserve(B, B->gmmFin, ngf)) return r;
-    B->tiles.n = B->tiles3.n = B->fdtdActive.n = B->gbUnits.n = B->gbBuckets.n = B->gmmAcc.n = B->gmmFin.n = 0;
+    B->tiles.n = B->tiles3.n = B->t3start.n = B->fdtdActive.n = B->gbUnits.n = B->gbBuckets.n = B->gmmAcc.n = B->gmmFin.n = 0;
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
         if (in.kind == PTK_INST_FDTD) {
@@ -480,15 +512,6 @@ int BuildLists(ptk_batch *B) {
                         B->tiles.host[B->tiles.n++] = int4{in.slot, c0, std::min(cols, ncol - c0), k0 | (k1 << 16)};
                 }
             }
-            if (three) {
-                const int ti3 = (int)((in.a + kStep3TileI) / kStep3TileI), tj3 = (in.b + kStep3TileJ) / kStep3TileJ;
-                const int slabs3 = (kdepth + depth3 - 1) / depth3;
-                for (int sl = 0; sl < slabs3; ++sl) {
-                    const int k0 = (int)((long long)sl * kdepth / slabs3), k1 = (int)((long long)(sl + 1) * kdepth / slabs3);
-                    for (int y = 0; y < tj3; ++y)
-                        for (int x = 0; x < ti3; ++x) B->tiles3.host[B->tiles3.n++] = int4{in.slot, x, y, k0 | (k1 << 16)};
-                }
-            }
             B->fdtdActive.host[B->fdtdActive.n++] = in.slot;
         } else if (in.kind == PTK_INST_GROUPBY) {
             const long long chunks = (in.a + in.gb.chunkRows - 1) / in.gb.chunkRows;
@@ -505,13 +528,19 @@ int BuildLists(ptk_batch *B) {
     B->hCounts[3] = (int)B->gbBuckets.n;
     B->hCounts[4] = (int)B->gmmAcc.n;
     B->hCounts[5] = (int)B->gmmFin.n;
+    if (three) {
+        BuildSchedule3(B, t3, B->grid3);
+        B->t3start.n = (size_t)B->grid3 + 1;
+    }
     B->hCounts[6] = (int)B->tiles3.n;
+    B->hCounts[7] = three ? B->grid3 : 0;
     auto up = [&](auto &l) -> int {
         if (l.n) HIPCHK(ptk_hip_memcpy_htod_async(l.dev, l.host, l.n * sizeof(*l.host), B->stream));
         return PTK_BATCH_OK;
     };
     if (int r = up(B->tiles)) return r;
     if (int r = up(B->tiles3)) return r;
+    if (int r = up(B->t3start)) return r;
     if (int r = up(B->fdtdActive)) return r;
     if (int r = up(B->gbUnits)) return r;
     if (int r = up(B->gbBuckets)) return r;
@@ -532,9 +561,9 @@ int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance, bool 
     const int *hc = B->hCounts;
     if (fdtd && steps == 3) {  // three whole steps per launch (tiles3)
         Args t;
-        t.ptr(B->tiles3.dev).ptr(B->dCounts + 6).ptr(B->dFdtd);
-        ++B->nLaunches;
-        HIPCHK(ptk_hip_launch(B->fStep3, grid(B->grid3, hc[6]), 1, 1, 64, 16, 1, 0, B->stream, t.b.data(), t.b.size()));
+        t.ptr(B->tiles3.dev).ptr(B->dCounts + 6).ptr(B->t3start.dev).ptr(B->dFdtd);
+        ++B->nLaunches;  // one block per virtual block of the schedule
+        HIPCHK(ptk_hip_launch(B->fStep3, (unsigned)B->grid3, 1, 1, 64, 16, 1, 0, B->stream, t.b.data(), t.b.size()));
     } else if (fdtd && B->curStep) {
         Args t;
         t.ptr(B->tiles.dev).ptr(B->dCounts + 0).ptr(B->dFdtd);
@@ -698,6 +727,14 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
     int r = PTK_BATCH_OK;
     if (ptk_hip_stream_create(&B->stream) || ptk_hip_event_create(&B->evStart, 1) || ptk_hip_event_create(&B->evStop, 1))
         return fail(Fail(PTK_BATCH_EHIP, std::string("stream/event: ") + ptk_hip_last_error_string()));
+    {
+        int dev = 0;
+        ptk_hip_device_props_t props;
+        if (ptk_hip_get_device(&dev) == PTK_HIP_OK && ptk_hip_get_device_props(dev, &props) == PTK_HIP_OK &&
+            props.compute_units > 0)
+            B->cus = props.compute_units;
+        B->grid3 = B->cus;  // the three-step schedule: one virtual block per CU
+    }
     const std::pair<ptk_hip_module *, const char *> mods[] = {
         {&B->mBatch, "batch.hsaco"}, {&B->mFdtd, "fdtd.hsaco"}, {&B->mGb, "groupby.hsaco"},
         {&B->mGmm, "gmm.hsaco"}, {&B->mUtil, "ptk_util.hsaco"}};
@@ -752,6 +789,7 @@ int ptk_batch_destroy(ptk_batch *B) {
     };
     freeList(B->tiles);
     freeList(B->tiles3);
+    freeList(B->t3start);
     freeList(B->fdtdActive);
     freeList(B->gbBuckets);
     freeList(B->gbUnits);
@@ -1010,10 +1048,10 @@ int ptk_batch_set_fdtd_steps(ptk_batch *B, int steps) {
     return PTK_BATCH_OK;
 }
 
-int ptk_batch_get_fdtd_steps(ptk_batch *B, int *steps, int *depth) {
+int ptk_batch_get_fdtd_steps(ptk_batch *B, int *steps, int *blocks) {
     if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
     if (steps) *steps = B->fdtdSteps;
-    if (depth) *depth = B->depth3;
+    if (blocks) *blocks = B->grid3;
     return PTK_BATCH_OK;
 }
 
@@ -1048,21 +1086,21 @@ int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
     // overlapped tiles; {0, planes, 3}: three steps per launch on overlapped tiles (compared per
     // step; the one-step whole-step tiles it falls back to near the end keep their shape)
     static const int cand[][3] = {{0, 4, 1}, {0, 8, 1}, {0, 16, 1}, {256, 8, 1}, {512, 16, 1}, {1024, 32, 1},
-                                  {0, 32, 3}, {0, 64, 3}, {0, 128, 3}};
-    static const int grids[] = {2048, 4096, 8192}, grids3[] = {512, 1024, 2048};
+                                  {0, 0, 3}};
+    static const int grids[] = {2048, 4096, 8192};
+    const int grids3[] = {B->cus, 2 * B->cus};  // virtual blocks of the three-step schedule
     double best = 1e30;
-    int bc = B->fdtdCols, bd = B->fdtdDepth, bg = B->fdtdGrid, bs = B->fdtdSteps, bd3 = B->depth3, bg3 = B->grid3;
+    int bc = B->fdtdCols, bd = B->fdtdDepth, bg = B->fdtdGrid, bs = B->fdtdSteps, bg3 = B->grid3;
     const int oneCols = bs == 3 ? 0 : bc, oneDepth = bd;  // the one-step shape kept beside three steps
     for (auto &c : cand) {
         const int T = c[2];
         B->fdtdSteps = T;
         B->fdtdCols = T == 3 ? 0 : c[0];
         B->fdtdDepth = T == 3 ? (oneCols == 0 ? oneDepth : 8) : c[1];
-        if (T == 3) B->depth3 = c[1];
-        if (int r = BuildLists(B)) return r;
-        if (T == 3 && !B->cur3) continue;  // an instance too large for the whole-step forms
-        for (int g : T == 3 ? grids3 : grids) {
+        for (int g : T == 3 ? std::vector<int>(grids3, grids3 + 2) : std::vector<int>(grids, grids + 3)) {
             (T == 3 ? B->grid3 : B->fdtdGrid) = g;
+            if (int r = BuildLists(B)) return r;  // the three-step schedule depends on the grid
+            if (T == 3 && !B->cur3) break;        // an instance too large for the whole-step forms
             if (int r = IssueBatched(B, true, false, false, false, false, T)) return r;  // warm
             HIPCHK(ptk_hip_event_record(B->evStart, B->stream));
             for (int rep = 0; rep < 2; ++rep)
@@ -1078,7 +1116,6 @@ int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
                 if (T == 3) {
                     bc = 0;
                     bd = B->fdtdDepth;
-                    bd3 = c[1];
                     bg3 = g;
                 } else {
                     bc = c[0];
@@ -1089,7 +1126,6 @@ int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
         }
     }
     B->fdtdSteps = bs;
-    B->depth3 = bd3;
     B->grid3 = bg3;
     B->fdtdCols = bc;
     B->fdtdDepth = bd;

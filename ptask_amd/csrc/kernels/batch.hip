@@ -15,7 +15,8 @@
 //   BatchFdtdAdvance          active FDTD instances: done += 1
 //   BatchFdtdStep             tiles {inst, tile x, tile y, k0 | k1 << 16}: a whole step on
 //                             overlapped 62 x 6-column tiles (the form when fdtd_cols == 0)
-//   BatchFdtdSteps3           the same tiles over 58 x 10 columns: three whole steps per launch
+//   BatchFdtdSteps3           items {inst, tile x, tile y, k0 | k1 << 16} over 58 x 10 columns in
+//                             per-block ranges: three whole steps per launch
 //   BatchGroupbyAccumulate/Reshuffle          units {inst, unit}: chunkRows rows
 //   BatchGroupbyMerge/Offsets                 units {inst, key bucket}
 //   BatchGmmAccumulate        units {inst, partial row b}
@@ -94,26 +95,33 @@ BatchFdtdStep(const int4 *__restrict__ tiles, const int *__restrict__ count, con
     }
 }
 
-// Three whole Yee steps per launch on overlapped tiles (overlapn_tile<3>, fdtd_fused.h): tiles
-// {inst, tile x, tile y, k0 | k1 << 16} of 58 x 10 owned columns; blocks of 64 x 16 threads.  A
-// launch reads set (done & 1) and writes the other set three steps later, so one BatchFdtdAdvance
-// per launch keeps the set index right.  Same instance conditions as BatchFdtdStep.
+// Three whole Yee steps per launch on overlapped tiles (overlapn_tile<3>, fdtd_fused.h) over a
+// persistent schedule of G virtual blocks (count[1]): virtual block b's items are
+// [starts[b], starts[b+1]) of `tiles`, each {inst, tile x, tile y, k0 | k1 << 16} of 58 x 10
+// owned columns, taken in order.  The executor builds the schedule (whole-depth tiles round-robin,
+// then equal runs of the remaining tiles' planes); blocks of 64 x 16 threads take virtual blocks
+// in grid strides, XCD-contiguously, so any grid size covers every item once.  A launch reads
+// set (done & 1) and writes the other set three steps later, so one BatchFdtdAdvance per launch
+// keeps the set index right.  Same instance conditions as BatchFdtdStep.
 extern "C" __global__ void __launch_bounds__(1024)
-BatchFdtdSteps3(const int4 *__restrict__ tiles, const int *__restrict__ count, const PTK_FDTD_INST *__restrict__ insts) {
+BatchFdtdSteps3(const int4 *__restrict__ tiles, const int *__restrict__ count, const int *__restrict__ starts,
+                const PTK_FDTD_INST *__restrict__ insts) {
     __shared__ OverlapNLds<3, 16, 1> L;
     if (blockDim.x != 64 || blockDim.y != 16 || blockDim.z != 1) return;  // geometry contract
-    const int n = *count, G = (int)gridDim.x < n ? (int)gridDim.x : n;  // as in BatchFdtdStep
-    if ((int)blockIdx.x >= G) return;
-    for (int t = (int)gb_xcd_block(blockIdx.x, (unsigned)G); t < n; t += G) {
-        const int4 tl = tiles[t];
-        const PTK_FDTD_INST &I = insts[tl.x];
-        if (!fits_buffer(I.p) || I.p.Nx < 1 || I.p.Ny < 1 || I.p.Nz < 1) continue;
-        const int cur = I.done & 1, nxt = cur ^ 1;
-        const Fields F{I.f[cur][0], I.f[cur][1], I.f[cur][2], I.f[cur][3], I.f[cur][4], I.f[cur][5],
-                       I.f[nxt][0], I.f[nxt][1], I.f[nxt][2], I.f[nxt][3], I.f[nxt][4], I.f[nxt][5]};
-        const int k0 = tl.w & 0xffff, k1 = (int)((unsigned)tl.w >> 16);
-        overlapn_tile<3, 16>(F, I.p, tl.y * 58, tl.z * 10, k0, k1, L);
-        lds_barrier();  // the next tile reuses LDS
+    const int G = count[1];
+    for (int vb = (int)blockIdx.x; vb < G; vb += (int)gridDim.x) {
+        const int b = (int)gb_xcd_block((unsigned)vb, (unsigned)G);
+        for (int t = starts[b]; t < starts[b + 1]; ++t) {
+            const int4 tl = tiles[t];
+            const PTK_FDTD_INST &I = insts[tl.x];
+            if (!fits_buffer(I.p) || I.p.Nx < 1 || I.p.Ny < 1 || I.p.Nz < 1) continue;
+            const int cur = I.done & 1, nxt = cur ^ 1;
+            const Fields F{I.f[cur][0], I.f[cur][1], I.f[cur][2], I.f[cur][3], I.f[cur][4], I.f[cur][5],
+                           I.f[nxt][0], I.f[nxt][1], I.f[nxt][2], I.f[nxt][3], I.f[nxt][4], I.f[nxt][5]};
+            const int k0 = tl.w & 0xffff, k1 = (int)((unsigned)tl.w >> 16);
+            overlapn_tile<3, 16>(F, I.p, tl.y * 58, tl.z * 10, k0, k1, L);
+            lds_barrier();  // the next item reuses LDS
+        }
     }
 }
 
