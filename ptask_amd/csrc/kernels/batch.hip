@@ -199,8 +199,11 @@ BatchGroupbyOffsets(const int2 *__restrict__ units, const int *__restrict__ coun
 extern "C" __global__ void __launch_bounds__(1024)
 BatchGroupbyReshuffle(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
     __shared__ GbPlaceLds L;
-    const int n = *count;
-    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+    // each XCD takes a contiguous run of units (consecutive chunks of an instance): see
+    // GroupbyReshuffle
+    const int n = *count, G = (int)gridDim.x < n ? (int)gridDim.x : n;
+    if ((int)blockIdx.x >= G) return;
+    for (int u = (int)gb_xcd_block(blockIdx.x, (unsigned)G); u < n; u += G) {
         const int2 w = units[u];
         const PTK_GROUPBY_INST &I = insts[w.x];
         if (!gb_keys_ok(I.numKeys) || I.n <= 0) continue;

@@ -57,7 +57,12 @@ GroupbyReshuffle(const int *__restrict__ keys, const int *__restrict__ vals, int
     __shared__ GbPlaceLds L;
     if (!gb_keys_ok(numKeys) || n <= 0 || (blockDim.x & 63) != 0) return;
     const PTK_GB_SCRATCH S = ptk_gb_scratch(scratch, n, numKeys);
-    for (long long c = blockIdx.x; c < S.chunks; c += gridDim.x)
+    // Consecutive chunks place a key's rows in consecutive slots of its group, so each XCD takes
+    // a contiguous run of chunks: the partial lines of rare keys' groups then fill up in one L2
+    // before they are written back.
+    const unsigned G = gridDim.x < S.chunks ? gridDim.x : (unsigned)S.chunks;
+    if (blockIdx.x >= G) return;
+    for (long long c = gb_xcd_block(blockIdx.x, G); c < S.chunks; c += G)
         gb_place_chunk(keys, vals, n, numKeys, c, offsets, S, outKeys, outVals, L);
 }
 
