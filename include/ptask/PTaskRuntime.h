@@ -87,6 +87,19 @@ UINT GetTaskThreadPoolSize();
 void SetTaskThreadPoolPolicy(THREADPOOLPOLICY ePolicy);
 void SetSchedulingMode(int mode);
 int GetSchedulingMode();
+/* Dispatch admission across all live graphs.  Each accelerator class (host, HIP) admits at most
+ * a number of dispatches at once (host: SetMaximumHostConcurrency; HIP: the launches being
+ * enqueued, 8 by default) and admits waiting ones best first: highest task priority
+ * (Task::SetPriority) in SCHEDMODE_PRIORITY, arrival order in the other modes (reference: the
+ * global ready queue of Scheduler.cpp:4551-4673).  GlobalQuiesce admits no more dispatches and
+ * returns when the admitted ones have finished; EndGlobalQuiescence resumes (reference:
+ * Scheduler::GlobalQuiesce / EndGlobalQuiescence, Scheduler.cpp:526-600, internal there).
+ * GetWaitingDispatchCount: dispatches waiting for admission. */
+void SetMaximumDeviceDispatchConcurrency(int n);
+int GetMaximumDeviceDispatchConcurrency();
+void GlobalQuiesce();
+void EndGlobalQuiescence();
+int GetWaitingDispatchCount();
 void SetForceSynchronous(BOOL b);
 BOOL GetForceSynchronous();
 void SetProfilePSDispatch(BOOL b);

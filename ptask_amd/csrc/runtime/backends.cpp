@@ -131,21 +131,10 @@ BOOL HostTask::PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vect
             default: args[a] = p; break;
         }
     }
-    // bound the number of host tasks computing at once (MaximumHostConcurrency)
-    RuntimeState &S = RT();
-    {
-        std::unique_lock<std::mutex> lk(S.hostSlotsLock);
-        S.hostSlotsCv.wait(lk, [&] { return S.hostSlotsInUse < S.maxHostConcurrency; });
-        S.hostSlotsInUse++;
-    }
+    // the host arbiter has admitted this dispatch (MaximumHostConcurrency slots)
     const long long reps = (long long)std::max(1, m_tgx) * std::max(1, m_tgy) * std::max(1, m_tgz);
     for (UINT it = 0; it < uiIterations; ++it)
         for (long long r = 0; r < reps; ++r) fn((UINT)args.size(), args.data());
-    {
-        std::lock_guard<std::mutex> g(S.hostSlotsLock);
-        S.hostSlotsInUse--;
-    }
-    S.hostSlotsCv.notify_one();
     return TRUE;
 }
 

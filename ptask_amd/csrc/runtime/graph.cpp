@@ -290,9 +290,16 @@ BOOL Task::AttemptDispatch(std::unique_lock<std::recursive_mutex> *pLock) {
         }
     }
 
-    // phase 2: the backend binds and launches without the graph lock
+    // phase 2: admission to the accelerator class shared by all live graphs, then the backend
+    // binds and launches, without the graph lock
     pLock->unlock();
-    BOOL ok = (uiIterations == 0) ? TRUE : PlatformSpecificDispatch(vIn, vOut, uiIterations);
+    BOOL ok = TRUE;
+    if (uiIterations != 0) {
+        DispatchArbiter &arb = GetAcceleratorClass() == ACCELERATOR_CLASS_HOST ? RT().hostArbiter : RT().hipArbiter;
+        arb.Acquire(RT().schedulingMode == SCHEDMODE_PRIORITY ? GetPriority() : 0);
+        ok = PlatformSpecificDispatch(vIn, vOut, uiIterations);
+        arb.Release();
+    }
     if (!ok) {
         for (size_t o = 0; o < nOut; ++o)
             if (vOut[o]) vOut[o]->SetControlSignal(DBCTLC_ERR);

@@ -176,6 +176,36 @@ public:
 };
 
 // ---------------------------------------------------------------------------------------------
+// Admission to one accelerator class's dispatch path, shared by every live graph (reference: the
+// global ready queue and accelerator assignment of Scheduler.cpp:4551-4673, quiescence
+// :526-700).  At most `slots` dispatches run at once; waiting dispatches are admitted best first:
+// the highest task priority when the scheduling mode is SCHEDMODE_PRIORITY, else arrival order.
+// While quiesced, none is admitted; Quiesce(TRUE) returns once the admitted ones have finished.
+class DispatchArbiter {
+public:
+    explicit DispatchArbiter(int nSlots) : m_nSlots(nSlots) {}
+    void Acquire(int nPriority);
+    void Release();
+    void SetSlots(int n);
+    int GetSlots();
+    void Quiesce(BOOL bQuiesce);
+    int GetWaiting();
+    unsigned long long GetAdmitted();
+
+private:
+    struct Ticket {
+        int prio;
+        unsigned long long seq;
+    };
+    std::mutex m_lock;
+    std::condition_variable m_cv;
+    int m_nSlots, m_nInUse = 0;
+    BOOL m_bQuiesced = FALSE;
+    unsigned long long m_uiSeq = 0, m_uiAdmitted = 0;
+    std::vector<Ticket> m_vWaiting;
+};
+
+// ---------------------------------------------------------------------------------------------
 // Process-wide state.
 struct RuntimeState {
     std::recursive_mutex lock;
@@ -201,10 +231,9 @@ struct RuntimeState {
     std::string lastError;
     std::atomic<unsigned long long> h2dCount{0}, h2dBytes{0}, d2hCount{0}, d2hBytes{0};
     std::atomic<UINT> nextDBUID{1};
-    // host-task concurrency limiter
-    std::mutex hostSlotsLock;
-    std::condition_variable hostSlotsCv;
-    int hostSlotsInUse = 0;
+    // dispatch admission per accelerator class: host tasks (maxHostConcurrency slots) and HIP
+    // tasks (enqueueing a launch; the device runs launched work concurrently)
+    DispatchArbiter hostArbiter{4}, hipArbiter{8};
     // templates used for descriptor blocks
     DatablockTemplate *sizeDescriptorTemplate = nullptr;
     DatablockTemplate *controlTemplate = nullptr;

@@ -17,6 +17,72 @@ RuntimeState &RT() {
     return *s;
 }
 
+// ---- DispatchArbiter ---------------------------------------------------------------------------
+
+void DispatchArbiter::Acquire(int nPriority) {
+    std::unique_lock<std::mutex> lk(m_lock);
+    const Ticket me{nPriority, m_uiSeq++};
+    m_vWaiting.push_back(me);
+    // best waiting ticket: highest priority, then earliest arrival
+    auto first = [&] {
+        for (const Ticket &w : m_vWaiting)
+            if (w.prio > me.prio || (w.prio == me.prio && w.seq < me.seq)) return false;
+        return true;
+    };
+    m_cv.wait(lk, [&] { return !m_bQuiesced && m_nInUse < m_nSlots && first(); });
+    for (size_t i = 0; i < m_vWaiting.size(); ++i)
+        if (m_vWaiting[i].seq == me.seq) {
+            m_vWaiting.erase(m_vWaiting.begin() + (long)i);
+            break;
+        }
+    m_nInUse++;
+    m_uiAdmitted++;
+    lk.unlock();
+    m_cv.notify_all();  // the next best may take a remaining slot
+}
+
+void DispatchArbiter::Release() {
+    {
+        std::lock_guard<std::mutex> g(m_lock);
+        m_nInUse--;
+    }
+    m_cv.notify_all();
+}
+
+void DispatchArbiter::SetSlots(int n) {
+    {
+        std::lock_guard<std::mutex> g(m_lock);
+        m_nSlots = n < 1 ? 1 : n;
+    }
+    m_cv.notify_all();
+}
+
+int DispatchArbiter::GetSlots() {
+    std::lock_guard<std::mutex> g(m_lock);
+    return m_nSlots;
+}
+
+void DispatchArbiter::Quiesce(BOOL bQuiesce) {
+    std::unique_lock<std::mutex> lk(m_lock);
+    m_bQuiesced = bQuiesce;
+    if (bQuiesce) {
+        m_cv.wait(lk, [&] { return m_nInUse == 0; });
+    } else {
+        lk.unlock();
+        m_cv.notify_all();
+    }
+}
+
+int DispatchArbiter::GetWaiting() {
+    std::lock_guard<std::mutex> g(m_lock);
+    return (int)m_vWaiting.size();
+}
+
+unsigned long long DispatchArbiter::GetAdmitted() {
+    std::lock_guard<std::mutex> g(m_lock);
+    return m_uiAdmitted;
+}
+
 const char *AccClassString(ACCELERATOR_CLASS cls) {
     static const char *names[] = {"ACCELERATOR_CLASS_DIRECT_X", "ACCELERATOR_CLASS_OPEN_CL",
                                   "ACCELERATOR_CLASS_CUDA",     "ACCELERATOR_CLASS_REFERENCE",
@@ -439,8 +505,22 @@ BOOL GetUseHost() { return RT().useHost; }
 BOOL GetUseHIP() { return RT().useHIP; }
 void SetMaximumConcurrency(int n) { RT().maxConcurrency = n; }
 int GetMaximumConcurrency() { return RT().maxConcurrency; }
-void SetMaximumHostConcurrency(int n) { RT().maxHostConcurrency = n < 1 ? 1 : n; }
+void SetMaximumHostConcurrency(int n) {
+    RT().maxHostConcurrency = n < 1 ? 1 : n;
+    RT().hostArbiter.SetSlots(RT().maxHostConcurrency);
+}
 int GetMaximumHostConcurrency() { return RT().maxHostConcurrency; }
+void SetMaximumDeviceDispatchConcurrency(int n) { RT().hipArbiter.SetSlots(n < 1 ? 1 : n); }
+int GetMaximumDeviceDispatchConcurrency() { return RT().hipArbiter.GetSlots(); }
+void GlobalQuiesce() {
+    RT().hostArbiter.Quiesce(TRUE);
+    RT().hipArbiter.Quiesce(TRUE);
+}
+void EndGlobalQuiescence() {
+    RT().hostArbiter.Quiesce(FALSE);
+    RT().hipArbiter.Quiesce(FALSE);
+}
+int GetWaitingDispatchCount() { return RT().hostArbiter.GetWaiting() + RT().hipArbiter.GetWaiting(); }
 void SetDefaultChannelCapacity(int n) { RT().channelCapacity = n < 1 ? 1 : n; }
 int GetDefaultChannelCapacity() { return RT().channelCapacity; }
 void SetTaskThreadPoolSize(UINT n) { RT().threadPoolSize = n; }
