@@ -141,7 +141,8 @@ protected:
  * (reference: ptask/MetaPort.h; general iteration MetaPort.cpp:927-1010). */
 class MetaPort : public InputPort {
 public:
-    MetaPort(DatablockTemplate *pTemplate, UINT uiId, const char *lpszBinding, UINT uiFormalIndex);
+    MetaPort(DatablockTemplate *pTemplate, UINT uiId, const char *lpszBinding, UINT uiFormalIndex,
+             UINT uiAllocationOutput = (UINT)-1);
     virtual ~MetaPort();
     BOOL IsFormalParameter() override { return FALSE; }
     BOOL IsReady() override;

@@ -277,3 +277,13 @@ void GroupbyCoalesce(unsigned int nArgs, void **a) {
     coalesce_int_float(7, args);
     free(counts);
 }
+
+/* GroupbyCoalesceSizes(groups, &coalescedBytes, &groupCountsBytes): the exact byte sizes of the
+ * coalesced layout (4 (rows + G)) and of the per-group counts (4 G), at least 4 each. */
+void GroupbyCoalesceSizes(unsigned int n, void **a) {
+    (void)n;
+    const long long *groups = (const long long *)a[0];
+    const long long G = groups[0], rows = groups[1];
+    *(int *)a[1] = (int)(4 * (rows + G > 0 ? rows + G : 1));
+    *(int *)a[2] = (int)(4 * (G > 0 ? G : 1));
+}

@@ -60,6 +60,7 @@ void GroupbyMerge(unsigned int n, void **a);
 void GroupbyReshuffle(unsigned int n, void **a);
 void coalesce_int_float(unsigned int n, void **a);
 void GroupbyCoalesce(unsigned int n, void **a);
+void GroupbyCoalesceSizes(unsigned int n, void **a);
 void HxComputation(unsigned int n, void **a);
 void HyComputation(unsigned int n, void **a);
 void HzComputation(unsigned int n, void **a);

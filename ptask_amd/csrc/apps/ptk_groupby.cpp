@@ -48,6 +48,7 @@ struct Options {
     bool profile = false;
     int gbGrid = 2048;  // at most this many blocks for the two chunk-parallel groupby kernels
     bool coalesce = false;  // append GroupbyCoalesce (the reference's CoalesceGroups layout)
+    bool exact = false;     // ... with its outputs allocated to their exact sizes
 };
 
 uint64_t Fnv1a(const void *p, size_t n) {
@@ -121,13 +122,15 @@ int main(int argc, char **argv) {
         else if (a == "--profile") o.profile = true;
         else if (a == "--gb-grid") o.gbGrid = atoi(next());
         else if (a == "--coalesce") o.coalesce = true;
+        else if (a == "--coalesce-exact") o.coalesce = o.exact = true;
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
     if (o.kernelFile.empty() || o.rows < 0 || o.numKeys < 0 || o.rows > 0x7fffffffLL ||
         (o.workload == "groupby" && (long long)o.numKeys + 1 > PTK_GB_MAX_KEYS)) {
         fprintf(stderr, "usage: ptk_groupby --kernels <groupby.hsaco|scan.hsaco|hosttasks.so> "
                         "[--workload groupby|pfxsum|scan2] [--rows N] [--keys K] [--dist zipf|uniform] "
-                        "[--zipf-s S] [--seed S] [--repeat R] [--warmup W] [--dump file] [--profile] [--coalesce]\n");
+                        "[--zipf-s S] [--seed S] [--repeat R] [--warmup W] [--dump file] [--profile] [--coalesce] "
+                        "[--coalesce-exact]\n");
         return 2;
     }
     Runtime::SetExitOnRuntimeFailure(FALSE);
@@ -222,14 +225,38 @@ int main(int argc, char **argv) {
                                                                    (UINT)(n + o.numKeys + 1), 1, 1);
             DatablockTemplate *tMeta = Runtime::GetDatablockTemplate((char *)"gbGroupCounts", sizeof(int32_t),
                                                                      (UINT)(o.numKeys + 1), 1, 1);
-            Port *kIn4[4] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"outVals", 0),
-                             Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"outKeys", 1),
-                             Runtime::CreatePort(INPUT_PORT, tGroups, uid++, (char *)"groups", 2),
-                             Runtime::CreatePort(INPUT_PORT, tOff, uid++, (char *)"offsets", 3)};
+            std::vector<Port *> kIn4 = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"outVals", 0),
+                                        Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"outKeys", 1),
+                                        Runtime::CreatePort(INPUT_PORT, tGroups, uid++, (char *)"groups", 2),
+                                        Runtime::CreatePort(INPUT_PORT, tOff, uid++, (char *)"offsets", 3)};
             Port *kOut2[2] = {Runtime::CreatePort(OUTPUT_PORT, tCo, uid++, (char *)"coalesced", 4),
                               Runtime::CreatePort(OUTPUT_PORT, tMeta, uid++, (char *)"groupCounts", 5)};
-            Task *tk = g->AddTask(K("GroupbyCoalesce"), 4, kIn4, 2, kOut2, "GroupbyCoalesce");
+            Port *sOut[2] = {nullptr, nullptr};
+            if (o.exact) {
+                // variable-size outputs (MetaPort.cpp:724, MF_ALLOCATION_SIZE): GroupbyCoalesceSizes
+                // turns the groups header into the two byte counts; each arrives on a meta port
+                // bound to the output it sizes (its in/out routing index), so the pulled blocks hold
+                // exactly rows + G and G words, the reference's wire format
+                DatablockTemplate *tBytes = Runtime::GetDatablockTemplate((char *)"gbBytes", sizeof(int32_t), 1, 1, 1);
+                Port *sIn[1] = {Runtime::CreatePort(INPUT_PORT, tGroups, uid++, (char *)"groups", 0)};
+                sOut[0] = Runtime::CreatePort(OUTPUT_PORT, tBytes, uid++, (char *)"coalescedBytes", 1);
+                sOut[1] = Runtime::CreatePort(OUTPUT_PORT, tBytes, uid++, (char *)"groupCountsBytes", 2);
+                Task *ts = g->AddTask(K("GroupbyCoalesceSizes"), 1, sIn, 2, sOut, "GroupbyCoalesceSizes");
+                geometry(ts, 1, 64);
+                g->AddInternalChannel(oOut[1], sIn[0], "offsets->sizes(groups)");
+                for (int q = 0; q < 2; ++q) {
+                    Port *m = Runtime::CreatePort(META_PORT, tBytes, uid++, (char *)(q ? "groupCountsSize" : "coalescedSize"),
+                                                  (UINT)-1, (UINT)q);
+                    m->SetMetaFunction(MF_ALLOCATION_SIZE);
+                    kIn4.push_back(m);
+                }
+            }
+            Task *tk = g->AddTask(K("GroupbyCoalesce"), (UINT)kIn4.size(), kIn4.data(), 2, kOut2, "GroupbyCoalesce");
             geometry(tk, chunkGrid, 512);
+            if (o.exact) {
+                g->AddInternalChannel(sOut[0], kIn4[4], "sizes->coalesce(coalescedBytes)");
+                g->AddInternalChannel(sOut[1], kIn4[5], "sizes->coalesce(groupCountsBytes)");
+            }
             g->AddInternalChannel(rOut[1], kIn4[0], "shuffle->coalesce(vals)");
             g->AddInternalChannel(rOut[0], kIn4[1], "shuffle->coalesce(keys)");
             g->AddInternalChannel(oOut[1], kIn4[2], "offsets->coalesce(groups)");

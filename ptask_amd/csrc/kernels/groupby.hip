@@ -81,3 +81,15 @@ GroupbyCoalesce(const int *__restrict__ vals, const int *__restrict__ keys, cons
     const long long chunks = (groups[1] + PTK_GB_CHUNK - 1) / PTK_GB_CHUNK;
     for (long long c = blockIdx.x; c < chunks; c += gridDim.x) gb_coalesce_chunk(vals, keys, groups, offsets, c, out, meta, L);
 }
+
+// Byte sizes of GroupbyCoalesce's two outputs for a graph that allocates them exactly (through
+// MF_ALLOCATION_SIZE meta ports, MetaPort.cpp:724): coalesced = 4 (rows + G), groupCounts = 4 G,
+// from the groups header {G, rows, errors}; at least 4 bytes each.  One thread.
+//   GroupbyCoalesceSizes(long long* groups, int* coalescedBytes, int* groupCountsBytes)
+extern "C" __global__ void __launch_bounds__(64)
+GroupbyCoalesceSizes(const long long *__restrict__ groups, int *coalescedBytes, int *groupCountsBytes) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const long long G = groups[0], rows = groups[1];
+    *coalescedBytes = (int)(4 * (rows + G > 0 ? rows + G : 1));
+    *groupCountsBytes = (int)(4 * (G > 0 ? G : 1));
+}

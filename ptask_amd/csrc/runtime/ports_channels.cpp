@@ -127,8 +127,11 @@ void StickyPort::Drain() {
 // MetaPort
 // =============================================================================================
 
-MetaPort::MetaPort(DatablockTemplate *pTemplate, UINT uiId, const char *lpszBinding, UINT uiFormalIndex)
-    : InputPort(pTemplate, uiId, lpszBinding, uiFormalIndex, (UINT)-1, META_PORT),
+// uiAllocationOutput: for an allocation-size meta port, the index of the output whose blocks it
+// sizes (the reference's CreatePort in/out index, MetaPort.cpp:724)
+MetaPort::MetaPort(DatablockTemplate *pTemplate, UINT uiId, const char *lpszBinding, UINT uiFormalIndex,
+                   UINT uiAllocationOutput)
+    : InputPort(pTemplate, uiId, lpszBinding, uiFormalIndex, uiAllocationOutput, META_PORT),
       m_pIterationBlock(nullptr), m_uiIterMax(0), m_uiIterCount(0), m_pAllocationPort(nullptr) {}
 
 MetaPort::~MetaPort() { PTSRELEASE(m_pIterationBlock); }

@@ -446,7 +446,7 @@ Port *CreatePort(PORTTYPE type, DatablockTemplate *pTemplate, UINT uiId, char *l
             p = new StickyPort(pTemplate, uiId, lpszVariableBinding, nKernelParameterIndex);
             break;
         case META_PORT:
-            p = new MetaPort(pTemplate, uiId, lpszVariableBinding, nKernelParameterIndex);
+            p = new MetaPort(pTemplate, uiId, lpszVariableBinding, nKernelParameterIndex, nInOutParmOutputIndex);
             break;
         case INITIALIZER_PORT:
             p = new InitializerPort(pTemplate, uiId, lpszVariableBinding, nKernelParameterIndex,

@@ -147,3 +147,28 @@ def test_hip_coalesce_equals_reference_layout(tmp_path, dist, n, nk):
         assert np.array_equal(np.sort(co[pos + 1:pos + 1 + cnt]), np.sort(want[pos + 1:pos + 1 + cnt]))
         pos += 1 + cnt
     assert pos == want.size
+
+
+@pytest.mark.parametrize("dist,n,nk", [("zipf", 300_001, 65535), ("uniform", 5000, 99)])
+def test_hip_coalesce_exact_sizes(tmp_path, dist, n, nk):
+    """--coalesce-exact on the GPU: the coalesced block holds exactly rows + G words and the
+    counts block G words (sizes from GroupbyCoalesceSizes through MF_ALLOCATION_SIZE meta
+    ports); keys at group starts and counts exact, values per group equal as multisets."""
+    from test_oracle_groupby import coalesce_brute, load_dump_by_sizes
+    d = str(tmp_path / "e.bin")
+    r = run_app("--kernels", paths.kernel_path("groupby.hsaco"), "--rows", n, "--num-keys", nk,
+                "--dist", dist, "--coalesce-exact", "--dump", d)
+    assert r["ok"] and r["backend"] == "hip"
+    keys, vals, out = load_dump_by_sizes(d, n, r)
+    g, head, offsets, eok, eov = oracle.groupby(keys, vals, nk)
+    want, wmeta = coalesce_brute(g, eov)
+    G = g.shape[0]
+    co, meta = out["coalesced"].view(np.int32), out["groupCounts"].view(np.int32)
+    assert co.size == int(head[1]) + G and meta.size == G
+    assert np.array_equal(meta, wmeta)
+    pos = 0
+    for key, cnt, _ in g:
+        assert co[pos] == key
+        assert np.array_equal(np.sort(co[pos + 1:pos + 1 + cnt]), np.sort(want[pos + 1:pos + 1 + cnt]))
+        pos += 1 + cnt
+    assert pos == co.size

@@ -224,6 +224,35 @@ def test_host_groupby_graph_coalesce(tmp_path):
     assert np.array_equal(co[:want.size], want) and np.array_equal(meta[:G], wmeta)
 
 
+def load_dump_by_sizes(path, n, r):
+    """Inputs (keys, vals) and each output of a groupby run, cut by the sizes the run reports."""
+    raw = np.fromfile(path, dtype=np.uint8)
+    keys, vals = raw[:4 * n].view(np.int32), raw[4 * n:8 * n].view(np.int32)
+    out, off = {}, 8 * n
+    for o in r["outputs"]:
+        out[o["name"]] = raw[off:off + o["bytes"]]
+        off += o["bytes"]
+    assert off == raw.size
+    return keys, vals, out
+
+
+@pytest.mark.parametrize("n,nk", [(20000, 511), (777, 0), (5000, 65535)])
+def test_host_groupby_graph_coalesce_exact_sizes(tmp_path, n, nk):
+    """--coalesce-exact: GroupbyCoalesceSizes feeds MF_ALLOCATION_SIZE meta ports, so the
+    coalesced block holds exactly rows + G words and the counts block G words (the reference's
+    variable-size CoalesceGroups output), with the same content as the fixed-capacity form."""
+    d = str(tmp_path / "e.bin")
+    r = run_app("--kernels", HOSTLIB, "--rows", n, "--num-keys", nk, "--coalesce-exact", "--dump", d)
+    assert r["ok"]
+    keys, vals, out = load_dump_by_sizes(d, n, r)
+    g, head, offsets, eok, eov = oracle.groupby(keys, vals, nk)
+    want, wmeta = coalesce_brute(g, eov)
+    G = g.shape[0]
+    assert out["coalesced"].size == 4 * (int(head[1]) + G) and out["groupCounts"].size == 4 * G
+    assert np.array_equal(out["coalesced"].view(np.int32), want)
+    assert np.array_equal(out["groupCounts"].view(np.int32), wmeta)
+
+
 def load_groupby_dump_prefix(path, n, nk):
     raw = np.fromfile(path, dtype=np.uint8)
     off = 0
