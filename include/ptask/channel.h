@@ -92,6 +92,7 @@ public:
     virtual void SetPredicationType(CHANNELENDPOINTTYPE e, CHANNELPREDICATE ePredicate,
                                     PREDICATE_FAILURE_ACTION pfa);
     virtual BOOL IsPredicated(CHANNELENDPOINTTYPE e) { return m_ePredicate[e] != CGATEFN_NONE; }
+    PREDICATE_FAILURE_ACTION GetPredicateFailureAction(CHANNELENDPOINTTYPE e) { return m_ePFA[e]; }
     BOOL PassesPredicate(CHANNELENDPOINTTYPE eEndpoint, Datablock *pBlock);
     void SetViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY e) { m_eViewPolicy = e; }
     VIEWMATERIALIZATIONPOLICY GetViewMaterializationPolicy() { return m_eViewPolicy; }
@@ -163,6 +164,7 @@ public:
     Datablock *Pull(DWORD dwTimeout = 0xFFFFFFFF) override;
     /* Control code stamped on produced blocks (e.g. DBCTLC_BEGINITERATION). */
     void SetInitialPropagatedControlSignal(CONTROLSIGNAL c) { m_luiControl = c; }
+    CONTROLSIGNAL GetInitialPropagatedControlSignal() { return m_luiControl; }
     Datablock *Produce();
 
 private:

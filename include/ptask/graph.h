@@ -78,6 +78,23 @@ public:
     virtual PTRESULT CheckGraphSemantics(BOOL bVerbose = TRUE, BOOL bFailOnWarning = FALSE);
     UINT GetTaskCount() { return (UINT)m_vTasks.size(); }
     Task *GetTaskByIndex(UINT i) { return m_vTasks[i]; }
+    /* Every channel in creation order, the members of a multi-channel included. */
+    UINT GetChannelCount() { return (UINT)m_vChannels.size(); }
+    Channel *GetChannelByIndex(UINT i) { return m_vChannels[i]; }
+    /* Writes the graph's structure (templates, kernels, tasks and ports, channels, bindings,
+     * predication, geometry) as XML (reference: Graph::Serialize, graph.cpp:6452-6468,
+     * XMLWriter::WriteGraph XMLWriter.cpp:124-245).  Fails (HandleError, FALSE) on what has no
+     * text form: a geometry estimator or a user-defined predicate function, duplicate port UIDs
+     * or channel names. */
+    virtual BOOL Serialize(const char *lpszFileName);
+    /* Builds the graph written by Serialize into this empty graph: templates are registered with
+     * the runtime (an existing template of the same name must match), kernels are loaded with
+     * Runtime::GetCompiledKernel, then tasks, channels and bindings are recreated (reference:
+     * Graph::Deserialize, graph.cpp:6477-6490, XMLReader::ReadGraph XMLReader.cpp:383-404).
+     * A kernel file recorded for CUDA (.ptx, .cu) is looked for as the HIP code object of the
+     * same path and stem (.hsaco), and CUDA tasks become HIP tasks.  Returns FALSE (after
+     * HandleError) on a malformed file or a kernel that cannot be loaded. */
+    virtual BOOL Deserialize(const char *lpszFileName);
     /* Total dispatches since Run(), across all tasks. */
     unsigned long long GetDispatchCount();
     /* Blocks until no task is ready or dispatching and every runner is idle (or the timeout in
@@ -99,8 +116,10 @@ public:
 private:
     void RunnerProc(UINT uiIndex);
     void RegisterChannel(Channel *pChannel);
+    std::string SerializeLocked();
 
     std::string m_strName;
+    bool m_bUnnamed;  // constructed without a name: Deserialize takes the file's
     GRAPHSTATE m_eState;
     std::recursive_mutex m_lock;
     std::condition_variable_any m_cvReady;

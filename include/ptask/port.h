@@ -49,6 +49,7 @@ public:
     METAFUNCTION GetMetaFunction() { return m_eMetaFunction; }
     /* A sticky or in/out block that survives across dispatches until replaced. */
     void SetPermanentBlock(BOOL b) { m_bPermanentBlock = b; }
+    BOOL IsPermanentBlock() { return m_bPermanentBlock; }
 
     virtual void BindChannel(Channel *pChannel);
     virtual void BindControlChannel(Channel *pChannel);
@@ -64,6 +65,7 @@ public:
     void AddControlPropagationPort(Port *pOutput) { m_vControlPropagationPorts.push_back(pOutput); }
     std::vector<Port *> &GetControlPropagationPorts() { return m_vControlPropagationPorts; }
     void AddControlPropagationChannel(Channel *p) { m_vControlPropagationChannels.push_back(p); }
+    std::vector<Channel *> &GetControlPropagationChannels() { return m_vControlPropagationChannels; }
     /* Pushes a control code to every output port / channel this port propagates to. */
     void PropagateControlSignal(CONTROLSIGNAL luiCode);
     void AddGatedPort(Port *pGated) { m_vGatedPorts.push_back(pGated); }
@@ -75,6 +77,7 @@ public:
     void SetDescribedPort(Port *p) { m_pDescribedPort = p; }
     Port *GetDescribedPort() { return m_pDescribedPort; }
     void AddIterationTarget(Port *p) { m_vIterationTargets.push_back(p); }
+    std::vector<Port *> &GetIterationTargets() { return m_vIterationTargets; }
 
 protected:
     PORTTYPE m_eType;
@@ -200,6 +203,7 @@ public:
     /* The controller saw EOF: flip the gate away from its initial state for one push
      * (reference: OutputPort::SignalGate, OutputPort.cpp:1363). */
     void SignalGate() { m_bGateOpen = !m_bInitiallyOpen; }
+    BOOL IsInitiallyOpen() { return m_bInitiallyOpen; }
     BOOL IsGateOpen() { return m_pControlPort == nullptr || m_bGateOpen; }
     BlockPool *GetBlockPool() { return m_pPool; }
     void ReleasePool();

@@ -41,11 +41,20 @@ public:
     /* Explicit launch geometry, honoured exactly (reference: cutask.cpp:633-697). */
     virtual void SetBlockAndGridSize(PTASKDIM3 grid, PTASKDIM3 block);
     virtual void SetGeometryEstimator(LPFNGEOMETRYESTIMATOR lpfn) { m_lpfnEstimator = lpfn; }
+    LPFNGEOMETRYESTIMATOR GetGeometryEstimator() { return m_lpfnEstimator; }
+    void GetComputeGeometry(int *tgx, int *tgy, int *tgz) { *tgx = m_tgx; *tgy = m_tgy; *tgz = m_tgz; }
+    /* TRUE once SetBlockAndGridSize was called; the grid and block then come from it. */
+    BOOL GetBlockAndGridSize(PTASKDIM3 *grid, PTASKDIM3 *block) {
+        *grid = m_grid;
+        *block = m_block;
+        return m_bExplicitGeometry;
+    }
     virtual void SetPriority(int nPTaskPrio, int nProxyOSPrio = 0);
     int GetPriority() { return m_nPriority; }
     BOOL SetAffinity(Accelerator *pAccelerator, AFFINITYTYPE eAffinityType);
     Accelerator *GetMandatoryAccelerator() { return m_pMandatoryAccelerator; }
     void SetUserCodeAllocatesMemory(BOOL b) { m_bUserCodeAllocates = b; }
+    BOOL GetUserCodeAllocatesMemory() { return m_bUserCodeAllocates; }
     ACCELERATOR_CLASS GetAcceleratorClass() { return m_eClass; }
     CompiledKernel *GetCompiledKernel() { return m_pKernel; }
     Graph *GetGraph() { return m_pGraph; }

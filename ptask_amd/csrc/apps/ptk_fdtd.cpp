@@ -82,6 +82,8 @@ struct Options {
     bool chain = false;  // run r+1 starts from run r's device-resident outputs
     int warmup = 1;      // chained runs before the timed ones (the first one uploads)
     std::string streams = "task";  // HIP stream policy: one per task, or one shared
+    std::string serialize;    // write the graph as XML (and stop, unless it was read from XML)
+    std::string deserialize;  // read the graph from XML instead of building it
 };
 
 bool Parse(int argc, char **argv, Options &o) {
@@ -118,6 +120,8 @@ bool Parse(int argc, char **argv, Options &o) {
         else if (a == "--chain") o.chain = true;
         else if (a == "--warmup") o.warmup = atoi(next("--warmup"));
         else if (a == "--streams") o.streams = next("--streams");
+        else if (a == "--serialize") o.serialize = next("--serialize");
+        else if (a == "--deserialize") o.deserialize = next("--deserialize");
         else {
             fprintf(stderr, "unknown option %s\n", a.c_str());
             return false;
@@ -139,7 +143,7 @@ bool Parse(int argc, char **argv, Options &o) {
         fprintf(stderr, "the overlap forms need (nx+1)(ny+1)(nz+1) < 2^29\n");
         return false;
     }
-    return !o.kernelFile.empty() && o.nx > 0 && o.ny > 0 && o.nz > 0 && o.steps > 0;
+    return (!o.kernelFile.empty() || !o.deserialize.empty()) && o.nx > 0 && o.ny > 0 && o.nz > 0 && o.steps > 0;
 }
 
 uint64_t Fnv1a(const void *p, size_t n) {
@@ -275,6 +279,47 @@ struct FdtdGraph {
         iters = Runtime::GetDatablockTemplate((char *)"fdtditers", sizeof(int), PTPARM_INT);
         g = new Graph("fdtd");
         return o.mode == "looped" ? BuildLooped() : BuildUnrolled();
+    }
+
+    // The graph from an XML file written by Graph::Serialize.  Its boundary is found by type and
+    // template: graph inputs carrying a field, the Config parameters or the iteration count, and
+    // one graph output per field.
+    bool Load() {
+        g = new Graph("fdtd");
+        if (!g->Deserialize(o.deserialize.c_str())) return false;
+        hip = g->GetTaskCount() && g->GetTaskByIndex(0)->GetAcceleratorClass() == ACCELERATOR_CLASS_HIP;
+        for (int f = 0; f < 6; ++f) field[f] = Runtime::GetDatablockTemplate((char *)(std::string(kFieldNames[f]) + "Size").c_str());
+        params = Runtime::GetDatablockTemplate((char *)"fdtdparms");
+        iters = Runtime::GetDatablockTemplate((char *)"fdtditers");
+        for (UINT i = 0; i < g->GetChannelCount(); ++i) {
+            Channel *c = g->GetChannelByIndex(i);
+            DatablockTemplate *t = c->GetTemplate();
+            if (c->GetType() == CT_GRAPH_INPUT) {
+                if (t == params) paramInputs.push_back((GraphInputChannel *)c);
+                else if (t == iters) iterInputs.push_back((GraphInputChannel *)c);
+                for (int f = 0; f < 6; ++f)
+                    if (t == field[f]) fieldInputs[f].push_back((GraphInputChannel *)c);
+            } else if (c->GetType() == CT_GRAPH_OUTPUT) {
+                for (int f = 0; f < 6; ++f)
+                    if (t == field[f]) {
+                        if (outputs[f]) return Runtime::HandleError("two graph outputs carry %s", kFieldNames[f]);
+                        outputs[f] = (GraphOutputChannel *)c;
+                    }
+            }
+        }
+        for (int f = 0; f < 6; ++f) {
+            if (!field[f] || fieldInputs[f].empty() || !outputs[f])
+                return Runtime::HandleError("%s has no graph input or output for %s", o.deserialize.c_str(), kFieldNames[f]);
+            // the field templates fix the grid the file was written for
+            int d1, d2, d3;
+            ptk_fdtd_extent(f, o.nx, o.ny, o.nz, &d1, &d2, &d3);
+            if (field[f]->GetXElementCount() != (UINT)d1 || field[f]->GetYElementCount() != (UINT)d2 ||
+                field[f]->GetZElementCount() != (UINT)d3)
+                return Runtime::HandleError("%s was written for another grid than %dx%dx%d", o.deserialize.c_str(), o.nx, o.ny, o.nz);
+        }
+        if (!params || paramInputs.empty() || (o.mode == "looped" && iterInputs.empty()))
+            return Runtime::HandleError("%s lacks the Config or iteration inputs", o.deserialize.c_str());
+        return true;
     }
 
     // producer[f] = (task index, output slot) of field f within one step
@@ -415,7 +460,7 @@ int main(int argc, char **argv) {
         fprintf(stderr, "usage: ptk_fdtd --kernels <fdtd.hsaco|hosttasks.so> [--n N | --nx --ny --nz] "
                         "[--steps S] [--mode looped|unrolled] [--blocks B] [--repeat R] [--seed S] "
                         "[--form six|halves|fused|overlap|overlap2|overlap3|overlap3p] [--geometry default|reference|compute] [--tile IxJxK] [--dump file] "
-                        "[--eager] [--profile] [--chain]\n");
+                        "[--eager] [--profile] [--chain] [--serialize graph.xml] [--deserialize graph.xml]\n");
         return 2;
     }
     Runtime::SetExitOnRuntimeFailure(FALSE);
@@ -430,9 +475,19 @@ int main(int argc, char **argv) {
     }
     FdtdGraph G;
     G.o = o;
-    if (!G.Build()) {
+    if (!(o.deserialize.empty() ? G.Build() : G.Load())) {
         fprintf(stderr, "graph construction failed: %s\n", Runtime::GetLastErrorText());
         return 1;
+    }
+    if (!o.serialize.empty()) {
+        // write the graph (built here or read from XML) and stop
+        BOOL ok = G.g->Serialize(o.serialize.c_str());
+        printf("{\"ok\": %s, \"serialized\": \"%s\", \"tasks\": %u, \"channels\": %u}\n", ok ? "true" : "false",
+               o.serialize.c_str(), G.g->GetTaskCount(), G.g->GetChannelCount());
+        G.g->Teardown();
+        Graph::DestroyGraph(G.g);
+        Runtime::Terminate();
+        return ok ? 0 : 1;
     }
     // seeded initial fields (element n of field f = ptk_rand_sym_f32(seed, f, n))
     std::vector<std::vector<float>> init(6);

@@ -49,6 +49,8 @@ struct Options {
     int gbGrid = 2048;  // at most this many blocks for the two chunk-parallel groupby kernels
     bool coalesce = false;  // append GroupbyCoalesce (the reference's CoalesceGroups layout)
     bool exact = false;     // ... with its outputs allocated to their exact sizes
+    std::string serialize;    // write the graph as XML (and stop)
+    std::string deserialize;  // read the graph from XML instead of building it
 };
 
 uint64_t Fnv1a(const void *p, size_t n) {
@@ -123,20 +125,22 @@ int main(int argc, char **argv) {
         else if (a == "--gb-grid") o.gbGrid = atoi(next());
         else if (a == "--coalesce") o.coalesce = true;
         else if (a == "--coalesce-exact") o.coalesce = o.exact = true;
+        else if (a == "--serialize") o.serialize = next();
+        else if (a == "--deserialize") o.deserialize = next();
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
-    if (o.kernelFile.empty() || o.rows < 0 || o.numKeys < 0 || o.rows > 0x7fffffffLL ||
+    if ((o.kernelFile.empty() && o.deserialize.empty()) || o.rows < 0 || o.numKeys < 0 || o.rows > 0x7fffffffLL ||
         (o.workload == "groupby" && (long long)o.numKeys + 1 > PTK_GB_MAX_KEYS)) {
         fprintf(stderr, "usage: ptk_groupby --kernels <groupby.hsaco|scan.hsaco|hosttasks.so> "
                         "[--workload groupby|pfxsum|scan2] [--rows N] [--keys K] [--dist zipf|uniform] "
                         "[--zipf-s S] [--seed S] [--repeat R] [--warmup W] [--dump file] [--profile] [--coalesce] "
-                        "[--coalesce-exact]\n");
+                        "[--coalesce-exact] [--serialize graph.xml] [--deserialize graph.xml]\n");
         return 2;
     }
     Runtime::SetExitOnRuntimeFailure(FALSE);
     if (o.profile) Runtime::SetProfilePSDispatch(TRUE);
     if (PTFAILED(Runtime::Initialize())) return 1;
-    const bool hip = ptaskutils::SelectAcceleratorClass(o.kernelFile.c_str()) == ACCELERATOR_CLASS_HIP;
+    bool hip = ptaskutils::SelectAcceleratorClass(o.kernelFile.c_str()) == ACCELERATOR_CLASS_HIP;
     const int n = (int)o.rows;
     std::vector<int32_t> keys, vals;
     MakeInputs(o, keys, vals);
@@ -162,7 +166,35 @@ int main(int argc, char **argv) {
     };
     const long long tableElems = 3ll * ((long long)o.numKeys + 1) + 1;
     const long long groupElems = 3 + 3ll * ((long long)o.numKeys + 1);
-    if (o.workload == "groupby") {
+    if (!o.deserialize.empty()) {
+        // the graph from an XML file written by Graph::Serialize; its inputs are found by the
+        // binding of the port each feeds, its outputs are the graph outputs in creation order
+        if (!g->Deserialize(o.deserialize.c_str())) return 1;
+        hip = g->GetTaskCount() && g->GetTaskByIndex(0)->GetAcceleratorClass() == ACCELERATOR_CLASS_HIP;
+        for (UINT i = 0; i < g->GetChannelCount(); ++i) {
+            Channel *c = g->GetChannelByIndex(i);
+            if (c->GetType() == CT_GRAPH_OUTPUT) {
+                outs.push_back((GraphOutputChannel *)c);
+                outNames.push_back(c->GetName());
+                continue;
+            }
+            if (c->GetType() != CT_GRAPH_INPUT) continue;
+            const std::string b = c->GetBoundPort(CE_DST)->GetVariableBinding();
+            GraphInputChannel *in = (GraphInputChannel *)c;
+            if (b == "keys" || b == "pin") keyIn.push_back(in);
+            else if (b == "vals") valIn.push_back(in);
+            else if (b == "n" || b == "N") nIn.push_back(in);
+            else if (b == "numKeys") kIn.push_back(in);
+            else {
+                fprintf(stderr, "%s: graph input %s feeds unknown port %s\n", o.deserialize.c_str(), c->GetName(), b.c_str());
+                return 1;
+            }
+        }
+        if (keyIn.empty() || outs.empty()) {
+            fprintf(stderr, "%s: no row input or no output\n", o.deserialize.c_str());
+            return 1;
+        }
+    } else if (o.workload == "groupby") {
         DatablockTemplate *tTable = Runtime::GetDatablockTemplate((char *)"gbTable", sizeof(int64_t), (UINT)tableElems, 1, 1);
         DatablockTemplate *tOff = Runtime::GetDatablockTemplate((char *)"gbOffsets", sizeof(int64_t), (UINT)(o.numKeys + 1), 1, 1);
         DatablockTemplate *tGroups = Runtime::GetDatablockTemplate((char *)"gbGroups", sizeof(int64_t), (UINT)groupElems, 1, 1);
@@ -299,6 +331,16 @@ int main(int argc, char **argv) {
     } else {
         fprintf(stderr, "unknown workload %s\n", o.workload.c_str());
         return 2;
+    }
+    if (!o.serialize.empty()) {
+        // write the graph (built here or read from XML) and stop
+        BOOL ok = g->Serialize(o.serialize.c_str());
+        printf("{\"ok\": %s, \"serialized\": \"%s\", \"tasks\": %u, \"channels\": %u}\n", ok ? "true" : "false",
+               o.serialize.c_str(), g->GetTaskCount(), g->GetChannelCount());
+        g->Teardown();
+        Graph::DestroyGraph(g);
+        Runtime::Terminate();
+        return ok ? 0 : 1;
     }
     g->Run();
     Datablock *bn = Runtime::AllocateDatablock(tInt, (void *)&n, sizeof(int), nullptr);

@@ -337,8 +337,8 @@ BOOL Task::AttemptDispatch(std::unique_lock<std::recursive_mutex> *pLock) {
 // =============================================================================================
 
 Graph::Graph(const char *lpszName)
-    : m_strName(lpszName ? lpszName : "graph"), m_eState(PTGS_INITIALIZING), m_bStopRequested(FALSE),
-      m_uiBusyRunners(0), m_bFinalized(FALSE) {
+    : m_strName(lpszName ? lpszName : "graph"), m_bUnnamed(lpszName == nullptr), m_eState(PTGS_INITIALIZING),
+      m_bStopRequested(FALSE), m_uiBusyRunners(0), m_bFinalized(FALSE) {
     AddRef();
 }
 

@@ -246,4 +246,10 @@ Accelerator *AcceleratorForMemorySpace(UINT uiMemSpace);
 // Fails loudly: HandleError with the HIP C ABI's last error text.
 BOOL CheckHip(int status, const char *what);
 
+namespace Runtime {
+// Adds a new template to the runtime's table under its name and returns it, or deletes it and
+// returns the template already registered under that name.
+DatablockTemplate *RegisterTemplate(DatablockTemplate *t);
+}  // namespace Runtime
+
 }  // namespace PTask

@@ -287,7 +287,7 @@ PTRESULT Terminate() {
     return PTASK_OK;
 }
 
-static DatablockTemplate *RegisterTemplate(DatablockTemplate *t) {
+DatablockTemplate *RegisterTemplate(DatablockTemplate *t) {
     RuntimeState &S = RT();
     std::lock_guard<std::recursive_mutex> g(S.lock);
     auto it = S.templates.find(t->GetTemplateName());
