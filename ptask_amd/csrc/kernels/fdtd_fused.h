@@ -228,6 +228,13 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// x-neighbours across the lanes of one wave (a tile row): lane l reads lane l + 1 through the
+// data-parallel-primitive mux (wave_shl:1), a VALU operand modifier instead of an LDS read.
+// Lane 63 reads 0; it is a halo lane of every tile.
+__device__ inline float lane_next(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+
 // Raw buffer access to one field array.  A byte offset at or beyond the array's size reads 0
 // and drops a store (the buffer unit's range check), so a lane outside a field's extent issues
 // the same instruction with bit 31 of its offset set instead of branching around it: every
@@ -530,8 +537,9 @@ __device__ inline void overlapn_tile(const Fields &F, const FDTD_PARAMS &p, int 
             const F3 ho = s == 1 ? S.h : h2[s - 1];  // H_{s-1}(p_s)
             eo[s] = ep;
             hn[s].x = ho.x + c.cH * ((eu.y - ep.y) * c.Cz - (E[2][y + 1][x] - ep.z) * c.Cy);
-            hn[s].y = ho.y + c.cH * ((E[2][y][x + 1] - ep.z) * c.Cx - (eu.x - ep.x) * c.Cz);
-            hn[s].z = ho.z + c.cH * ((E[0][y + 1][x] - ep.x) * c.Cy - (E[1][y][x + 1] - ep.y) * c.Cx);
+            // Ez and Ey at i + 1 from the next lane (the same values its LDS slots hold)
+            hn[s].y = ho.y + c.cH * ((lane_next(ep.z) - ep.z) * c.Cx - (eu.x - ep.x) * c.Cz);
+            hn[s].z = ho.z + c.cH * ((E[0][y + 1][x] - ep.x) * c.Cy - (lane_next(ep.y) - ep.y) * c.Cx);
             L.h[hb][s - 1][0][y][x] = hn[s].x;
             L.h[hb][s - 1][1][y][x] = hn[s].y;
             L.h[hb][s - 1][2][y][x] = hn[s].z;
