@@ -400,7 +400,9 @@ __device__ inline void overlap_tile(const Fields &F, const FDTD_PARAMS &p, int i
 //   barrier
 // Step 0 is the input: E_0(m), E_0(m+1) and H_0(m) are loaded two planes ahead.  Step s-1's
 // values two and one planes behind its own (those of planes p_s and p_s+1) are carried in
-// registers (for T >= 3, the E values are read back from the own slots of the LDS planes).  E_s
+// registers (for T >= 3, the E values are read back from the own slots of the LDS planes).  Ez
+// and Ey at i + 1 come from the next lane, so no Ey plane of E_0 is stored, nor, for T < 3, of
+// any step (an Ey slot is only read by its own lane).  E_s
 // planes are double-buffered by iteration parity (E_s(p_s) is written in B of iteration m and
 // read as a plane in A of iteration m+2); H_s planes are written in A and read in B of the same
 // iteration (HB = 1, two barriers per plane), or by parity (HB = 2, one barrier per plane).  A
@@ -511,7 +513,6 @@ __device__ inline void overlapn_tile(const Fields &F, const FDTD_PARAMS &p, int 
     pad_stores();
     const int y = r + 1, x = l + 1;
     L.e[0][ms & 1][0][y][x] = S0.e.x;
-    L.e[0][ms & 1][1][y][x] = S0.e.y;
     L.e[0][ms & 1][2][y][x] = S0.e.z;
     // carried own values of steps 1..T: planes one and two behind the step's own (index 0 unused)
     F3 h1[T + 1], h2[T + 1], e1[T + 1], e2[T + 1];
@@ -545,7 +546,6 @@ __device__ inline void overlapn_tile(const Fields &F, const FDTD_PARAMS &p, int 
             L.h[hb][s - 1][2][y][x] = hn[s].z;
         }
         L.e[0][b ^ 1][0][y][x] = S.e.x;
-        L.e[0][b ^ 1][1][y][x] = S.e.y;
         L.e[0][b ^ 1][2][y][x] = S.e.z;
         lds_barrier();
         // ---- B: E_s(p_s) for every step (graphfdtd.cpp:243-312: interior cells update, faces
@@ -566,7 +566,7 @@ __device__ inline void overlapn_tile(const Fields &F, const FDTD_PARAMS &p, int 
             en[s].z = (iIn && jIn) ? wz : e.z;
             if (s < T) {  // E_s(p_s) in this parity's buffer was last read in A
                 L.e[s][b][0][y][x] = en[s].x;
-                L.e[s][b][1][y][x] = en[s].y;
+                if (kEFromLds) L.e[s][b][1][y][x] = en[s].y;  // read only at its own slot
                 L.e[s][b][2][y][x] = en[s].z;
             }
         }
