@@ -42,6 +42,9 @@ struct Options {
     int repeat = 1, warmup = 0;
     unsigned long long seed = 0x6a5d39;
     bool profile = false;
+    // HIP stream policy: one shared stream (each EM step is a chain of two dependent tasks, which
+    // then needs no cross-stream event waits), or one per task
+    std::string streams = "shared";
 };
 
 uint64_t Fnv1a(const void *p, size_t n) {
@@ -112,6 +115,7 @@ int main(int argc, char **argv) {
         else if (a == "--data") o.data = next();
         else if (a == "--dump") o.dump = next();
         else if (a == "--profile") o.profile = true;
+        else if (a == "--streams") o.streams = next();
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
     const long long tiles = (o.n + 127) / 128;
@@ -120,10 +124,10 @@ int main(int argc, char **argv) {
         o.k < 1 || o.k > PTK_GMM_MAX_K || o.k > o.n || o.steps < 1 || o.repeat < 1 ||
         (o.mode != "looped" && o.mode != "unrolled") || (long long)o.n * o.d > 0x3fffffffLL ||
         (!o.init.empty() && o.init != "strided" && o.init != "components") ||
-        (o.init == "components" && !o.data.empty())) {
+        (o.init == "components" && !o.data.empty()) || (o.streams != "shared" && o.streams != "task")) {
         fprintf(stderr, "usage: ptk_gmm --kernels <gmm.hsaco|hosttasks.so> [--n N] [--d D<=32] "
                         "[--k K<=64] [--steps S] [--mode looped|unrolled] [--init strided|components] [--blocks B] [--repeat R] "
-                        "[--warmup W] [--seed S] [--data x.f32] [--dump file] [--profile]\n");
+                        "[--warmup W] [--seed S] [--data x.f32] [--dump file] [--profile] [--streams shared|task]\n");
         return 2;
     }
     std::vector<float> X;
@@ -169,6 +173,7 @@ int main(int argc, char **argv) {
     ptk_gmm_fill_table(&dims, model0.data());
 
     Runtime::SetExitOnRuntimeFailure(FALSE);
+    Runtime::SetTaskStreamPolicy(o.streams == "shared" ? PTK_STREAMS_SHARED : PTK_STREAMS_PER_TASK);
     if (o.profile) Runtime::SetProfilePSDispatch(TRUE);
     if (PTFAILED(Runtime::Initialize())) return 1;
     const bool hip = ptaskutils::SelectAcceleratorClass(o.kernelFile.c_str()) == ACCELERATOR_CLASS_HIP;
