@@ -317,8 +317,8 @@ def host_graph_baseline(runs: int = 5) -> dict:
 def main() -> int:
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(every GPU box here has a single MI355X)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the path runs on a "
+              "single MI355X (how it would shard across GPUs is a design, not a measured mode)", file=sys.stderr)
         return 2
     from ptask_amd import hip as H
     if H.device_count() < 1:
