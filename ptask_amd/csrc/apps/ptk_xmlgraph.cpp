@@ -22,7 +22,10 @@
 // out2 = 1.5 * A, out3 = s3 * A.
 //
 // Usage: ptk_xmlgraph (--kernels lib.so | --read graph.xml) [--write graph.xml] [--run]
-//        [--n N] [--s3 S]
+//        [--n N] [--s3 S] [--unwritable estimator|predicate|uid|channel]
+// --unwritable adds what has no text form to the built graph (a geometry estimator function, a
+// user-defined predicate function, a second port with an existing UniqueID, a second channel
+// with an existing name), for the tests of Serialize's refusals.
 // --write without --run writes the graph (built or read) and stops.  --run pushes a seeded A,
 // pulls both outputs, checks them against the closed forms and prints one JSON object with
 // their checksums.
@@ -41,7 +44,7 @@ using namespace PTask;
 namespace {
 
 struct Options {
-    std::string kernels, read, write;
+    std::string kernels, read, write, unwritable;
     bool run = false;
     int n = 64;
     float s3 = 2.0f;
@@ -56,6 +59,13 @@ uint64_t Fnv1a(const void *p, size_t n) {
     }
     return h;
 }
+
+void __stdcall Estimate(UINT, Datablock **, PTASKDIM3 *block, PTASKDIM3 *grid) {
+    *block = PTASKDIM3(1, 1, 1);
+    *grid = PTASKDIM3(1, 1, 1);
+}
+
+BOOL __stdcall Predicate(Channel *, Datablock *, PREDICATE_FAILURE_ACTION) { return TRUE; }
 
 Graph *Build(const Options &o) {
     DatablockTemplate *tVec = Runtime::GetDatablockTemplate((char *)"xgVec", sizeof(float), (UINT)o.n, 1, 1);
@@ -134,6 +144,21 @@ Graph *Build(const Options &o) {
     g->AddOutputChannel(op[0], "probe.out")->SetPredicationType(CE_SRC, CGATEFN_OPEN_ON_ENDITERATION);
     g->BindIterationScope(ip[3], ip[0]);
     g->BindDescriptorPort(ip[0], ip[2], DF_SIZE);
+
+    if (o.unwritable == "estimator") {
+        tp->SetGeometryEstimator(Estimate);
+    } else if (o.unwritable == "predicate") {
+        pa->SetPredicator(CE_DST, Predicate);
+    } else if (o.unwritable == "uid" || o.unwritable == "channel") {
+        // a second probe-like task; its input port reuses scale1.A's UniqueID, or its input
+        // channel reuses the name "s2"
+        Port *in[3] = {Runtime::CreatePort(INPUT_PORT, tVec, o.unwritable == "uid" ? i1[0]->GetUID() : uid++, (char *)"A", 0),
+                       Runtime::CreatePort(STICKY_PORT, tS, uid++, (char *)"s", 1),
+                       Runtime::CreatePort(STICKY_PORT, tN, uid++, (char *)"n", 2)};
+        Port *out[1] = {Runtime::CreatePort(OUTPUT_PORT, tVec, uid++, (char *)"out", 3)};
+        g->AddTask(k, 3, in, 1, out, "extra");
+        g->AddInputChannel(in[0], o.unwritable == "channel" ? "s2" : "extra.A");
+    }
     return g;
 }
 
@@ -153,6 +178,7 @@ int main(int argc, char **argv) {
         else if (a == "--run") o.run = true;
         else if (a == "--n") o.n = atoi(next());
         else if (a == "--s3") o.s3 = (float)atof(next());
+        else if (a == "--unwritable") o.unwritable = next();
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
     if ((o.kernels.empty() == o.read.empty()) || o.n < 1) {

@@ -11,7 +11,8 @@ Checked on the CPU with host tasks:
   remainder task) against the oracle bit for bit, the exact-size groupby graph byte for byte;
 - a file holding only the reference writer's elements (no optional additions) still loads and
   runs;
-- malformed files and template mismatches fail with a message and a nonzero exit.
+- malformed files and template mismatches fail with a message and a nonzero exit, and
+  Serialize refuses function pointers and duplicate port UIDs or channel names without writing.
 
 On the GPU: HIP FDTD graphs read from XML equal the built graphs bit for bit, and a file recorded
 for CUDA (kernel file .ptx, accelerator class 2) loads onto the HIP code object of the same stem.
@@ -295,3 +296,17 @@ def test_gpu_cuda_recorded_file_loads_onto_hip(tmp_path):
     want = direct(nx, ny, nz, steps)
     for f, got in enumerate(load_dump(dump, nx, ny, nz)):
         assert np.array_equal(got, want[f].ravel()), f
+
+
+@pytest.mark.parametrize("what,message", [
+    ("estimator", "geometry estimator function"),
+    ("predicate", "user-defined predicate function"),
+    ("uid", "share UniqueID"),
+    ("channel", "two channels are named 's2'"),
+])
+def test_serialize_refuses_what_has_no_text_form(hostlib, tmp_path, what, message):
+    """Function pointers and ambiguous names: Serialize fails with a message and writes nothing."""
+    a = tmp_path / "a.xml"
+    p = run(XMLAPP, "--kernels", hostlib, "--unwritable", what, "--write", str(a), ok=False)
+    assert p.returncode != 0 and message in p.stderr, p.stderr
+    assert not a.exists() and not (tmp_path / "a.xml.tmp").exists()
