@@ -188,6 +188,9 @@ struct XmlNode {
 class XmlParser {
 public:
     explicit XmlParser(const std::string &s) : m_s(s), m_i(0) {}
+    // A graph document nests at most a handful of levels; deeper input is refused instead of
+    // recursing without bound.
+    static constexpr int kMaxDepth = 256;
 
     void Document(XmlNode *root) {
         Misc();
@@ -239,8 +242,14 @@ private:
         else if (ent == "quot") *out += '"';
         else if (ent == "apos") *out += '\'';
         else if (!ent.empty() && ent[0] == '#') {
-            unsigned long c = ent.size() > 1 && (ent[1] == 'x' || ent[1] == 'X') ? strtoul(ent.c_str() + 2, nullptr, 16)
-                                                                                  : strtoul(ent.c_str() + 1, nullptr, 10);
+            const bool hex = ent.size() > 1 && (ent[1] == 'x' || ent[1] == 'X');
+            const char *digits = ent.c_str() + (hex ? 2 : 1);
+            char *end = nullptr;
+            const unsigned long c = strtoul(digits, &end, hex ? 16 : 10);
+            // a character reference names one Unicode scalar value: digits only, not NUL, not a
+            // surrogate, at most U+10FFFF
+            if (!*digits || *end || c == 0 || c > 0x10FFFF || (c >= 0xD800 && c <= 0xDFFF))
+                Fail("bad character reference &" + ent + ";");
             if (c < 0x80) *out += (char)c;
             else if (c < 0x800) { *out += (char)(0xC0 | (c >> 6)); *out += (char)(0x80 | (c & 63)); }
             else if (c < 0x10000) { *out += (char)(0xE0 | (c >> 12)); *out += (char)(0x80 | ((c >> 6) & 63)); *out += (char)(0x80 | (c & 63)); }
@@ -249,7 +258,8 @@ private:
             Fail("unknown entity &" + ent + ";");
         }
     }
-    void Element(XmlNode *n) {
+    void Element(XmlNode *n, int depth = 0) {
+        if (depth >= kMaxDepth) Fail("elements nested deeper than " + std::to_string(kMaxDepth) + " levels");
         ++m_i;  // '<'
         n->tag = Name();
         // attributes: skipped, quoted values may hold '>'
@@ -288,7 +298,7 @@ private:
             } else if (At("<?")) Skip("?>");
             else if (At("<")) {
                 n->kids.emplace_back();
-                Element(&n->kids.back());
+                Element(&n->kids.back(), depth + 1);
             } else if (m_s[m_i] == '&') Entity(&text);
             else text += m_s[m_i++];
         }

@@ -86,5 +86,6 @@ def read_dump(path: str, N: int, D: int, K: int, steps: int):
     X = np.frombuffer(raw[:nx], dtype=np.float32).reshape(N, D)
     mu0 = np.frombuffer(raw[nx:nx + K * D * 8], dtype=np.float64).reshape(K, D)
     model = np.frombuffer(raw[nx + K * D * 8:], dtype=np.float64)
-    assert model.size == model_elems(D, K, steps), "dump size does not match N, D, K, steps"
+    if model.size != model_elems(D, K, steps):
+        raise ValueError("dump size does not match N, D, K, steps")
     return X, mu0, parse_model(model, D, K, steps)

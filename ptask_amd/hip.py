@@ -151,7 +151,8 @@ class DeviceBuffer:
     def upload(self, host, stream: "Stream | None" = None) -> None:
         import numpy as np
         a = np.ascontiguousarray(host)
-        assert a.nbytes <= self.nbytes, (a.nbytes, self.nbytes)
+        if a.nbytes > self.nbytes:
+            raise ValueError(f"upload of {a.nbytes} bytes into a {self.nbytes}-byte device buffer")
         if stream is None:
             check(lib().ptk_hip_memcpy_htod(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p),
                                             a.nbytes), "memcpy_htod")
@@ -163,11 +164,18 @@ class DeviceBuffer:
 
     def download(self, out) -> None:
         import numpy as np
-        assert out.flags["C_CONTIGUOUS"] and out.nbytes <= self.nbytes
+        if not out.flags["C_CONTIGUOUS"]:
+            raise ValueError("download needs a C-contiguous host array")
+        if out.nbytes > self.nbytes:
+            raise ValueError(f"download of {out.nbytes} bytes from a {self.nbytes}-byte device buffer")
         check(lib().ptk_hip_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
                                         out.nbytes), "memcpy_dtoh")
 
     def copy_from(self, other: "DeviceBuffer", nbytes: int, stream: "Stream") -> None:
+        nbytes = int(nbytes)
+        if nbytes < 0 or nbytes > self.nbytes or nbytes > other.nbytes:
+            raise ValueError(f"copy of {nbytes} bytes between device buffers of {other.nbytes} "
+                             f"(source) and {self.nbytes} (destination) bytes")
         check(lib().ptk_hip_memcpy_dtod_async(C.c_void_p(self.ptr), C.c_void_p(other.ptr), nbytes,
                                               stream.handle), "memcpy_dtod_async")
 

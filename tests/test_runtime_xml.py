@@ -245,6 +245,12 @@ def test_reference_schema_file_loads_and_runs(tmp_path):
     (lambda t: t.replace("<Operation>ScaleVector</Operation>", "<Operation>NoSuchKernel</Operation>"),
      "could not be loaded"),
     (lambda t: t.replace("</ScopedPort>", "9</ScopedPort>"), "unknown port"),
+    (lambda t: t.replace("</Graph>", "<Z>" * 300 + "</Z>" * 300 + "</Graph>"), "nested deeper"),
+    (lambda t: t.replace("<Operation>ScaleVector", "<Operation>Scale&#;Vector"), "bad character reference"),
+    (lambda t: t.replace("<Operation>ScaleVector", "<Operation>Scale&#0;Vector"), "bad character reference"),
+    (lambda t: t.replace("<Operation>ScaleVector", "<Operation>Scale&#x110000;Vector"),
+     "bad character reference"),
+    (lambda t: t.replace("<Operation>ScaleVector", "<Operation>Scale&#12a;Vector"), "bad character reference"),
 ])
 def test_malformed_files_fail_loudly(hostlib, tmp_path, edit, message):
     a, b = str(tmp_path / "a.xml"), str(tmp_path / "b.xml")
