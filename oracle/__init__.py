@@ -124,9 +124,10 @@ def groupby(keys: np.ndarray, vals: np.ndarray, num_keys: int):
     return groups[3:3 + 3 * G].reshape(G, 3), groups[:3].copy(), offsets, ok, ov
 
 
-def gmm_em(X: np.ndarray, means_init: np.ndarray, iters: int):
+def gmm_em(X: np.ndarray, means_init: np.ndarray, iters: int, threads: int = 0):
     """Diagonal-covariance EM as CPU_EM drives NR3's Gaumixmod (oracle/gmm.c): the constructor's
-    E+M step, then `iters` more.  Returns (mu[K,D], var[K,D], frac[K], loglike[iters+1])."""
+    E+M step, then `iters` more.  Returns (mu[K,D], var[K,D], frac[K], loglike[iters+1]).
+    threads > 0 runs the multi-threaded form (oracle/parallel.c), bitwise equal to the serial one."""
     L = lib()
     X = np.ascontiguousarray(X, dtype=np.float64)
     N, D = X.shape
@@ -135,16 +136,23 @@ def gmm_em(X: np.ndarray, means_init: np.ndarray, iters: int):
     var = np.empty((K, D), np.float64)
     frac = np.empty(K, np.float64)
     ll = np.empty(iters + 1, np.float64)
-    fn = L.ptk_oracle_gmm_em
-    fn.argtypes = [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4
-    fn(X.ctypes.data, N, D, K, iters, mu.ctypes.data, var.ctypes.data, frac.ctypes.data,
-       ll.ctypes.data)
+    args = [X.ctypes.data, N, D, K, iters, mu.ctypes.data, var.ctypes.data, frac.ctypes.data,
+            ll.ctypes.data]
+    if threads > 0:
+        fn = L.ptk_oracle_gmm_em_mt
+        fn.argtypes = [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4 + [C.c_int]
+        fn(*args, threads)
+    else:
+        fn = L.ptk_oracle_gmm_em
+        fn.argtypes = [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4
+        fn(*args)
     return mu, var, frac, ll
 
 
-def pca_gs(X: np.ndarray, K: int, J: int, center: bool = True):
+def pca_gs(X: np.ndarray, K: int, J: int, center: bool = True, threads: int = 0):
     """Gram-Schmidt PCA as pca_cpu runs it (oracle/pca.c).  X is M x N (any layout; it is passed
-    column-major).  Returns (T[M,K], P[N,K], R[M,N], L[K])."""
+    column-major).  Returns (T[M,K], P[N,K], R[M,N], L[K]).  threads > 0 runs the multi-threaded
+    form (oracle/parallel.c), bitwise equal to the serial one."""
     L_ = lib()
     X = np.asfortranarray(X, dtype=np.float64)
     M, N = X.shape
@@ -152,11 +160,26 @@ def pca_gs(X: np.ndarray, K: int, J: int, center: bool = True):
     P = np.zeros((N, K), order="F")
     R = np.zeros((M, N), order="F")
     Lv = np.zeros(K)
-    fn = L_.ptk_oracle_pca_gs
-    fn.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4
-    fn(X.ctypes.data, M, N, K, J, int(center), T.ctypes.data, P.ctypes.data, R.ctypes.data,
-       Lv.ctypes.data)
+    args = [X.ctypes.data, M, N, K, J, int(center), T.ctypes.data, P.ctypes.data, R.ctypes.data,
+            Lv.ctypes.data]
+    if threads > 0:
+        fn = L_.ptk_oracle_pca_gs_mt
+        fn.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4 + [C.c_int]
+        fn(*args, threads)
+    else:
+        fn = L_.ptk_oracle_pca_gs
+        fn.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4
+        fn(*args)
     return T, P, R, Lv
+
+
+def default_threads() -> int:
+    """Threads for the oracle's multi-threaded forms: OMP_NUM_THREADS when set, else the CPUs
+    this process may run on, at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, min(16, len(os.sched_getaffinity(0))))
 
 
 def fdtd_update(field: int, own: np.ndarray, a: np.ndarray, b: np.ndarray,

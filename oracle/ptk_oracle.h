@@ -52,6 +52,13 @@ void ptk_oracle_gmm_em(const double *X, long long N, int D, int K, int iters, do
 void ptk_oracle_pca_gs(const double *X, int M, int N, int K, int J, int center, double *T, double *P,
                        double *R, double *L);
 
+/* Multi-threaded forms (oracle/parallel.c): bitwise equal to the serial ones for any thread
+ * count (threads <= 0: the OpenMP default). */
+void ptk_oracle_gmm_em_mt(const double *X, long long N, int D, int K, int iters, double *mu,
+                          double *var, double *frac, double *loglike, int threads);
+void ptk_oracle_pca_gs_mt(const double *X, int M, int N, int K, int J, int center, double *T,
+                          double *P, double *R, double *L, int threads);
+
 /* Host-task ABI forms (void f(UINT, void**)), loaded by name by the HostAccelerator. */
 void pfxsum(unsigned int n, void **a);
 void GroupbyAccumulate(unsigned int n, void **a);

@@ -6,8 +6,9 @@
 // Request mix (weights --mix F:G:B): FDTD cubic edge uniform on [32,192] with 20..100 steps,
 // GMM N uniform on [10K, 1M], D = 16, K uniform on [8, 32], --gmm-steps EM steps, groupby rows
 // uniform on [1M, 16M] (Zipf 1.1 over 65536 keys).  Latency = completion - scheduled arrival.
-// --identity N re-runs N of the requests alone on a fresh executor and compares their result
-// checksums with the in-mix ones.  Prints one JSON object.
+// --identity N re-runs N of the requests alone, each on a fresh executor, and compares their
+// result checksums with the in-mix ones; --identity -1 re-runs every request alone (one at a
+// time on one fresh executor, so no other instance is resident).  Prints one JSON object.
 
 #include <algorithm>
 #include <chrono>
@@ -196,7 +197,24 @@ int main(int argc, char **argv) {
     }
     // identity: some requests again, each alone on a fresh executor
     int idChecked = 0, idEqual = 0;
-    if (o.identity > 0) {
+    if (o.identity < 0) {
+        ptk_batch *A = nullptr;
+        if (ptk_batch_create(o.kernels.c_str(), arena, mode, &A)) {
+            fprintf(stderr, "create: %s\n", ptk_batch_last_error());
+            return 1;
+        }
+        if (o.direct) ptk_batch_set_launch(A, 0);
+        for (size_t i = 0; i < reqs.size(); ++i) {
+            Req alone = reqs[i];
+            unsigned long long a[8], m[8];
+            const bool eq = !SubmitReq(A, alone, 0) && !ptk_batch_drain(A) && !ptk_batch_checksums(A, alone.id, a) &&
+                            !ptk_batch_checksums(B, reqs[i].id, m) && !memcmp(a, m, sizeof(a));
+            if (!eq) fprintf(stderr, "identity: request %zu (kind %d) differs alone and in the mix\n", i, reqs[i].kind);
+            ++idChecked;
+            idEqual += eq ? 1 : 0;
+        }
+        ptk_batch_destroy(A);
+    } else if (o.identity > 0) {
         const size_t stride = std::max<size_t>(1, reqs.size() / (size_t)o.identity);
         for (size_t i = 0; i < reqs.size() && idChecked < o.identity; i += stride) {
             ptk_batch *A = nullptr;

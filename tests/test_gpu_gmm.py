@@ -68,8 +68,8 @@ def em_on_device(hip, gmod, X, K, steps, blocks):
     return gmm.parse_model(out, D, K, steps), mu0
 
 
-def check_against_oracle(X, mu0, m, steps, label):
-    mu, var, frac, ll = oracle.gmm_em(X.astype(np.float64), mu0, steps - 1)
+def check_against_oracle(X, mu0, m, steps, label, threads=0):
+    mu, var, frac, ll = oracle.gmm_em(X.astype(np.float64), mu0, steps - 1, threads=threads)
     scale = float(np.abs(X).max())
     e_ll = float(np.max(np.abs(m.loglike - ll) / np.abs(ll)))
     e_mu = float(np.max(np.abs(m.mu - mu))) / scale

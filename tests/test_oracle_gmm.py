@@ -80,3 +80,17 @@ def test_reference_fixture_recovers_generating_mixture():
     assert np.allclose(frac, [0.6, 0.3, 0.1], atol=0.02)
     gm = sklearn_em(X, [[5.0], [9.0], [20.0]], 20)
     assert np.allclose(mu, gm.means_, rtol=1e-9)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_form_is_bitwise_the_serial_one(threads):
+    """oracle/parallel.c splits work only across points and components and keeps every sum in
+    point order, so it equals the serial EM bit for bit for any thread count."""
+    rng = np.random.default_rng(threads)
+    N, D, K = 3001, 7, 5
+    X = np.concatenate([rng.normal(c, 0.7, (N // K + 1, D)) for c in range(K)])[:N]
+    init = X[np.arange(K) * (N // K)]
+    a = oracle.gmm_em(X, init, 4)
+    b = oracle.gmm_em(X, init, 4, threads=threads)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

@@ -71,3 +71,16 @@ def test_first_component_one_iteration_by_hand():
     assert np.allclose(P[:, 0], p) and np.isclose(L[0], np.linalg.norm(t))
     assert np.allclose(T[:, 0], t)            # T = (t / L) * L
     assert np.allclose(R, X - np.outer(t, p))
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+@pytest.mark.parametrize("center", [True, False])
+def test_threaded_form_is_bitwise_the_serial_one(threads, center):
+    """oracle/parallel.c splits work only across matrix rows or columns and keeps every sum in
+    index order, so it equals the serial PCA bit for bit (row blocks of 512 are crossed here)."""
+    rng = np.random.default_rng(threads)
+    X = rng.standard_normal((1500, 37))
+    a = oracle.pca_gs(X, K=4, J=3, center=center)
+    b = oracle.pca_gs(X, K=4, J=3, center=center, threads=threads)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

@@ -44,8 +44,8 @@ def run_pca(kernels, tmp_path, M, N, K, J, *extra, timeout=300):
     return js, X, T, P, R, L
 
 
-def compare(X, T, P, R, L, K, J, center, tol):
-    eT, eP, eR, eL = oracle.pca_gs(X.astype(np.float64), K, J, center)
+def compare(X, T, P, R, L, K, J, center, tol, threads=0):
+    eT, eP, eR, eL = oracle.pca_gs(X.astype(np.float64), K, J, center, threads=threads)
     scale = float(np.abs(X).max())  # the residual can vanish (K = rank): measure against X
     err = {"L": float(np.max(np.abs(L - eL) / eL)),
            "P": float(np.max(np.abs(P - eP))),
