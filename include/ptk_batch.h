@@ -8,7 +8,7 @@
  * one EM-GMM graph (E+M per iteration, ptaskUnitTest/graphemgmm.cpp:124-211).  There is no
  * counterpart of the executor in the reference: it is the batching and serving layer of this
  * build, and its instances compute exactly what the single graphs compute (bitwise for FDTD,
- * for the groupby keys, counts and sums, and for the GMM model).
+ * for every groupby output, and for the GMM model).
  *
  * Lifecycle: submit (queued, inputs described by seeds) -> admitted at an iteration boundary
  * when its HBM footprint fits the arena (inputs generated on the device, datablocks resident
@@ -109,8 +109,9 @@ int ptk_batch_drain(ptk_batch *b);
 int ptk_batch_status(ptk_batch *b, int id, int *state, double *t_submit, double *t_admit, double *t_done);
 /* Checksums of a completed instance, taken on the device at completion:
  *   FDTD: one per field (bit patterns, position-weighted), [6] unused;
- *   groupby: [0] groups (header and the G triples), [1] reshuffled keys (position-weighted), [2] multiset of (key, value)
- *            rows of the output, [3] multiset of (key, value) rows of the input;
+ *   groupby: [0] groups (header and the G triples), [1] reshuffled keys, [2] reshuffled values (both
+ *            position-weighted), [3] multiset of (key, value) rows of the input, [4] multiset of (key, value) rows of
+ *            the output (equal to [3] when every key is valid);
  *   GMM: [0] model block. */
 int ptk_batch_checksums(ptk_batch *b, int id, unsigned long long out[8]);
 /* Copy a result array of a kept instance to the host.  FDTD: which = field 0..5 (current set);

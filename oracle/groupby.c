@@ -110,8 +110,9 @@ void pfxsum(unsigned int n, void **a) {
 }
 
 /* The chunk / bucket pipeline of the HIP kernels (include/ptk_groupby.h) restated on the host:
- * the same scratch layout, with each chunk's distinct keys in ascending key order and the merge
- * taking chunks in order (one of the orders the device may produce). */
+ * the same scratch layout, with each chunk's distinct keys in ascending key order (the device
+ * writes them in any order inside a bucket; nothing downstream depends on it); merge bases,
+ * group bases and ranks follow row order, as on the device. */
 static int cmp_i32(const void *x, const void *y) {
     const int32_t a = *(const int32_t *)x, b = *(const int32_t *)y;
     return (a > b) - (a < b);
@@ -165,27 +166,56 @@ void GroupbyAccumulate(unsigned int n_, void **a) {
     free(tmp);
 }
 
+/* GroupbyMerge(scratch, n, numKeys): per merge group, each entry's base (the key's rows in the
+ * group's earlier chunks) replaces its sum; the group's per-key totals go to gcnt / gsum. */
 void GroupbyMerge(unsigned int n_, void **a) {
+    (void)n_;
+    const long long n = IARG(a, 1);
+    const int numKeys = IARG(a, 2);
+    if (n < 0 || numKeys < 0 || (long long)numKeys + 1 > PTK_GB_MAX_KEYS) return;
+    PTK_GB_SCRATCH S = ptk_gb_scratch(a[0], n, numKeys);
+    const long long KP = S.buckets * PTK_GB_BUCKET;
+    memset(S.gcnt, 0, sizeof(unsigned) * (size_t)(S.groups * KP));
+    memset(S.gsum, 0, sizeof(uint64_t) * (size_t)(S.groups * KP));
+    for (long long c = 0; c < S.chunks; ++c) {  /* within a merge group, chunks in row order */
+        unsigned *gc = S.gcnt + (c / PTK_GB_GROUP) * KP;
+        uint64_t *gs = S.gsum + (c / PTK_GB_GROUP) * KP;
+        const unsigned d = S.boff[c * (S.buckets + 1) + S.buckets];
+        for (unsigned i = 0; i < d; ++i) {
+            PTK_GB_ENTRY *e = S.entry + c * PTK_GB_CHUNK + i;
+            gs[e->key] += (uint64_t)e->sum;
+            e->sum = (long long)gc[e->key];  /* the entry's base inside its group replaces its sum */
+            gc[e->key] += e->count;
+        }
+    }
+}
+
+/* GroupbyTotals(scratch, n, numKeys, table): groups in row order give each key's per-group base
+ * (replacing gcnt) and its count and sum; the bucket totals and the error count. */
+void GroupbyTotals(unsigned int n_, void **a) {
     (void)n_;
     const long long n = IARG(a, 1);
     const int numKeys = IARG(a, 2);
     int64_t *table = (int64_t *)a[3];
     if (n < 0 || numKeys < 0 || (long long)numKeys + 1 > PTK_GB_MAX_KEYS) return;
     PTK_GB_SCRATCH S = ptk_gb_scratch(a[0], n, numKeys);
+    const long long KP = S.buckets * PTK_GB_BUCKET;
     uint64_t *cnt = (uint64_t *)table, *sum = cnt + (numKeys + 1), *err = cnt + 3 * ((size_t)numKeys + 1);
-    memset(cnt, 0, sizeof(uint64_t) * 2 * ((size_t)numKeys + 1));
-    uint64_t errors = 0;
-    for (long long c = 0; c < S.chunks; ++c) {
-        const unsigned d = S.boff[c * (S.buckets + 1) + S.buckets];
-        for (unsigned i = 0; i < d; ++i) {
-            const long long at = c * PTK_GB_CHUNK + i;
-            PTK_GB_ENTRY *e = S.entry + at;
-            sum[e->key] += (uint64_t)e->sum;
-            e->sum = (long long)cnt[e->key];  /* the entry's base replaces its sum */
-            cnt[e->key] += e->count;
+    for (long long k = 0; k < KP; ++k) {
+        uint64_t t = 0, v = 0;
+        for (long long g = 0; g < S.groups; ++g) {
+            const unsigned c = S.gcnt[g * KP + k];
+            S.gcnt[g * KP + k] = (unsigned)t;
+            t += c;
+            v += S.gsum[g * KP + k];
         }
-        errors += S.berr[c];
+        if (k <= numKeys) {
+            cnt[k] = t;
+            sum[k] = v;
+        }
     }
+    uint64_t errors = 0;
+    for (long long c = 0; c < S.chunks; ++c) errors += S.berr[c];
     *err = errors;
     for (long long b = 0; b < S.buckets; ++b) {
         uint64_t rows = 0, nz = 0;
@@ -217,7 +247,8 @@ void GroupbyReshuffle(unsigned int n_, void **a) {
         const long long r0 = c * PTK_GB_CHUNK, r1 = r0 + PTK_GB_CHUNK < n ? r0 + PTK_GB_CHUNK : n;
         const PTK_GB_ENTRY *e = S.entry + c * PTK_GB_CHUNK;
         const unsigned d = S.boff[c * (S.buckets + 1) + S.buckets];
-        for (unsigned i = 0; i < d; ++i) next[i] = offsets[e[i].key] + e[i].sum;  /* sum holds the base */
+        const unsigned *gbase = S.gcnt + (c / PTK_GB_GROUP) * (S.buckets * PTK_GB_BUCKET);
+        for (unsigned i = 0; i < d; ++i) next[i] = offsets[e[i].key] + gbase[e[i].key] + e[i].sum;  /* sum: the merge base */
         for (long long r = r0; r < r1; ++r) {
             const int32_t k = keys[r];
             if (k < 0 || k > numKeys || d == 0) continue;

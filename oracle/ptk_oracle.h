@@ -64,6 +64,7 @@ void pfxsum(unsigned int n, void **a);
 void GroupbyAccumulate(unsigned int n, void **a);
 void GroupbyOffsets(unsigned int n, void **a);
 void GroupbyMerge(unsigned int n, void **a);
+void GroupbyTotals(unsigned int n, void **a);
 void GroupbyReshuffle(unsigned int n, void **a);
 void coalesce_int_float(unsigned int n, void **a);
 void GroupbyCoalesce(unsigned int n, void **a);

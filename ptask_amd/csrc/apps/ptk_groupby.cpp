@@ -1,10 +1,11 @@
 // ptk_groupby.cpp -- the groupby and prefix-sum graphs as PTask applications (the roles of
 // ptaskUnitTest/graphgroupby.cpp and ptaskUnitTest/cpfxsum.cpp), against ptaskapi.h only.
 //
-// --workload groupby: four tasks per run (graphgroupby.cpp builds the same stages:
+// --workload groupby: five tasks per run (graphgroupby.cpp builds the same stages:
 //   allocate/count/prefix-sum/reshuffle, :106-387; the layouts are in include/ptk_groupby.h):
 //     GroupbyAccumulate (keys, vals, n, numKeys)                         -> scratch
-//     GroupbyMerge      (scratch in/out, n, numKeys)                     -> scratch, table
+//     GroupbyMerge      (scratch in/out, n, numKeys)                     -> scratch
+//     GroupbyTotals     (scratch in/out, n, numKeys)                     -> scratch, table
 //     GroupbyOffsets    (table, numKeys, scratch)                        -> offsets, groups
 //     GroupbyReshuffle  (keys, vals, n, numKeys, offsets, scratch)       -> outKeys, outVals
 //   and with --coalesce the reference's CoalesceGroups layout (HostTasks.cpp:131-197):
@@ -210,15 +211,22 @@ int main(int argc, char **argv) {
         Port *aOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch", 4)};
         Task *ta = g->AddTask(K("GroupbyAccumulate"), 4, aIn, 1, aOut, "GroupbyAccumulate");
         geometry(ta, chunkGrid, 512);
-        // GroupbyMerge: one block per key bucket; scratch in/out, table out (every count, sum and
-        // the error count are written)
+        // GroupbyMerge: one block per (key bucket, merge group) unit; scratch in/out
         Port *mIn[3] = {Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 0, 0),
                         Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", 1),
                         Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 2)};
-        Port *mOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch(out)", 0),
+        Port *mOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch(out)", 0)};
+        Task *tm = g->AddTask(K("GroupbyMerge"), 3, mIn, 1, mOut, "GroupbyMerge");
+        geometry(tm, (int)std::min(65535LL, buckets * ptk_gb_groups(n)), PTK_GB_BUCKET);
+        // GroupbyTotals: one block per key bucket; scratch in/out, table out (every count, sum and
+        // the error count are written)
+        Port *tIn[3] = {Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 0, 0),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", 1),
+                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 2)};
+        Port *tOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch(out)", 0),
                          Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table", 3)};
-        Task *tm = g->AddTask(K("GroupbyMerge"), 3, mIn, 2, mOut, "GroupbyMerge");
-        geometry(tm, buckets, 1024);
+        Task *tt = g->AddTask(K("GroupbyTotals"), 3, tIn, 2, tOut, "GroupbyTotals");
+        geometry(tt, buckets, PTK_GB_BUCKET);
         Port *oIn[3] = {Runtime::CreatePort(INPUT_PORT, tTable, uid++, (char *)"table", 0),
                         Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 1),
                         Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 2)};
@@ -235,17 +243,19 @@ int main(int argc, char **argv) {
         Port *rOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outKeys", 6),
                          Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outVals", 7)};
         Task *tr = g->AddTask(K("GroupbyReshuffle"), 6, rIn, 2, rOut, "GroupbyReshuffle");
-        geometry(tr, chunkGrid, 512);
+        geometry(tr, chunkGrid, PTK_GB_PLACE_THREADS);
         keyIn = {g->AddInputChannel(aIn[0], "keys->acc"), g->AddInputChannel(rIn[0], "keys->shuffle")};
         valIn = {g->AddInputChannel(aIn[1], "vals->acc"), g->AddInputChannel(rIn[1], "vals->shuffle")};
         nIn = {g->AddInputChannel(aIn[2], "n->acc"), g->AddInputChannel(mIn[1], "n->merge"),
-               g->AddInputChannel(rIn[2], "n->shuffle")};
+               g->AddInputChannel(tIn[1], "n->totals"), g->AddInputChannel(rIn[2], "n->shuffle")};
         kIn = {g->AddInputChannel(aIn[3], "nk->acc"), g->AddInputChannel(mIn[2], "nk->merge"),
-               g->AddInputChannel(oIn[1], "nk->offsets"), g->AddInputChannel(rIn[3], "nk->shuffle")};
+               g->AddInputChannel(tIn[2], "nk->totals"), g->AddInputChannel(oIn[1], "nk->offsets"),
+               g->AddInputChannel(rIn[3], "nk->shuffle")};
         g->AddInternalChannel(aOut[0], mIn[0], "acc->merge");
-        g->AddInternalChannel(mOut[1], oIn[0], "merge->offsets(table)");
-        g->AddInternalChannel(mOut[0], oIn[2], "merge->offsets(scratch)");
-        g->AddInternalChannel(mOut[0], rIn[5], "merge->shuffle(scratch)");
+        g->AddInternalChannel(mOut[0], tIn[0], "merge->totals");
+        g->AddInternalChannel(tOut[1], oIn[0], "totals->offsets(table)");
+        g->AddInternalChannel(tOut[0], oIn[2], "totals->offsets(scratch)");
+        g->AddInternalChannel(tOut[0], rIn[5], "totals->shuffle(scratch)");
         g->AddInternalChannel(oOut[0], rIn[4], "offsets->shuffle");
         outs = {g->AddOutputChannel(oOut[1], "groups"), g->AddOutputChannel(rOut[0], "outKeys"),
                 g->AddOutputChannel(rOut[1], "outVals")};

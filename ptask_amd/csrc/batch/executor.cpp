@@ -7,7 +7,7 @@
 //   * batched: one captured hipGraph of the persistent multi-instance kernels of batch.hsaco,
 //     whose work-list lengths live in device memory so the same graph serves every mix, or
 //   * plain: the baseline, one kernel launch per task per instance (the six Yee tasks of a step,
-//     the four groupby tasks, the two EM tasks) in submission order on one stream.
+//     the five groupby tasks, the two EM tasks) in submission order on one stream.
 // Every HIP call goes through the C ABI of libptk_hip.so.
 
 #include <algorithm>
@@ -42,6 +42,10 @@ int Fail(int code, const std::string &msg) {
 
 constexpr unsigned long long kAlign = 256;
 constexpr int kFdtdCap = 16384, kGbCap = 8192, kGmmCap = 8192;
+// device work-list lengths: FDTD tiles, FDTD instances, groupby chunk units, groupby buckets,
+// GMM accumulate units, GMM finalize units, three-step items, three-step blocks, groupby merge
+// units
+constexpr int kCounts = 9;
 
 unsigned long long Up(unsigned long long x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
@@ -147,10 +151,10 @@ struct ptk_batch {
     ptk_hip_event evStart = nullptr, evStop = nullptr;
     ptk_hip_module mBatch = nullptr, mFdtd = nullptr, mGb = nullptr, mGmm = nullptr, mUtil = nullptr;
     ptk_hip_function fH = nullptr, fE = nullptr, fStep = nullptr, fStep3 = nullptr, fAdv = nullptr, fGbAcc = nullptr,
-                     fGbMerge = nullptr, fGbOff = nullptr, fGbRes = nullptr, fGmmAcc = nullptr, fGmmFin = nullptr, fInitFdtd = nullptr,
+                     fGbMerge = nullptr, fGbTot = nullptr, fGbOff = nullptr, fGbRes = nullptr, fGmmAcc = nullptr, fGmmFin = nullptr, fInitFdtd = nullptr,
                      fInitGb = nullptr, fInitGmm = nullptr, fInitGmmModel = nullptr, fSumWords = nullptr,
                      fSumPairs = nullptr, fSumGroups = nullptr, fSumJobs = nullptr, fFill = nullptr;
-    ptk_hip_function fSix[6] = {}, fPGbAcc = nullptr, fPGbMerge = nullptr, fPGbOff = nullptr,
+    ptk_hip_function fSix[6] = {}, fPGbAcc = nullptr, fPGbMerge = nullptr, fPGbTot = nullptr, fPGbOff = nullptr,
                      fPGbRes = nullptr,
                      fPGmmAcc = nullptr, fPGmmFin = nullptr;
     void *arenaBase = nullptr;
@@ -165,7 +169,7 @@ struct ptk_batch {
     List<int4> tiles, tiles3;  // FDTD tiles of the current form; three-step items (when cur3)
     List<int> t3start;         // three-step items of virtual block b: [t3start[b], t3start[b+1])
     List<int> fdtdActive;
-    List<int2> gbUnits, gbBuckets, gmmAcc, gmmFin;
+    List<int2> gbUnits, gbBuckets, gbMerge, gmmAcc, gmmFin;
     List<PTK_CSUM_JOB> csumJobs;
     double *dCdf = nullptr;
     int cdfKeys = -1;
@@ -472,7 +476,7 @@ int BuildLists(ptk_batch *B) {
         }
     const size_t nt3 = three ? t3.size() + (size_t)B->grid3 : 0;
     B->cur3 = three;
-    size_t nt = FdtdTiles(B, cols, depth), nf = 0, ngu = 0, ngi = 0, nga = 0, ngf = 0;
+    size_t nt = FdtdTiles(B, cols, depth), nf = 0, ngu = 0, ngi = 0, ngm = 0, nga = 0, ngf = 0;
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
         if (in.kind == PTK_INST_FDTD) {
@@ -480,6 +484,7 @@ int BuildLists(ptk_batch *B) {
         } else if (in.kind == PTK_INST_GROUPBY) {
             ngu += (size_t)((in.a + in.gb.chunkRows - 1) / in.gb.chunkRows);
             ngi += (size_t)ptk_gb_buckets(in.b);
+            ngm += (size_t)(ptk_gb_buckets(in.b) * ptk_gb_groups(in.a));
         } else {
             nga += (size_t)in.gm.d.blocks;
             ngf += (size_t)in.gm.d.K + 1;
@@ -491,9 +496,10 @@ int BuildLists(ptk_batch *B) {
     if (int r = Reserve(B, B->fdtdActive, nf)) return r;
     if (int r = Reserve(B, B->gbUnits, ngu)) return r;
     if (int r = Reserve(B, B->gbBuckets, ngi)) return r;
+    if (int r = Reserve(B, B->gbMerge, ngm)) return r;
     if (int r = Reserve(B, B->gmmAcc, nga)) return r;
     if (int r = Reserve(B, B->gmmFin, ngf)) return r;
-    B->tiles.n = B->tiles3.n = B->t3start.n = B->fdtdActive.n = B->gbUnits.n = B->gbBuckets.n = B->gmmAcc.n = B->gmmFin.n = 0;
+    B->tiles.n = B->tiles3.n = B->t3start.n = B->fdtdActive.n = B->gbUnits.n = B->gbBuckets.n = B->gbMerge.n = B->gmmAcc.n = B->gmmFin.n = 0;
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
         if (in.kind == PTK_INST_FDTD) {
@@ -517,6 +523,8 @@ int BuildLists(ptk_batch *B) {
             const long long chunks = (in.a + in.gb.chunkRows - 1) / in.gb.chunkRows;
             for (long long c = 0; c < chunks; ++c) B->gbUnits.host[B->gbUnits.n++] = int2{in.slot, (int)c};
             for (long long q = 0; q < ptk_gb_buckets(in.b); ++q) B->gbBuckets.host[B->gbBuckets.n++] = int2{in.slot, (int)q};
+            for (long long q = 0; q < ptk_gb_buckets(in.b) * ptk_gb_groups(in.a); ++q)
+                B->gbMerge.host[B->gbMerge.n++] = int2{in.slot, (int)q};
         } else {
             for (int b = 0; b < in.gm.d.blocks; ++b) B->gmmAcc.host[B->gmmAcc.n++] = int2{in.slot, b};
             for (int k = 0; k <= in.gm.d.K; ++k) B->gmmFin.host[B->gmmFin.n++] = int2{in.slot, k};
@@ -534,6 +542,7 @@ int BuildLists(ptk_batch *B) {
     }
     B->hCounts[6] = (int)B->tiles3.n;
     B->hCounts[7] = three ? B->grid3 : 0;
+    B->hCounts[8] = (int)B->gbMerge.n;
     auto up = [&](auto &l) -> int {
         if (l.n) HIPCHK(ptk_hip_memcpy_htod_async(l.dev, l.host, l.n * sizeof(*l.host), B->stream));
         return PTK_BATCH_OK;
@@ -544,9 +553,10 @@ int BuildLists(ptk_batch *B) {
     if (int r = up(B->fdtdActive)) return r;
     if (int r = up(B->gbUnits)) return r;
     if (int r = up(B->gbBuckets)) return r;
+    if (int r = up(B->gbMerge)) return r;
     if (int r = up(B->gmmAcc)) return r;
     if (int r = up(B->gmmFin)) return r;
-    HIPCHK(ptk_hip_memcpy_htod_async(B->dCounts, B->hCounts, 8 * sizeof(int), B->stream));
+    HIPCHK(ptk_hip_memcpy_htod_async(B->dCounts, B->hCounts, kCounts * sizeof(int), B->stream));
     return PTK_BATCH_OK;
 }
 
@@ -589,15 +599,17 @@ int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance, bool 
         if (int r = B->Launch(B->fGmmFin, grid(512, hc[5]), 1, 1, 256, f)) return r;
     }
     if (gb) {
-        Args a, m, o, s;
+        Args a, m, t, o, s;
         a.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
-        m.ptr(B->gbBuckets.dev).ptr(B->dCounts + 3).ptr(B->dGb);
+        m.ptr(B->gbMerge.dev).ptr(B->dCounts + 8).ptr(B->dGb);
+        t.ptr(B->gbBuckets.dev).ptr(B->dCounts + 3).ptr(B->dGb);
         o.ptr(B->gbBuckets.dev).ptr(B->dCounts + 3).ptr(B->dGb);
         s.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
         if (int r = B->Launch(B->fGbAcc, grid(1024, hc[2]), 1, 1, 512, a)) return r;
-        if (int r = B->Launch(B->fGbMerge, grid(2048, hc[3]), 1, 1, 1024, m)) return r;
+        if (int r = B->Launch(B->fGbMerge, grid(4096, hc[8]), 1, 1, PTK_GB_BUCKET, m)) return r;
+        if (int r = B->Launch(B->fGbTot, grid(2048, hc[3]), 1, 1, PTK_GB_BUCKET, t)) return r;
         if (int r = B->Launch(B->fGbOff, grid(2048, hc[3]), 1, 1, PTK_GB_BUCKET, o)) return r;
-        if (int r = B->Launch(B->fGbRes, grid(1024, hc[2]), 1, 1, 512, s)) return r;
+        if (int r = B->Launch(B->fGbRes, grid(1024, hc[2]), 1, 1, PTK_GB_PLACE_THREADS, s)) return r;
     }
     return PTK_BATCH_OK;
 }
@@ -622,15 +634,18 @@ int IssuePlain(ptk_batch *B, const Inst &in) {
         const PTK_GROUPBY_INST &d = in.gb;
         const unsigned grid = (unsigned)std::max(1LL, std::min(2048LL, ptk_gb_chunks(d.n)));
         const unsigned buckets = (unsigned)ptk_gb_buckets(d.numKeys);
-        Args a, m, o, s;
+        const unsigned units = (unsigned)std::min(65535LL, ptk_gb_buckets(d.numKeys) * ptk_gb_groups(d.n));
+        Args a, m, t, o, s;
         a.ptr(d.keys).ptr(d.vals).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.scratch);
-        m.ptr(d.scratch).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.table);
+        m.ptr(d.scratch).add<int>((int)d.n).add<int>(d.numKeys);
+        t.ptr(d.scratch).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.table);
         o.ptr(d.table).add<int>(d.numKeys).ptr(d.scratch).ptr(d.offsets).ptr(d.groups);
         s.ptr(d.keys).ptr(d.vals).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.offsets).ptr(d.scratch).ptr(d.outKeys).ptr(d.outVals);
         if (int r = B->Launch(B->fPGbAcc, grid, 1, 1, 512, a)) return r;
-        if (int r = B->Launch(B->fPGbMerge, buckets, 1, 1, 1024, m)) return r;
+        if (int r = B->Launch(B->fPGbMerge, units, 1, 1, PTK_GB_BUCKET, m)) return r;
+        if (int r = B->Launch(B->fPGbTot, buckets, 1, 1, PTK_GB_BUCKET, t)) return r;
         if (int r = B->Launch(B->fPGbOff, buckets, 1, 1, PTK_GB_BUCKET, o)) return r;
-        if (int r = B->Launch(B->fPGbRes, grid, 1, 1, 512, s)) return r;
+        if (int r = B->Launch(B->fPGbRes, grid, 1, 1, PTK_GB_PLACE_THREADS, s)) return r;
     } else {
         const PTK_GMM_INST &d = in.gm;
         Args a, f;
@@ -667,8 +682,9 @@ int Retire(ptk_batch *B, const std::vector<int> &done) {
         } else if (in.kind == PTK_INST_GROUPBY) {
             jobs.push_back({in.gb.groups, nullptr, 0, s + 0, 2, 0});
             jobs.push_back({in.gb.outKeys, nullptr, in.a, s + 1, 0, 0});
-            jobs.push_back({in.gb.outKeys, in.gb.outVals, in.a, s + 2, 1, 0});
+            jobs.push_back({in.gb.outVals, nullptr, in.a, s + 2, 0, 0});
             jobs.push_back({in.gb.keys, in.gb.vals, in.a, s + 3, 1, 0});
+            jobs.push_back({in.gb.outKeys, in.gb.outVals, in.a, s + 4, 1, 0});
         } else {
             jobs.push_back({in.gm.model, nullptr, 2 * ptk_gmm_model_elems(&in.gm.d), s + 0, 0, 0});
         }
@@ -744,7 +760,7 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
     const std::pair<ptk_hip_function *, const char *> bf[] = {
         {&B->fH, "BatchFdtdH"}, {&B->fE, "BatchFdtdE"}, {&B->fStep, "BatchFdtdStep"},
         {&B->fStep3, "BatchFdtdSteps3"}, {&B->fAdv, "BatchFdtdAdvance"},
-        {&B->fGbAcc, "BatchGroupbyAccumulate"}, {&B->fGbMerge, "BatchGroupbyMerge"},
+        {&B->fGbAcc, "BatchGroupbyAccumulate"}, {&B->fGbMerge, "BatchGroupbyMerge"}, {&B->fGbTot, "BatchGroupbyTotals"},
         {&B->fGbOff, "BatchGroupbyOffsets"}, {&B->fGbRes, "BatchGroupbyReshuffle"},
         {&B->fGmmAcc, "BatchGmmAccumulate"}, {&B->fGmmFin, "BatchGmmFinalize"},
         {&B->fInitFdtd, "BatchInitFdtd"}, {&B->fInitGb, "BatchInitGroupby"}, {&B->fInitGmm, "BatchInitGmm"},
@@ -758,7 +774,7 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
     for (int t = 0; t < 6; ++t)
         if ((r = Load(B->mFdtd, six[t], &B->fSix[t]))) return fail(r);
     if ((r = Load(B->mGb, "GroupbyAccumulate", &B->fPGbAcc)) ||
-        (r = Load(B->mGb, "GroupbyMerge", &B->fPGbMerge)) ||
+        (r = Load(B->mGb, "GroupbyMerge", &B->fPGbMerge)) || (r = Load(B->mGb, "GroupbyTotals", &B->fPGbTot)) ||
         (r = Load(B->mGb, "GroupbyOffsets", &B->fPGbOff)) || (r = Load(B->mGb, "GroupbyReshuffle", &B->fPGbRes)) ||
         (r = Load(B->mGmm, "GmmAccumulate", &B->fPGmmAcc)) || (r = Load(B->mGmm, "GmmFinalize", &B->fPGmmFin)) ||
         (r = Load(B->mUtil, "PtkFillWords", &B->fFill)))
@@ -769,12 +785,12 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
     if (ptk_hip_malloc((void **)&B->dFdtd, sizeof(PTK_FDTD_INST) * kFdtdCap) ||
         ptk_hip_malloc((void **)&B->dGb, sizeof(PTK_GROUPBY_INST) * kGbCap) ||
         ptk_hip_malloc((void **)&B->dGmm, sizeof(PTK_GMM_INST) * kGmmCap) ||
-        ptk_hip_malloc((void **)&B->dCounts, 8 * sizeof(int)) || ptk_hip_host_malloc((void **)&B->hCounts, 8 * sizeof(int)))
+        ptk_hip_malloc((void **)&B->dCounts, kCounts * sizeof(int)) || ptk_hip_host_malloc((void **)&B->hCounts, kCounts * sizeof(int)))
         return fail(Fail(PTK_BATCH_EHIP, std::string("tables: ") + ptk_hip_last_error_string()));
     for (int i = kFdtdCap - 1; i >= 0; --i) B->freeFdtd.push_back(i);
     for (int i = kGbCap - 1; i >= 0; --i) B->freeGb.push_back(i);
     for (int i = kGmmCap - 1; i >= 0; --i) B->freeGmm.push_back(i);
-    memset(B->hCounts, 0, 8 * sizeof(int));
+    memset(B->hCounts, 0, kCounts * sizeof(int));
     *out = B;
     return PTK_BATCH_OK;
 }
@@ -792,6 +808,7 @@ int ptk_batch_destroy(ptk_batch *B) {
     freeList(B->t3start);
     freeList(B->fdtdActive);
     freeList(B->gbBuckets);
+    freeList(B->gbMerge);
     freeList(B->gbUnits);
     freeList(B->gmmAcc);
     freeList(B->gmmFin);

@@ -18,7 +18,8 @@
 //   BatchFdtdSteps3           items {inst, tile x, tile y, k0 | k1 << 16} over 58 x 10 columns in
 //                             per-block ranges: three whole steps per launch
 //   BatchGroupbyAccumulate/Reshuffle          units {inst, unit}: chunkRows rows
-//   BatchGroupbyMerge/Offsets                 units {inst, key bucket}
+//   BatchGroupbyMerge                         units {inst, merge unit (bucket, merge group)}
+//   BatchGroupbyTotals/Offsets                units {inst, key bucket}
 //   BatchGmmAccumulate        units {inst, partial row b}
 //   BatchGmmFinalize          units {inst, component k (K = log-likelihood block)}
 //   BatchInit*                device-side input generation at admission
@@ -167,17 +168,34 @@ BatchGroupbyAccumulate(const int2 *__restrict__ units, const int *__restrict__ c
     }
 }
 
-// Units {inst, bucket}, taken in an XCD-contiguous order (neighbouring buckets share lines).
-extern "C" __global__ void __launch_bounds__(1024)
+// Units {inst, merge unit} (bucket-major (bucket, merge group) pairs, gb_merge_unit), taken in
+// an XCD-contiguous order (neighbouring units share lines); blocks of PTK_GB_BUCKET threads.
+extern "C" __global__ void __launch_bounds__(PTK_GB_BUCKET)
 BatchGroupbyMerge(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
     __shared__ GbMergeLds L;
+    if (blockDim.x != PTK_GB_BUCKET) return;
     const int n = *count;
     for (int u = (int)gb_xcd_block(blockIdx.x, gridDim.x); u < n; u += gridDim.x) {
         const int2 w = units[u];
         const PTK_GROUPBY_INST &I = insts[w.x];
         if (!gb_keys_ok(I.numKeys)) continue;
         const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
-        if (w.y < S.buckets) gb_merge_bucket(S, I.numKeys, w.y, I.table, L);
+        if (w.y < S.buckets * S.groups) gb_merge_unit(S, w.y, L);
+    }
+}
+
+// Units {inst, bucket}; blocks of PTK_GB_BUCKET threads.
+extern "C" __global__ void __launch_bounds__(PTK_GB_BUCKET)
+BatchGroupbyTotals(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ unsigned long long red[16];
+    if (blockDim.x != PTK_GB_BUCKET) return;
+    const int n = *count;
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const int2 w = units[u];
+        const PTK_GROUPBY_INST &I = insts[w.x];
+        if (!gb_keys_ok(I.numKeys)) continue;
+        const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
+        if (w.y < S.buckets) gb_totals_bucket(S, I.numKeys, w.y, I.table, red);
     }
 }
 
@@ -196,9 +214,10 @@ BatchGroupbyOffsets(const int2 *__restrict__ units, const int *__restrict__ coun
     }
 }
 
-extern "C" __global__ void __launch_bounds__(1024)
+extern "C" __global__ void __launch_bounds__(PTK_GB_PLACE_THREADS)
 BatchGroupbyReshuffle(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
     __shared__ GbPlaceLds L;
+    if (blockDim.x != PTK_GB_PLACE_THREADS) return;
     // each XCD takes a contiguous run of units (consecutive chunks of an instance): see
     // GroupbyReshuffle
     const int n = *count, G = (int)gridDim.x < n ? (int)gridDim.x : n;

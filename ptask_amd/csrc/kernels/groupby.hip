@@ -4,19 +4,23 @@
 // per-key counts in a device hash table, prefix-sums them (single block) and reshuffles the
 // input so equal keys are contiguous; its kernels are not in the tree.  This pipeline keeps that
 // shape, adds an int32 value column aggregated to int64 sum and count per key, and replaces the
-// device hash table by per-chunk LDS tables merged per key bucket (no global atomics; the
-// bodies are in groupby_body.h, the layouts in include/ptk_groupby.h):
+// device hash table by per-chunk LDS tables merged per key bucket (no global atomics on the data;
+// every output, the order of rows inside a group included, is bit-reproducible; the bodies are
+// in groupby_body.h, the layouts in include/ptk_groupby.h):
 //
 //   GroupbyAccumulate(int* keys, int* vals, int n, int numKeys, void* scratch)     -> scratch
-//   GroupbyMerge(void* scratch, int n, int numKeys, long long* table)  -> table; scratch in/out
+//   GroupbyMerge(void* scratch, int n, int numKeys)                      scratch in/out
+//   GroupbyTotals(void* scratch, int n, int numKeys, long long* table) -> table; scratch in/out
 //   GroupbyOffsets(long long* table, int numKeys, void* scratch, long long* offsets,
 //                  long long* groups)
 //   GroupbyReshuffle(int* keys, int* vals, int n, int numKeys, long long* offsets,
 //                    void* scratch, int* outKeys, int* outVals)
 //
-// Launch: Accumulate and Reshuffle take any grid of 64k-thread blocks (blocks stride over the
-// ceil(n / PTK_GB_CHUNK) chunks); Merge any grid (blocks stride over the buckets); Offsets
-// blocks of exactly PTK_GB_BUCKET threads, any grid.  Keys lie in [0, numKeys] as in the
+// Launch: Accumulate takes any grid of 64k-thread blocks (blocks stride over the
+// ceil(n / PTK_GB_CHUNK) chunks); Reshuffle any grid of PTK_GB_PLACE_THREADS-thread blocks
+// (the same chunks); Merge any grid of PTK_GB_BUCKET-thread blocks (blocks stride over the
+// buckets x merge groups units); Totals and Offsets blocks of exactly PTK_GB_BUCKET threads, any
+// grid.  Keys lie in [0, numKeys] as in the
 // reference test (graphgroupby.cpp:428); a key outside is counted in the table's error slot and
 // otherwise ignored, so no access leaves its array.  numKeys + 1 <= PTK_GB_MAX_KEYS (checked).
 #include "groupby_body.h"
@@ -31,14 +35,24 @@ GroupbyAccumulate(const int *__restrict__ keys, const int *__restrict__ vals, in
     for (long long c = blockIdx.x; c < S.chunks; c += gridDim.x) gb_accumulate_chunk(keys, vals, n, numKeys, c, S, L);
 }
 
-// Neighbouring buckets share cache lines of the entry and offset arrays, so each XCD (blocks are
-// dealt to the 8 XCDs round-robin) takes a contiguous run of buckets.
-extern "C" __global__ void __launch_bounds__(1024) GroupbyMerge(void *scratch, int n, int numKeys, long long *table) {
+// Units (bucket, merge group), bucket-major; neighbouring units share cache lines of the entry
+// and offset arrays, so each XCD (blocks are dealt to the 8 XCDs round-robin) takes a contiguous
+// run of units.  Blocks of exactly PTK_GB_BUCKET threads, any grid.
+extern "C" __global__ void __launch_bounds__(PTK_GB_BUCKET) GroupbyMerge(void *scratch, int n, int numKeys) {
     __shared__ GbMergeLds L;
-    if (!gb_keys_ok(numKeys) || n < 0 || (blockDim.x & 63) != 0) return;
+    if (!gb_keys_ok(numKeys) || n < 0 || blockDim.x != PTK_GB_BUCKET) return;
     const PTK_GB_SCRATCH S = ptk_gb_scratch(scratch, n, numKeys);
-    const unsigned b = gb_xcd_block(blockIdx.x, gridDim.x);
-    for (long long r = b; r < S.buckets; r += gridDim.x) gb_merge_bucket(S, numKeys, (int)r, table, L);
+    const long long units = S.buckets * S.groups;
+    for (long long u = gb_xcd_block(blockIdx.x, gridDim.x); u < units; u += gridDim.x) gb_merge_unit(S, u, L);
+}
+
+// One block per bucket (any grid; blocks stride over the buckets), blocks of exactly
+// PTK_GB_BUCKET threads.
+extern "C" __global__ void __launch_bounds__(PTK_GB_BUCKET) GroupbyTotals(void *scratch, int n, int numKeys, long long *table) {
+    __shared__ unsigned long long red[16];
+    if (!gb_keys_ok(numKeys) || n < 0 || blockDim.x != PTK_GB_BUCKET) return;
+    const PTK_GB_SCRATCH S = ptk_gb_scratch(scratch, n, numKeys);
+    for (long long r = blockIdx.x; r < S.buckets; r += gridDim.x) gb_totals_bucket(S, numKeys, (int)r, table, red);
 }
 
 extern "C" __global__ void __launch_bounds__(PTK_GB_BUCKET)
@@ -50,12 +64,12 @@ GroupbyOffsets(long long *table, int numKeys, void *scratch, long long *offsets,
     for (long long r = blockIdx.x; r < S.buckets; r += gridDim.x) gb_offsets_bucket(table, numKeys, S, (int)r, offsets, groups, L);
 }
 
-extern "C" __global__ void __launch_bounds__(1024)
+extern "C" __global__ void __launch_bounds__(PTK_GB_PLACE_THREADS)
 GroupbyReshuffle(const int *__restrict__ keys, const int *__restrict__ vals, int n, int numKeys,
                  const long long *__restrict__ offsets, void *scratch, int *__restrict__ outKeys,
                  int *__restrict__ outVals) {
     __shared__ GbPlaceLds L;
-    if (!gb_keys_ok(numKeys) || n <= 0 || (blockDim.x & 63) != 0) return;
+    if (!gb_keys_ok(numKeys) || n <= 0 || blockDim.x != PTK_GB_PLACE_THREADS) return;
     const PTK_GB_SCRATCH S = ptk_gb_scratch(scratch, n, numKeys);
     // Consecutive chunks place a key's rows in consecutive slots of its group, so each XCD takes
     // a contiguous run of chunks: the partial lines of rare keys' groups then fill up in one L2

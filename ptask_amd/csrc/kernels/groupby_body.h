@@ -2,24 +2,29 @@
 // kernels (groupby.hip) and the batched multi-instance kernels (batch.hip).  Layouts of the
 // table, groups and scratch arrays: include/ptk_groupby.h.
 //
-// No pass uses a global atomic.  Per-key sums are exact int64 (wrapping) additions, whose result
-// does not depend on their order, so every output except the order of values inside one group
-// is bit-reproducible.
+// No pass uses a global atomic on the data.  Per-key sums are exact int64 (wrapping) additions,
+// whose result does not depend on their order, and every position is computed in row order, so
+// every output -- including the order of rows inside a group, which is input order -- is
+// bit-reproducible.
 //
 //   accumulate  a block aggregates one chunk of PTK_GB_CHUNK rows in an LDS hash table of
 //               GB_SLOTS = 2 x PTK_GB_CHUNK slots (a chunk has at most PTK_GB_CHUNK distinct
 //               keys, so the table never fills and every probe sequence ends), then writes the
 //               chunk's distinct keys with their count and sum, grouped by key bucket.
-//   merge       a block owns one bucket of PTK_GB_BUCKET keys: it adds every chunk's entries of
-//               that bucket in LDS -- the value an entry's count is added to is the number of
-//               rows of its key in the chunks merged before it, i.e. the start of the chunk's
-//               range inside the key's group, which replaces the entry's sum in place -- and
-//               writes the bucket's count and sum.
+//   merge       a block owns one (bucket, merge group) unit: PTK_GB_BUCKET keys over
+//               PTK_GB_GROUP consecutive chunks.  It stages the chunks' entries of the bucket
+//               in LDS, 64 chunks at a time, as a per-key mask of the chunks holding the key;
+//               the thread of a key walks the staged chunks in chunk order, so the base it
+//               writes into each entry (the key's rows in the group's earlier chunks) follows
+//               row order.  The unit's per-key totals go to gcnt / gsum.
+//   totals      a block per bucket turns the groups' per-key totals into per-group bases (in
+//               group order) and writes the bucket's counts, sums and totals.
 //   offsets     a block per bucket: the bucket totals of the lower buckets give its starting row
 //               and group index; a block scan over its keys gives offsets and group records.
-//   reshuffle   a block re-reads one chunk: each distinct key's first output row is
-//               offsets[key] + its merge base; rows are staged in LDS in output order with LDS
-//               cursors and stored in that order.
+//   reshuffle   a block re-reads one chunk: each distinct key's first output row is offsets[key]
+//               + its group base + its merge base; a row's rank among the chunk's rows of its key
+//               is counted in row order (per wave with ballots, then wave bases); rows are staged
+//               in LDS in output order and stored in that order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -179,69 +184,166 @@ __device__ inline void gb_accumulate_chunk(const int *__restrict__ keys, const i
     __syncthreads();
 }
 
+#define GB_MERGE_BATCH 16  // chunks a merge unit stages at once: one bit of a 64-bit mask each
+#define GB_MERGE_LOADS 4   // entries a thread loads before it stages them
+
 struct GbMergeLds {
-    unsigned long long cnt[PTK_GB_BUCKET];
+    unsigned tab[GB_MERGE_BATCH][PTK_GB_BUCKET];  // per staged chunk and key: index of the key's
+                                                  // entry in the chunk's bucket segment << 16 | count
+    unsigned long long mask[PTK_GB_BUCKET];       // per key: the staged chunks holding it
     unsigned long long sum[PTK_GB_BUCKET];
+    unsigned segS[PTK_GB_GROUP];                  // per chunk of the group: its bucket segment
+    unsigned ebase[GB_MERGE_BATCH];               // per staged chunk: index of its segment's first entry
+    unsigned pre[PTK_GB_GROUP + 1];               // and the exclusive scan of the segment sizes
     unsigned long long red[16];
 };
 
-// Bucket r: adds every chunk's entries of the bucket (the running count an entry is added to is
-// its base), writes count and sum of the bucket's keys to the table and the bucket totals to
-// S.btot; bucket 0 also writes the error count.  blockDim.x a multiple of 64, at most 1024.
-__device__ inline void gb_merge_bucket(const PTK_GB_SCRATCH &S, int numKeys, int r, long long *table, GbMergeLds &L) {
+// Merge unit u = r * S.groups + g (bucket r, merge group g); blockDim.x == PTK_GB_BUCKET, thread k
+// owns key r * PTK_GB_BUCKET + k.  Writes each entry's base inside the group and the group's
+// per-key totals (gcnt, gsum).  The entries of a batch of chunks are loaded by all threads at
+// once (flat over the batch's segments), so a unit waits for memory about once per batch; a
+// thread then walks the staged chunks holding its key in chunk order.
+__device__ inline void gb_merge_unit(const PTK_GB_SCRATCH &S, long long u, GbMergeLds &L) {
     const int NB = (int)S.buckets;
-    for (int k = threadIdx.x; k < PTK_GB_BUCKET; k += blockDim.x) {
-        L.cnt[k] = 0;
-        L.sum[k] = 0;
-    }
-    __syncthreads();
+    const long long NG = S.groups;
+    const int r = (int)(u / NG);
+    const long long g = u % NG;
+    const int k = threadIdx.x, lane = k & 63, w = k >> 6;
     const unsigned key0 = (unsigned)r * PTK_GB_BUCKET;
-    constexpr int kAhead = 4;  // chunks whose segment bounds are loaded together
-    for (long long c0 = threadIdx.x; c0 < S.chunks; c0 += (long long)kAhead * blockDim.x) {
-        unsigned s[kAhead], e[kAhead];
-#pragma unroll
-        for (int j = 0; j < kAhead; ++j) {
-            const long long c = c0 + (long long)j * blockDim.x;
-            s[j] = e[j] = 0;
-            if (c < S.chunks) {
-                s[j] = S.boff[c * (NB + 1) + r];
-                e[j] = S.boff[c * (NB + 1) + r + 1];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kAhead; ++j) {
-            const long long c = c0 + (long long)j * blockDim.x;
-            for (unsigned i = s[j]; i < e[j]; ++i) {
-                const long long at = c * PTK_GB_CHUNK + i;
-                const PTK_GB_ENTRY E = S.entry[at];
-                const unsigned k = (unsigned)E.key - key0;  // < PTK_GB_BUCKET by construction
-                if (k >= PTK_GB_BUCKET) continue;
-                atomicAdd(&L.sum[k], (unsigned long long)E.sum);
-                S.entry[at].sum = (long long)atomicAdd(&L.cnt[k], (unsigned long long)E.count);  // the base
-            }
-        }
+    const long long KP = (long long)NB * PTK_GB_BUCKET;
+    const long long cg0 = g * PTK_GB_GROUP;
+    const int nc = (int)(cg0 + PTK_GB_GROUP < S.chunks ? PTK_GB_GROUP : S.chunks - cg0);
+    // the group's bucket segments (thread k: chunk cg0 + k) and their flat positions
+    unsigned len = 0;
+    if (k < nc) {
+        const unsigned *bo = S.boff + (cg0 + k) * (NB + 1) + r;
+        const unsigned s0 = bo[0] < PTK_GB_CHUNK ? bo[0] : PTK_GB_CHUNK;
+        const unsigned s1 = bo[1] < PTK_GB_CHUNK ? bo[1] : PTK_GB_CHUNK;
+        L.segS[k] = s0;
+        len = s1 > s0 ? s1 - s0 : 0;
+        len = len < PTK_GB_BUCKET ? len : PTK_GB_BUCKET;  // a segment holds distinct keys of one bucket
     }
+    const unsigned long long incl = wave_incl(len);
+    L.sum[k] = 0;
+    L.mask[k] = 0;
+    __syncthreads();  // the previous unit is done with red
+    if (lane == 63) L.red[w] = incl;
     __syncthreads();
-    unsigned long long rows = 0, nz = 0;
-    for (int k = threadIdx.x; k < PTK_GB_BUCKET; k += blockDim.x) {
-        const long long key = (long long)key0 + k;
-        if (key > numKeys) continue;
-        gb_count(table, numKeys)[key] = L.cnt[k];
-        gb_sum(table, numKeys)[key] = L.sum[k];
-        rows += L.cnt[k];
-        nz += L.cnt[k] ? 1 : 0;
+    unsigned long long pre = incl - len;
+    for (int q = 0; q < w; ++q) pre += L.red[q];
+    L.pre[k] = (unsigned)pre;
+    if (k == PTK_GB_BUCKET - 1) L.pre[PTK_GB_BUCKET] = (unsigned)(pre + len);
+    __syncthreads();
+    unsigned run = 0;  // rows of the key in the chunks walked so far
+    for (int j0 = 0; j0 < nc; j0 += GB_MERGE_BATCH) {
+        const int nb = nc - j0 < GB_MERGE_BATCH ? nc - j0 : GB_MERGE_BATCH;
+        const unsigned F0 = L.pre[j0], F1 = L.pre[j0 + nb];
+        for (unsigned f0 = F0 + k; f0 < F1; f0 += GB_MERGE_LOADS * PTK_GB_BUCKET) {
+            PTK_GB_ENTRY e[GB_MERGE_LOADS];
+            int jj[GB_MERGE_LOADS];
+            unsigned ii[GB_MERGE_LOADS];
+#pragma unroll
+            for (int q = 0; q < GB_MERGE_LOADS; ++q) {
+                const unsigned f = f0 + q * PTK_GB_BUCKET;
+                jj[q] = -1;
+                if (f < F1) {
+                    int lo = j0, hi = j0 + nb;  // the last chunk whose segment starts at or before f
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (L.pre[mid] <= f) lo = mid;
+                        else hi = mid;
+                    }
+                    jj[q] = lo;
+                    ii[q] = f - L.pre[lo];
+                    e[q] = S.entry[(cg0 + lo) * PTK_GB_CHUNK + L.segS[lo] + ii[q]];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < GB_MERGE_LOADS; ++q) {
+                if (jj[q] < 0) continue;
+                const unsigned kk = (unsigned)e[q].key - key0;
+                if (kk >= PTK_GB_BUCKET) continue;  // only for an inconsistent scratch
+                const int j = jj[q] - j0;
+                atomicOr(&L.mask[kk], 1ull << j);
+                L.tab[j][kk] = (ii[q] << 16) | (e[q].count & 0xFFFFu);
+                atomicAdd(&L.sum[kk], (unsigned long long)e[q].sum);
+            }
+        }
+        if (k < nb) L.ebase[k] = (unsigned)((cg0 + j0 + k) * PTK_GB_CHUNK + L.segS[j0 + k]);
+        __syncthreads();
+        unsigned long long m = L.mask[k];
+        L.mask[k] = 0;
+        while (m) {  // the key's chunks in chunk order, up to four per round (their reads overlap)
+            int j[4];
+            unsigned v[4], eb[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                j[q] = m ? __builtin_ctzll(m) : -1;
+                m &= m - 1;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (j[q] >= 0) {
+                    v[q] = L.tab[j[q]][k];
+                    eb[q] = L.ebase[j[q]];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (j[q] >= 0) {
+                    S.entry[eb[q] + (v[q] >> 16)].sum = (long long)run;
+                    run += v[q] & 0xFFFFu;
+                }
+            }
+        }
+        __syncthreads();
     }
-    rows = block_sum(rows, L.red);
-    nz = block_sum(nz, L.red);
-    if (threadIdx.x == 0) {
+    S.gsum[g * KP + key0 + k] = L.sum[k];
+    S.gcnt[g * KP + key0 + k] = run;
+}
+
+// Bucket r (blockDim.x == PTK_GB_BUCKET, thread k owns key r * PTK_GB_BUCKET + k), after every
+// merge unit: turns the groups' per-key totals into per-group bases in group order, writes the
+// keys' counts and sums to the table, the bucket's rows and non-empty keys to S.btot and (bucket
+// 0) the error count.
+__device__ inline void gb_totals_bucket(const PTK_GB_SCRATCH &S, int numKeys, int r, long long *table,
+                                        unsigned long long *red) {
+    const int NB = (int)S.buckets, k = threadIdx.x;
+    const long long NG = S.groups, KP = (long long)NB * PTK_GB_BUCKET;
+    const long long key = (long long)r * PTK_GB_BUCKET + k;
+    unsigned long long tot = 0, sum = 0;
+    constexpr int kT = 8;  // groups whose totals are loaded together
+    for (long long q0 = 0; q0 < NG; q0 += kT) {
+        unsigned c[kT];
+        unsigned long long v[kT];
+#pragma unroll
+        for (int q = 0; q < kT; ++q) {
+            c[q] = q0 + q < NG ? S.gcnt[(q0 + q) * KP + key] : 0u;
+            v[q] = q0 + q < NG ? S.gsum[(q0 + q) * KP + key] : 0ull;
+        }
+#pragma unroll
+        for (int q = 0; q < kT; ++q) {
+            if (q0 + q < NG) S.gcnt[(q0 + q) * KP + key] = (unsigned)tot;
+            tot += c[q];
+            sum += v[q];
+        }
+    }
+    const bool in = key <= numKeys;
+    if (in) {
+        gb_count(table, numKeys)[key] = tot;
+        gb_sum(table, numKeys)[key] = sum;
+    }
+    const unsigned long long rows = block_sum(in ? tot : 0, red);
+    const unsigned long long nz = block_sum(in && tot ? 1 : 0, red);
+    if (k == 0) {
         S.btot[2 * r] = (long long)rows;
         S.btot[2 * r + 1] = (long long)nz;
     }
     if (r == 0) {
         unsigned long long err = 0;
-        for (long long c = threadIdx.x; c < S.chunks; c += blockDim.x) err += S.berr[c];
-        err = block_sum(err, L.red);
-        if (threadIdx.x == 0) *gb_errors(table, numKeys) = err;
+        for (long long c = k; c < S.chunks; c += blockDim.x) err += S.berr[c];
+        err = block_sum(err, red);
+        if (k == 0) *gb_errors(table, numKeys) = err;
     }
     __syncthreads();
 }
@@ -306,85 +408,149 @@ __device__ inline void gb_offsets_bucket(long long *table, int numKeys, const PT
     __syncthreads();
 }
 
+#define GB_PLACE_WAVES (PTK_GB_PLACE_THREADS / 64)
+#define GB_PLACE_ROWS (PTK_GB_CHUNK / PTK_GB_PLACE_THREADS)  // rows per thread
+#define GB_ENTRY_BITS 11                                     // entry indices < PTK_GB_CHUNK
+
+#define GB_KEY_MASK 0x7FFFFu  // keys < PTK_GB_MAX_KEYS = 2^19; a hash slot holds key | entry << 19
+
 struct GbPlaceLds {
-    unsigned key[GB_SLOTS];
-    unsigned next[GB_SLOTS];   // next chunk-local position of the key's rows
-    int delta[GB_SLOTS];       // output row - chunk-local position, for the key's rows
-    int2 stage[PTK_GB_CHUNK];  // per chunk-local position: {output row, key}
-    int sval[PTK_GB_CHUNK];    // and the value
+    union {
+        unsigned slot[GB_SLOTS];  // hash of the chunk's distinct keys: key | entry index << 19
+        struct {                  // once the rows are ranked: the rows in output order
+            int row[PTK_GB_CHUNK], key[PTK_GB_CHUNK], val[PTK_GB_CHUNK];
+        } stage;
+    } u;
+    int out0[PTK_GB_CHUNK];                             // per entry: output row of its first row
+    unsigned short start[PTK_GB_CHUNK];                 // per entry: chunk-local position of its first row
+    unsigned short wcnt[GB_PLACE_WAVES][PTK_GB_CHUNK];  // per wave and entry: rows, then the wave's base
     unsigned long long red[16];
 };
 
-// Chunk c: places its rows at offsets[key] + base(chunk, key) + (rank inside the chunk), so equal
-// keys are contiguous and the keys output is exact; the order of values within a group is not
-// fixed.  The chunk's rows are first staged in LDS in output order (each distinct key's rows
-// contiguous), then stored in that order, so a key's run of rows is written with consecutive
-// stores.  blockDim.x a multiple of 64; begins and ends with a barrier.
+// Chunk c: places its rows at offsets[key] + group base + merge base + (rank among the chunk's
+// rows of the key, in row order), so a group holds its rows in input order.  Wave w counts the
+// rows [w, w + 1) * PTK_GB_CHUNK / GB_PLACE_WAVES of the chunk in row order: a row's rank in its
+// wave is the wave's running count of its entry plus the lower lanes of the same step holding the
+// same entry (ballots over the entry index bits); the lower waves' counts are added afterwards.
+// The chunk's rows are staged in LDS in output order and stored in that order, so a key's run
+// of rows is written with consecutive stores.  blockDim.x == PTK_GB_PLACE_THREADS; begins and
+// ends with a barrier.
 __device__ inline void gb_place_chunk(const int *__restrict__ keys, const int *__restrict__ vals, long long n,
                                       int numKeys, long long c, const long long *__restrict__ offsets,
                                       const PTK_GB_SCRATCH &S, int *__restrict__ outKeys, int *__restrict__ outVals,
                                       GbPlaceLds &L) {
     const long long r0 = c * PTK_GB_CHUNK, r1 = r0 + PTK_GB_CHUNK < n ? r0 + PTK_GB_CHUNK : n;
     const int NB = (int)S.buckets;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int s = threadIdx.x; s < GB_SLOTS; s += blockDim.x) L.key[s] = GB_EMPTY;
+    const unsigned *gbase = S.gcnt + (c / PTK_GB_GROUP) * ((long long)NB * PTK_GB_BUCKET);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int s = tid; s < GB_SLOTS; s += PTK_GB_PLACE_THREADS) L.u.slot[s] = GB_EMPTY;
+    for (int i = tid; i < GB_PLACE_WAVES * PTK_GB_CHUNK / 2; i += PTK_GB_PLACE_THREADS)
+        ((unsigned *)&L.wcnt[0][0])[i] = 0;
+    // the rows of this thread (row order: step s of wave w covers 64 consecutive rows)
+    const long long wr0 = r0 + (long long)w * (PTK_GB_CHUNK / GB_PLACE_WAVES);
+    int kr[GB_PLACE_ROWS], vr[GB_PLACE_ROWS];
+#pragma unroll
+    for (int s = 0; s < GB_PLACE_ROWS; ++s) {
+        const long long row = wr0 + 64 * s + lane;
+        kr[s] = row < r1 ? keys[row] : -1;
+        vr[s] = row < r1 ? vals[row] : 0;
+    }
     unsigned D = S.boff[c * (NB + 1) + NB];  // distinct keys of the chunk (<= PTK_GB_CHUNK)
     D = D < PTK_GB_CHUNK ? D : PTK_GB_CHUNK;
     // chunk-local start of each entry's rows: exclusive scan of the counts in entry order, each
     // thread taking a run of consecutive entries
-    const unsigned per = (D + blockDim.x - 1) / blockDim.x, i0 = threadIdx.x * per;
+    const unsigned per = (D + PTK_GB_PLACE_THREADS - 1) / PTK_GB_PLACE_THREADS, i0 = tid * per;
+    const PTK_GB_ENTRY *ent = S.entry + c * PTK_GB_CHUNK;
     unsigned long long mine = 0;
-    for (unsigned i = i0; i < i0 + per && i < D; ++i) mine += S.entry[c * PTK_GB_CHUNK + i].count;
+    for (unsigned i = i0; i < i0 + per && i < D; ++i) mine += ent[i].count;
     const unsigned long long incl = wave_incl(mine);
-    __syncthreads();  // the slots are cleared; red is free
+    __syncthreads();  // the slots and counters are cleared; red is free
     if (lane == 63) L.red[w] = incl;
     __syncthreads();
     unsigned long long start = incl - mine;
-    for (int q = 0; q < w && q < nw; ++q) start += L.red[q];
-    const PTK_GB_ENTRY *ent = S.entry + c * PTK_GB_CHUNK;
+    for (int q = 0; q < w; ++q) start += L.red[q];
     for (unsigned i = i0; i < i0 + per && i < D; ++i) {
         const int k = ent[i].key;
         if (k >= 0 && k <= numKeys) {
-            const long long out0 = offsets[k] + ent[i].sum;  // sum holds the merge base
+            const long long out0 = offsets[k] + (long long)gbase[k] + ent[i].sum;  // sum holds the merge base
             unsigned h = gb_hash((unsigned)k) & (GB_SLOTS - 1);
             for (int p = 0; p < GB_SLOTS; ++p) {  // distinct keys: the first empty slot is ours
-                if (atomicCAS(&L.key[h], GB_EMPTY, (unsigned)k) == GB_EMPTY) {
-                    L.next[h] = (unsigned)start;
-                    L.delta[h] = (int)(out0 - (long long)start);
+                if (atomicCAS(&L.u.slot[h], GB_EMPTY, (unsigned)k | (i << 19)) == GB_EMPTY) break;
+                h = (h + 1) & (GB_SLOTS - 1);
+            }
+            L.start[i] = (unsigned short)(start < PTK_GB_CHUNK ? start : PTK_GB_CHUNK);
+            L.out0[i] = (int)out0;
+        }
+        start += ent[i].count;
+    }
+    __syncthreads();
+    unsigned er[GB_PLACE_ROWS], rk[GB_PLACE_ROWS];
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int s = 0; s < GB_PLACE_ROWS; ++s) {
+        const int k = kr[s];
+        unsigned e = 0;
+        bool use = false;
+        if (k >= 0 && k <= numKeys) {
+            unsigned h = gb_hash((unsigned)k) & (GB_SLOTS - 1);
+            for (int p = 0; p < GB_SLOTS; ++p) {
+                const unsigned cur = L.u.slot[h];
+                if (cur == GB_EMPTY) break;  // only for an inconsistent scratch
+                if ((cur & GB_KEY_MASK) == (unsigned)k) {
+                    e = cur >> 19;
+                    use = true;
                     break;
                 }
                 h = (h + 1) & (GB_SLOTS - 1);
             }
         }
-        start += ent[i].count;
+        // lanes of this step holding the same entry
+        unsigned long long peers = __ballot(use);
+#pragma unroll
+        for (int b = 0; b < GB_ENTRY_BITS; ++b) {
+            const bool bit = (e >> b) & 1u;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const unsigned rank = (unsigned)__popcll(peers & below);
+        unsigned old = 0;
+        if (use) old = L.wcnt[w][e];
+        if (use && rank == 0) L.wcnt[w][e] = (unsigned short)(old + (unsigned)__popcll(peers));
+        er[s] = use ? e : 0xFFFFFFFFu;
+        rk[s] = old + rank;
+    }
+    __syncthreads();  // the hash slots are dead from here on: their memory stages the rows
+    for (unsigned e = tid; e < D; e += PTK_GB_PLACE_THREADS) {  // the lower waves' rows of each entry
+        unsigned b = 0;
+#pragma unroll
+        for (int q = 0; q < GB_PLACE_WAVES; ++q) {
+            const unsigned t = L.wcnt[q][e];
+            L.wcnt[q][e] = (unsigned short)b;
+            b += t;
+        }
     }
     __syncthreads();
-    gb_for_rows(keys, vals, r0, r1, [&](int k, int v) {
-        if (k < 0 || k > numKeys) return;
-        unsigned h = gb_hash((unsigned)k) & (GB_SLOTS - 1);
-        for (int p = 0; p < GB_SLOTS; ++p) {
-            const unsigned cur = L.key[h];
-            if (cur == (unsigned)k) {
-                const unsigned lp = atomicAdd(&L.next[h], 1u);
-                if (lp < PTK_GB_CHUNK) {  // always, for a consistent scratch
-                    L.stage[lp] = make_int2(L.delta[h] + (int)lp, k);
-                    L.sval[lp] = v;
-                }
-                return;
-            }
-            if (cur == GB_EMPTY) return;  // only for an inconsistent scratch
-            h = (h + 1) & (GB_SLOTS - 1);
+#pragma unroll
+    for (int s = 0; s < GB_PLACE_ROWS; ++s) {
+        if (er[s] == 0xFFFFFFFFu) continue;
+        const unsigned e = er[s];
+        const unsigned d = L.wcnt[w][e] + rk[s];  // the row's rank among the chunk's rows of its key
+        const unsigned pos = (unsigned)L.start[e] + d;
+        if (pos < PTK_GB_CHUNK) {  // always, for a consistent scratch
+            L.u.stage.row[pos] = L.out0[e] + (int)d;
+            L.u.stage.key[pos] = kr[s];
+            L.u.stage.val[pos] = vr[s];
         }
-    });
+    }
     __syncthreads();
     unsigned long long rows = 0;  // valid rows of the chunk: every local position below is filled
-    for (int q = 0; q < nw; ++q) rows += L.red[q];
+    for (int q = 0; q < GB_PLACE_WAVES; ++q) rows += L.red[q];
     rows = rows < PTK_GB_CHUNK ? rows : PTK_GB_CHUNK;
-    for (unsigned p = threadIdx.x; p < rows; p += blockDim.x) {
-        const int2 st = L.stage[p];
-        if (st.x >= 0 && st.x < n) {  // always, for a consistent scratch
-            outKeys[st.x] = st.y;
-            outVals[st.x] = L.sval[p];
+    for (unsigned p = tid; p < rows; p += PTK_GB_PLACE_THREADS) {
+        const int row = L.u.stage.row[p];
+        if (row >= 0 && row < n) {  // always, for a consistent scratch
+            outKeys[row] = L.u.stage.key[p];
+            outVals[row] = L.u.stage.val[p];
         }
     }
     __syncthreads();

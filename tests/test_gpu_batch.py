@@ -1,8 +1,7 @@
 """The batched / served executor (include/ptk_batch.h) on the GPU.
 
 Bar: every instance computes what its graph computes alone -- FDTD fields bit-exact against the
-oracle, groupby groups/offsets/reshuffled keys bit-exact and values equal per group as
-multisets, GMM within the tolerance of the single-graph GMM kernels (log-likelihood 1e-4
+oracle, every groupby output bit-exact (values of a group in input order), GMM within the tolerance of the single-graph GMM kernels (log-likelihood 1e-4
 relative) -- in plain per-task dispatch and in batched captured iterations alike, and the
 result checksums of an instance are identical alone, in a mix, in either mode, and when
 admission by footprint delays it.
@@ -60,11 +59,9 @@ def test_groupby_instances_match_oracle(hip, mode):
             assert np.array_equal(ex.read(i, 3, np.int64), offsets)
             out_k, out_v = ex.read(i, 4, np.int32), ex.read(i, 5, np.int32)
             assert np.array_equal(out_k, ok)
-            for key, cnt, _ in g[:50]:
-                o = offsets[key]
-                assert np.array_equal(np.sort(out_v[o:o + cnt]), np.sort(ov[o:o + cnt]))
+            assert np.array_equal(out_v, ov)  # rows of a group in input order
             c = ex.checksums(i)
-            assert c[2] == c[3]  # the output rows are a permutation of the input rows
+            assert c[4] == c[3]  # the output rows are a permutation of the input rows
 
 
 def gmm_initial_means(X, K, seed):

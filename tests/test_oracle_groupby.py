@@ -136,12 +136,12 @@ def test_host_pfxsum_graph_equals_oracle(tmp_path):
         assert np.array_equal(out, oracle.pfxsum(keys)), n
 
 
-@pytest.mark.parametrize("n,nk", [(1, 0), (2049, 513), (10_000, 1000), (6000, 255)])
+@pytest.mark.parametrize("n,nk", [(1, 0), (2049, 513), (10_000, 1000), (6000, 255), (1_100_000, 4095)])
 def test_host_task_forms_pipeline(n, nk):
-    """The host-task forms of the four groupby tasks (the chunk/bucket scratch layout of
+    """The host-task forms of the five groupby tasks (the chunk/bucket scratch layout of
     include/ptk_groupby.h restated on the host) chained as the graph chains them, with invalid
-    keys, equal the oracle's pipeline; with chunks merged in order the values come out in input
-    order, so every output is exact."""
+    keys, equal the oracle's pipeline exactly, values in input order included (1.1M rows span three
+    merge groups)."""
     import ctypes as C
 
     from ptask_amd import groupby as GB
@@ -164,7 +164,8 @@ def test_host_task_forms_pipeline(n, nk):
 
     p = lambda a: a.ctypes.data
     call("GroupbyAccumulate", p(keys), p(vals), n, nk, p(scratch))
-    call("GroupbyMerge", p(scratch), n, nk, p(table))
+    call("GroupbyMerge", p(scratch), n, nk)
+    call("GroupbyTotals", p(scratch), n, nk, p(table))
     call("GroupbyOffsets", p(table), nk, p(scratch), p(offsets), p(groups))
     call("GroupbyReshuffle", p(keys), p(vals), n, nk, p(offsets), p(scratch), p(ok), p(ov))
     g, head, eoff, eok, eov = oracle.groupby(keys, vals, nk)
