@@ -1,0 +1,79 @@
+// groupby_direct.hip -- libptk_groupby.so: the groupby graph's task kernels as direct library
+// calls (include/ptk_groupby_api.h).
+//
+// The kernels are the graph's own (groupby.hip is compiled into this library unchanged), launched
+// on the caller's stream in the order the graph dispatches them, with the geometry the groupby
+// application gives its tasks.  Results are therefore bitwise the graph's.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <string>
+
+#include "groupby.hip"
+#include "ptk_groupby_api.h"
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int Fail(int code, const std::string &what) {
+    g_lastError = what;
+    return code;
+}
+
+int HipFail(hipError_t e, const char *what) {
+    return Fail(PTK_GB_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+long long Align256(long long b) { return (b + 255) / 256 * 256; }
+
+// table (count | sum | cursor | errors), then the scratch of include/ptk_groupby.h
+long long TableBytes(int numKeys) { return Align256(8 * (3 * ((long long)numKeys + 1) + 1)); }
+
+bool ShapeOk(long long n, int numKeys) {
+    return n >= 1 && n <= INT_MAX && numKeys >= 0 && (long long)numKeys + 1 <= PTK_GB_MAX_KEYS;
+}
+
+// The five launches on `stream`; table and scratch lie in device memory the caller provides.
+int Launch(const int *keys, const int *vals, int n, int numKeys, long long *table, void *scratch, long long *offsets,
+           long long *groups, int *outKeys, int *outVals, hipStream_t stream) {
+    const long long chunks = ptk_gb_chunks(n), buckets = ptk_gb_buckets(numKeys);
+    const unsigned chunkGrid = (unsigned)(chunks < 2048 ? chunks : 2048);
+    const long long units = buckets * ptk_gb_groups(n);
+    const unsigned unitGrid = (unsigned)(units < 65535 ? units : 65535);
+    hipLaunchKernelGGL(GroupbyAccumulate, dim3(chunkGrid), dim3(512), 0, stream, keys, vals, n, numKeys, scratch);
+    hipLaunchKernelGGL(GroupbyMerge, dim3(unitGrid), dim3(PTK_GB_BUCKET), 0, stream, scratch, n, numKeys);
+    hipLaunchKernelGGL(GroupbyTotals, dim3((unsigned)buckets), dim3(PTK_GB_BUCKET), 0, stream, scratch, n, numKeys,
+                       table);
+    hipLaunchKernelGGL(GroupbyOffsets, dim3((unsigned)buckets), dim3(PTK_GB_BUCKET), 0, stream, table, numKeys,
+                       scratch, offsets, groups);
+    hipLaunchKernelGGL(GroupbyReshuffle, dim3(chunkGrid), dim3(PTK_GB_PLACE_THREADS), 0, stream, keys, vals, n,
+                       numKeys, (const long long *)offsets, scratch, outKeys, outVals);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PTK_GB_OK : HipFail(e, "groupby launch");
+}
+
+}  // namespace
+
+extern "C" long long ptk_groupby_workspace_bytes(long long n, int numKeys) {
+    if (!ShapeOk(n, numKeys)) return -1;
+    return TableBytes(numKeys) + ptk_gb_scratch_bytes(n, numKeys);
+}
+
+extern "C" int ptk_groupby_run(const int *keys, const int *vals, long long n, int numKeys, void *workspace,
+                               unsigned long long workspaceBytes, long long *offsets, long long *groups, int *outKeys,
+                               int *outVals, void *stream) {
+    if (!ShapeOk(n, numKeys)) return Fail(PTK_GB_EINVAL, "ptk_groupby_run: n must be in [1, 2^31 - 1] and numKeys in [0, 2^19 - 2]");
+    if (!keys || !vals || !workspace || !offsets || !groups || !outKeys || !outVals)
+        return Fail(PTK_GB_EINVAL, "ptk_groupby_run: null device pointer");
+    if (((unsigned long long)workspace & 15) != 0) return Fail(PTK_GB_EINVAL, "ptk_groupby_run: workspace is not 16-byte aligned");
+    const long long need = ptk_groupby_workspace_bytes(n, numKeys);
+    if (workspaceBytes < (unsigned long long)need)
+        return Fail(PTK_GB_ESPACE, "ptk_groupby_run: workspace of " + std::to_string(workspaceBytes) + " bytes, " +
+                                       std::to_string(need) + " needed");
+    long long *table = (long long *)workspace;
+    void *scratch = (char *)workspace + TableBytes(numKeys);
+    return Launch(keys, vals, (int)n, numKeys, table, scratch, offsets, groups, outKeys, outVals, (hipStream_t)stream);
+}
+
+extern "C" const char *ptk_groupby_last_error(void) { return g_lastError.c_str(); }

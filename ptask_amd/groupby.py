@@ -41,3 +41,43 @@ def scratch_offsets(n: int, num_keys: int) -> dict:
 
 def scratch_bytes(n: int, num_keys: int) -> int:
     return scratch_offsets(n, num_keys)["total"]
+
+
+# ---- the direct-call library (include/ptk_groupby_api.h, ptask_amd/lib/libptk_groupby.so) ----
+
+OK, EINVAL, ESPACE, EHIP = 0, 1, 2, 3
+_lib = None
+
+
+def lib():
+    """ctypes handle of libptk_groupby.so with its argument types bound; fails loudly when the
+    library is not built."""
+    import ctypes as C
+    import os
+
+    from . import paths
+    global _lib
+    if _lib is None:
+        path = paths.lib_path("libptk_groupby.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        L.ptk_groupby_workspace_bytes.restype = C.c_longlong
+        L.ptk_groupby_workspace_bytes.argtypes = [C.c_longlong, C.c_int]
+        L.ptk_groupby_run.restype = C.c_int
+        L.ptk_groupby_run.argtypes = ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_void_p,
+                                       C.c_ulonglong] + [C.c_void_p] * 5)
+        L.ptk_groupby_last_error.restype = C.c_char_p
+        L.ptk_groupby_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def declared_symbols() -> list[str]:
+    """Names of the functions declared in include/ptk_groupby_api.h."""
+    import os
+    import re
+
+    from . import paths
+    text = open(os.path.join(paths.include_dir(), "ptk_groupby_api.h")).read()
+    return re.findall(r"^\s*(?:int|long long|unsigned long long|const char \*)\s*(ptk_groupby_\w+)\s*\(", text, re.M)
