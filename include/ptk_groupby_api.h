@@ -44,6 +44,18 @@ int ptk_groupby_run(const int *keys, const int *vals, long long n, int numKeys, 
                     unsigned long long workspaceBytes, long long *offsets, long long *groups, int *outKeys,
                     int *outVals, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  As ptk_groupby_run,
+ * but the workspace is a static device array of PTK_GB_NOALLOC_BYTES inside the library, one per
+ * GPU, held until the process ends; the call allocates nothing and fails with PTK_GB_ESPACE when
+ * ptk_groupby_workspace_bytes(n, numKeys) exceeds it (about 3.5M rows over 65536 keys).  The
+ * array keeps the last call's contents; nothing in it needs clearing between calls.  Calls that
+ * overlap on one GPU (other streams, other threads) share the array and race: serialise them. */
+#define PTK_GB_NOALLOC_BYTES (64ull << 20)
+unsigned long long ptk_groupby_noalloc_capacity(void);
+int ptk_groupby_run_noalloc(const int *keys, const int *vals, long long n, int numKeys, long long *offsets,
+                            long long *groups, int *outKeys, int *outVals, void *stream);
+
 const char *ptk_groupby_last_error(void);
 
 #ifdef __cplusplus

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <mutex>
 #include <string>
 
 #include "groupby.hip"
@@ -74,6 +75,59 @@ extern "C" int ptk_groupby_run(const int *keys, const int *vals, long long n, in
     long long *table = (long long *)workspace;
     void *scratch = (char *)workspace + TableBytes(numKeys);
     return Launch(keys, vals, (int)n, numKeys, table, scratch, offsets, groups, outKeys, outVals, (hipStream_t)stream);
+}
+
+// ---- the no-allocation entry point ------------------------------------------------------------
+// Its workspace is this file-scope device array: each GPU the library runs on holds one copy for
+// the life of the process.  Host code never passes the array's name to a kernel (on the host it
+// is a stand-in, not a device address): the first call on a device, with that device current,
+// asks the runtime for the device's own address of the array and keeps it per device.
+static __device__ __align__(256) unsigned char scratch[PTK_GB_NOALLOC_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void *g_scratchPtr[kMaxDevices] = {};
+
+int ScratchOfCurrentDevice(void **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return HipFail(e, "ptk_groupby_run_noalloc: hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return Fail(PTK_GB_EINVAL, "ptk_groupby_run_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return HipFail(e, "ptk_groupby_run_noalloc: hipGetSymbolAddress");
+        if (!p) return Fail(PTK_GB_EHIP, "ptk_groupby_run_noalloc: no device address for the workspace");
+        g_scratchPtr[dev] = p;
+    }
+    *out = g_scratchPtr[dev];
+    return PTK_GB_OK;
+}
+
+}  // namespace
+
+extern "C" unsigned long long ptk_groupby_noalloc_capacity(void) { return PTK_GB_NOALLOC_BYTES; }
+
+extern "C" int ptk_groupby_run_noalloc(const int *keys, const int *vals, long long n, int numKeys, long long *offsets,
+                                       long long *groups, int *outKeys, int *outVals, void *stream) {
+    if (!ShapeOk(n, numKeys))
+        return Fail(PTK_GB_EINVAL, "ptk_groupby_run_noalloc: n must be in [1, 2^31 - 1] and numKeys in [0, 2^19 - 2]");
+    if (!keys || !vals || !offsets || !groups || !outKeys || !outVals)
+        return Fail(PTK_GB_EINVAL, "ptk_groupby_run_noalloc: null device pointer");
+    const long long need = ptk_groupby_workspace_bytes(n, numKeys);
+    if ((unsigned long long)need > PTK_GB_NOALLOC_BYTES)
+        return Fail(PTK_GB_ESPACE, "ptk_groupby_run_noalloc: " + std::to_string(n) + " rows over " +
+                                       std::to_string((long long)numKeys + 1) + " keys need " + std::to_string(need) +
+                                       " bytes of workspace; the static workspace holds " +
+                                       std::to_string((unsigned long long)PTK_GB_NOALLOC_BYTES));
+    void *ws = nullptr;
+    if (int r = ScratchOfCurrentDevice(&ws)) return r;
+    long long *table = (long long *)ws;
+    void *s = (char *)ws + TableBytes(numKeys);
+    return Launch(keys, vals, (int)n, numKeys, table, s, offsets, groups, outKeys, outVals, (hipStream_t)stream);
 }
 
 extern "C" const char *ptk_groupby_last_error(void) { return g_lastError.c_str(); }

@@ -67,6 +67,11 @@ def lib():
         L.ptk_groupby_run.restype = C.c_int
         L.ptk_groupby_run.argtypes = ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_void_p,
                                        C.c_ulonglong] + [C.c_void_p] * 5)
+        L.ptk_groupby_noalloc_capacity.restype = C.c_ulonglong
+        L.ptk_groupby_noalloc_capacity.argtypes = []
+        L.ptk_groupby_run_noalloc.restype = C.c_int
+        L.ptk_groupby_run_noalloc.argtypes = ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_int]
+                                              + [C.c_void_p] * 5)
         L.ptk_groupby_last_error.restype = C.c_char_p
         L.ptk_groupby_last_error.argtypes = []
         _lib = L

@@ -55,7 +55,7 @@ def zipf_rows(n, nk, seed):
     return keys, vals
 
 
-def run_direct(hip, keys, vals, nk):
+def run_direct(hip, keys, vals, nk, fn="run"):
     """One library call on device buffers; returns (groups, offsets, outKeys, outVals)."""
     n, K = keys.size, nk + 1
     L = GB.lib()
@@ -64,9 +64,12 @@ def run_direct(hip, keys, vals, nk):
     dv.upload(vals)
     off, grp = hip.DeviceBuffer(8 * K), hip.DeviceBuffer(8 * (3 + 3 * K))
     ok, ov = hip.DeviceBuffer(4 * n), hip.DeviceBuffer(4 * n)
-    ws_bytes = L.ptk_groupby_workspace_bytes(n, nk)
-    ws = hip.DeviceBuffer(ws_bytes)
-    rc = L.ptk_groupby_run(dk.ptr, dv.ptr, n, nk, ws.ptr, ws_bytes, off.ptr, grp.ptr, ok.ptr, ov.ptr, None)
+    if fn == "run":
+        ws_bytes = L.ptk_groupby_workspace_bytes(n, nk)
+        ws = hip.DeviceBuffer(ws_bytes)
+        rc = L.ptk_groupby_run(dk.ptr, dv.ptr, n, nk, ws.ptr, ws_bytes, off.ptr, grp.ptr, ok.ptr, ov.ptr, None)
+    else:
+        rc = L.ptk_groupby_run_noalloc(dk.ptr, dv.ptr, n, nk, off.ptr, grp.ptr, ok.ptr, ov.ptr, None)
     assert rc == 0, L.ptk_groupby_last_error()
     hip.synchronize()
     out = [np.zeros(3 + 3 * K, np.int64), np.zeros(K, np.int64), np.zeros(n, np.int32), np.zeros(n, np.int32)]
@@ -92,3 +95,15 @@ def test_direct_call_equals_oracle(hip, n, nk):
     keys, vals = zipf_rows(n, nk, seed=n)
     check_against_oracle(keys, vals, nk, run_direct(hip, keys, vals, nk))
 
+
+def test_noalloc_rejects_inputs_beyond_its_static_workspace():
+    L = GB.lib()
+    cap = L.ptk_groupby_noalloc_capacity()
+    assert cap == 64 << 20
+    p = C.c_void_p(0x1000)  # never dereferenced: the size check fails first
+    n = 8_000_000
+    assert L.ptk_groupby_workspace_bytes(n, 65535) > cap
+    assert L.ptk_groupby_run_noalloc(p, p, n, 65535, p, p, p, p, None) == GB.ESPACE
+    assert b"static workspace" in L.ptk_groupby_last_error()
+    assert L.ptk_groupby_run_noalloc(p, p, 0, 65535, p, p, p, p, None) == GB.EINVAL
+    assert L.ptk_groupby_run_noalloc(None, p, 10, 5, p, p, p, p, None) == GB.EINVAL
