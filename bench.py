@@ -28,12 +28,15 @@ Graph forms measured (all bit-identical to the six reference updates); `value` i
 
 The same line also carries, under "workloads", the other graphs of the hot path:
   captured -- the same 384^3 graph as one instance of the batched executor: plain per-task
-             dispatch of the six reference tasks vs one hipGraph-captured iteration per step
+             dispatch of the six reference tasks, the batched kernels launched directly, and the
+             same kernels as one hipGraph replay per iteration (capture alone vs plain dispatch)
   batched -- 2048 FDTD graphs (edge U{16..96}, 50 steps) + 512 groupby graphs (1M-8M rows, Zipf 1.1
              over 65,536 keys) through the batched executor, plain per-task dispatch vs captured
-             batched iterations (instances/s, both on the same instances, result checksums compared)
-  served  -- Poisson arrivals of FDTD / GMM / groupby graph instances at a few rates: p50/p99
-             latency and graphs/s, continuous batching vs plain dispatch, alone-vs-mix identity
+             batched iterations (instances/s, both on the same instances, result checksums
+             compared; tile shapes from an untimed autotuning pass)
+  served  -- Poisson arrivals of FDTD / GMM / groupby graph instances, both modes swept up to
+             saturation: p50/p99 latency, graphs/s, capacity and rate within a p99 bound,
+             continuous batching vs plain dispatch, every request's result identical alone
   gmm     -- EM-GMM N=2M, D=32, K=64, 25 EM steps (points x steps per second)
   pca     -- Gram-Schmidt PCA 65536 x 2048, 16 components, 10 power iterations
 
@@ -67,8 +70,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--suite", default="captured,batched,served,gmm,pca",
                     help="extra workloads after the FDTD headline (comma list; empty for none)")
-    ap.add_argument("--serve-seconds", type=float, default=5.0)
-    ap.add_argument("--serve-qps", default="20,100,300")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-qps", default="25,50,100,200,400,800,1600,3200")
     return ap.parse_args()
 
 
@@ -135,9 +138,24 @@ def kernel_roofline(r: dict, n: int) -> dict:
             "launches_timed": int(dom["dispatches"])}
 
 
+# Algorithmic HBM bytes per cell-step of each graph form: every array its tasks read once and
+# every array they write once, binary32, per time step.
+FORM_BYTES_PER_CELL_STEP = {"six": 96, "halves": 72, "overlap": 48, "fused": 48, "overlap2": 24,
+                            "overlap3": 16, "overlap3p": 16}
+
+
 def form_roofline(r: dict, n: int, form: str) -> dict:
+    """The dominant kernel's roofline, plus the form's own rate: cell-steps per second of the whole
+    graph times the form's algorithmic bytes per cell-step (for the six-task form, whose three H
+    and three E tasks run concurrently on their own streams, this is the meaningful figure: each
+    kernel's duration is stretched by its concurrent siblings)."""
     roof = kernel_roofline(r, n)
     roof["traffic"] = measured_traffic(roof["kernel"], n, form)
+    b = FORM_BYTES_PER_CELL_STEP.get(form)
+    if b:
+        gbs = r["mcups"] * 1e6 * b / 1e9
+        roof["graph"] = {"bytes_per_cell_step": b, "achieved": round(gbs, 1),
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)}
     return roof
 
 
@@ -167,13 +185,28 @@ def measured_traffic(kernel: str, n: int, form: str):
     return data.get(kernel.split("_")[0], {}).get("traffic_bytes")
 
 
+def hbm_ceiling() -> dict:
+    """The measured streaming ceiling of this GPU beside the 8 TB/s peak: PtkStreamMix (six arrays
+    read and three written with 16-byte accesses, the access mix of a half-step FDTD kernel
+    without its stencil), PtkReadWords (reads only) and PtkCopyWords (copy), HIP-event timed."""
+    from ptask_amd import paths
+    out = {"unit": "GB/s", "peak": HBM_PEAK_GBS}
+    for mode, mib, grid in (("mix", 217, 1024), ("read", 1024, 4096), ("copy", 1024, 1024)):
+        r = run_app("ptk_copy", ["--kernels", paths.KERNEL_DIR, "--mode", mode, "--mib", mib,
+                                 "--repeat", 10, "--grid", grid])
+        out[mode] = r.get("gbs", r.get("read_plus_write_gbs"))
+    return out
+
+
 def leg_batched() -> dict:
     from ptask_amd import paths
     r = run_app("ptk_batch", ["--kernels", paths.KERNEL_DIR])
     pl, ba = r["modes"]["plain"], r["modes"]["batched"]
     return {"workload": "2048 FDTD graphs (cubic edge U{16..96}, 50 steps) + 512 groupby graphs "
                         "(rows U[1M,8M], Zipf 1.1 over 65,536 keys), one executor, all resident",
-            "unit": "graph instances/s",
+            "unit": "graph instances/s", "target_speedup": 5.0,
+            "autotune": "separate untimed pass over the same mix; its choice is applied to the timed run",
+            "autotune_pass_ms": r.get("autotune_pass_ms"), "tuning": r.get("tuning"),
             "plain_exec": round(pl["exec_instances_per_s"], 1),
             "batched_exec": round(ba["exec_instances_per_s"], 1),
             "speedup_exec": round(ba["exec_instances_per_s"] / pl["exec_instances_per_s"], 2),
@@ -186,38 +219,66 @@ def leg_batched() -> dict:
             "footprint_bytes": r["footprint_bytes"]}
 
 
-def leg_served(seconds: float, rates: list) -> dict:
+SERVE_P99_BOUND_MS = 1000.0
+
+
+def serve_run(mode: str, qps: float, seconds: float, identity: int = 0) -> dict:
     from ptask_amd import paths
-    runs = []
-    for q in rates:
-        r = run_app("ptk_serve", ["--kernels", paths.KERNEL_DIR, "--qps", q, "--duration", seconds,
-                                  "--identity", 4])
-        runs.append({k: r[k] for k in ("mode", "qps", "requests", "graphs_per_s", "p50_ms", "p99_ms",
-                                       "p50_ms_by_kind", "p99_ms_by_kind", "mean_active",
-                                       "identity_checked", "identity_equal")})
-    q = rates[len(rates) // 2]
-    r = run_app("ptk_serve", ["--kernels", paths.KERNEL_DIR, "--qps", q, "--duration", seconds,
-                              "--identity", 0, "--mode", "plain"])
-    runs.append({k: r[k] for k in ("mode", "qps", "requests", "graphs_per_s", "p50_ms", "p99_ms",
-                                   "p50_ms_by_kind", "p99_ms_by_kind", "mean_active")})
-    return {"workload": "Poisson arrivals; FDTD edge U{32..192} x 20..100 steps, GMM N U[10K,1M] D=16 "
-                        "K U{8..32} 20 EM steps, groupby rows U[1M,16M]; mix 0.5:0.25:0.25",
-            "latency_kinds": ["fdtd", "gmm", "groupby"], "runs": runs}
+    r = run_app("ptk_serve", ["--kernels", paths.KERNEL_DIR, "--qps", qps, "--duration", seconds,
+                              "--identity", identity, "--mode", mode])
+    keep = ("mode", "qps", "requests", "graphs_per_s", "offered_per_s", "p50_ms", "p99_ms",
+            "p50_ms_by_kind", "p99_ms_by_kind", "mean_active", "identity_checked", "identity_equal")
+    return {k: r[k] for k in keep if k in r}
+
+
+def leg_served(seconds: float, rates: list) -> dict:
+    """Both modes swept upwards in offered load until they saturate (graphs/s below 85% of the
+    offered rate, or p99 above 10 s).  Per mode: the service capacity (largest graphs/s any run
+    sustained) and the largest offered rate served within the p99 bound with at least 95% of the
+    offered rate completed; their ratio batched / plain is the served speedup."""
+    out = {"workload": "Poisson arrivals; FDTD edge U{32..192} x 20..100 steps, GMM N U[10K,1M] D=16 "
+                       "K U{8..32} 20 EM steps, groupby rows U[1M,16M]; mix 0.5:0.25:0.25",
+           "latency_kinds": ["fdtd", "gmm", "groupby"], "p99_bound_ms": SERVE_P99_BOUND_MS,
+           "target_speedup": 5.0, "modes": {}}
+    for mode in ("batched", "plain"):
+        runs = []
+        for q in rates:
+            r = serve_run(mode, float(q), seconds)
+            runs.append(r)
+            progress(f"served {mode} {q} QPS: {r['graphs_per_s']:.1f} graphs/s, p99 {r['p99_ms']:.1f} ms")
+            if r["graphs_per_s"] < 0.85 * r["offered_per_s"] or r["p99_ms"] > 10000:
+                break
+        ok = [r for r in runs if r["p99_ms"] <= SERVE_P99_BOUND_MS and r["graphs_per_s"] >= 0.95 * r["offered_per_s"]]
+        out["modes"][mode] = {"runs": runs,
+                              "capacity_graphs_per_s": round(max(r["graphs_per_s"] for r in runs), 2),
+                              "max_qps_within_p99": max((r["qps"] for r in ok), default=0.0)}
+    b, p = out["modes"]["batched"], out["modes"]["plain"]
+    out["capacity_speedup"] = round(b["capacity_graphs_per_s"] / max(p["capacity_graphs_per_s"], 1e-9), 2)
+    out["qps_within_p99_speedup"] = (round(b["max_qps_within_p99"] / p["max_qps_within_p99"], 2)
+                                     if p["max_qps_within_p99"] else None)
+    ident = serve_run("batched", float(rates[min(2, len(rates) - 1)]), seconds, identity=-1)
+    out["identity"] = {"qps": ident["qps"], "requests": ident["requests"],
+                       "checked": ident["identity_checked"], "equal": ident["identity_equal"]}
+    return out
 
 
 def leg_captured(n: int, steps: int) -> dict:
-    """The 384^3 FDTD graph as one instance of the batched executor: plain per-task dispatch
-    (the six reference Yee tasks launched one by one, every step) against hipGraph-captured
-    iterations (one replayed graph per three steps, the three-step kernel on overlapped tiles,
-    then one-step graphs for the remaining steps % 3).  Device
-    time of the iterations from the executor's events; wall time includes admission (field
-    generation on the device) and retirement (result checksums)."""
+    """The 384^3 FDTD graph as one instance of the batched executor, three ways:
+      plain    -- per-task dispatch of the six reference Yee tasks, one launch each, every step;
+      direct   -- the batched iteration's kernels (three steps per launch on overlapped tiles, then
+                  one-step launches for steps % 3) launched directly, one launch per kernel;
+      captured -- the same kernels as one hipGraph replay per iteration.
+    direct vs captured is the effect of capture alone; plain vs captured combines it with the
+    fused kernels.  Device time of the iterations from the executor's events; wall time includes
+    admission (field generation on the device) and retirement (result checksums)."""
     from ptask_amd import batch
     out = {"workload": f"FDTD {n}^3 binary32, {steps} time steps, one executor instance",
            "unit": "Mcell-updates/s"}
     sums = {}
-    for name, mode in (("plain", batch.PLAIN), ("captured", batch.BATCHED)):
+    for name, mode, graphs in (("plain", batch.PLAIN, False), ("direct", batch.BATCHED, False),
+                               ("captured", batch.BATCHED, True)):
         with batch.Executor(8 << 30, mode) as ex:
+            ex.set_launch(graphs)
             t0 = time.perf_counter()
             i = ex.submit_fdtd(n, n, n, steps, 1, keep=True)
             ex.drain()
@@ -230,8 +291,9 @@ def leg_captured(n: int, steps: int) -> dict:
                          "graph_launches": st["graph_launches"],
                          "fdtd_steps_per_launch": ex.fdtd_steps()[0] if mode == batch.BATCHED else 1}
         progress(f"captured leg {name}: {out[name]['value']} Mcell-updates/s")
-    out["speedup"] = round(out["captured"]["value"] / out["plain"]["value"], 2)
-    out["checksums_equal"] = sums["plain"] == sums["captured"]
+    out["speedup_capture_alone"] = round(out["captured"]["value"] / out["direct"]["value"], 3)
+    out["speedup_vs_plain"] = round(out["captured"]["value"] / out["plain"]["value"], 2)
+    out["checksums_equal"] = sums["plain"] == sums["captured"] == sums["direct"]
     return out
 
 
@@ -283,9 +345,20 @@ def cpu_baseline(n: int, seconds: float) -> dict:
     rate = n ** 3 * steps / el / 1e6
     del cur, F
     return {"value": round(rate, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model(), "cpus_available": len(os.sched_getaffinity(0)),
             "sample": f"{n}^3 grid, {steps} time step(s) of the binary32 oracle on one core "
                       f"({el:.1f} s)",
             "host_graph_64": host_graph_baseline()}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def host_graph_baseline(runs: int = 5) -> dict:
@@ -332,6 +405,11 @@ def main() -> int:
     head_form = max(results, key=lambda f: results[f]["mcups"])
     head = results[head_form]
     roof = form_roofline(head, n, head_form)
+    try:
+        ceiling = hbm_ceiling()
+        roof["measured_ceiling"] = ceiling
+    except Exception as e:  # reported, the headline still stands
+        progress(f"hbm ceiling probe failed: {e}")
     line = {
         "metric": METRIC,
         "value": round(head["mcups"], 1),

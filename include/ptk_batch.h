@@ -130,6 +130,13 @@ int ptk_batch_set_tiles(ptk_batch *b, int fdtd_cols, int fdtd_depth, int gb_rows
 int ptk_batch_get_tiles(ptk_batch *b, int *fdtd_cols, int *fdtd_depth, int *gb_rows);
 int ptk_batch_autotune(ptk_batch *b, double *best_ms);
 
+/* The autotuner's choice as five integers {fdtd_cols, fdtd_depth, fdtd_grid, fdtd_steps_per_launch,
+ * fdtd_blocks3} (tile shape, persistent grid of the one-step kernels, steps per launch, virtual
+ * blocks of the three-step schedule), so a tuning run's decision can be applied to later executors
+ * of the same size class without tuning inside their timed work.  set validates every field. */
+int ptk_batch_get_tuning(ptk_batch *b, int out[5]);
+int ptk_batch_set_tuning(ptk_batch *b, const int in[5]);
+
 /* FDTD time steps per batched launch with the whole-step form (fdtd_cols == 0): 3 (the default)
  * advances every resident FDTD instance three steps per launch on overlapped 58 x 10-column tiles
  * (BatchFdtdSteps3) while each has at least three steps left, then one step per launch for the

@@ -1091,6 +1091,31 @@ int ptk_batch_get_tiles(ptk_batch *B, int *cols, int *depth, int *gbRows) {
 // rewrites the next field set from the current one, so repeating it changes nothing) and keeps
 // the fastest per time step.  Groupby chunk rows are fixed at
 // admission and are not tuned here.
+int ptk_batch_get_tuning(ptk_batch *B, int out[5]) {
+    if (!B || !out) return Fail(PTK_BATCH_EINVAL, "null argument");
+    out[0] = B->fdtdCols;
+    out[1] = B->fdtdDepth;
+    out[2] = B->fdtdGrid;
+    out[3] = B->fdtdSteps;
+    out[4] = B->grid3;
+    return PTK_BATCH_OK;
+}
+
+int ptk_batch_set_tuning(ptk_batch *B, const int in[5]) {
+    if (!B || !in) return Fail(PTK_BATCH_EINVAL, "null argument");
+    if (in[0] < 0 || in[1] < 1 || in[1] > 65535 || in[2] < 1 || in[2] > (1 << 20) || (in[3] != 1 && in[3] != 3) ||
+        in[4] < 1 || in[4] > (1 << 16))
+        return Fail(PTK_BATCH_EINVAL, "tuning out of range");
+    B->fdtdCols = in[0];
+    B->fdtdDepth = in[1];
+    B->fdtdGrid = in[2];
+    B->fdtdSteps = in[3];
+    B->grid3 = in[4];
+    DropGraphs(B);  // the captured iterations hold the grid sizes
+    B->dirty = true;
+    return PTK_BATCH_OK;
+}
+
 int ptk_batch_autotune(ptk_batch *B, double *best_ms) {
     if (!B) return Fail(PTK_BATCH_EINVAL, "null executor");
     bool anyFdtd = false;

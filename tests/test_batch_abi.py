@@ -8,7 +8,7 @@ from ptask_amd import batch, fdtd, gmm, groupby
 def test_library_exports_every_declared_symbol():
     L = batch.lib()
     names = batch.declared_symbols()
-    assert len(names) == 21
+    assert len(names) == 23
     for n in names:
         assert hasattr(L, n), n
 
