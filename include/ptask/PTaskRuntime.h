@@ -15,6 +15,7 @@ class Channel;
 class CompiledKernel;
 class Datablock;
 class DatablockTemplate;
+class Graph;
 class Port;
 class Task;
 
@@ -124,6 +125,25 @@ int GetTaskStreamPolicy();
 /* Blocks kept per output-port pool before recycled blocks are freed instead. */
 void SetDefaultBlockPoolSize(UINT n);
 UINT GetDefaultBlockPoolSize();
+
+/* ---- graph helpers (reference PTaskRuntime.h:2441-2560): forwarders to the Graph, Port and
+ * Channel calls, for applications written against the Runtime:: form ---------------------- */
+void SetPortMetaFunction(Port *pPort, METAFUNCTION eCanonicalMetaFunction);
+Task *AddTask(Graph *pGraph, CompiledKernel *pKernel, UINT uiInputPortCount, Port **pvInputPorts,
+              UINT uiOutputPorts, Port **pvOutputPorts, char *lpszTaskName);
+void BindDerivedPort(Graph *pGraph, Port *pDescribedPort, Port *pDescriberPort, DESCRIPTORFUNC func = DF_SIZE);
+void RunGraph(Graph *pGraph);
+void Push(Channel *pChannel, Datablock *pBlock);
+Datablock *Pull(Channel *pChannel, DWORD dwTimeout = INFINITE);
+PTRESULT CheckGraphSemantics(Graph *pGraph, BOOL bVerbose = TRUE, BOOL bFailOnWarning = FALSE);
+/* Accelerators the runtime may use per class: the devices enumerated at Initialize unless set
+ * (before Initialize) to fewer (reference: g_nPhysicalAccelerators). */
+UINT GetPhysicalAcceleratorCount();
+void SetPhysicalAcceleratorCount(UINT nAcceleratorCount);
+/* TRUE when bCondition holds; otherwise a warning with the text and FALSE (the reference also
+ * asserts in debug builds). */
+BOOL CheckInvariantCondition(BOOL bCondition, char *lpszErrorText = NULL);
+BOOL CheckInvariantCondition(BOOL bCondition, std::string strErrorText);
 
 /* ---- diagnostics -------------------------------------------------------------------------- */
 /* Prints the message; then exits the process unless SetExitOnRuntimeFailure(FALSE), in which

@@ -212,6 +212,7 @@ struct RuntimeState {
     bool initialized = false;
     BOOL useHost = TRUE, useHIP = TRUE;
     int maxConcurrency = 0;         // HIP devices to use (0 = all)
+    UINT physicalAccelerators = 0;  // devices per class the runtime may open (0 = all found)
     int maxHostConcurrency = 4;     // host tasks running at once
     int channelCapacity = 4;
     UINT threadPoolSize = 0;        // graph runner threads (0 = automatic)

@@ -243,6 +243,7 @@ PTRESULT Initialize() {
         int n = 0;
         if (ptk_hip_get_device_count(&n) != PTK_HIP_OK) n = 0;
         if (S.maxConcurrency > 0 && n > S.maxConcurrency) n = S.maxConcurrency;
+        if (S.physicalAccelerators > 0 && n > (int)S.physicalAccelerators) n = (int)S.physicalAccelerators;
         for (int d = 0; d < n; ++d) {
             HIPAccelerator *a = new HIPAccelerator((UINT)(1 + d), d, (UINT)(1 + d));
             a->AddRef();
@@ -494,6 +495,60 @@ Accelerator *GetAccelerator(ACCELERATOR_CLASS eClass, UINT uiIndex) {
 UINT GetAcceleratorCount(ACCELERATOR_CLASS eClass) {
     auto *v = ListFor(eClass);
     return v ? (UINT)v->size() : 0;
+}
+
+// ---- graph helpers: the Runtime:: forms of Graph, Port and Channel calls -----------------------
+// (reference PTaskRuntime.cpp:4690-4790, PTaskRuntime.h:2441-2560)
+
+void SetPortMetaFunction(Port *pPort, METAFUNCTION eCanonicalMetaFunction) {
+    if (pPort) pPort->SetMetaFunction(eCanonicalMetaFunction);
+}
+
+Task *AddTask(Graph *pGraph, CompiledKernel *pKernel, UINT uiInputPortCount, Port **pvInputPorts, UINT uiOutputPorts,
+              Port **pvOutputPorts, char *lpszTaskName) {
+    if (!pGraph) {
+        HandleError("AddTask: no graph");
+        return nullptr;
+    }
+    return pGraph->AddTask(pKernel, uiInputPortCount, pvInputPorts, uiOutputPorts, pvOutputPorts, lpszTaskName);
+}
+
+void BindDerivedPort(Graph *pGraph, Port *pDescribedPort, Port *pDescriberPort, DESCRIPTORFUNC func) {
+    if (pGraph) pGraph->BindDescriptorPort(pDescribedPort, pDescriberPort, func);
+}
+
+void RunGraph(Graph *pGraph) {
+    if (pGraph) pGraph->Run();
+}
+
+void Push(Channel *pChannel, Datablock *pBlock) {
+    if (pChannel) pChannel->Push(pBlock);
+}
+
+Datablock *Pull(Channel *pChannel, DWORD dwTimeout) { return pChannel ? pChannel->Pull(dwTimeout) : nullptr; }
+
+PTRESULT CheckGraphSemantics(Graph *pGraph, BOOL bVerbose, BOOL bFailOnWarning) {
+    return pGraph ? pGraph->CheckGraphSemantics(bVerbose, bFailOnWarning) : PTASK_ERR;
+}
+
+UINT GetPhysicalAcceleratorCount() {
+    RuntimeState &S = RT();
+    if (S.physicalAccelerators) return S.physicalAccelerators;
+    return (UINT)S.hipAccelerators.size();
+}
+
+void SetPhysicalAcceleratorCount(UINT nAcceleratorCount) { RT().physicalAccelerators = nAcceleratorCount; }
+
+BOOL CheckInvariantCondition(BOOL bCondition, char *lpszErrorText) {
+    if (bCondition) return TRUE;
+    Warning(lpszErrorText ? lpszErrorText : "invariant violated");
+    return FALSE;
+}
+
+BOOL CheckInvariantCondition(BOOL bCondition, std::string strErrorText) {
+    if (bCondition) return TRUE;
+    Warning(strErrorText.c_str());
+    return FALSE;
 }
 
 void SetUseHost(BOOL b) { RT().useHost = b; }

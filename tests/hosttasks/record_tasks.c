@@ -44,3 +44,16 @@ void ScaleVector(unsigned int nArgs, void **a) {
     float *out = (float *)a[3];
     for (int i = 0; i < n; ++i) out[i] = s * A[i];
 }
+
+/* ScaleInPlace(float* A, float s, VECADD_PARAMS n): A[i] *= s for i < n, in place -- the vector
+ * task of ptaskUnitTest/iteration.cpp run_simple_iteration_task (an in/out port, a sticky
+ * scalar, the element count from a port derived from A). */
+void ScaleInPlace(unsigned int nArgs, void **a) {
+    (void)nArgs;
+    float *A = (float *)a[0];
+    float s;
+    const uintptr_t bits = (uintptr_t)a[1];
+    memcpy(&s, &bits, sizeof(s));
+    const int n = (int)(intptr_t)a[2];
+    for (int i = 0; i < n; ++i) A[i] = s * A[i];
+}
