@@ -47,6 +47,14 @@ public:
     BOOL IsMarshallable() { return m_bMarshallable; }
     void SetMetaFunction(METAFUNCTION e) { m_eMetaFunction = e; }
     METAFUNCTION GetMetaFunction() { return m_eMetaFunction; }
+    /* Materialise this port's block in the memory of the task's dependent accelerator of class
+     * accClass (instance nIndex) and pass a device pointer to a dependent host task (reference:
+     * Port::BindDependentAccelerator, port.h:1185).  ACCELERATOR_CLASS_CUDA requests bind to the
+     * HIP accelerator on this build. */
+    virtual void BindDependentAccelerator(ACCELERATOR_CLASS accClass, int nIndex);
+    virtual BOOL HasDependentAcceleratorBinding() { return m_bDependentBinding; }
+    ACCELERATOR_CLASS GetDependentAcceleratorClass() { return m_eDependentClass; }
+    int GetDependentAcceleratorIndex() { return m_nDependentIndex; }
     /* A sticky or in/out block that survives across dispatches until replaced. */
     void SetPermanentBlock(BOOL b) { m_bPermanentBlock = b; }
     BOOL IsPermanentBlock() { return m_bPermanentBlock; }
@@ -99,6 +107,9 @@ protected:
     std::vector<std::pair<Port *, DESCRIPTORFUNC>> m_vDescriptorPorts;
     Port *m_pDescribedPort;
     std::vector<Port *> m_vIterationTargets;
+    BOOL m_bDependentBinding = FALSE;
+    ACCELERATOR_CLASS m_eDependentClass = ACCELERATOR_CLASS_UNKNOWN;
+    int m_nDependentIndex = 0;
 };
 
 /* A port that consumes blocks.  Switch ("control") channels are preferred over normal ones;

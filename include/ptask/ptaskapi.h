@@ -24,5 +24,6 @@
 #include "StickyPort.h"
 #include "MetaPort.h"
 #include "shrperft.h"
+#include "hosttask.h"
 
 #endif

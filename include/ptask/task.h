@@ -63,6 +63,18 @@ public:
     Port *GetInputPort(UINT i) { return m_vInputs[i]; }
     OutputPort *GetOutputPort(UINT i) { return m_vOutputs[i]; }
     UINT GetDispatchCount() { return m_uiDispatchCount; }
+    /* Dependent accelerators (reference: Task::BindDependentAcceleratorClass, task.h:768,
+     * task.cpp): a host task that also needs nRequiredInstances (1) accelerators of class cls to
+     * run -- its ports bound with Port::BindDependentAccelerator receive device pointers, and with
+     * bRequestDependentPSObjects the task is called in the LPFNDEPHOSTTASKEX form with the
+     * dependent stream (hosttask.h).  ACCELERATOR_CLASS_CUDA requests bind to HIP on this build.
+     * FALSE (and an error) for a second binding of a class or a count other than 1. */
+    virtual BOOL BindDependentAcceleratorClass(ACCELERATOR_CLASS cls, int nRequiredInstances = 1,
+                                               BOOL bRequestDependentPSObjects = FALSE);
+    virtual int GetDependentBindingClassCount() { return (int)m_vDependentClasses.size(); }
+    virtual BOOL HasDependentAcceleratorBindings() { return !m_vDependentClasses.empty(); }
+    ACCELERATOR_CLASS GetDependentAcceleratorClass(int i) { return m_vDependentClasses[(size_t)i]; }
+    BOOL GetRequestDependentPSObjects() { return m_bRequestDependentPSObjects; }
     /* Device time of the last dispatch in ms (filled when Runtime::SetProfilePSDispatch(TRUE),
      * from events around the launches, as ptask/cutask.cpp:709-776). */
     double GetLastDispatchDeviceMs() { return m_dLastDeviceMs; }
@@ -117,6 +129,8 @@ protected:
     UINT m_uiDispatchCount;
     double m_dLastDeviceMs;
     double m_dTotalDeviceMs;
+    std::vector<ACCELERATOR_CLASS> m_vDependentClasses;
+    BOOL m_bRequestDependentPSObjects = FALSE;
 };
 
 }  // namespace PTask

@@ -99,14 +99,27 @@ BOOL HostAccelerator::Compile(const char *lpszFile, const char *lpszOperation,
 HostTask::HostTask(Graph *pGraph, CompiledKernel *pKernel, const char *lpszName)
     : Task(pGraph, pKernel, ACCELERATOR_CLASS_HOST, lpszName) {}
 
+HostTask::~HostTask() { ReleaseDeviceResources(); }
+
+void HostTask::ReleaseDeviceResources() {
+    if (m_pDepContext) {
+        m_pDepAccelerator->MakeDeviceContextCurrent();
+        m_pDepContext->SynchronizeContext();
+        m_pDepContext->Release();
+        m_pDepContext = nullptr;
+    }
+}
+
 // reference: HostTask::PlatformSpecificDispatchNoDependences, hosttask.cpp:305-343 -- buffers
 // bind as host pointers, INT/FLOAT values are bit-copied into the argument slot, by-value
 // structs are passed by pointer, and the function runs once per point of the geometry.
 BOOL HostTask::PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vector<Datablock *> &vOut,
                                         UINT uiIterations) {
     Accelerator *host = HostAcceleratorInstance();
-    LPFNHOSTTASK fn = (LPFNHOSTTASK)m_pKernel->GetPlatformSpecificBinary(host);
-    if (!fn) return Runtime::HandleError("%s: host function not resolved", GetTaskName());
+    void *binary = m_pKernel->GetPlatformSpecificBinary(host);
+    if (!binary) return Runtime::HandleError("%s: host function not resolved", GetTaskName());
+    if (HasDependentAcceleratorBindings()) return DispatchWithDependences((FARPROC)binary, vIn, vOut, uiIterations);
+    LPFNHOSTTASK fn = (LPFNHOSTTASK)binary;
     std::vector<void *> args(m_vBindings.size(), nullptr);
     for (size_t a = 0; a < m_vBindings.size(); ++a) {
         Binding &b = m_vBindings[a];
@@ -135,6 +148,100 @@ BOOL HostTask::PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vect
     const long long reps = (long long)std::max(1, m_tgx) * std::max(1, m_tgy) * std::max(1, m_tgz);
     for (UINT it = 0; it < uiIterations; ++it)
         for (long long r = 0; r < reps; ++r) fn((UINT)args.size(), args.data());
+    return TRUE;
+}
+
+// reference: HostTask::PlatformSpecificDispatchWithDependences(Ex), hosttask.cpp:357-563.  Arguments
+// of ports bound to the dependent accelerator are that device's pointers (the block is made valid
+// there, in order after its producers on the task's dependent stream); the others bind as in a
+// plain host task.  EX form (platform objects requested): the function gets the stream and the
+// runtime orders later readers after the work it enqueued there.  Legacy form: the stream is
+// drained before the call and the device after it, so the function may use the device freely.
+BOOL HostTask::DispatchWithDependences(FARPROC binary, std::vector<Datablock *> &vIn,
+                                       std::vector<Datablock *> &vOut, UINT uiIterations) {
+    if (!m_pDepAccelerator) {
+        m_pDepAccelerator = DefaultHIPAccelerator();
+        if (!m_pDepAccelerator) return Runtime::HandleError("%s: no HIP accelerator for the dependent binding", GetTaskName());
+    }
+    m_pDepAccelerator->MakeDeviceContextCurrent();
+    if (!m_pDepContext) {
+        m_pDepContext = static_cast<HIPAsyncContext *>(m_pDepAccelerator->CreateAsyncContext(ASYNCCTXT_TASK));
+        if (!m_pDepContext) return Runtime::HandleError("%s: dependent stream creation failed", GetTaskName());
+    }
+    const size_t nArgs = m_vBindings.size();
+    if (nArgs > (size_t)MAXARGS) return Runtime::HandleError("%s: %zu arguments, at most %d", GetTaskName(), nArgs, MAXARGS);
+    std::vector<void *> args(nArgs, nullptr), devBindings(nArgs, nullptr);
+    std::vector<BOOL> isDep(nArgs, FALSE);
+    std::vector<Datablock *> blocks(nArgs, nullptr);
+    struct Touch { PBuffer *buf; BOOL write; };
+    std::vector<Touch> touched;
+    void *device = (void *)(intptr_t)m_pDepAccelerator->GetDeviceOrdinal();
+    for (size_t a = 0; a < nArgs; ++a) {
+        Binding &b = m_vBindings[a];
+        Datablock *blk = b.bInput ? vIn[b.uiSlot] : vOut[b.uiSlot];
+        if (!blk) return Runtime::HandleError("%s: no block for argument %zu", GetTaskName(), a);
+        blocks[a] = blk;
+        const bool buffer = b.eType != PTPARM_INT && b.eType != PTPARM_FLOAT && b.eType != PTPARM_BYVALSTRUCT;
+        if (buffer && b.pPort->HasDependentAcceleratorBinding()) {
+            void *dptr = blk->AcquireDeviceView(m_pDepAccelerator, m_pDepContext, b.bWritable, b.bInput ? TRUE : FALSE);
+            if (!dptr) return Runtime::HandleError("%s: no device view for %s", GetTaskName(), b.pPort->GetVariableBinding());
+            args[a] = dptr;
+            isDep[a] = TRUE;
+            devBindings[a] = device;
+            touched.push_back({blk->GetPlatformBuffer(m_pDepAccelerator->GetMemorySpaceId()), b.bWritable});
+            continue;
+        }
+        void *p = blk->AcquireHostView(b.bWritable, b.bInput ? TRUE : FALSE);
+        switch (b.eType) {
+            case PTPARM_INT: {
+                int v = 0;
+                memcpy(&v, p, sizeof(int));
+                args[a] = (void *)(intptr_t)v;
+                break;
+            }
+            case PTPARM_FLOAT: {
+                float f = 0.f;
+                memcpy(&f, p, sizeof(float));
+                uintptr_t bits = 0;
+                memcpy(&bits, &f, sizeof(float));
+                args[a] = (void *)bits;
+                break;
+            }
+            default: args[a] = p; break;
+        }
+    }
+    for (Touch &t : touched) t.buf->WaitOutstanding(m_pDepContext, t.write);
+    void *deps[1] = {device};
+    void *streams[1] = {m_bRequestDependentPSObjects ? (void *)m_pDepContext->GetStream() : nullptr};
+    const long long reps = (long long)std::max(1, m_tgx) * std::max(1, m_tgy) * std::max(1, m_tgz);
+    if (m_bRequestDependentPSObjects) {
+        DEPENDENTCONTEXT ctx;
+        memset(&ctx, 0, sizeof(ctx));
+        ctx.cbDependentContext = sizeof(ctx);
+        ctx.nArguments = (UINT)nArgs;
+        ctx.nDependentAccelerators = 1;
+        ctx.ppArguments = args.data();
+        ctx.ppDatablocks = blocks.data();
+        ctx.pbIsDependentBinding = isDep.data();
+        ctx.pvDeviceBindings = devBindings.data();
+        ctx.pDependentDevices = deps;
+        ctx.pStreams = streams;
+        ctx.lpszTaskName = const_cast<char *>(GetTaskName());
+        LPFNDEPHOSTTASKEX fn = (LPFNDEPHOSTTASKEX)binary;
+        for (UINT it = 0; it < uiIterations; ++it)
+            for (long long r = 0; r < reps; ++r) fn(&ctx);
+        SyncPoint *sp = m_pDepContext->CreateSyncPoint();
+        if (!sp) return FALSE;
+        for (Touch &t : touched) t.buf->AddOutstanding(sp, t.write);
+        sp->Release();
+        if (RT().forceSynchronous) m_pDepContext->SynchronizeContext();
+    } else {
+        if (!m_pDepContext->SynchronizeContext()) return FALSE;
+        LPFNDEPHOSTTASK fn = (LPFNDEPHOSTTASK)binary;
+        for (UINT it = 0; it < uiIterations; ++it)
+            for (long long r = 0; r < reps; ++r) fn((UINT)nArgs, args.data(), isDep.data(), devBindings.data(), 1, deps);
+        if (!CheckHip(ptk_hip_device_synchronize(), "device synchronize after a dependent host task")) return FALSE;
+    }
     return TRUE;
 }
 

@@ -52,6 +52,24 @@ BOOL Task::SetAffinity(Accelerator *pAccelerator, AFFINITYTYPE eType) {
     return TRUE;
 }
 
+// reference: Task::BindDependentAcceleratorClass (task.cpp): one instance per class, recorded
+// for the dispatch; the dependent accelerator is the HIP device (CUDA requests map to it).
+BOOL Task::BindDependentAcceleratorClass(ACCELERATOR_CLASS cls, int nRequiredInstances,
+                                         BOOL bRequestDependentPSObjects) {
+    if (cls == ACCELERATOR_CLASS_CUDA) cls = ACCELERATOR_CLASS_HIP;
+    if (m_eClass != ACCELERATOR_CLASS_HOST)
+        return Runtime::HandleError("%s: dependent accelerators are for host tasks", GetTaskName());
+    if (cls != ACCELERATOR_CLASS_HIP)
+        return Runtime::HandleError("%s: dependent accelerator class %d has no backend here", GetTaskName(), (int)cls);
+    if (nRequiredInstances != 1 ||
+        std::find(m_vDependentClasses.begin(), m_vDependentClasses.end(), cls) != m_vDependentClasses.end())
+        return Runtime::HandleError("%s: malformed dependent binding: %d instance(s), at most one per class",
+                                   GetTaskName(), nRequiredInstances);
+    m_vDependentClasses.push_back(cls);
+    m_bRequestDependentPSObjects = bRequestDependentPSObjects;
+    return TRUE;
+}
+
 void Task::AttachPorts(UINT nIn, Port **ppIn, UINT nOut, Port **ppOut) {
     for (UINT i = 0; i < nIn; ++i) {
         ppIn[i]->AddRef();

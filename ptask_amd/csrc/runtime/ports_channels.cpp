@@ -29,6 +29,12 @@ Port::~Port() {
 
 BOOL Port::IsFormalParameter() { return m_uiFormalIndex != (UINT)-1; }
 
+void Port::BindDependentAccelerator(ACCELERATOR_CLASS accClass, int nIndex) {
+    m_bDependentBinding = TRUE;
+    m_eDependentClass = accClass == ACCELERATOR_CLASS_CUDA ? ACCELERATOR_CLASS_HIP : accClass;
+    m_nDependentIndex = nIndex;
+}
+
 void Port::BindChannel(Channel *pChannel) { m_vChannels.push_back(pChannel); }
 void Port::BindControlChannel(Channel *pChannel) { m_vControlChannels.push_back(pChannel); }
 

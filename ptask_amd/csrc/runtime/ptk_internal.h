@@ -41,7 +41,6 @@ private:
 // ---------------------------------------------------------------------------------------------
 // Host backend: memory space 0, host functions called in place (reference: hostaccelerator.cpp,
 // hosttask.cpp).  The oracle path of every parity check runs here.
-typedef void(__stdcall *LPFNHOSTTASK)(UINT nArguments, void **ppArguments);
 
 class HostAsyncContext : public AsyncContext {
 public:
@@ -81,13 +80,23 @@ private:
     std::map<std::string, void *> m_mLibraries;
 };
 
+class HIPAccelerator;
+class HIPAsyncContext;
+
 class HostTask : public Task {
 public:
     HostTask(Graph *pGraph, CompiledKernel *pKernel, const char *lpszName);
+    ~HostTask() override;
+    void ReleaseDeviceResources() override;
 
 protected:
     BOOL PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vector<Datablock *> &vOut,
                                   UINT uiIterations) override;
+    // dispatch with a dependent accelerator (hosttask.h forms)
+    BOOL DispatchWithDependences(FARPROC fn, std::vector<Datablock *> &vIn, std::vector<Datablock *> &vOut,
+                                 UINT uiIterations);
+    HIPAccelerator *m_pDepAccelerator = nullptr;
+    HIPAsyncContext *m_pDepContext = nullptr;
 };
 
 // ---------------------------------------------------------------------------------------------
