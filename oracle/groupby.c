@@ -179,7 +179,7 @@ void GroupbyMerge(unsigned int n_, void **a) {
     memset(S.gsum, 0, sizeof(uint64_t) * (size_t)(S.groups * KP));
     for (long long c = 0; c < S.chunks; ++c) {  /* within a merge group, chunks in row order */
         unsigned *gc = S.gcnt + (c / PTK_GB_GROUP) * KP;
-        uint64_t *gs = S.gsum + (c / PTK_GB_GROUP) * KP;
+        unsigned long long *gs = S.gsum + (c / PTK_GB_GROUP) * KP;
         const unsigned d = S.boff[c * (S.buckets + 1) + S.buckets];
         for (unsigned i = 0; i < d; ++i) {
             PTK_GB_ENTRY *e = S.entry + c * PTK_GB_CHUNK + i;

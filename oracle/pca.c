@@ -15,7 +15,9 @@
  *
  * Pinned by tests/test_oracle_pca.py: numpy's SVD at convergence on matrices with a known,
  * well-separated spectrum; the exact identity X_centred = T P^T + R; orthonormal P; and the
- * one-iteration case computed independently with numpy.
+ * one-iteration case computed independently with numpy.  The BLAS host-task forms at the end of
+ * this file are pinned by tests/test_pca_blas_graph.py: the reference's unrolled graph of them
+ * must equal ptk_oracle_pca_gs bit for bit, with and without centring.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -254,4 +256,85 @@ void PcaScaleT(unsigned int n, void **a) {
             float *t = T + (size_t)k * d->Mp + m;
             *t = (float)((double)*t * PCA_L(S, d)[k]);
         }
+}
+
+/* ---- host-task forms of the reference PCA graph's BLAS tasks (ptaskUnitTest/graphcupca.cpp:294-299)
+ * with the by-value parameter structs of graphcupca.cpp:161-220 (passed by pointer to a host
+ * task); the same operations as pca_cpu's cpublasD* routines (pca_cpu.cpp:147-265). */
+typedef struct { int N, OFFSETX, OFFSETY, INCX, INCY; } PTK_DCOPY;
+typedef struct { int N; double ALPHA; int OFFSETX, OFFSETY, INCX, INCY; } PTK_DAXPY;
+typedef struct { _Bool TRANSPOSE; int M, N; double ALPHA, BETA; int LDA, OFFSETX, OFFSETY, INCX, INCY; } PTK_DGEMV;
+typedef struct { int N, OFFSETX, OFFSETY, INCX; } PTK_DNRM2;
+typedef struct { int N; double ALPHA_CONST; int INCX, OFFSETX, OFFSETALPHA_VAR; _Bool INVERT_ALPHA_VAR; } PTK_DSCAL;
+typedef struct { int OFFSETY, INCX, INCY, LDA, M, N, OFFSETALPHA_VAR, OFFSETX; } PTK_DGER;
+
+void ptaskDcopy(unsigned int n, void **a) {
+    (void)n;
+    const double *X = (const double *)a[0];
+    double *Y = (double *)a[1];
+    const PTK_DCOPY *p = (const PTK_DCOPY *)a[2];
+    for (long long i = 0; i < p->N; ++i) Y[p->OFFSETY + i * p->INCY] = X[p->OFFSETX + i * p->INCX];
+}
+
+void ptaskDaxpy(unsigned int n, void **a) {
+    (void)n;
+    const double *X = (const double *)a[0];
+    double *Y = (double *)a[1];
+    const PTK_DAXPY *p = (const PTK_DAXPY *)a[2];
+    for (long long i = 0; i < p->N; ++i) Y[p->OFFSETY + i * p->INCY] += p->ALPHA * X[p->OFFSETX + i * p->INCX];
+}
+
+void ptaskDgemv(unsigned int n, void **a) {
+    (void)n;
+    const double *A = (const double *)a[0];
+    const PTK_DGEMV *p = (const PTK_DGEMV *)a[3];
+    const double *x = (const double *)a[1] + p->OFFSETX;
+    double *y = (double *)a[2] + p->OFFSETY;
+    const int rows = p->TRANSPOSE ? p->N : p->M;
+    for (long long row = 0; row < rows; ++row) {
+        double result = p->BETA * y[row * p->INCY], ax;
+        if (!p->TRANSPOSE) {
+            ax = A[row] * x[0];
+            for (int i = 1; i < p->N; ++i) ax += A[row + (long long)i * p->LDA] * x[(long long)i * p->INCX];
+        } else {
+            ax = A[row * p->LDA] * x[0];
+            for (int i = 1; i < p->M; ++i) ax += A[i + row * p->LDA] * x[(long long)i * p->INCX];
+        }
+        result += p->ALPHA * ax;
+        y[row * p->INCY] = result;
+    }
+}
+
+void ptaskDnrm2(unsigned int n, void **a) {
+    (void)n;
+    const double *X = (const double *)a[0];
+    double *Y = (double *)a[1];
+    const PTK_DNRM2 *p = (const PTK_DNRM2 *)a[2];
+    double sum = 0.0;
+    for (int i = 0; i < p->N; ++i) {
+        const double v = X[p->OFFSETX + (long long)i * p->INCX];
+        sum += v * v;
+    }
+    Y[p->OFFSETY] = sqrt(sum);
+}
+
+void ptaskDscal(unsigned int n, void **a) {
+    (void)n;
+    const double *V = (const double *)a[0];
+    double *X = (double *)a[1];
+    const PTK_DSCAL *p = (const PTK_DSCAL *)a[2];
+    const double s = p->INVERT_ALPHA_VAR ? 1.0 / V[p->OFFSETALPHA_VAR] : V[p->OFFSETALPHA_VAR];
+    const double alpha = p->ALPHA_CONST * s;
+    for (long long i = 0; i < p->N; ++i) X[p->OFFSETX + i * p->INCX] *= alpha;
+}
+
+void ptaskDger(unsigned int n, void **a) {
+    (void)n;
+    const double *V = (const double *)a[0], *X = (const double *)a[1], *Y = (const double *)a[2];
+    double *A = (double *)a[3];
+    const PTK_DGER *p = (const PTK_DGER *)a[4];
+    const double alpha = -V[p->OFFSETALPHA_VAR];
+    for (long long col = 0; col < p->N; ++col)
+        for (long long row = 0; row < p->M; ++row)
+            A[col * p->LDA + row] += alpha * X[p->OFFSETX + row * p->INCX] * Y[p->OFFSETY + col * p->INCY];
 }

@@ -368,3 +368,4 @@ PcaScaleT(float *__restrict__ T, float *__restrict__ S, PCA_DIMS d) {
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
         T[i] = (float)((double)T[i] * L[i / d.Mp]);
 }
+#include "pca_blas.h"
