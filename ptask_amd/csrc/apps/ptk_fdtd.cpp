@@ -84,6 +84,9 @@ struct Options {
     std::string streams = "task";  // HIP stream policy: one per task, or one shared
     std::string serialize;    // write the graph as XML (and stop, unless it was read from XML)
     std::string deserialize;  // read the graph from XML instead of building it
+    int graphs = 1;           // independent graphs run at once (graph g: seed + g, priority g)
+    std::string priorities;   // comma-separated task priority per graph (default: all equal)
+    std::string sched = "priority";  // dispatch admission across graphs: priority | fifo
 };
 
 bool Parse(int argc, char **argv, Options &o) {
@@ -122,6 +125,9 @@ bool Parse(int argc, char **argv, Options &o) {
         else if (a == "--streams") o.streams = next("--streams");
         else if (a == "--serialize") o.serialize = next("--serialize");
         else if (a == "--deserialize") o.deserialize = next("--deserialize");
+        else if (a == "--graphs") o.graphs = atoi(next("--graphs"));
+        else if (a == "--priorities") o.priorities = next("--priorities");
+        else if (a == "--sched") o.sched = next("--sched");
         else {
             fprintf(stderr, "unknown option %s\n", a.c_str());
             return false;
@@ -143,7 +149,8 @@ bool Parse(int argc, char **argv, Options &o) {
         fprintf(stderr, "the overlap forms need (nx+1)(ny+1)(nz+1) < 2^29\n");
         return false;
     }
-    return (!o.kernelFile.empty() || !o.deserialize.empty()) && o.nx > 0 && o.ny > 0 && o.nz > 0 && o.steps > 0;
+    return (!o.kernelFile.empty() || !o.deserialize.empty()) && o.nx > 0 && o.ny > 0 && o.nz > 0 && o.steps > 0 &&
+           o.graphs >= 1 && o.graphs <= 64;
 }
 
 uint64_t Fnv1a(const void *p, size_t n) {
@@ -452,6 +459,110 @@ struct FdtdGraph {
     }
 };
 
+// --graphs G: G independent graphs of the same form live at once (the concurrent-graph scenario
+// of rttests.cpp:936-1333 and the per-instance priorities of graphbench.cpp:752-845).  Graph g
+// starts from seed + g and its tasks run at priority priorities[g]; every graph's inputs are
+// pushed before any output is pulled, so their dispatches contend for the device together.
+// Dumps go to <dump>.<g>.
+int RunConcurrent(const Options &o) {
+    std::vector<int> prio(o.graphs, 0);
+    if (!o.priorities.empty()) {
+        size_t i = 0;
+        for (int g = 0; g < o.graphs; ++g) {
+            size_t j = o.priorities.find(',', i);
+            prio[g] = atoi(o.priorities.substr(i, j == std::string::npos ? std::string::npos : j - i).c_str());
+            if (j == std::string::npos) {
+                for (int h = g + 1; h < o.graphs; ++h) prio[h] = prio[g];
+                break;
+            }
+            i = j + 1;
+        }
+    }
+    std::vector<FdtdGraph> G(o.graphs);
+    for (int g = 0; g < o.graphs; ++g) {
+        G[g].o = o;
+        G[g].o.seed = o.seed + (unsigned long long)g;
+        if (!G[g].Build()) {
+            fprintf(stderr, "graph %d construction failed: %s\n", g, Runtime::GetLastErrorText());
+            return 1;
+        }
+        for (UINT t = 0; t < G[g].g->GetTaskCount(); ++t) G[g].g->GetTaskByIndex(t)->SetPriority(prio[g]);
+    }
+    FDTD_PARAMS prm = ptk_fdtd_make_params(o.nx, o.ny, o.nz, o.courant);
+    for (auto &fg : G) {
+        fg.g->Run(FALSE);
+        Datablock *cfg = Runtime::AllocateDatablock(fg.params, &prm, sizeof(prm), nullptr);
+        for (GraphInputChannel *c : fg.paramInputs) c->Push(cfg);
+        cfg->Release();
+    }
+    std::vector<std::vector<std::vector<float>>> fin(o.graphs, std::vector<std::vector<float>>(6));
+    std::vector<unsigned long long> disp0(o.graphs);
+    CHighResolutionTimer timer(gran_msec);
+    for (int g = 0; g < o.graphs; ++g) {
+        FdtdGraph &fg = G[g];
+        disp0[g] = fg.g->GetDispatchCount();
+        for (int f = 0; f < 6; ++f) {
+            std::vector<float> init((size_t)ptk_fdtd_field_elems(f, o.nx, o.ny, o.nz));
+            for (size_t n = 0; n < init.size(); ++n) init[n] = ptk_rand_sym_f32(fg.o.seed, (uint64_t)f, n);
+            fin[g][f].resize(init.size());
+            Datablock *db = Runtime::AllocateDatablock(fg.field[f], init.data(), (UINT)(init.size() * sizeof(float)), nullptr);
+            for (GraphInputChannel *c : fg.fieldInputs[f]) c->Push(db);
+            db->Release();
+        }
+        if (o.mode == "looped") {
+            int iters = o.steps / StepsPerTask(o.form);
+            Datablock *it = Runtime::AllocateDatablock(fg.iters, &iters, sizeof(int), nullptr);
+            for (GraphInputChannel *c : fg.iterInputs) c->Push(it);
+            it->Release();
+        }
+    }
+    bool ok = true;
+    for (int g = 0; g < o.graphs && ok; ++g)
+        for (int f = 0; f < 6; ++f) {
+            Datablock *res = G[g].outputs[f]->Pull(600000);
+            if (!res) {
+                fprintf(stderr, "graph %d: no output on %s: %s\n", g, kFieldNames[f], Runtime::GetLastErrorText());
+                ok = false;
+                break;
+            }
+            res->Lock();
+            const float *p = (const float *)res->GetDataPointer(FALSE);
+            fin[g][f].assign(p, p + fin[g][f].size());
+            res->Unlock();
+            res->Release();
+        }
+    const double ms = timer.elapsed(false);
+    std::string js;
+    for (int g = 0; g < o.graphs; ++g) {
+        FdtdGraph &fg = G[g];
+        char buf[160];
+        snprintf(buf, sizeof(buf), "%s{\"graph\": %d, \"seed\": %llu, \"priority\": %d, \"dispatches\": %llu, \"fnv1a\": [",
+                 g ? ", " : "", g, fg.o.seed, prio[g], (unsigned long long)(fg.g->GetDispatchCount() - disp0[g]));
+        js += buf;
+        for (int f = 0; f < 6; ++f) {
+            snprintf(buf, sizeof(buf), "%s\"%016llx\"", f ? ", " : "",
+                     (unsigned long long)Fnv1a(fin[g][f].data(), fin[g][f].size() * sizeof(float)));
+            js += buf;
+        }
+        js += "]}";
+        fg.g->Stop();
+        fg.g->Teardown();
+        Graph::DestroyGraph(fg.g);
+        if (ok && !o.dump.empty()) {
+            FILE *fp = fopen((o.dump + "." + std::to_string(g)).c_str(), "wb");
+            for (int f = 0; fp && f < 6; ++f) fwrite(fin[g][f].data(), sizeof(float), fin[g][f].size(), fp);
+            if (fp) fclose(fp);
+        }
+    }
+    printf("{\"ok\": %s, \"mode\": \"%s\", \"form\": \"%s\", \"backend\": \"%s\", \"grid\": [%d, %d, %d], "
+           "\"steps\": %d, \"graphs\": %d, \"sched\": \"%s\", \"ms\": %.4f, \"results\": [%s]}\n",
+           ok ? "true" : "false", o.mode.c_str(), o.form.c_str(), G[0].hip ? "hip" : "host", o.nx, o.ny, o.nz,
+           o.steps, o.graphs, o.sched.c_str(), ms, js.c_str());
+    fflush(stdout);
+    Runtime::Terminate();
+    return ok ? 0 : 1;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -460,7 +571,8 @@ int main(int argc, char **argv) {
         fprintf(stderr, "usage: ptk_fdtd --kernels <fdtd.hsaco|hosttasks.so> [--n N | --nx --ny --nz] "
                         "[--steps S] [--mode looped|unrolled] [--blocks B] [--repeat R] [--seed S] "
                         "[--form six|halves|fused|overlap|overlap2|overlap3|overlap3p] [--geometry default|reference|compute] [--tile IxJxK] [--dump file] "
-                        "[--eager] [--profile] [--chain] [--serialize graph.xml] [--deserialize graph.xml]\n");
+                        "[--eager] [--profile] [--chain] [--serialize graph.xml] [--deserialize graph.xml] "
+                        "[--graphs G [--priorities p0,p1,...] [--sched priority|fifo]]\n");
         return 2;
     }
     Runtime::SetExitOnRuntimeFailure(FALSE);
@@ -469,10 +581,19 @@ int main(int argc, char **argv) {
     if (o.profile) Runtime::SetProfilePSDispatch(TRUE);
     Runtime::SetTaskStreamPolicy(o.streams == "shared" ? PTK_STREAMS_SHARED : PTK_STREAMS_PER_TASK);
     if (o.mode == "unrolled") Runtime::SetDefaultChannelCapacity(std::max(4, o.blocks));
+    if (o.graphs > 1) {
+        if (!o.deserialize.empty() || !o.serialize.empty() || o.chain || o.blocks != 1 ||
+            (o.sched != "priority" && o.sched != "fifo")) {
+            fprintf(stderr, "--graphs runs built graphs once each: no --chain, --blocks, --serialize or --deserialize\n");
+            return 2;
+        }
+        Runtime::SetSchedulingMode(o.sched == "fifo" ? SCHEDMODE_FIFO : SCHEDMODE_PRIORITY);
+    }
     if (PTFAILED(Runtime::Initialize())) {
         fprintf(stderr, "Runtime::Initialize failed\n");
         return 1;
     }
+    if (o.graphs > 1) return RunConcurrent(o);
     FdtdGraph G;
     G.o = o;
     if (!(o.deserialize.empty() ? G.Build() : G.Load())) {
