@@ -60,10 +60,15 @@ Port *CreatePort(PORTTYPE type, DatablockTemplate *pTemplate, UINT uiId,
                  char *lpszVariableBinding = NULL, UINT nKernelParameterIndex = (UINT)-1,
                  UINT nInOutParmOutputIndex = (UINT)-1);
 PTRESULT SetTaskAffinity(Task *pTask, Accelerator *pAccelerator, AFFINITYTYPE eAffinityType);
+/* By accelerator id (ACCELERATOR_DESCRIPTOR::uiAcceleratorId) among the accelerators of the task's
+ * class; PTASK_ERR_INVALID_PARAMETER for an id of no such accelerator (reference
+ * PTaskRuntime.h:472-502, both forms). */
+PTRESULT SetTaskAffinity(Task *pTask, UINT uiAcceleratorId, AFFINITYTYPE eAffinityType);
+PTRESULT SetTaskAffinity(Task *pTask, UINT *pvAcceleratorIds, AFFINITYTYPE *pvAffinityTypes, UINT nAcceleratorIds);
 
 /* ---- accelerators --------------------------------------------------------------------------- */
 /* The uiEnumerationIndex-th enabled accelerator of a class (reference:
- * Runtime::EnumerateAccelerators); the descriptor is owned by the caller (delete). */
+ * Runtime::EnumerateAccelerators); the descriptor is malloc'd and owned by the caller (free). */
 PTRESULT EnumerateAccelerators(ACCELERATOR_CLASS eClass, UINT uiEnumerationIndex,
                                ACCELERATOR_DESCRIPTOR **ppDescriptor);
 Accelerator *GetAccelerator(ACCELERATOR_CLASS eClass, UINT uiIndex);

@@ -468,6 +468,32 @@ PTRESULT SetTaskAffinity(Task *pTask, Accelerator *pAccelerator, AFFINITYTYPE eA
     return pTask->SetAffinity(pAccelerator, eAffinityType) ? PTASK_OK : PTASK_ERR;
 }
 
+// the accelerator of the task's class with this id (ids as EnumerateAccelerators reports them)
+static Accelerator *AcceleratorById(Task *pTask, UINT uiAcceleratorId) {
+    const ACCELERATOR_CLASS c = pTask->GetAcceleratorClass();
+    for (UINT i = 0; i < GetAcceleratorCount(c); ++i) {
+        Accelerator *a = GetAccelerator(c, i);
+        if (a && a->GetAcceleratorId() == uiAcceleratorId) return a;
+    }
+    return nullptr;
+}
+
+PTRESULT SetTaskAffinity(Task *pTask, UINT uiAcceleratorId, AFFINITYTYPE eAffinityType) {
+    if (!pTask) return PTASK_ERR_INVALID_PARAMETER;
+    Accelerator *a = AcceleratorById(pTask, uiAcceleratorId);
+    if (!a) return PTASK_ERR_INVALID_PARAMETER;
+    return SetTaskAffinity(pTask, a, eAffinityType);
+}
+
+PTRESULT SetTaskAffinity(Task *pTask, UINT *pvAcceleratorIds, AFFINITYTYPE *pvAffinityTypes, UINT nAcceleratorIds) {
+    if (!pTask || (nAcceleratorIds && (!pvAcceleratorIds || !pvAffinityTypes))) return PTASK_ERR_INVALID_PARAMETER;
+    for (UINT i = 0; i < nAcceleratorIds; ++i) {
+        PTRESULT r = SetTaskAffinity(pTask, pvAcceleratorIds[i], pvAffinityTypes[i]);
+        if (PTFAILED(r)) return r;
+    }
+    return PTASK_OK;
+}
+
 static std::vector<Accelerator *> *ListFor(ACCELERATOR_CLASS eClass) {
     if (eClass == ACCELERATOR_CLASS_HIP) return &RT().hipAccelerators;
     if (eClass == ACCELERATOR_CLASS_HOST) return &RT().hostAccelerators;
@@ -480,7 +506,9 @@ PTRESULT EnumerateAccelerators(ACCELERATOR_CLASS eClass, UINT uiEnumerationIndex
     *ppDescriptor = nullptr;
     Accelerator *a = GetAccelerator(eClass, uiEnumerationIndex);
     if (!a) return PTASK_ERR_NOT_FOUND;
-    ACCELERATOR_DESCRIPTOR *d = new ACCELERATOR_DESCRIPTOR();
+    // malloc'd, as the reference's callers free() it (PTaskRuntime.cpp:1140, graphgroupby.cpp:538)
+    ACCELERATOR_DESCRIPTOR *d = (ACCELERATOR_DESCRIPTOR *)calloc(1, sizeof(ACCELERATOR_DESCRIPTOR));
+    if (!d) return PTASK_ERR;
     a->GetDescriptor(d);
     *ppDescriptor = d;
     return PTASK_OK;
