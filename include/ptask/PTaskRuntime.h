@@ -60,6 +60,32 @@ Port *CreatePort(PORTTYPE type, DatablockTemplate *pTemplate, UINT uiId,
                  char *lpszVariableBinding = NULL, UINT nKernelParameterIndex = (UINT)-1,
                  UINT nInOutParmOutputIndex = (UINT)-1);
 PTRESULT SetTaskAffinity(Task *pTask, Accelerator *pAccelerator, AFFINITYTYPE eAffinityType);
+
+/* ---- block pools (reference PTaskRuntime.h:111-112, GlobalPoolManager.h) ------------------
+ * Runtime-wide pools of pre-allocated blocks (host storage, and device storage on the first HIP
+ * accelerator): typed by template, or untyped by data size.  AllocateDatablock takes a free block
+ * from the template's pool, else from the smallest untyped pool that holds the size; released
+ * blocks go back.  Called before Initialize, the pools are allocated by Initialize; after, at
+ * once.  nBlocks 0: the default block pool size. */
+void RequireBlockPool(int nDataSize, int nMetaSize, int nTemplateSize, int nBlocks = 0);
+void RequireBlockPool(DatablockTemplate *pTemplate, int nBlocks);
+
+/* ---- memory statistics and failure injection (no counterpart in the reference) -----------
+ * A host or device allocation that fails makes the runtime free the free blocks of every pool
+ * and try again, then once more after every accelerator has finished its work.  These count
+ * what happened; SetAllocationFailureInjection makes the next nFailures allocation attempts in a
+ * memory space (0 host, 1 the HIP device) fail, for tests of that path. */
+typedef struct _ptk_memory_statistics {
+    unsigned long long ullAllocationFailures;      /* allocations whose first attempt failed */
+    unsigned long long ullAllocationsRecovered;    /* of those, allocations a retry satisfied */
+    unsigned long long ullAllocationsUnrecovered;  /* of those, allocations that failed for good */
+    unsigned long long ullBytesReclaimed;          /* storage freed from pools for retries */
+    unsigned long long ullPooledBlockHits;         /* pooled blocks handed out by AllocateDatablock / RequestPooledBlock */
+    unsigned long long ullPooledBlockMisses;       /* requests a pool could not serve from its free blocks */
+} PTK_MEMORY_STATISTICS;
+void GetMemoryStatistics(PTK_MEMORY_STATISTICS *pStatistics);
+void ResetMemoryStatistics();
+void SetAllocationFailureInjection(UINT uiMemorySpaceId, int nFailures);
 /* By accelerator id (ACCELERATOR_DESCRIPTOR::uiAcceleratorId) among the accelerators of the task's
  * class; PTASK_ERR_INVALID_PARAMETER for an id of no such accelerator (reference
  * PTaskRuntime.h:472-502, both forms). */

@@ -71,12 +71,17 @@ public:
     /* Makes the block a copy-free alias target: called when an output block is recycled so a
      * later writer does not trigger a pointless host->device population. */
     void MarkDeviceOnly(UINT uiMemSpace);
+    /* Allocates the block's storage on pAcc now, leaving every view's coherence state as it is
+     * (pre-allocated pool blocks); FALSE when the allocation failed. */
+    BOOL ReserveDeviceBuffer(Accelerator *pAcc);
+    /* Bytes of host and device storage the block holds. */
+    size_t GetFootprintBytes();
 
 protected:
     void OnLastRelease() override;
 
 private:
-    void EnsureHostBuffer();
+    BOOL EnsureHostBuffer();
     void SyncHostFromDevice(UINT uiMemSpace);
 
     DatablockTemplate *m_pTemplate;

@@ -24,6 +24,7 @@ namespace PTask {
 
 class CompiledKernel;
 class Task;
+class BlockPool;
 class Port;
 class Datablock;
 
@@ -113,8 +114,24 @@ public:
     void NotifyRoom() { m_cvRoom.notify_all(); }
     BOOL IsAlive() { return m_eState != PTGS_TEARINGDOWN && m_eState != PTGS_TEARDOWNCOMPLETE; }
 
+    /* ---- graph-scoped block pools (reference graph.h:139-199, ScopedPoolManager) ------------
+     * Pre-allocated blocks (host storage, and device storage when a HIP accelerator exists) that
+     * RequestPooledBlock hands out and that return to the pool on their last Release; released
+     * at Teardown.  nBlocks 0: the runtime's default block pool size. */
+    BOOL RequireBlockPool(DatablockTemplate *pTemplate, int nDataSize, int nMetaSize, int nTemplateSize,
+                          int nBlocks = 0);
+    BOOL RequireBlockPool(int nDataSize, int nMetaSize, int nTemplateSize, int nBlocks = 0);
+    BOOL RequireBlockPool(DatablockTemplate *pTemplate, int nBlocks = 0);
+    /* A free block of the template's pool (else of the smallest untyped pool holding uiDataSize
+     * bytes), with one reference; NULL when none is free. */
+    Datablock *RequestPooledBlock(DatablockTemplate *pTemplate, UINT uiDataSize, UINT uiMetaSize,
+                                  UINT uiTemplateSize);
+
 private:
     void RunnerProc(UINT uiIndex);
+    void ReleaseScopedPools();
+    std::map<DatablockTemplate *, BlockPool *> m_mTypedPools;
+    std::map<UINT, BlockPool *> m_mUntypedPools;
     void RegisterChannel(Channel *pChannel);
     std::string SerializeLocked();
 

@@ -84,6 +84,9 @@ struct Options {
     std::string streams = "task";  // HIP stream policy: one per task, or one shared
     std::string serialize;    // write the graph as XML (and stop, unless it was read from XML)
     std::string deserialize;  // read the graph from XML instead of building it
+    std::string pool;         // input blocks from a runtime-wide ("global") or graph ("graph") pool
+    int poolBlocks = 6;       // blocks per field pool
+    int injectHost = 0, injectDevice = 0;  // allocation failures forced before every run after the first
     int graphs = 1;           // independent graphs run at once (graph g: seed + g, priority g)
     std::string priorities;   // comma-separated task priority per graph (default: all equal)
     std::string sched = "priority";  // dispatch admission across graphs: priority | fifo
@@ -125,6 +128,10 @@ bool Parse(int argc, char **argv, Options &o) {
         else if (a == "--streams") o.streams = next("--streams");
         else if (a == "--serialize") o.serialize = next("--serialize");
         else if (a == "--deserialize") o.deserialize = next("--deserialize");
+        else if (a == "--pool") o.pool = next("--pool");
+        else if (a == "--pool-blocks") o.poolBlocks = atoi(next("--pool-blocks"));
+        else if (a == "--inject-host-failures") o.injectHost = atoi(next("--inject-host-failures"));
+        else if (a == "--inject-device-failures") o.injectDevice = atoi(next("--inject-device-failures"));
         else if (a == "--graphs") o.graphs = atoi(next("--graphs"));
         else if (a == "--priorities") o.priorities = next("--priorities");
         else if (a == "--sched") o.sched = next("--sched");
@@ -150,7 +157,8 @@ bool Parse(int argc, char **argv, Options &o) {
         return false;
     }
     return (!o.kernelFile.empty() || !o.deserialize.empty()) && o.nx > 0 && o.ny > 0 && o.nz > 0 && o.steps > 0 &&
-           o.graphs >= 1 && o.graphs <= 64;
+           o.graphs >= 1 && o.graphs <= 64 && (o.pool.empty() || o.pool == "global" || o.pool == "graph") &&
+           o.poolBlocks >= 1;
 }
 
 uint64_t Fnv1a(const void *p, size_t n) {
@@ -600,6 +608,16 @@ int main(int argc, char **argv) {
         fprintf(stderr, "graph construction failed: %s\n", Runtime::GetLastErrorText());
         return 1;
     }
+    if (o.pool == "global") {
+        for (int f = 0; f < 6; ++f) Runtime::RequireBlockPool(G.field[f], o.poolBlocks);
+    } else if (o.pool == "graph") {
+        for (int f = 0; f < 6; ++f)
+            if (!G.g->RequireBlockPool(G.field[f], o.poolBlocks)) {
+                fprintf(stderr, "graph block pool: %s\n", Runtime::GetLastErrorText());
+                return 1;
+            }
+    }
+    Runtime::ResetMemoryStatistics();
     if (!o.serialize.empty()) {
         // write the graph (built here or read from XML) and stop
         BOOL ok = G.g->Serialize(o.serialize.c_str());
@@ -651,6 +669,10 @@ int main(int argc, char **argv) {
             chainTimer.reset();
         }
         Runtime::ResetTransferStatistics();
+        if (r > 0) {
+            Runtime::SetAllocationFailureInjection(HOST_MEMORY_SPACE_ID, o.injectHost);
+            if (hipAcc) Runtime::SetAllocationFailureInjection(hipAcc->GetMemorySpaceId(), o.injectDevice);
+        }
         dispatch0 = G.g->GetDispatchCount();
         timer.reset();
         for (int b = 0; b < blocks; ++b) {
@@ -658,14 +680,25 @@ int main(int argc, char **argv) {
                 Datablock *db = nullptr;
                 if (!carried.empty()) {
                     db = carried[f];
+                } else if (o.pool == "graph") {
+                    db = G.g->RequestPooledBlock(G.field[f], (UINT)(init[f].size() * sizeof(float)), 0, 0);
+                    void *dst = db ? db->GetDataPointer(TRUE) : nullptr;
+                    if (dst) memcpy(dst, init[f].data(), init[f].size() * sizeof(float));
+                    else if (db) db->Release(), db = nullptr;
                 } else {
                     db = Runtime::AllocateDatablock(G.field[f], init[f].data(),
                                                     (UINT)(init[f].size() * sizeof(float)), nullptr);
+                }
+                if (!db) {
+                    fprintf(stderr, "run %d: no block for %s: %s\n", r, kFieldNames[f], Runtime::GetLastErrorText());
+                    ok = false;
+                    break;
                 }
                 for (GraphInputChannel *c : G.fieldInputs[f]) c->Push(db);
                 db->Release();
             }
             carried.clear();
+            if (!ok) break;
             if (o.mode == "looped") {
                 int iters = o.steps / StepsPerTask(o.form);  // graph iterations
                 Datablock *it = Runtime::AllocateDatablock(G.iters, &iters, sizeof(int), nullptr);
@@ -756,7 +789,13 @@ int main(int argc, char **argv) {
         printf("%s{\"field\": \"%s\", \"fnv1a\": \"%016llx\", \"sum\": %.17g}", f ? ", " : "", kFieldNames[f],
                (unsigned long long)Fnv1a(final[f].data(), final[f].size() * sizeof(float)), s);
     }
-    printf("], \"run_ms\": [");
+    Runtime::PTK_MEMORY_STATISTICS ms = {};
+    Runtime::GetMemoryStatistics(&ms);
+    printf("], \"memory\": {\"allocation_failures\": %llu, \"recovered\": %llu, \"unrecovered\": %llu, "
+           "\"bytes_reclaimed\": %llu, \"pool_hits\": %llu, \"pool_misses\": %llu}",
+           ms.ullAllocationFailures, ms.ullAllocationsRecovered, ms.ullAllocationsUnrecovered, ms.ullBytesReclaimed,
+           ms.ullPooledBlockHits, ms.ullPooledBlockMisses);
+    printf(", \"run_ms\": [");
     for (size_t r = 0; r < runMs.size(); ++r) printf("%s%.4f", r ? ", " : "", runMs[r]);
     printf("]}\n");
     fflush(stdout);

@@ -298,13 +298,11 @@ void HIPAsyncContext::RecycleEvent(void *pEvent) {
     m_vFreeEvents.push_back((ptk_hip_event)pEvent);
 }
 
+// A failed allocation leaves the buffer empty without reporting: the caller
+// (AllocDeviceStorage) reclaims pooled memory, retries, and reports only the final failure.
 PHIPBuffer::PHIPBuffer(Accelerator *pAcc, size_t cb) : PBuffer(pAcc, cb) {
     pAcc->MakeDeviceContextCurrent();
-    int rc = ptk_hip_malloc(&m_pBuffer, cb);
-    if (rc != PTK_HIP_OK) {
-        m_pBuffer = nullptr;
-        CheckHip(rc, "device allocation");
-    }
+    if (ptk_hip_malloc(&m_pBuffer, cb) != PTK_HIP_OK) m_pBuffer = nullptr;
 }
 
 PHIPBuffer::~PHIPBuffer() {

@@ -184,14 +184,15 @@ void Datablock::OnLastRelease() {
     else delete this;
 }
 
-void Datablock::EnsureHostBuffer() {
-    if (m_pHost) return;
+BOOL Datablock::EnsureHostBuffer() {
+    if (m_pHost) return TRUE;
     size_t cb = m_cbAllocated ? m_cbAllocated : 1;
-    // page-aligned and zeroed: a block nobody has written reads as zeros everywhere
-    void *p = nullptr;
-    if (posix_memalign(&p, 4096, (cb + 4095) & ~size_t(4095)) != 0 || !p) {
+    // page-aligned and zeroed: a block nobody has written reads as zeros everywhere; a failed
+    // allocation is retried after pooled blocks are reclaimed
+    void *p = AllocHostStorage(cb);
+    if (!p) {
         Runtime::HandleError("out of host memory allocating a %u-byte datablock", m_cbAllocated);
-        return;
+        return FALSE;
     }
     memset(p, 0, cb);
     m_pHost = (unsigned char *)p;
@@ -201,6 +202,7 @@ void Datablock::EnsureHostBuffer() {
             if (m_eState[i] == BSTATE_SHARED || m_eState[i] == BSTATE_EXCLUSIVE) anyValid = TRUE;
         m_eState[HOST_MEMORY_SPACE_ID] = anyValid ? BSTATE_INVALID : BSTATE_SHARED;
     }
+    return TRUE;
 }
 
 void Datablock::SyncHostFromDevice(UINT ms) {
@@ -216,7 +218,10 @@ void Datablock::SyncHostFromDevice(UINT ms) {
 
 void *Datablock::AcquireHostView(BOOL bWriteable, BOOL bPopulate) {
     Lock();
-    EnsureHostBuffer();
+    if (!EnsureHostBuffer()) {
+        Unlock();
+        return nullptr;
+    }
     BUFFER_COHERENCE_STATE &hs = m_eState[HOST_MEMORY_SPACE_ID];
     if (bPopulate && hs != BSTATE_SHARED && hs != BSTATE_EXCLUSIVE) {
         for (UINT ms = 1; ms < PTK_MAX_MEMORY_SPACES; ++ms) {
@@ -259,8 +264,8 @@ void *Datablock::AcquireDeviceView(Accelerator *pAcc, AsyncContext *pCtx, BOOL b
     }
     Lock();
     if (!m_pBuffers[ms]) {
-        m_pBuffers[ms] = pAcc->NewPlatformSpecificBuffer(m_cbAllocated ? m_cbAllocated : 4);
-        if (!m_pBuffers[ms] || !m_pBuffers[ms]->GetBuffer()) {
+        m_pBuffers[ms] = AllocDeviceStorage(pAcc, m_cbAllocated ? m_cbAllocated : 4);
+        if (!m_pBuffers[ms]) {
             Unlock();
             return nullptr;
         }
@@ -293,6 +298,23 @@ void *Datablock::AcquireDeviceView(Accelerator *pAcc, AsyncContext *pCtx, BOOL b
     return p;
 }
 
+BOOL Datablock::ReserveDeviceBuffer(Accelerator *pAcc) {
+    const UINT ms = pAcc ? pAcc->GetMemorySpaceId() : HOST_MEMORY_SPACE_ID;
+    if (ms == HOST_MEMORY_SPACE_ID || ms >= PTK_MAX_MEMORY_SPACES) return FALSE;
+    Lock();
+    if (!m_pBuffers[ms]) m_pBuffers[ms] = AllocDeviceStorage(pAcc, m_cbAllocated ? m_cbAllocated : 4);
+    const BOOL ok = m_pBuffers[ms] != nullptr;
+    Unlock();
+    return ok;
+}
+
+size_t Datablock::GetFootprintBytes() {
+    size_t cb = m_pHost ? m_cbAllocated : 0;
+    for (UINT i = 1; i < PTK_MAX_MEMORY_SPACES; ++i)
+        if (m_pBuffers[i]) cb += m_pBuffers[i]->GetSize();
+    return cb;
+}
+
 PBuffer *Datablock::GetPlatformBuffer(UINT ms) { return ms < PTK_MAX_MEMORY_SPACES ? m_pBuffers[ms] : nullptr; }
 void Datablock::SetPoolOwner(BlockPool *pPool) { m_pPoolOwner = pPool; }
 BlockPool *Datablock::GetPoolOwner() { return m_pPoolOwner; }
@@ -322,8 +344,8 @@ void Datablock::GrowDataBuffer(UINT uiNewSize) {
     if (uiNewSize > m_cbAllocated) {
         // grow on the host; device copies are dropped and re-created on next use
         AcquireHostView(FALSE, TRUE);
-        void *p = nullptr;
-        if (posix_memalign(&p, 4096, (uiNewSize + 4095) & ~size_t(4095)) != 0 || !p) {
+        void *p = AllocHostStorage(uiNewSize);
+        if (!p) {
             Unlock();
             Runtime::HandleError("out of host memory growing a datablock to %u bytes", uiNewSize);
             return;
@@ -379,9 +401,11 @@ BUFFER_COHERENCE_STATE Datablock::GetCoherenceState(UINT ms) {
 BlockPool::BlockPool(DatablockTemplate *pTemplate, UINT uiMaxFree)
     : m_pTemplate(pTemplate), m_uiMaxFree(uiMaxFree), m_bClosed(FALSE) {
     if (m_pTemplate) m_pTemplate->AddRef();
+    RegisterPool(this);
 }
 
 BlockPool::~BlockPool() {
+    UnregisterPool(this);  // first: a reclaim walking the registry may be waiting to visit
     for (Datablock *b : m_vFree) {
         b->SetPoolOwner(nullptr);
         delete b;
@@ -389,24 +413,73 @@ BlockPool::~BlockPool() {
     if (m_pTemplate) m_pTemplate->Release();
 }
 
-Datablock *BlockPool::Request(UINT cb) {
+Datablock *BlockPool::Request(UINT cb, BOOL bAtLeast, BOOL bCreate) {
+    Datablock *reuse = nullptr;
     {
         std::lock_guard<std::mutex> g(m_lock);
+        size_t best = m_vFree.size();
         for (size_t i = 0; i < m_vFree.size(); ++i) {
-            Datablock *b = m_vFree[i];
-            if (b->GetDataBufferAllocatedSizeBytes() == cb) {
-                m_vFree.erase(m_vFree.begin() + (long)i);
-                b->ResetForReuse();
-                b->AddRef();
-                return b;
+            const UINT have = m_vFree[i]->GetDataBufferAllocatedSizeBytes();
+            if (have == cb) {
+                best = i;
+                break;
             }
+            if (bAtLeast && have > cb &&
+                (best == m_vFree.size() || have < m_vFree[best]->GetDataBufferAllocatedSizeBytes()))
+                best = i;
+        }
+        if (best < m_vFree.size()) {
+            reuse = m_vFree[best];
+            m_vFree.erase(m_vFree.begin() + (long)best);
         }
     }
+    if (reuse) {
+        reuse->ResetForReuse();
+        if (reuse->GetDataBufferLogicalSizeBytes() != cb) reuse->GrowDataBuffer(cb);  // sets the logical size
+        reuse->AddRef();
+        return reuse;
+    }
+    if (!bCreate) return nullptr;
     Datablock *b = new Datablock(m_pTemplate, cb, 0, DBCTLC_NONE);
     b->SetPoolOwner(this);
     AddRef();  // each live pooled block keeps its pool
     b->AddRef();
     return b;
+}
+
+UINT BlockPool::Prefill(UINT n, UINT cb, UINT cbMeta, Accelerator *pDevice) {
+    UINT added = 0;
+    for (UINT i = 0; i < n; ++i) {
+        Datablock *b = new Datablock(m_pTemplate, cb, cbMeta, DBCTLC_NONE);
+        // host storage now; device storage now too, so the first dispatch allocates nothing
+        if (!b->AcquireHostView(FALSE, FALSE) || (pDevice && !b->ReserveDeviceBuffer(pDevice))) {
+            delete b;
+            break;
+        }
+        b->SetPoolOwner(this);
+        AddRef();
+        std::lock_guard<std::mutex> g(m_lock);
+        m_vFree.push_back(b);
+        ++added;
+    }
+    return added;
+}
+
+size_t BlockPool::Trim() {
+    std::vector<Datablock *> v;
+    {
+        std::lock_guard<std::mutex> g(m_lock);
+        v.swap(m_vFree);
+    }
+    size_t cb = 0;
+    for (Datablock *b : v) {
+        cb += b->GetFootprintBytes();
+        b->SetPoolOwner(nullptr);
+        delete b;
+    }
+    // the freed blocks' references on this pool go last: the caller holds one of its own
+    for (size_t i = 0; i < v.size(); ++i) Release();
+    return cb;
 }
 
 void BlockPool::Return(Datablock *pBlock) {

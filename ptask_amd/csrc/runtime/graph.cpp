@@ -625,6 +625,7 @@ void Graph::Teardown() {
     for (Channel *c : m_vChannels) c->Release();
     m_vChannels.clear();
     m_mChannels.clear();
+    ReleaseScopedPools();
     m_eState = PTGS_TEARDOWNCOMPLETE;
 }
 

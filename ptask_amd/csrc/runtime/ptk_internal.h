@@ -22,13 +22,21 @@ class BlockPool : public ReferenceCounted {
 public:
     BlockPool(DatablockTemplate *pTemplate, UINT uiMaxFree);
     ~BlockPool() override;
-    // A free block with at least cb bytes, or a new one; returned with one reference.
-    Datablock *Request(UINT cb);
+    // A free block of exactly cb bytes (bAtLeast: the smallest free block of at least cb bytes,
+    // its logical size set to cb), or a new one of cb bytes; returned with one reference.
+    // bCreate FALSE: nullptr instead of a new block.
+    Datablock *Request(UINT cb, BOOL bAtLeast = FALSE, BOOL bCreate = TRUE);
     // Called by Datablock::OnLastRelease for blocks owned by this pool.
     void Return(Datablock *pBlock);
     // No more recycling: free blocks are deleted now, returning blocks are deleted later.
     void Close();
     size_t FreeCount();
+    // Adds n free blocks of cb data bytes (and cbMeta metadata bytes) with host storage and, when
+    // pDevice is given, storage on that accelerator; returns how many were added.
+    UINT Prefill(UINT n, UINT cb, UINT cbMeta, Accelerator *pDevice);
+    // Deletes the free blocks (their host and device storage) for an allocation that failed;
+    // returns the bytes freed.  The pool stays open.
+    size_t Trim();
 
 private:
     std::mutex m_lock;
@@ -37,6 +45,23 @@ private:
     UINT m_uiMaxFree;
     BOOL m_bClosed;
 };
+
+// ---------------------------------------------------------------------------------------------
+// Memory: allocation with recovery and runtime-wide pools (pools.cpp).  A failed allocation
+// reclaims the free blocks of every live pool and tries again, then once more after every
+// accelerator has finished its work (reference: the forced collection and retry of
+// PCUBuffer.cpp:979-1015 with the collector of ptgc.cpp:326).
+void *AllocHostStorage(size_t cb);                    // page-aligned, free() releases it
+PBuffer *AllocDeviceStorage(Accelerator *pAcc, size_t cb);
+size_t ReclaimPooledMemory();                         // bytes freed from every pool's free list
+void RegisterPool(BlockPool *pPool);
+void UnregisterPool(BlockPool *pPool);
+// Runtime-wide pools of Runtime::RequireBlockPool (reference GlobalPoolManager.cpp:120-250):
+// typed by template, or untyped by data size; a request takes the template's pool, else the
+// smallest untyped pool whose blocks hold cb bytes.
+Datablock *RequestGlobalPooledBlock(DatablockTemplate *pTemplate, UINT cb, UINT cbMeta);
+void AllocateGlobalPools();
+void DestroyGlobalPools();
 
 // ---------------------------------------------------------------------------------------------
 // Host backend: memory space 0, host functions called in place (reference: hostaccelerator.cpp,
@@ -244,6 +269,10 @@ struct RuntimeState {
     // dispatch admission per accelerator class: host tasks (maxHostConcurrency slots) and HIP
     // tasks (enqueueing a launch; the device runs launched work concurrently)
     DispatchArbiter hostArbiter{4}, hipArbiter{8};
+    // allocation recovery and pools (pools.cpp)
+    std::atomic<unsigned long long> allocFailures{0}, allocRecovered{0}, allocUnrecovered{0}, bytesReclaimed{0};
+    std::atomic<unsigned long long> poolHits{0}, poolMisses{0};
+    std::atomic<int> injectedFailures[PTK_MAX_MEMORY_SPACES] = {};
     // templates used for descriptor blocks
     DatablockTemplate *sizeDescriptorTemplate = nullptr;
     DatablockTemplate *controlTemplate = nullptr;

@@ -260,6 +260,7 @@ PTRESULT Initialize() {
     S.controlTemplate = new DatablockTemplate("__ptk_control", sizeof(CONTROLSIGNAL), PTPARM_BYVALSTRUCT);
     S.controlTemplate->AddRef();
     S.initialized = true;
+    AllocateGlobalPools();
     return PTASK_OK;
 }
 
@@ -269,6 +270,7 @@ PTRESULT Terminate() {
     RuntimeState &S = RT();
     std::lock_guard<std::recursive_mutex> g(S.lock);
     if (!S.initialized) return PTASK_ERR_UNINITIALIZED;
+    DestroyGlobalPools();  // while the accelerators that hold their device storage are open
     for (CompiledKernel *k : S.kernels) delete k;
     S.kernels.clear();
     for (auto &kv : S.templates) kv.second->Release();
@@ -344,13 +346,26 @@ Datablock *AllocateDatablock(DatablockTemplate *pTemplate, void *pInitData, UINT
     (void)pDestChannel;
     UINT cb = cbInitData;
     if (pTemplate && cb == 0) cb = pTemplate->GetDatablockByteCount(DBDATA_IDX);
-    Datablock *b = new Datablock(pTemplate, cb, 0, uiBlockControlCode);
-    b->AddRef();
-    if (flags) b->SetAccessFlags(flags);
+    // a pooled block when a runtime-wide pool holds one (RequireBlockPool), else a new block
+    Datablock *b = cb ? RequestGlobalPooledBlock(pTemplate, cb, 0) : nullptr;
+    if (b) {
+        if (uiBlockControlCode) b->SetControlSignal(uiBlockControlCode);
+        b->SetAccessFlags(flags ? flags : PT_ACCESS_DEFAULT);
+    } else {
+        b = new Datablock(pTemplate, cb, 0, uiBlockControlCode);
+        b->AddRef();
+        if (flags) b->SetAccessFlags(flags);
+    }
     if (pInitData && cbInitData) {
         b->Lock();
-        memcpy(b->GetDataPointer(TRUE), pInitData, cbInitData);
+        // a recycled block's old contents are read back only when the new data covers part of it
+        void *dst = b->AcquireHostView(TRUE, cbInitData < b->GetDataBufferLogicalSizeBytes());
+        if (dst) memcpy(dst, pInitData, cbInitData);
         b->Unlock();
+        if (!dst) {  // out of host memory even after reclaiming pooled blocks (reported)
+            b->Release();
+            return nullptr;
+        }
     }
     if (pTemplate && pTemplate->DescribesRecordStream() == FALSE && pTemplate->GetStride() &&
         !pTemplate->IsScalarParameter())
@@ -361,7 +376,12 @@ Datablock *AllocateDatablock(DatablockTemplate *pTemplate, void *pInitData, UINT
 Datablock *AllocateDatablock(DatablockTemplate *pTemplate, UINT uiDataSize, UINT uiMetaSize,
                              UINT uiTemplateSize, CONTROLSIGNAL uiBlockControlCode) {
     (void)uiTemplateSize;
-    Datablock *b = new Datablock(pTemplate, uiDataSize, uiMetaSize, uiBlockControlCode);
+    Datablock *b = uiDataSize ? RequestGlobalPooledBlock(pTemplate, uiDataSize, uiMetaSize) : nullptr;
+    if (b) {
+        if (uiBlockControlCode) b->SetControlSignal(uiBlockControlCode);
+        return b;
+    }
+    b = new Datablock(pTemplate, uiDataSize, uiMetaSize, uiBlockControlCode);
     b->AddRef();
     return b;
 }
