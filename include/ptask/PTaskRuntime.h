@@ -137,7 +137,15 @@ BOOL GetForceSynchronous();
 void SetProfilePSDispatch(BOOL b);
 BOOL GetProfilePSDispatch();
 void SetDebugMode(BOOL b);
+/* The graph monitor watchdog (reference PTaskRuntime.h:1476,1599-1619; default threshold
+ * 10000 ms as in PTaskRuntime.cpp:99): graphs started while it is on get a monitor thread that
+ * reports a stall (Graph::ReportGraphState to stderr).  GetWatchdogReportCount counts the
+ * reports of every graph (no reference counterpart). */
 void SetUseGraphMonitorWatchdog(BOOL b);
+BOOL GetUseGraphMonitorWatchdog();
+void SetDispatchWatchdogThreshold(DWORD dwThreshold);
+DWORD GetDispatchWatchdogThreshold();
+unsigned long long GetWatchdogReportCount();
 void SetDefaultViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY e);
 void SetDefaultOutputViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY e);
 VIEWMATERIALIZATIONPOLICY GetDefaultOutputViewMaterializationPolicy();

@@ -7,10 +7,12 @@
 #ifndef PTK_GRAPH_H
 #define PTK_GRAPH_H
 
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <map>
 #include <mutex>
+#include <ostream>
 #include <string>
 #include <thread>
 #include <vector>
@@ -114,6 +116,16 @@ public:
     void NotifyRoom() { m_cvRoom.notify_all(); }
     BOOL IsAlive() { return m_eState != PTGS_TEARINGDOWN && m_eState != PTGS_TEARDOWNCOMPLETE; }
 
+    /* ---- diagnostics (reference Graph::ReportGraphState, graph.h:1476; the monitor of
+     * GraphMonitorProc, graph.cpp:2477-2560) ----------------------------------------------------
+     * A textual state dump: every task with its dispatch count, whether it is dispatching, and the
+     * readiness of each input port; every channel holding blocks.  With a path, the DOT rendering
+     * is written there too.  While Runtime::SetUseGraphMonitorWatchdog(TRUE) was in force at Run,
+     * a monitor thread dumps the state to stderr when no dispatch has completed for
+     * Runtime::GetDispatchWatchdogThreshold() ms although work is pending (once per stall). */
+    void ReportGraphState(std::ostream &ios, const char *lpszDiagnosticDOTFilePath, BOOL bForceGraphDisplay);
+    UINT GetWatchdogReportCount() { return m_uiWatchdogReports; }
+
     /* ---- graph-scoped block pools (reference graph.h:139-199, ScopedPoolManager) ------------
      * Pre-allocated blocks (host storage, and device storage when a HIP accelerator exists) that
      * RequestPooledBlock hands out and that return to the pool on their last Release; released
@@ -129,7 +141,14 @@ public:
 
 private:
     void RunnerProc(UINT uiIndex);
+    void MonitorProc();
+    void StopMonitor();
+    BOOL HasPendingWorkLocked();
     void ReleaseScopedPools();
+    std::thread m_tMonitor;
+    std::condition_variable_any m_cvMonitor;
+    BOOL m_bMonitorStop = FALSE;
+    std::atomic<UINT> m_uiWatchdogReports{0};
     std::map<DatablockTemplate *, BlockPool *> m_mTypedPools;
     std::map<UINT, BlockPool *> m_mUntypedPools;
     void RegisterChannel(Channel *pChannel);

@@ -18,6 +18,8 @@
 //               SCHEDMODE_PRIORITY; by arrival in SCHEDMODE_FIFO.
 //   pause       one RecordTask graph stopped after Run: inputs pushed while it is stopped are
 //               not dispatched until Run resumes it.
+//   stall       a ScaleVector graph missing its scalar: the graph monitor watchdog (--watchdog-ms)
+//               reports the stall, then the missing input arrives and the task runs.
 //   concurrent  G graphs, each one ScaleVector task whose MF_SIMPLE_ITERATOR meta port repeats
 //               its dispatch --iterations times, all running at once; every output is checked.
 // Prints one JSON object.
@@ -41,7 +43,7 @@ namespace {
 
 struct Options {
     std::string kernels, scenario = "contend", mode = "priority", priorities;
-    int graphs = 2, tasks = 4, hostSlots = 1, iterations = 5, elems = 1024;
+    int graphs = 2, tasks = 4, hostSlots = 1, iterations = 5, elems = 1024, watchdogMs = 0;
 };
 
 std::vector<int> ParseList(const std::string &s) {
@@ -278,6 +280,66 @@ int Concurrent(const Options &o) {
     return bad ? 1 : 0;
 }
 
+// stall: one ScaleVector graph whose vector arrives while its scalar does not, so the task can
+// never dispatch: with the graph monitor watchdog on (threshold --watchdog-ms) the stall is
+// reported.  Then the missing inputs arrive, the task runs, and an idle graph with nothing pending
+// draws no further report.
+int Stall(const Options &o) {
+    DatablockTemplate *tVec = Runtime::GetDatablockTemplate((char *)"vec", sizeof(float), (UINT)o.elems, 1, 1);
+    DatablockTemplate *tScale = Runtime::GetDatablockTemplate((char *)"scale", sizeof(float), PTPARM_FLOAT);
+    DatablockTemplate *tN = Runtime::GetDatablockTemplate((char *)"n", sizeof(int), PTPARM_INT);
+    CompiledKernel *k = Runtime::GetCompiledKernel((char *)o.kernels.c_str(), (char *)"ScaleVector");
+    if (!k) return 1;
+    UINT uid = 0;
+    Graph *g = new Graph("stall");
+    Port *in[3] = {Runtime::CreatePort(INPUT_PORT, tVec, uid++, (char *)"A", 0),
+                   Runtime::CreatePort(STICKY_PORT, tScale, uid++, (char *)"scalar", 1),
+                   Runtime::CreatePort(STICKY_PORT, tN, uid++, (char *)"N", 2)};
+    Port *out = Runtime::CreatePort(OUTPUT_PORT, tVec, uid++, (char *)"Aout", 3);
+    Task *t = g->AddTask(k, 3, in, 1, &out, "scale");
+    if (!t) return 1;
+    t->SetComputeGeometry(1, 1, 1);
+    GraphInputChannel *ca = g->AddInputChannel(in[0], "A"), *cs = g->AddInputChannel(in[1], "scalar");
+    GraphInputChannel *cn = g->AddInputChannel(in[2], "N");
+    GraphOutputChannel *co = g->AddOutputChannel(out, "Aout");
+    g->Run();
+    std::vector<float> A((size_t)o.elems);
+    for (int e = 0; e < o.elems; ++e) A[(size_t)e] = 0.5f * (float)e;
+    const float s = 3.0f;
+    const int n = o.elems;
+    Datablock *ba = Runtime::AllocateDatablock(tVec, A.data(), (UINT)(n * sizeof(float)), nullptr);
+    ca->Push(ba);
+    ba->Release();
+    std::this_thread::sleep_for(std::chrono::milliseconds(4 * o.watchdogMs));
+    const UINT during = g->GetWatchdogReportCount();
+    Datablock *bs = Runtime::AllocateDatablock(tScale, (void *)&s, sizeof(s), nullptr);
+    Datablock *bn = Runtime::AllocateDatablock(tN, (void *)&n, sizeof(n), nullptr);
+    cs->Push(bs);
+    cn->Push(bn);
+    bs->Release();
+    bn->Release();
+    bool ok = true;
+    Datablock *r = co->Pull(60000);
+    if (!r) {
+        ok = false;
+    } else {
+        r->Lock();
+        const float *p = (const float *)r->GetDataPointer(FALSE);
+        for (int e = 0; e < n; ++e) ok = ok && p[e] == s * A[(size_t)e];
+        r->Unlock();
+        r->Release();
+    }
+    std::this_thread::sleep_for(std::chrono::milliseconds(4 * o.watchdogMs));
+    const UINT after = g->GetWatchdogReportCount();
+    g->Stop();
+    g->Teardown();
+    Graph::DestroyGraph(g);
+    printf("{\"scenario\": \"stall\", \"ok\": %s, \"reports_during_stall\": %u, \"reports_after\": %u, "
+           "\"runtime_reports\": %llu}\n",
+           ok ? "true" : "false", during, after, Runtime::GetWatchdogReportCount());
+    return ok ? 0 : 1;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -300,6 +362,7 @@ int main(int argc, char **argv) {
         else if (a == "--host-slots") o.hostSlots = atoi(next());
         else if (a == "--iterations") o.iterations = atoi(next());
         else if (a == "--elems") o.elems = atoi(next());
+        else if (a == "--watchdog-ms") o.watchdogMs = atoi(next());
         else {
             fprintf(stderr, "unknown option %s\n", a.c_str());
             return 2;
@@ -307,15 +370,19 @@ int main(int argc, char **argv) {
     }
     if (o.kernels.empty() || o.graphs < 1 || o.tasks < 1 || o.hostSlots < 1 || o.iterations < 1 || o.elems < 1 ||
         (o.mode != "priority" && o.mode != "fifo")) {
-        fprintf(stderr, "usage: ptk_multigraph --kernels <hosttasks.so> [--scenario contend|pause|concurrent] "
+        fprintf(stderr, "usage: ptk_multigraph --kernels <hosttasks.so> [--scenario contend|pause|concurrent|stall] "
                         "[--mode priority|fifo] [--graphs G] [--tasks K] [--priorities p0,p1,...] "
-                        "[--host-slots S] [--iterations I] [--elems N]\n");
+                        "[--host-slots S] [--iterations I] [--elems N] [--watchdog-ms T]\n");
         return 2;
     }
     Runtime::SetExitOnRuntimeFailure(FALSE);
     Runtime::SetUseHIP(FALSE);  // host tasks only
     Runtime::SetMaximumHostConcurrency(o.hostSlots);
     Runtime::SetSchedulingMode(o.mode == "priority" ? SCHEDMODE_PRIORITY : SCHEDMODE_FIFO);
+    if (o.watchdogMs > 0) {
+        Runtime::SetUseGraphMonitorWatchdog(TRUE);
+        Runtime::SetDispatchWatchdogThreshold((DWORD)o.watchdogMs);
+    }
     // enough runner threads per graph for every task of a graph to wait for admission at once
     Runtime::SetTaskThreadPoolSize((UINT)std::max(o.tasks, 1));
     if (PTFAILED(Runtime::Initialize())) {
@@ -326,6 +393,7 @@ int main(int argc, char **argv) {
     if (o.scenario == "contend") r = Contend(o);
     else if (o.scenario == "pause") r = Pause(o);
     else if (o.scenario == "concurrent") r = Concurrent(o);
+    else if (o.scenario == "stall") r = Stall(o);
     else fprintf(stderr, "unknown scenario %s\n", o.scenario.c_str());
     Runtime::Terminate();
     return r;

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cstring>
 #include <fstream>
+#include <iostream>
 #include <map>
 
 #include "ptk_internal.h"
@@ -539,6 +540,89 @@ void Graph::Run(BOOL bSingleThreaded) {
     m_eState = PTGS_RUNNING;
     for (UINT i = 0; i < n; ++i) m_vRunners.emplace_back(&Graph::RunnerProc, this, i);
     for (Task *t : m_vTasks) SignalTask(t);
+    if (RT().useWatchdog && !m_tMonitor.joinable()) {
+        m_bMonitorStop = FALSE;
+        m_tMonitor = std::thread(&Graph::MonitorProc, this);
+    }
+}
+
+// Work that should lead to a dispatch: a queued task, a running dispatch, or a block waiting in a
+// channel or an input port.  Graph lock held.
+BOOL Graph::HasPendingWorkLocked() {
+    if (!m_qReady.empty() || m_uiBusyRunners) return TRUE;
+    for (Channel *c : m_vChannels)
+        if (c->GetType() != CT_GRAPH_OUTPUT && c->GetQueueDepth()) return TRUE;
+    return FALSE;
+}
+
+// reference: Graph::GraphMonitorProc, graph.cpp:2477-2560 -- wake a few times per threshold;
+// progress is a change in the graph's completed dispatch count; a running graph that makes none
+// for the threshold while work is pending is reported once, until it progresses again.
+void Graph::MonitorProc() {
+    std::unique_lock<std::recursive_mutex> lk(m_lock);
+    unsigned long long last = 0;
+    for (Task *t : m_vTasks) last += t->GetDispatchCount();
+    auto since = std::chrono::steady_clock::now();
+    bool reported = false;
+    while (!m_bMonitorStop) {
+        const DWORD threshold = RT().watchdogThresholdMs;
+        const DWORD period = std::max<DWORD>(10, std::min<DWORD>(threshold / 4, 500));
+        m_cvMonitor.wait_for(lk, std::chrono::milliseconds(period));
+        if (m_bMonitorStop) break;
+        unsigned long long now = 0;
+        for (Task *t : m_vTasks) now += t->GetDispatchCount();
+        const auto tnow = std::chrono::steady_clock::now();
+        if (now != last || m_eState != PTGS_RUNNING) {
+            last = now;
+            since = tnow;
+            reported = false;
+            continue;
+        }
+        const auto idleMs = std::chrono::duration_cast<std::chrono::milliseconds>(tnow - since).count();
+        if (reported || idleMs <= (long long)threshold || !HasPendingWorkLocked()) continue;
+        reported = true;
+        m_uiWatchdogReports++;
+        RT().watchdogReports++;
+        std::cerr << "graph monitor: graph " << m_strName << " has completed no dispatch for " << idleMs
+                  << " ms with work pending; hung?  Graph state:\n";
+        ReportGraphState(std::cerr, nullptr, TRUE);
+    }
+}
+
+void Graph::StopMonitor() {
+    std::thread t;
+    {
+        std::lock_guard<std::recursive_mutex> g(m_lock);
+        m_bMonitorStop = TRUE;
+        m_cvMonitor.notify_all();
+        t.swap(m_tMonitor);
+    }
+    if (t.joinable()) t.join();
+}
+
+void Graph::ReportGraphState(std::ostream &ios, const char *lpszDiagnosticDOTFilePath, BOOL) {
+    std::lock_guard<std::recursive_mutex> g(m_lock);
+    static const char *kStates[] = {"initializing", "runnable", "running", "quiescing", "tearing down",
+                                    "torn down"};
+    const int st = (int)m_eState;
+    ios << "graph " << m_strName << ": " << (st >= 0 && st < 6 ? kStates[st] : "?") << ", " << m_vTasks.size()
+        << " tasks, " << m_qReady.size() << " queued, " << m_uiBusyRunners << " dispatching\n";
+    for (Task *t : m_vTasks) {
+        ios << "  task " << t->GetTaskName() << ": " << t->GetDispatchCount() << " dispatches"
+            << (t->IsDispatching() ? ", dispatching" : "") << "\n";
+        for (UINT i = 0; i < t->GetInputPortCount(); ++i) {
+            Port *p = t->GetInputPort(i);
+            const BOOL ready = p->GetPortType() == INITIALIZER_PORT || static_cast<InputPort *>(p)->IsReady();
+            ios << "    in  " << (p->GetVariableBinding() ? p->GetVariableBinding() : "?") << ": "
+                << (ready ? "ready" : "waiting") << "\n";
+        }
+    }
+    for (Channel *c : m_vChannels)
+        if (c->GetQueueDepth())
+            ios << "  channel " << (c->GetName() ? c->GetName() : "?") << ": " << c->GetQueueDepth()
+                << " block(s) queued\n";
+    ios.flush();
+    if (lpszDiagnosticDOTFilePath) WriteDOTFile(lpszDiagnosticDOTFilePath, TRUE);
 }
 
 // Runner thread: pop a queued task, dispatch it if ready, re-queue it after a dispatch (it may
@@ -576,6 +660,7 @@ BOOL Graph::WaitQuiescent(DWORD dwTimeout) {
 }
 
 void Graph::Stop() {
+    StopMonitor();
     std::vector<std::thread> runners;
     {
         std::lock_guard<std::recursive_mutex> g(m_lock);

@@ -269,6 +269,10 @@ struct RuntimeState {
     // dispatch admission per accelerator class: host tasks (maxHostConcurrency slots) and HIP
     // tasks (enqueueing a launch; the device runs launched work concurrently)
     DispatchArbiter hostArbiter{4}, hipArbiter{8};
+    // graph monitor watchdog
+    std::atomic<BOOL> useWatchdog{FALSE};
+    std::atomic<DWORD> watchdogThresholdMs{10000};
+    std::atomic<unsigned long long> watchdogReports{0};
     // allocation recovery and pools (pools.cpp)
     std::atomic<unsigned long long> allocFailures{0}, allocRecovered{0}, allocUnrecovered{0}, bytesReclaimed{0};
     std::atomic<unsigned long long> poolHits{0}, poolMisses{0};

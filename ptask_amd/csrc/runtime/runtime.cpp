@@ -636,7 +636,11 @@ BOOL GetForceSynchronous() { return RT().forceSynchronous; }
 void SetProfilePSDispatch(BOOL b) { RT().profilePSDispatch = b; }
 BOOL GetProfilePSDispatch() { return RT().profilePSDispatch; }
 void SetDebugMode(BOOL) {}
-void SetUseGraphMonitorWatchdog(BOOL) {}
+void SetUseGraphMonitorWatchdog(BOOL b) { RT().useWatchdog = b; }
+BOOL GetUseGraphMonitorWatchdog() { return RT().useWatchdog; }
+void SetDispatchWatchdogThreshold(DWORD dwThreshold) { RT().watchdogThresholdMs = dwThreshold ? dwThreshold : 1; }
+DWORD GetDispatchWatchdogThreshold() { return RT().watchdogThresholdMs; }
+unsigned long long GetWatchdogReportCount() { return RT().watchdogReports; }
 void SetDefaultViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY) {}
 void SetDefaultOutputViewMaterializationPolicy(VIEWMATERIALIZATIONPOLICY e) { RT().outputViewPolicy = e; }
 VIEWMATERIALIZATIONPOLICY GetDefaultOutputViewMaterializationPolicy() { return RT().outputViewPolicy; }

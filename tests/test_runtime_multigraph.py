@@ -64,3 +64,22 @@ def test_stopped_graph_dispatches_nothing_until_run(hostlib):
 def test_many_concurrent_graphs_with_iterated_tasks(hostlib, graphs, slots):
     r = run(hostlib, "--scenario", "concurrent", "--graphs", graphs, "--iterations", 7, "--host-slots", slots)
     assert r["ok"] and r["bad_graphs"] == 0 and r["dispatches"] == graphs
+
+
+def test_watchdog_reports_a_stalled_graph_once(hostlib):
+    # graph.cpp:2477-2560: a running graph with pending work and no dispatch for the threshold is
+    # reported with its state; it is reported once per stall, and an idle graph is not reported
+    p = subprocess.run([APP, "--kernels", hostlib, "--scenario", "stall", "--watchdog-ms", "100"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    assert r["ok"] and r["reports_during_stall"] == 1 and r["reports_after"] == 1
+    assert "graph monitor: graph stall" in p.stderr
+    assert "scalar: waiting" in p.stderr and "channel A: 1 block(s) queued" in p.stderr
+
+
+def test_watchdog_quiet_while_graphs_progress(hostlib):
+    p = subprocess.run([APP, "--kernels", hostlib, "--scenario", "concurrent", "--graphs", "4", "--iterations", "20",
+                        "--host-slots", "2", "--watchdog-ms", "200"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0 and json.loads(p.stdout.strip().splitlines()[-1])["ok"]
+    assert "graph monitor" not in p.stderr

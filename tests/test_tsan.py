@@ -66,7 +66,8 @@ def test_fdtd_tsan_results_equal_regular_build():
      "--host-slots", 1],
     ["--scenario", "pause"],
     ["--scenario", "concurrent", "--graphs", 6, "--iterations", 5, "--host-slots", 3],
-], ids=["contend", "pause", "concurrent"])
+    ["--scenario", "stall", "--watchdog-ms", 50],
+], ids=["contend", "pause", "concurrent", "watchdog"])
 def test_multigraph_race_free(recordlib, args):
     p = run_tsan("ptk_multigraph", "--kernels", recordlib, *args)
     assert p.returncode == 0
