@@ -45,6 +45,7 @@ struct Options {
     // HIP stream policy: one shared stream (each EM step is a chain of two dependent tasks, which
     // then needs no cross-stream event waits), or one per task
     std::string streams = "shared";
+    std::string accumulate;  // the accumulate kernel: valu (GmmAccumulate) or mfma (the HIP default)
 };
 
 uint64_t Fnv1a(const void *p, size_t n) {
@@ -116,6 +117,7 @@ int main(int argc, char **argv) {
         else if (a == "--dump") o.dump = next();
         else if (a == "--profile") o.profile = true;
         else if (a == "--streams") o.streams = next();
+        else if (a == "--accumulate") o.accumulate = next();
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
     const long long tiles = (o.n + 127) / 128;
@@ -124,10 +126,12 @@ int main(int argc, char **argv) {
         o.k < 1 || o.k > PTK_GMM_MAX_K || o.k > o.n || o.steps < 1 || o.repeat < 1 ||
         (o.mode != "looped" && o.mode != "unrolled") || (long long)o.n * o.d > 0x3fffffffLL ||
         (!o.init.empty() && o.init != "strided" && o.init != "components") ||
-        (o.init == "components" && !o.data.empty()) || (o.streams != "shared" && o.streams != "task")) {
+        (o.init == "components" && !o.data.empty()) || (o.streams != "shared" && o.streams != "task") ||
+        (!o.accumulate.empty() && o.accumulate != "valu" && o.accumulate != "mfma")) {
         fprintf(stderr, "usage: ptk_gmm --kernels <gmm.hsaco|hosttasks.so> [--n N] [--d D<=32] "
                         "[--k K<=64] [--steps S] [--mode looped|unrolled] [--init strided|components] [--blocks B] [--repeat R] "
-                        "[--warmup W] [--seed S] [--data x.f32] [--dump file] [--profile] [--streams shared|task]\n");
+                        "[--warmup W] [--seed S] [--data x.f32] [--dump file] [--profile] [--streams shared|task] "
+                        "[--accumulate valu|mfma]\n");
         return 2;
     }
     std::vector<float> X;
@@ -182,7 +186,12 @@ int main(int argc, char **argv) {
     DatablockTemplate *tPart = Runtime::GetDatablockTemplate((char *)"gmmPartials", sizeof(double), (UINT)partElems, 1, 1);
     DatablockTemplate *tDims = Runtime::GetDatablockTemplate((char *)"gmmDims", sizeof(GMM_DIMS), PTPARM_BYVALSTRUCT);
     DatablockTemplate *tIter = Runtime::GetDatablockTemplate((char *)"gmmIters", sizeof(int), PTPARM_INT);
-    CompiledKernel *kAcc = Runtime::GetCompiledKernel((char *)o.kernelFile.c_str(), (char *)"GmmAccumulate");
+    // the accumulate task: GmmAccumulate on the VALU, or GmmAccumulateMfma with its products on
+    // the matrix cores (same arguments and partial rows; HIP code objects only)
+    if (o.accumulate.empty())
+        o.accumulate = ptaskutils::SelectAcceleratorClass(o.kernelFile.c_str()) == ACCELERATOR_CLASS_HIP ? "mfma" : "valu";
+    const char *accName = o.accumulate == "mfma" ? "GmmAccumulateMfma" : "GmmAccumulate";
+    CompiledKernel *kAcc = Runtime::GetCompiledKernel((char *)o.kernelFile.c_str(), (char *)accName);
     CompiledKernel *kFin = Runtime::GetCompiledKernel((char *)o.kernelFile.c_str(), (char *)"GmmFinalize");
     if (!kAcc || !kFin) {
         fprintf(stderr, "GMM kernels unavailable: %s\n", Runtime::GetLastErrorText());
@@ -355,12 +364,13 @@ int main(int argc, char **argv) {
     const double msRun = o.repeat ? timedMs / o.repeat : 0.0;
     char llText[40] = "null";
     if (std::isfinite(ll)) snprintf(llText, sizeof(llText), "%.17g", ll);
-    printf("{\"ok\": %s, \"backend\": \"%s\", \"mode\": \"%s\", \"init\": \"%s\", \"n\": %lld, \"d\": %d, \"k\": %d, "
+    printf("{\"ok\": %s, \"backend\": \"%s\", \"accumulate\": \"%s\", \"mode\": \"%s\", \"init\": \"%s\", \"n\": %lld, \"d\": %d, \"k\": %d, "
            "\"steps\": %d, \"blocks\": %d, \"runs\": %d, \"timed_runs\": %d, \"timed_ms\": %.4f, "
            "\"ms_per_run\": %.4f, \"point_steps_per_s\": %.1f, \"steps_done\": %.0f, \"loglike\": %s, "
            "\"h2d_count_timed\": %llu, \"d2h_count_timed\": %llu, \"tasks\": [%s], \"model_fnv1a\": \"%016llx\", "
            "\"min_var\": %s, \"nonpositive_var\": %d, \"min_frac\": %s, \"loglike_history\": [%s]}\n",
-           ok ? "true" : "false", hip ? "hip" : "host", o.mode.c_str(), o.init.c_str(), o.n, o.d, o.k, o.steps, o.blocks,
+           ok ? "true" : "false", hip ? "hip" : "host", o.accumulate.c_str(), o.mode.c_str(), o.init.c_str(), o.n, o.d, o.k,
+           o.steps, o.blocks,
            runs, o.repeat, timedMs, msRun, msRun > 0 ? (double)o.n * o.steps / (msRun / 1e3) : 0.0,
            stepsDone, llText, h2d, d2h, taskJson.c_str(),
            (unsigned long long)Fnv1a(result.data(), result.size() * sizeof(double)), Num(minVar).c_str(), badVar,

@@ -51,6 +51,22 @@ struct GmmShared {
     double sLL[4];
 };
 
+// Coordinates of tile t (zero for padding points and dimensions, nothing for t >= t1) into this
+// thread's 2 NC registers: element i = tid + 256 j of the TILE x Dp tile.
+template <int NC>
+__device__ __forceinline__ void gmm_fetch_tile(const float *__restrict__ X, const GMM_DIMS &d, long long t,
+                                               long long t1, float *pf) {
+    const int Dp = 4 * NC, D = d.D;
+    const long long p0 = t * TILE;
+    const int np = t < t1 ? (int)min((long long)TILE, (long long)d.N - p0) : 0;
+    const float *__restrict__ xt = X + p0 * D;
+#pragma unroll
+    for (int j = 0; j < 2 * NC; ++j) {
+        const int i = (int)threadIdx.x + 256 * j, p = i / Dp, m = i - p * Dp;
+        pf[j] = (p < np && m < D) ? xt[(long long)p * D + m] : 0.f;
+    }
+}
+
 // NC = float4 chunks per point (Dp / 4), a compile-time constant so that the chunk loops unroll
 // without guards and a component's table row arrives in one batch of scalar loads.
 // Tiles t0, t0 + tstep, ... (< t1) of the points into one row of partial sums.
@@ -98,15 +114,20 @@ __device__ __forceinline__ void gmm_accumulate_tiles(const float *__restrict__ X
     }
     double ll = 0;
 
+    // The next tile's coordinates are loaded into registers while the current tile computes
+    // (2 NC values per thread: TILE x Dp / 256), so the global-memory latency stays hidden.
+    float pf[2 * NC];
+    gmm_fetch_tile<NC>(X, d, t0, t1, pf);
     for (long long t = t0; t < t1; t += tstep) {
         const long long p0 = t * TILE;
         const int np = (int)min((long long)TILE, (long long)d.N - p0);
-        const float *__restrict__ xt = X + p0 * D;
-        for (int i = tid; i < TILE * Dp; i += 256) {
-            const int p = i / Dp, m = i - p * Dp;
-            sX[p * XS + m] = (p < np && m < D) ? xt[(long long)p * D + m] : 0.f;
+#pragma unroll
+        for (int j = 0; j < 2 * NC; ++j) {
+            const int i = tid + 256 * j, p = i / Dp, m = i - p * Dp;
+            sX[p * XS + m] = pf[j];
         }
         __syncthreads();
+        gmm_fetch_tile<NC>(X, d, t + tstep, t1, pf);
 
         // ---- E-phase: log responsibilities of point ep for components [k0, k1)
         float x[4 * NC];

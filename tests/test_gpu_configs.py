@@ -67,12 +67,13 @@ def test_fdtd_384_six_task_graph(tmp_path):
     assert_fields_bitwise(load_dump(d, 384, 384, 384), fdtd_oracle(384, 2, 6))
 
 
-def test_gmm_2m_d32_k64_graph(tmp_path):
-    print("case config gmm N=2M D=32 K=64 3 steps", flush=True)
+@pytest.mark.parametrize("accumulate", ["valu", "mfma"])
+def test_gmm_2m_d32_k64_graph(tmp_path, accumulate):
+    print(f"case config gmm N=2M D=32 K=64 3 steps {accumulate}", flush=True)
     js, X, mu0, m = run_gmm(paths.kernel_path("gmm.hsaco"), tmp_path, 2_000_000, 32, 64, 3,
-                            "--mode", "looped", timeout=600)
+                            "--mode", "looped", "--accumulate", accumulate, timeout=600)
     assert js["ok"] and js["backend"] == "hip" and js["nonpositive_var"] == 0
-    check_against_oracle(X, mu0, m, 3, "graph N=2M D=32 K=64", threads=oracle.default_threads())
+    check_against_oracle(X, mu0, m, 3, f"graph N=2M D=32 K=64 {accumulate}", threads=oracle.default_threads())
 
 
 def test_pca_65536x2048_k16_graph(tmp_path):

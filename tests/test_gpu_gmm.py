@@ -44,8 +44,8 @@ def initial_model(X, K, steps):
     return m, mu0
 
 
-def em_on_device(hip, gmod, X, K, steps, blocks):
-    """`steps` GmmAccumulate + GmmFinalize launches through the C ABI; returns the final model."""
+def em_on_device(hip, gmod, X, K, steps, blocks, accumulate="GmmAccumulate"):
+    """`steps` accumulate + GmmFinalize launches through the C ABI; returns the final model."""
     N, D = X.shape
     assert 1 <= D <= gmm.MAX_D and 1 <= K <= gmm.MAX_K and blocks >= 1
     dims = gmm.Dims(N, D, K, blocks, steps)
@@ -56,7 +56,7 @@ def em_on_device(hip, gmod, X, K, steps, blocks):
     dM.upload(model)
     dP = hip.DeviceBuffer(8 * blocks * gmm.partial_stride(D, K))
     fin_grid = K + 1
-    acc, fin = gmod.function("GmmAccumulate"), gmod.function("GmmFinalize")
+    acc, fin = gmod.function(accumulate), gmod.function("GmmFinalize")
     for _ in range(steps):
         hip.launch(acc, (blocks, 1, 1), (256, 1, 1),
                    hip.ArgPack().ptr(dX.ptr).raw(bytes(dims)).ptr(dM.ptr).ptr(dP.ptr))
@@ -90,39 +90,54 @@ def check_against_oracle(X, mu0, m, steps, label, threads=0):
     (1000, 1, 1, 3, 1, 1008), (5000, 3, 2, 6, 7, 5023), (4097, 8, 33, 5, 33, 2),
     (20000, 32, 64, 6, 157, 20288), (129, 2, 5, 4, 2, 148), (50000, 16, 32, 8, 391, 50144),
     (3000, 31, 17, 5, 24, 3234)])
-def test_kernels_match_oracle(hip, gmod, N, D, K, steps, blocks, seed):
-    print(f"case gmm kernels N={N} D={D} K={K} steps={steps} blocks={blocks}", flush=True)
+@pytest.mark.parametrize("accumulate", ["GmmAccumulate", "GmmAccumulateMfma"])
+def test_kernels_match_oracle(hip, gmod, N, D, K, steps, blocks, seed, accumulate):
+    print(f"case gmm kernels {accumulate} N={N} D={D} K={K} steps={steps} blocks={blocks}", flush=True)
     X = mixture(N, D, K, seed=seed)
-    m, mu0 = em_on_device(hip, gmod, X, K, steps, blocks)
-    check_against_oracle(X, mu0, m, steps, f"N={N} D={D} K={K}")
+    m, mu0 = em_on_device(hip, gmod, X, K, steps, blocks, accumulate)
+    check_against_oracle(X, mu0, m, steps, f"{accumulate} N={N} D={D} K={K}")
 
 
-def test_kernels_are_deterministic(hip, gmod):
-    print("case gmm determinism", flush=True)
+@pytest.mark.parametrize("accumulate", ["GmmAccumulate", "GmmAccumulateMfma"])
+def test_kernels_are_deterministic(hip, gmod, accumulate):
+    print(f"case gmm determinism {accumulate}", flush=True)
     X = mixture(30000, 12, 20, seed=3)
-    a, _ = em_on_device(hip, gmod, X, 20, 5, 200)
-    b, _ = em_on_device(hip, gmod, X, 20, 5, 200)
+    a, _ = em_on_device(hip, gmod, X, 20, 5, 200, accumulate)
+    b, _ = em_on_device(hip, gmod, X, 20, 5, 200, accumulate)
     for f in ("mu", "var", "frac", "loglike"):
         assert np.array_equal(getattr(a, f), getattr(b, f))
 
 
-def test_reference_fixture_on_device(hip, gmod):
+@pytest.mark.parametrize("accumulate", ["GmmAccumulate", "GmmAccumulateMfma"])
+def test_reference_fixture_on_device(hip, gmod, accumulate):
     """dist_3 (1-D, 1000 points) for 21 steps, as CPU_EM runs its 20 iterations after the
     constructor (graphemgmm.cpp:315-317)."""
-    print("case gmm dist_3", flush=True)
+    print(f"case gmm dist_3 {accumulate}", flush=True)
     import os
     x = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "dist_3.txt"))
     X = x.astype(np.float32).reshape(-1, 1)
-    m, mu0 = em_on_device(hip, gmod, X, 3, 21, 8)
-    check_against_oracle(X, mu0, m, 21, "dist_3")
+    m, mu0 = em_on_device(hip, gmod, X, 3, 21, 8, accumulate)
+    check_against_oracle(X, mu0, m, 21, f"dist_3 {accumulate}")
+
+
+def test_mfma_first_step_equals_valu(hip, gmod):
+    """The first EM step (var = 1e-10) keeps the VALU E-step in GmmAccumulateMfma: its
+    log-likelihood is bitwise GmmAccumulate's (the M-step sums differ only in rounding)."""
+    print("case gmm mfma first step", flush=True)
+    X = mixture(20000, 32, 64, seed=20288)
+    a, _ = em_on_device(hip, gmod, X, 64, 1, 157, "GmmAccumulate")
+    b, _ = em_on_device(hip, gmod, X, 64, 1, 157, "GmmAccumulateMfma")
+    assert a.loglike[0] == b.loglike[0]
+    assert np.allclose(a.mu, b.mu, rtol=1e-5, atol=1e-5)
 
 
 @pytest.mark.parametrize("mode", ["looped", "unrolled"])
-def test_hip_graph_matches_oracle(tmp_path, mode):
+@pytest.mark.parametrize("accumulate", ["valu", "mfma"])
+def test_hip_graph_matches_oracle(tmp_path, mode, accumulate):
     js, X, mu0, m = run_gmm(KERNELS, tmp_path, 40000, 8, 16, 7, "--mode", mode, "--blocks", "200",
-                            "--init", "strided")
-    assert js["ok"] and js["backend"] == "hip"
-    check_against_oracle(X, mu0, m, 7, f"graph {mode}")
+                            "--init", "strided", "--accumulate", accumulate)
+    assert js["ok"] and js["backend"] == "hip" and js["accumulate"] == accumulate
+    check_against_oracle(X, mu0, m, 7, f"graph {mode} {accumulate}")
 
 
 def test_hip_graph_forms_reruns_and_residency(tmp_path):
