@@ -27,6 +27,7 @@ namespace PTask {
 class CompiledKernel;
 class Task;
 class BlockPool;
+class GraphReplay;
 class Port;
 class Datablock;
 
@@ -126,6 +127,18 @@ public:
     void ReportGraphState(std::ostream &ios, const char *lpszDiagnosticDOTFilePath, BOOL bForceGraphDisplay);
     UINT GetWatchdogReportCount() { return m_uiWatchdogReports; }
 
+    /* ---- launch capture (no reference counterpart) ---------------------------------------------
+     * EnableLaunchCapture(TRUE) before Run: the next run is recorded and, if every task is a HIP
+     * task that dispatches exactly once per run (no meta ports, no predicated channels), every
+     * later run replays it as one hipGraph launch: inputs are copied into staging buffers, the
+     * outputs are fresh pooled blocks, and each task's dispatch count still advances by one per
+     * run.  New values on a sticky port, or a recording that does not fit, return the graph to
+     * ordinary dispatch.  IsLaunchCaptureArmed: replays are active. */
+    BOOL EnableLaunchCapture(BOOL bEnable);
+    BOOL IsLaunchCaptureArmed();
+    UINT GetLaunchCaptureReplayCount();
+    class GraphReplay *GetReplay() { return m_pReplay; }
+
     /* ---- graph-scoped block pools (reference graph.h:139-199, ScopedPoolManager) ------------
      * Pre-allocated blocks (host storage, and device storage when a HIP accelerator exists) that
      * RequestPooledBlock hands out and that return to the pool on their last Release; released
@@ -151,6 +164,7 @@ private:
     std::atomic<UINT> m_uiWatchdogReports{0};
     std::map<DatablockTemplate *, BlockPool *> m_mTypedPools;
     std::map<UINT, BlockPool *> m_mUntypedPools;
+    class GraphReplay *m_pReplay = nullptr;
     void RegisterChannel(Channel *pChannel);
     std::string SerializeLocked();
 

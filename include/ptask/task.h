@@ -63,6 +63,8 @@ public:
     Port *GetInputPort(UINT i) { return m_vInputs[i]; }
     OutputPort *GetOutputPort(UINT i) { return m_vOutputs[i]; }
     UINT GetDispatchCount() { return m_uiDispatchCount; }
+    /* A dispatch that a replay of the graph's captured launches performed for this task. */
+    void NoteReplayedDispatch() { m_uiDispatchCount++; }
     /* Dependent accelerators (reference: Task::BindDependentAcceleratorClass, task.h:768,
      * task.cpp): a host task that also needs nRequiredInstances (1) accelerators of class cls to
      * run -- its ports bound with Port::BindDependentAccelerator receive device pointers, and with

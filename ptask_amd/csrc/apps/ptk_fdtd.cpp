@@ -87,6 +87,8 @@ struct Options {
     std::string pool;         // input blocks from a runtime-wide ("global") or graph ("graph") pool
     int poolBlocks = 6;       // blocks per field pool
     int injectHost = 0, injectDevice = 0;  // allocation failures forced before every run after the first
+    bool capture = false;     // record the first run's launches and replay them as one hipGraph
+    int seedStep = 0;         // run r starts from seed + r * seedStep (the dump is the last run's)
     int graphs = 1;           // independent graphs run at once (graph g: seed + g, priority g)
     std::string priorities;   // comma-separated task priority per graph (default: all equal)
     std::string sched = "priority";  // dispatch admission across graphs: priority | fifo
@@ -133,6 +135,8 @@ bool Parse(int argc, char **argv, Options &o) {
         else if (a == "--inject-host-failures") o.injectHost = atoi(next("--inject-host-failures"));
         else if (a == "--inject-device-failures") o.injectDevice = atoi(next("--inject-device-failures"));
         else if (a == "--graphs") o.graphs = atoi(next("--graphs"));
+        else if (a == "--capture") o.capture = true;
+        else if (a == "--seed-step") o.seedStep = atoi(next("--seed-step"));
         else if (a == "--priorities") o.priorities = next("--priorities");
         else if (a == "--sched") o.sched = next("--sched");
         else {
@@ -580,7 +584,7 @@ int main(int argc, char **argv) {
                         "[--steps S] [--mode looped|unrolled] [--blocks B] [--repeat R] [--seed S] "
                         "[--form six|halves|fused|overlap|overlap2|overlap3|overlap3p] [--geometry default|reference|compute] [--tile IxJxK] [--dump file] "
                         "[--eager] [--profile] [--chain] [--serialize graph.xml] [--deserialize graph.xml] "
-                        "[--graphs G [--priorities p0,p1,...] [--sched priority|fifo]]\n");
+                        "[--graphs G [--priorities p0,p1,...] [--sched priority|fifo]] [--capture]\n");
         return 2;
     }
     Runtime::SetExitOnRuntimeFailure(FALSE);
@@ -618,6 +622,10 @@ int main(int argc, char **argv) {
             }
     }
     Runtime::ResetMemoryStatistics();
+    if (o.capture && !G.g->EnableLaunchCapture(TRUE)) {
+        fprintf(stderr, "launch capture: %s\n", Runtime::GetLastErrorText());
+        return 1;
+    }
     if (!o.serialize.empty()) {
         // write the graph (built here or read from XML) and stop
         BOOL ok = G.g->Serialize(o.serialize.c_str());
@@ -646,7 +654,7 @@ int main(int argc, char **argv) {
     unsigned long long h2d0 = 0, d2h0 = 0, h2db = 0, d2hb = 0;
     std::vector<float> final[6];
     // chained: warmup runs, then the timed runs, then one run whose outputs are downloaded
-    const int runs = o.mode == "looped" ? (o.chain ? o.warmup + o.repeat + 1 : o.repeat) : 1;
+    const int runs = o.chain ? o.warmup + o.repeat + 1 : o.repeat;
     std::vector<std::pair<double, UINT>> profileBase;
     const int blocks = o.mode == "looped" ? 1 : o.blocks;
     unsigned long long dispatch0 = 0;
@@ -673,6 +681,10 @@ int main(int argc, char **argv) {
             Runtime::SetAllocationFailureInjection(HOST_MEMORY_SPACE_ID, o.injectHost);
             if (hipAcc) Runtime::SetAllocationFailureInjection(hipAcc->GetMemorySpaceId(), o.injectDevice);
         }
+        if (o.seedStep && r > 0 && carried.empty())  // this run's initial fields (off the clock)
+            for (int f = 0; f < 6; ++f)
+                for (size_t n = 0; n < init[f].size(); ++n)
+                    init[f][n] = ptk_rand_sym_f32(o.seed + (unsigned long long)(r * o.seedStep), (uint64_t)f, n);
         dispatch0 = G.g->GetDispatchCount();
         timer.reset();
         for (int b = 0; b < blocks; ++b) {
@@ -758,6 +770,8 @@ int main(int argc, char **argv) {
             taskJson += buf;
         }
     }
+    const bool captureArmed = G.g->IsLaunchCaptureArmed();
+    const unsigned captureReplays = G.g->GetLaunchCaptureReplayCount();
     G.g->Stop();
     G.g->Teardown();
     Graph::DestroyGraph(G.g);
@@ -791,7 +805,9 @@ int main(int argc, char **argv) {
     }
     Runtime::PTK_MEMORY_STATISTICS ms = {};
     Runtime::GetMemoryStatistics(&ms);
-    printf("], \"memory\": {\"allocation_failures\": %llu, \"recovered\": %llu, \"unrecovered\": %llu, "
+    printf("], \"capture\": {\"requested\": %s, \"armed\": %s, \"replays\": %u}",
+           o.capture ? "true" : "false", captureArmed ? "true" : "false", captureReplays);
+    printf(", \"memory\": {\"allocation_failures\": %llu, \"recovered\": %llu, \"unrecovered\": %llu, "
            "\"bytes_reclaimed\": %llu, \"pool_hits\": %llu, \"pool_misses\": %llu}",
            ms.ullAllocationFailures, ms.ullAllocationsRecovered, ms.ullAllocationsUnrecovered, ms.ullBytesReclaimed,
            ms.ullPooledBlockHits, ms.ullPooledBlockMisses);

@@ -467,6 +467,8 @@ BOOL HIPTask::PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vecto
     struct Touch { PBuffer *buf; BOOL write; };
     std::vector<Touch> touched;
     touched.reserve(m_vBindings.size());
+    GraphReplay *replay = m_pGraph->GetReplay();
+    std::vector<std::pair<UINT, Datablock *>> ptrArgs;  // for a launch capture being recorded
     for (Binding &b : m_vBindings) {
         Datablock *blk = b.bInput ? vIn[b.uiSlot] : vOut[b.uiSlot];
         if (!blk) return Runtime::HandleError("%s: argument %s has no block", GetTaskName(),
@@ -483,12 +485,15 @@ BOOL HIPTask::PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vecto
                                                b.pPort->GetVariableBinding());
         memcpy(m_vArgs.data() + b.uiOffset, &dptr, sizeof(void *));
         touched.push_back({blk->GetPlatformBuffer(m_pAccelerator->GetMemorySpaceId()), b.bWritable});
+        if (replay) ptrArgs.push_back(std::make_pair(b.uiOffset, blk));
     }
     // (2) order after every producer (and, for writes, every reader) on other streams
     for (Touch &t : touched) t.buf->WaitOutstanding(m_pContext, t.write);
     // (3) launch
     PTASKDIM3 grid, block;
     GeometryFor(vIn, &grid, &block);
+    if (replay && replay->GetState() == GraphReplay::RECORDING)
+        replay->OnLaunch(this, m_pAccelerator, fn, grid, block, m_vArgs.data(), m_uiArgBytes, ptrArgs, uiIterations);
     ptk_hip_stream s = m_pContext->GetStream();
     const BOOL prof = Runtime::GetProfilePSDispatch();
     ptk_hip_event evStart = nullptr, evEnd = nullptr;

@@ -240,6 +240,8 @@ static Datablock *MakeDescriptorBlock(Datablock *pDescribed, DESCRIPTORFUNC eFun
 
 // reference: Task::Dispatch, task.cpp:3822-3906
 BOOL Task::AttemptDispatch(std::unique_lock<std::recursive_mutex> *pLock) {
+    GraphReplay *replay = m_pGraph->GetReplay();
+    if (replay && replay->IsArmed()) return replay->TryReplay();  // no task runs on its own then
     if (!IsReadyForDispatch()) return FALSE;
     ComputeParameterLayout();
     m_bDispatching = TRUE;
@@ -348,6 +350,7 @@ BOOL Task::AttemptDispatch(std::unique_lock<std::recursive_mutex> *pLock) {
         if (vOut[o]) m_vOutputs[o]->Push(vOut[o]);
     m_uiDispatchCount++;
     m_bDispatching = FALSE;
+    if (replay) replay->OnDispatchDone(this);
     return TRUE;
 }
 
@@ -546,6 +549,27 @@ void Graph::Run(BOOL bSingleThreaded) {
     }
 }
 
+BOOL Graph::EnableLaunchCapture(BOOL bEnable) {
+    std::lock_guard<std::recursive_mutex> g(m_lock);
+    if (m_eState == PTGS_RUNNING) return Runtime::HandleError("%s: EnableLaunchCapture while running", GetName());
+    if (m_pReplay) {
+        delete m_pReplay;
+        m_pReplay = nullptr;
+    }
+    if (bEnable) m_pReplay = new GraphReplay(this);
+    return TRUE;
+}
+
+BOOL Graph::IsLaunchCaptureArmed() {
+    std::lock_guard<std::recursive_mutex> g(m_lock);
+    return m_pReplay && m_pReplay->IsArmed();
+}
+
+UINT Graph::GetLaunchCaptureReplayCount() {
+    std::lock_guard<std::recursive_mutex> g(m_lock);
+    return m_pReplay ? m_pReplay->GetReplayCount() : 0;
+}
+
 // Work that should lead to a dispatch: a queued task, a running dispatch, or a block waiting in a
 // channel or an input port.  Graph lock held.
 BOOL Graph::HasPendingWorkLocked() {
@@ -711,6 +735,10 @@ void Graph::Teardown() {
     m_vChannels.clear();
     m_mChannels.clear();
     ReleaseScopedPools();
+    if (m_pReplay) {
+        delete m_pReplay;
+        m_pReplay = nullptr;
+    }
     m_eState = PTGS_TEARDOWNCOMPLETE;
 }
 

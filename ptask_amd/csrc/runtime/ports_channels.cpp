@@ -397,6 +397,7 @@ void Channel::PushDescriptors(Datablock *pBlock) {
 }
 
 void Channel::Enqueue(Datablock *pBlock) {
+    if (m_eType == CT_GRAPH_OUTPUT && m_pGraph->GetReplay()) m_pGraph->GetReplay()->OnOutputEnqueue(this, pBlock);
     pBlock->AddRef();
     m_q.push_back(pBlock);
     if (m_pDstPort) m_pGraph->SignalPortTask(m_pDstPort);
@@ -432,6 +433,7 @@ BOOL Channel::PushToQueue(Datablock *pBlock, DWORD dwTimeout) {
         else if (!m_pGraph->RoomAvailable().wait_for(lk, std::chrono::milliseconds(dwTimeout), room))
             return FALSE;
         if (!m_pGraph->IsAlive()) return FALSE;
+        if (m_eType == CT_GRAPH_INPUT && m_pGraph->GetReplay()) m_pGraph->GetReplay()->OnInputPush(this, pBlock);
     }
     Enqueue(pBlock);
     return TRUE;

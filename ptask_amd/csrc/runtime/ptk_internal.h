@@ -6,6 +6,7 @@
 #include <condition_variable>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -190,6 +191,7 @@ public:
     void ReleaseDeviceResources() override;
     // The stream this task launches on (created on first dispatch).
     HIPAsyncContext *GetContext() { return m_pContext; }
+    Accelerator *GetHIPAccelerator() { return (Accelerator *)m_pAccelerator; }
 
 protected:
     BOOL PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vector<Datablock *> &vOut,
@@ -207,6 +209,62 @@ private:
 public:
     double GetTotalDispatchDeviceMsResolved();
     double GetTotalDispatchDeviceMs() override { return GetTotalDispatchDeviceMsResolved(); }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Launch capture of a graph (replay.cpp): a recorded run replayed as one hipGraph launch.
+class GraphReplay {
+public:
+    enum State { RECORDING, ARMED, FAILED };
+    explicit GraphReplay(Graph *pGraph);
+    ~GraphReplay();
+    State GetState() { return m_eState; }
+    BOOL IsArmed() { return m_eState == ARMED; }
+    UINT GetReplayCount() { return m_uiReplays; }
+    UINT GetLaunchesPerReplay() { return m_uiLaunchesPerReplay; }
+    // recording hooks: graph input and output channel enqueues (graph lock held), HIP launches
+    // (any runner thread), completed dispatches (graph lock held)
+    void OnInputPush(Channel *pChannel, Datablock *pBlock);
+    void OnOutputEnqueue(Channel *pChannel, Datablock *pBlock);
+    void OnLaunch(Task *pTask, Accelerator *pAcc, ptk_hip_function fn, const PTASKDIM3 &grid, const PTASKDIM3 &block,
+                  const unsigned char *args, UINT cbArgs, const std::vector<std::pair<UINT, Datablock *>> &ptrs,
+                  UINT uiIterations);
+    void OnDispatchDone(Task *pTask);
+    // graph lock held: one replay when every data input holds a block; TRUE when it ran
+    BOOL TryReplay();
+    // the stream's work is finished and every resource returned
+    void Release();
+
+private:
+    struct Launch {
+        Task *pTask;
+        ptk_hip_function fn;
+        PTASKDIM3 grid, block;
+        std::vector<unsigned char> args;
+        std::vector<std::pair<UINT, Datablock *>> ptrs;  // (argument offset, block)
+        UINT uiIterations;
+    };
+    struct Staging {
+        void *dptr = nullptr;
+        size_t bytes = 0, logical = 0;
+        bool in = false, out = false;
+    };
+    void Fail(const char *why);
+    void Seal();
+
+    Graph *m_pGraph;
+    State m_eState = RECORDING;
+    std::mutex m_recLock;
+    Accelerator *m_pAccelerator = nullptr;
+    HIPAsyncContext *m_pContext = nullptr;
+    std::vector<Launch> m_vLaunches;
+    std::map<Channel *, Datablock *> m_mInputs, m_mOutputs;  // recorded blocks (keys once sealed)
+    bool m_bKeysOnly = false;
+    std::set<Task *> m_sDispatched;
+    std::map<Datablock *, Staging> m_mStaging;
+    std::vector<Datablock *> m_vPinned;
+    ptk_hip_graph m_hGraph = nullptr;
+    UINT m_uiReplays = 0, m_uiLaunchesPerReplay = 0;
 };
 
 // ---------------------------------------------------------------------------------------------
