@@ -294,7 +294,33 @@ def leg_captured(n: int, steps: int) -> dict:
     out["speedup_capture_alone"] = round(out["captured"]["value"] / out["direct"]["value"], 3)
     out["speedup_vs_plain"] = round(out["captured"]["value"] / out["plain"]["value"], 2)
     out["checksums_equal"] = sums["plain"] == sums["captured"] == sums["direct"]
+    out["ptaskapi"] = leg_captured_ptaskapi(n)
     return out
+
+
+def leg_captured_ptaskapi(n: int) -> dict:
+    """The reference's six-task graph built through ptaskapi (unrolled, graphfdtd.cpp:735-739), run
+    three times: task-by-task dispatch by the runtime vs the runtime's launch capture (the first
+    run recorded, the later runs replayed as one hipGraph launch each; Graph::EnableLaunchCapture).
+    Same kernels both ways, so the ratio is the effect of capture alone.  Chained runs: each run
+    starts from the previous run's device-resident outputs, and the timed region covers three runs
+    after one warm-up run (recorded, under capture) from a drained device to completion."""
+    from ptask_amd import paths
+    res = {}
+    for edge, steps in ((n, 20), (32, 50)):
+        base = ["--kernels", paths.kernel_path("fdtd.hsaco"), "--form", "six", "--mode", "unrolled",
+                "--n", edge, "--steps", steps, "--repeat", 3, "--warmup", 1, "--chain", "--seed", 3]
+        plain, cap = run_app("ptk_fdtd", base), run_app("ptk_fdtd", base + ["--capture"])
+        res[f"{edge}^3x{steps}"] = {
+            "plain_ms_per_run": plain["ms_per_run"], "captured_ms_per_run": cap["ms_per_run"],
+            "plain_mcups": round(edge ** 3 * steps / (plain["ms_per_run"] / 1e3) / 1e6, 1),
+            "captured_mcups": round(edge ** 3 * steps / (cap["ms_per_run"] / 1e3) / 1e6, 1),
+            "speedup": round(plain["ms_per_run"] / cap["ms_per_run"], 3),
+            "tasks_per_run": plain["dispatches_last_run"], "captured_dispatches_per_run": cap["dispatches_last_run"],
+            "replays": cap["capture"]["replays"],
+            "checksums_equal": [c["fnv1a"] for c in plain["checksums"]] == [c["fnv1a"] for c in cap["checksums"]]}
+        progress(f"ptaskapi capture {edge}^3 x {steps}: plain {plain['ms_per_run']} ms, captured {cap['ms_per_run']} ms")
+    return res
 
 
 def leg_gmm() -> dict:

@@ -468,7 +468,7 @@ BOOL HIPTask::PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vecto
     std::vector<Touch> touched;
     touched.reserve(m_vBindings.size());
     GraphReplay *replay = m_pGraph->GetReplay();
-    std::vector<std::pair<UINT, Datablock *>> ptrArgs;  // for a launch capture being recorded
+    std::vector<GraphReplay::PtrArg> ptrArgs;  // for a launch capture being recorded
     for (Binding &b : m_vBindings) {
         Datablock *blk = b.bInput ? vIn[b.uiSlot] : vOut[b.uiSlot];
         if (!blk) return Runtime::HandleError("%s: argument %s has no block", GetTaskName(),
@@ -485,7 +485,7 @@ BOOL HIPTask::PlatformSpecificDispatch(std::vector<Datablock *> &vIn, std::vecto
                                                b.pPort->GetVariableBinding());
         memcpy(m_vArgs.data() + b.uiOffset, &dptr, sizeof(void *));
         touched.push_back({blk->GetPlatformBuffer(m_pAccelerator->GetMemorySpaceId()), b.bWritable});
-        if (replay) ptrArgs.push_back(std::make_pair(b.uiOffset, blk));
+        if (replay) ptrArgs.push_back({b.uiOffset, blk, b.bWritable});
     }
     // (2) order after every producer (and, for writes, every reader) on other streams
     for (Touch &t : touched) t.buf->WaitOutstanding(m_pContext, t.write);

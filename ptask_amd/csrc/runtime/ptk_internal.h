@@ -226,9 +226,13 @@ public:
     // (any runner thread), completed dispatches (graph lock held)
     void OnInputPush(Channel *pChannel, Datablock *pBlock);
     void OnOutputEnqueue(Channel *pChannel, Datablock *pBlock);
+    struct PtrArg {
+        UINT uiOffset;      // byte offset of the pointer in the packed arguments
+        Datablock *pBlock;
+        BOOL bWrite;        // the launch writes the block
+    };
     void OnLaunch(Task *pTask, Accelerator *pAcc, ptk_hip_function fn, const PTASKDIM3 &grid, const PTASKDIM3 &block,
-                  const unsigned char *args, UINT cbArgs, const std::vector<std::pair<UINT, Datablock *>> &ptrs,
-                  UINT uiIterations);
+                  const unsigned char *args, UINT cbArgs, const std::vector<PtrArg> &ptrs, UINT uiIterations);
     void OnDispatchDone(Task *pTask);
     // graph lock held: one replay when every data input holds a block; TRUE when it ran
     BOOL TryReplay();
@@ -241,7 +245,7 @@ private:
         ptk_hip_function fn;
         PTASKDIM3 grid, block;
         std::vector<unsigned char> args;
-        std::vector<std::pair<UINT, Datablock *>> ptrs;  // (argument offset, block)
+        std::vector<PtrArg> ptrs;
         UINT uiIterations;
     };
     struct Staging {
@@ -251,6 +255,7 @@ private:
     };
     void Fail(const char *why);
     void Seal();
+    bool CaptureLaunches();
 
     Graph *m_pGraph;
     State m_eState = RECORDING;

@@ -48,7 +48,7 @@ void GraphReplay::Release() {
     for (Datablock *b : m_vPinned) b->Release();
     m_vPinned.clear();
     for (Launch &l : m_vLaunches)
-        for (auto &pa : l.ptrs) pa.second->Release();
+        for (auto &pa : l.ptrs) pa.pBlock->Release();
     m_vLaunches.clear();
     if (!m_bKeysOnly) {
         for (auto &kv : m_mInputs) kv.second->Release();
@@ -92,7 +92,7 @@ void GraphReplay::OnOutputEnqueue(Channel *pChannel, Datablock *pBlock) {
 
 void GraphReplay::OnLaunch(Task *pTask, Accelerator *pAcc, ptk_hip_function fn, const PTASKDIM3 &grid,
                            const PTASKDIM3 &block, const unsigned char *args, UINT cbArgs,
-                           const std::vector<std::pair<UINT, Datablock *>> &ptrs, UINT uiIterations) {
+                           const std::vector<PtrArg> &ptrs, UINT uiIterations) {
     std::lock_guard<std::mutex> g(m_recLock);
     if (m_eState != RECORDING) return;
     if (m_pAccelerator && m_pAccelerator != pAcc) return Fail("tasks run on more than one accelerator");
@@ -104,7 +104,7 @@ void GraphReplay::OnLaunch(Task *pTask, Accelerator *pAcc, ptk_hip_function fn, 
     l.block = block;
     l.args.assign(args, args + cbArgs);
     l.ptrs = ptrs;
-    for (auto &pa : l.ptrs) pa.second->AddRef();
+    for (auto &pa : l.ptrs) pa.pBlock->AddRef();
     l.uiIterations = uiIterations;
     m_vLaunches.push_back(std::move(l));
 }
@@ -161,22 +161,41 @@ void GraphReplay::Seal() {
     std::set<Datablock *> pinned;
     for (Launch &l : m_vLaunches)
         for (auto &pa : l.ptrs)
-            if (!m_mStaging.count(pa.second) && pinned.insert(pa.second).second) {
-                pa.second->AddRef();
-                m_vPinned.push_back(pa.second);
+            if (!m_mStaging.count(pa.pBlock) && pinned.insert(pa.pBlock).second) {
+                pa.pBlock->AddRef();
+                m_vPinned.push_back(pa.pBlock);
             }
-    // capture the re-targeted launches in recorded order
+    if (!CaptureLaunches()) return Fail("stream capture failed");
+    m_uiLaunchesPerReplay = 0;
+    for (Launch &l : m_vLaunches) m_uiLaunchesPerReplay += l.uiIterations;
+    // the recorded input and output blocks go back to the application and the pools; their
+    // addresses stay as the keys of the staging buffers, never dereferenced again
+    for (Launch &l : m_vLaunches)
+        for (auto &pa : l.ptrs) pa.pBlock->Release();
+    m_vLaunches.clear();
+    for (auto &kv : m_mInputs) kv.second->Release();
+    for (auto &kv : m_mOutputs) kv.second->Release();
+    m_bKeysOnly = true;
+    m_eState = ARMED;
+}
+
+// The re-targeted launches in recorded order, captured on the replay stream.  One stream: a
+// capture forked over several streams (events recorded on the capturing stream, waited on by the
+// others) ended the process with a segmentation fault inside the HIP runtime on the first run,
+// so the captured graph is a chain and independent launches (the three H or E updates of a Yee
+// step) run one after another.
+bool GraphReplay::CaptureLaunches() {
     ptk_hip_stream s = m_pContext->GetStream();
-    if (ptk_hip_stream_begin_capture(s) != PTK_HIP_OK) return Fail("stream capture could not begin");
+    if (ptk_hip_stream_begin_capture(s) != PTK_HIP_OK) return false;
     bool ok = true;
     for (Launch &l : m_vLaunches) {
         std::vector<unsigned char> args = l.args;
         for (auto &pa : l.ptrs) {
             void *p = nullptr;
-            auto it = m_mStaging.find(pa.second);
+            auto it = m_mStaging.find(pa.pBlock);
             if (it != m_mStaging.end()) p = it->second.dptr;
-            else p = pa.second->GetPlatformBuffer(m_pAccelerator->GetMemorySpaceId())->GetBuffer();
-            memcpy(args.data() + pa.first, &p, sizeof(void *));
+            else p = pa.pBlock->GetPlatformBuffer(m_pAccelerator->GetMemorySpaceId())->GetBuffer();
+            memcpy(args.data() + pa.uiOffset, &p, sizeof(void *));
         }
         for (UINT it = 0; it < l.uiIterations && ok; ++it)
             ok = ptk_hip_launch(l.fn, (unsigned)l.grid.x, (unsigned)l.grid.y, (unsigned)l.grid.z, (unsigned)l.block.x,
@@ -184,19 +203,12 @@ void GraphReplay::Seal() {
     }
     ptk_hip_graph graph = nullptr;
     const int rc = ptk_hip_stream_end_capture(s, &graph);
-    if (!ok || rc != PTK_HIP_OK || !graph) return Fail("stream capture failed");
+    if (!ok || rc != PTK_HIP_OK || !graph) {
+        if (graph) ptk_hip_graph_destroy(graph);
+        return false;
+    }
     m_hGraph = graph;
-    m_uiLaunchesPerReplay = 0;
-    for (Launch &l : m_vLaunches) m_uiLaunchesPerReplay += l.uiIterations;
-    // the recorded input and output blocks go back to the application and the pools; their
-    // addresses stay as the keys of the staging buffers, never dereferenced again
-    for (Launch &l : m_vLaunches)
-        for (auto &pa : l.ptrs) pa.second->Release();
-    m_vLaunches.clear();
-    for (auto &kv : m_mInputs) kv.second->Release();
-    for (auto &kv : m_mOutputs) kv.second->Release();
-    m_bKeysOnly = true;
-    m_eState = ARMED;
+    return true;
 }
 
 // ---- replay -----------------------------------------------------------------------------------
