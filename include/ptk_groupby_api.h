@@ -2,9 +2,9 @@
  * ptk_groupby_api.h -- the groupby pipeline as direct library calls (libptk_groupby.so), for
  * callers that hold their rows in device memory outside a PTask graph.
  *
- * One call runs the same five task kernels the groupby graph dispatches (GroupbyAccumulate,
- * GroupbyMerge, GroupbyTotals, GroupbyOffsets, GroupbyReshuffle; layouts in ptk_groupby.h) on
- * the caller's stream, in stream order, without synchronising.  Results are bitwise those of the
+ * One call runs the same seven task kernels the groupby graph dispatches (GroupbyHistogram,
+ * GroupbyBucketScan, GroupbyPartition, GroupbyKeyCount, GroupbyKeyScan, GroupbyReshuffle,
+ * GroupbyTotals; layouts in ptk_groupby.h) on the caller's stream, in stream order, without synchronising.  Results are bitwise those of the
  * graph: per-key int64 counts and sums, the groups block {G, rows, errors, then G {key, count,
  * sum} in key order}, offsets, and the rows grouped by ascending key with the rows of a group in
  * input order.  Keys lie in [0, numKeys]; other keys are counted as errors and dropped.
@@ -28,7 +28,7 @@ enum {
 };
 
 /* Bytes of device workspace ptk_groupby_run needs for n rows over keys [0, numKeys] (the
- * per-key table and the chunk / merge-group scratch of ptk_groupby.h); -1 for a bad shape. */
+ * per-key table and the chunk / tile scratch of ptk_groupby.h); -1 for a bad shape. */
 long long ptk_groupby_workspace_bytes(long long n, int numKeys);
 
 /* The groupby graph (ptaskUnitTest/graphgroupby.cpp:332-387: count unique keys, allocate,
@@ -48,7 +48,7 @@ int ptk_groupby_run(const int *keys, const int *vals, long long n, int numKeys, 
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  As ptk_groupby_run,
  * but the workspace is a static device array of PTK_GB_NOALLOC_BYTES inside the library, one per
  * GPU, held until the process ends; the call allocates nothing and fails with PTK_GB_ESPACE when
- * ptk_groupby_workspace_bytes(n, numKeys) exceeds it (about 3.5M rows over 65536 keys).  The
+ * ptk_groupby_workspace_bytes(n, numKeys) exceeds it (about 11M rows over 65536 keys).  The
  * array keeps the last call's contents; nothing in it needs clearing between calls.  Calls that
  * overlap on one GPU (other streams, other threads) share the array and race: serialise them. */
 #define PTK_GB_NOALLOC_BYTES (64ull << 20)

@@ -109,161 +109,200 @@ void pfxsum(unsigned int n, void **a) {
     ptk_oracle_pfxsum_i32((const int32_t *)a[0], (int32_t *)a[1], IARG(a, 2));
 }
 
-/* The chunk / bucket pipeline of the HIP kernels (include/ptk_groupby.h) restated on the host:
- * the same scratch layout, with each chunk's distinct keys in ascending key order (the device
- * writes them in any order inside a bucket; nothing downstream depends on it); merge bases,
- * group bases and ranks follow row order, as on the device. */
-static int cmp_i32(const void *x, const void *y) {
-    const int32_t a = *(const int32_t *)x, b = *(const int32_t *)y;
-    return (a > b) - (a < b);
+/* The two-level radix partition of the HIP kernels (include/ptk_groupby.h) restated on the host
+ * with the same scratch layout: each pass fills the arrays the device pass fills, with the same
+ * values (counts, prefixes and positions are functions of the input in row order, so a
+ * sequential walk over the rows gives what the device's ballot ranks give). */
+#define GB_SCRATCH_OR_RETURN(S, sp, n, numKeys)                                                \
+    if ((n) < 0 || (numKeys) < 0 || (long long)(numKeys) + 1 > PTK_GB_MAX_KEYS) return;           \
+    PTK_GB_SCRATCH S = ptk_gb_scratch((sp), (n), (numKeys))
+
+/* GroupbyHistogram(keys, n, numKeys, scratch): rows per bucket per chunk, invalid keys per chunk. */
+void GroupbyHistogram(unsigned int n_, void **a) {
+    (void)n_;
+    const int32_t *keys = (const int32_t *)a[0];
+    const long long n = IARG(a, 1);
+    const int numKeys = IARG(a, 2);
+    GB_SCRATCH_OR_RETURN(S, a[3], n, numKeys);
+    const long long NB = S.buckets;
+    for (long long c = 0; c < S.chunks; ++c) {
+        unsigned *h = S.hist + c * NB;
+        memset(h, 0, sizeof(unsigned) * (size_t)NB);
+        unsigned err = 0;
+        for (long long r = c * PTK_GB_CHUNK; r < (c + 1) * PTK_GB_CHUNK && r < n; ++r) {
+            if (keys[r] < 0 || keys[r] > numKeys) ++err;
+            else ++h[keys[r] / PTK_GB_BUCKET];
+        }
+        S.berr[c] = err;
+    }
 }
 
-void GroupbyAccumulate(unsigned int n_, void **a) {
+/* GroupbyBucketScan(scratch, n, numKeys): per bucket, the rows of the earlier chunks (in place)
+ * and the bucket's rows. */
+void GroupbyBucketScan(unsigned int n_, void **a) {
+    (void)n_;
+    const long long n = IARG(a, 1);
+    const int numKeys = IARG(a, 2);
+    GB_SCRATCH_OR_RETURN(S, a[0], n, numKeys);
+    const long long NB = S.buckets;
+    for (long long b = 0; b < NB; ++b) {
+        unsigned run = 0;
+        for (long long c = 0; c < S.chunks; ++c) {
+            const unsigned v = S.hist[c * NB + b];
+            S.hist[c * NB + b] = run;
+            run += v;
+        }
+        S.btot[b] = run;
+    }
+}
+
+/* GroupbyPartition(keys, vals, n, numKeys, scratch): bucket and tile starts, the tile table, the
+ * invalid keys; each valid row's low key byte and value at bstart[bucket] + its rank among the
+ * bucket's rows (row order). */
+void GroupbyPartition(unsigned int n_, void **a) {
     (void)n_;
     const int32_t *keys = (const int32_t *)a[0], *vals = (const int32_t *)a[1];
     const long long n = IARG(a, 2);
     const int numKeys = IARG(a, 3);
-    if (n <= 0 || numKeys < 0 || (long long)numKeys + 1 > PTK_GB_MAX_KEYS) return;
-    PTK_GB_SCRATCH S = ptk_gb_scratch(a[4], n, numKeys);
-    int32_t *tmp = (int32_t *)malloc(sizeof(int32_t) * PTK_GB_CHUNK);
-    for (long long c = 0; c < S.chunks; ++c) {
-        const long long r0 = c * PTK_GB_CHUNK, r1 = r0 + PTK_GB_CHUNK < n ? r0 + PTK_GB_CHUNK : n;
-        unsigned err = 0, m = 0;
-        for (long long r = r0; r < r1; ++r) {
-            if (keys[r] < 0 || keys[r] > numKeys) ++err;
-            else tmp[m++] = keys[r];
+    GB_SCRATCH_OR_RETURN(S, a[4], n, numKeys);
+    const long long NB = S.buckets;
+    unsigned rows = 0, tiles = 0;
+    for (long long b = 0; b < NB; ++b) {
+        S.bstart[b] = rows;
+        S.tstart[b] = tiles;
+        const unsigned nt = (S.btot[b] + PTK_GB_TILE - 1) / PTK_GB_TILE;
+        for (unsigned t = 0; t < nt; ++t) {
+            const unsigned r0 = rows + t * PTK_GB_TILE, end = rows + S.btot[b];
+            unsigned *ti = S.tinfo + 4ull * (tiles + t);
+            ti[0] = (unsigned)b;
+            ti[1] = r0;
+            ti[2] = r0 + PTK_GB_TILE < end ? r0 + PTK_GB_TILE : end;
+            ti[3] = 0;
         }
-        qsort(tmp, m, sizeof(int32_t), cmp_i32);
-        PTK_GB_ENTRY *e = S.entry + c * PTK_GB_CHUNK;
-        unsigned d = 0;
-        for (unsigned i = 0; i < m; ++i)
-            if (i == 0 || tmp[i] != tmp[i - 1]) {
-                e[d].key = tmp[i];
-                e[d].count = 0;
-                e[d].sum = 0;
-                ++d;
-            }
-        for (long long r = r0; r < r1; ++r) {  /* add each row to its entry (binary search) */
+        rows += S.btot[b];
+        tiles += nt;
+    }
+    S.bstart[NB] = rows;
+    S.tstart[NB] = tiles;
+    uint64_t err = 0;
+    for (long long c = 0; c < S.chunks; ++c) err += S.berr[c];
+    S.head[0] = err;
+    unsigned *dest = (unsigned *)malloc(sizeof(unsigned) * (size_t)NB);
+    for (long long c = 0; c < S.chunks; ++c) {
+        for (long long b = 0; b < NB; ++b) dest[b] = S.bstart[b] + S.hist[c * NB + b];
+        for (long long r = c * PTK_GB_CHUNK; r < (c + 1) * PTK_GB_CHUNK && r < n; ++r) {
             const int32_t k = keys[r];
             if (k < 0 || k > numKeys) continue;
-            unsigned lo = 0, hi = d;
-            while (hi - lo > 1) {
-                const unsigned mid = (lo + hi) / 2;
-                if (e[mid].key <= k) lo = mid;
-                else hi = mid;
-            }
-            e[lo].count += 1;
-            e[lo].sum = (long long)((uint64_t)e[lo].sum + (uint64_t)(int64_t)vals[r]);
+            const unsigned g = dest[k / PTK_GB_BUCKET]++;
+            S.tlow[g] = (unsigned char)(k & (PTK_GB_BUCKET - 1));
+            S.tval[g] = vals[r];
         }
-        unsigned *boff = S.boff + c * (S.buckets + 1);
-        unsigned i = 0;
-        for (long long b = 0; b <= S.buckets; ++b) {  /* entries of buckets < b */
-            while (i < d && e[i].key / PTK_GB_BUCKET < b) ++i;
-            boff[b] = i;
-        }
-        S.berr[c] = err;
     }
-    free(tmp);
+    free(dest);
 }
 
-/* GroupbyMerge(scratch, n, numKeys): per merge group, each entry's base (the key's rows in the
- * group's earlier chunks) replaces its sum; the group's per-key totals go to gcnt / gsum. */
-void GroupbyMerge(unsigned int n_, void **a) {
+/* GroupbyKeyCount(scratch, n, numKeys): rows per key (low byte) per tile. */
+void GroupbyKeyCount(unsigned int n_, void **a) {
     (void)n_;
     const long long n = IARG(a, 1);
     const int numKeys = IARG(a, 2);
-    if (n < 0 || numKeys < 0 || (long long)numKeys + 1 > PTK_GB_MAX_KEYS) return;
-    PTK_GB_SCRATCH S = ptk_gb_scratch(a[0], n, numKeys);
-    const long long KP = S.buckets * PTK_GB_BUCKET;
-    memset(S.gcnt, 0, sizeof(unsigned) * (size_t)(S.groups * KP));
-    memset(S.gsum, 0, sizeof(uint64_t) * (size_t)(S.groups * KP));
-    for (long long c = 0; c < S.chunks; ++c) {  /* within a merge group, chunks in row order */
-        unsigned *gc = S.gcnt + (c / PTK_GB_GROUP) * KP;
-        unsigned long long *gs = S.gsum + (c / PTK_GB_GROUP) * KP;
-        const unsigned d = S.boff[c * (S.buckets + 1) + S.buckets];
-        for (unsigned i = 0; i < d; ++i) {
-            PTK_GB_ENTRY *e = S.entry + c * PTK_GB_CHUNK + i;
-            gs[e->key] += (uint64_t)e->sum;
-            e->sum = (long long)gc[e->key];  /* the entry's base inside its group replaces its sum */
-            gc[e->key] += e->count;
+    GB_SCRATCH_OR_RETURN(S, a[0], n, numKeys);
+    for (unsigned t = 0; t < S.tstart[S.buckets]; ++t) {
+        unsigned *c2 = S.c2 + (size_t)t * PTK_GB_BUCKET;
+        memset(c2, 0, sizeof(unsigned) * PTK_GB_BUCKET);
+        for (unsigned r = S.tinfo[4ull * t + 1]; r < S.tinfo[4ull * t + 2]; ++r) ++c2[S.tlow[r]];
+    }
+}
+
+/* GroupbyKeyScan(scratch, n, numKeys, offsets): per key, the rows in the bucket's earlier tiles
+ * (in place), its count, its first output row; the non-empty keys per half bucket. */
+void GroupbyKeyScan(unsigned int n_, void **a) {
+    (void)n_;
+    const long long n = IARG(a, 1);
+    const int numKeys = IARG(a, 2);
+    int64_t *offsets = (int64_t *)a[3];
+    GB_SCRATCH_OR_RETURN(S, a[0], n, numKeys);
+    for (long long b = 0; b < S.buckets; ++b) {
+        unsigned row = S.bstart[b], nz[2] = {0, 0};
+        for (int k = 0; k < PTK_GB_BUCKET; ++k) {
+            unsigned run = 0;
+            for (unsigned t = S.tstart[b]; t < S.tstart[b + 1]; ++t) {
+                unsigned *c = S.c2 + (size_t)t * PTK_GB_BUCKET + k;
+                const unsigned v = *c;
+                *c = run;
+                run += v;
+            }
+            const long long key = b * PTK_GB_BUCKET + k;
+            S.kcnt[key] = key <= numKeys ? run : 0u;
+            if (key <= numKeys) {
+                offsets[key] = row;
+                row += run;
+                nz[k / (PTK_GB_BUCKET / 2)] += run ? 1u : 0u;
+            }
+        }
+        S.nz[2 * b] = nz[0];
+        S.nz[2 * b + 1] = nz[1];
+    }
+}
+
+/* GroupbyReshuffle(scratch, n, numKeys, offsets, outKeys, outVals): each tile's rows at
+ * offsets[key] + the key's rows in the earlier tiles + its rank in the tile (row order); the
+ * tile's per-key sums. */
+void GroupbyReshuffle(unsigned int n_, void **a) {
+    (void)n_;
+    const long long n = IARG(a, 1);
+    const int numKeys = IARG(a, 2);
+    const int64_t *offsets = (const int64_t *)a[3];
+    int32_t *outKeys = (int32_t *)a[4], *outVals = (int32_t *)a[5];
+    GB_SCRATCH_OR_RETURN(S, a[0], n, numKeys);
+    int64_t cur[PTK_GB_BUCKET];
+    for (unsigned t = 0; t < S.tstart[S.buckets]; ++t) {
+        const long long b = S.tinfo[4ull * t], key0 = b * PTK_GB_BUCKET;
+        uint64_t *s2 = S.s2 + (size_t)t * PTK_GB_BUCKET;
+        for (int k = 0; k < PTK_GB_BUCKET; ++k) {
+            cur[k] = (key0 + k <= numKeys ? offsets[key0 + k] : 0) + S.c2[(size_t)t * PTK_GB_BUCKET + k];
+            s2[k] = 0;
+        }
+        for (unsigned r = S.tinfo[4ull * t + 1]; r < S.tinfo[4ull * t + 2]; ++r) {
+            const unsigned k = S.tlow[r];
+            const int64_t g = cur[k]++;
+            outKeys[g] = (int32_t)(key0 + k);
+            outVals[g] = S.tval[r];
+            s2[k] += (uint64_t)(int64_t)S.tval[r];
         }
     }
 }
 
-/* GroupbyTotals(scratch, n, numKeys, table): groups in row order give each key's per-group base
- * (replacing gcnt) and its count and sum; the bucket totals and the error count. */
+/* GroupbyTotals(scratch, n, numKeys, table, groups): per key its count and its sum over the
+ * bucket's tiles (tile order); the group records in key order and the header. */
 void GroupbyTotals(unsigned int n_, void **a) {
     (void)n_;
     const long long n = IARG(a, 1);
     const int numKeys = IARG(a, 2);
-    int64_t *table = (int64_t *)a[3];
-    if (n < 0 || numKeys < 0 || (long long)numKeys + 1 > PTK_GB_MAX_KEYS) return;
-    PTK_GB_SCRATCH S = ptk_gb_scratch(a[0], n, numKeys);
-    const long long KP = S.buckets * PTK_GB_BUCKET;
+    int64_t *table = (int64_t *)a[3], *groups = (int64_t *)a[4];
+    GB_SCRATCH_OR_RETURN(S, a[0], n, numKeys);
     uint64_t *cnt = (uint64_t *)table, *sum = cnt + (numKeys + 1), *err = cnt + 3 * ((size_t)numKeys + 1);
-    for (long long k = 0; k < KP; ++k) {
-        uint64_t t = 0, v = 0;
-        for (long long g = 0; g < S.groups; ++g) {
-            const unsigned c = S.gcnt[g * KP + k];
-            S.gcnt[g * KP + k] = (unsigned)t;
-            t += c;
-            v += S.gsum[g * KP + k];
-        }
-        if (k <= numKeys) {
-            cnt[k] = t;
-            sum[k] = v;
-        }
-    }
-    uint64_t errors = 0;
-    for (long long c = 0; c < S.chunks; ++c) errors += S.berr[c];
-    *err = errors;
-    for (long long b = 0; b < S.buckets; ++b) {
-        uint64_t rows = 0, nz = 0;
-        for (long long k = b * PTK_GB_BUCKET; k < (b + 1) * PTK_GB_BUCKET && k <= numKeys; ++k) {
-            rows += cnt[k];
-            nz += cnt[k] ? 1 : 0;
-        }
-        S.btot[2 * b] = (int64_t)rows;
-        S.btot[2 * b + 1] = (int64_t)nz;
-    }
-}
-
-void GroupbyOffsets(unsigned int n, void **a) {
-    (void)n;
-    gb_offsets((const int64_t *)a[0], IARG(a, 1), (int64_t *)a[3], (int64_t *)a[4]);
-}
-
-void GroupbyReshuffle(unsigned int n_, void **a) {
-    (void)n_;
-    const int32_t *keys = (const int32_t *)a[0], *vals = (const int32_t *)a[1];
-    const long long n = IARG(a, 2);
-    const int numKeys = IARG(a, 3);
-    const int64_t *offsets = (const int64_t *)a[4];
-    int32_t *outKeys = (int32_t *)a[6], *outVals = (int32_t *)a[7];
-    if (n <= 0 || numKeys < 0 || (long long)numKeys + 1 > PTK_GB_MAX_KEYS) return;
-    PTK_GB_SCRATCH S = ptk_gb_scratch(a[5], n, numKeys);
-    int64_t *next = (int64_t *)malloc(sizeof(int64_t) * PTK_GB_CHUNK);
-    for (long long c = 0; c < S.chunks; ++c) {
-        const long long r0 = c * PTK_GB_CHUNK, r1 = r0 + PTK_GB_CHUNK < n ? r0 + PTK_GB_CHUNK : n;
-        const PTK_GB_ENTRY *e = S.entry + c * PTK_GB_CHUNK;
-        const unsigned d = S.boff[c * (S.buckets + 1) + S.buckets];
-        const unsigned *gbase = S.gcnt + (c / PTK_GB_GROUP) * (S.buckets * PTK_GB_BUCKET);
-        for (unsigned i = 0; i < d; ++i) next[i] = offsets[e[i].key] + gbase[e[i].key] + e[i].sum;  /* sum: the merge base */
-        for (long long r = r0; r < r1; ++r) {
-            const int32_t k = keys[r];
-            if (k < 0 || k > numKeys || d == 0) continue;
-            unsigned lo = 0, hi = d;
-            while (hi - lo > 1) {
-                const unsigned mid = (lo + hi) / 2;
-                if (e[mid].key <= k) lo = mid;
-                else hi = mid;
+    uint64_t G = 0;
+    for (long long b = 0; b < S.buckets; ++b)
+        for (int k = 0; k < PTK_GB_BUCKET; ++k) {
+            const long long key = b * PTK_GB_BUCKET + k;
+            if (key > numKeys) break;
+            uint64_t v = 0;
+            for (unsigned t = S.tstart[b]; t < S.tstart[b + 1]; ++t) v += S.s2[(size_t)t * PTK_GB_BUCKET + k];
+            cnt[key] = S.kcnt[key];
+            sum[key] = v;
+            if (S.kcnt[key]) {
+                int64_t *e = groups + 3 + 3 * G;
+                e[0] = key;
+                e[1] = (int64_t)S.kcnt[key];
+                e[2] = (int64_t)v;
+                ++G;
             }
-            const int64_t pos = next[lo]++;
-            outKeys[pos] = k;
-            outVals[pos] = vals[r];
         }
-    }
-    free(next);
+    *err = S.head[0];
+    groups[0] = (int64_t)G;
+    groups[1] = (int64_t)S.bstart[S.buckets];
+    groups[2] = (int64_t)S.head[0];
 }
 
 /* CoalesceGroups (HostTasks/HostTasks.cpp:131-197) with the ABI of its coalesce_int_float entry

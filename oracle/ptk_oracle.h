@@ -61,11 +61,13 @@ void ptk_oracle_pca_gs_mt(const double *X, int M, int N, int K, int J, int cente
 
 /* Host-task ABI forms (void f(UINT, void**)), loaded by name by the HostAccelerator. */
 void pfxsum(unsigned int n, void **a);
-void GroupbyAccumulate(unsigned int n, void **a);
-void GroupbyOffsets(unsigned int n, void **a);
-void GroupbyMerge(unsigned int n, void **a);
-void GroupbyTotals(unsigned int n, void **a);
+void GroupbyHistogram(unsigned int n, void **a);
+void GroupbyBucketScan(unsigned int n, void **a);
+void GroupbyPartition(unsigned int n, void **a);
+void GroupbyKeyCount(unsigned int n, void **a);
+void GroupbyKeyScan(unsigned int n, void **a);
 void GroupbyReshuffle(unsigned int n, void **a);
+void GroupbyTotals(unsigned int n, void **a);
 void coalesce_int_float(unsigned int n, void **a);
 void GroupbyCoalesce(unsigned int n, void **a);
 void GroupbyCoalesceSizes(unsigned int n, void **a);

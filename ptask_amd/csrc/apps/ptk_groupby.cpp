@@ -1,13 +1,16 @@
 // ptk_groupby.cpp -- the groupby and prefix-sum graphs as PTask applications (the roles of
 // ptaskUnitTest/graphgroupby.cpp and ptaskUnitTest/cpfxsum.cpp), against ptaskapi.h only.
 //
-// --workload groupby: five tasks per run (graphgroupby.cpp builds the same stages:
-//   allocate/count/prefix-sum/reshuffle, :106-387; the layouts are in include/ptk_groupby.h):
-//     GroupbyAccumulate (keys, vals, n, numKeys)                         -> scratch
-//     GroupbyMerge      (scratch in/out, n, numKeys)                     -> scratch
-//     GroupbyTotals     (scratch in/out, n, numKeys)                     -> scratch, table
-//     GroupbyOffsets    (table, numKeys, scratch)                        -> offsets, groups
-//     GroupbyReshuffle  (keys, vals, n, numKeys, offsets, scratch)       -> outKeys, outVals
+// --workload groupby: seven tasks per run (graphgroupby.cpp builds the stages count, prefix-sum,
+//   reshuffle, :106-387; here as a stable two-level radix partition; the layouts are in
+//   include/ptk_groupby.h):
+//     GroupbyHistogram  (keys, n, numKeys)                               -> scratch
+//     GroupbyBucketScan (scratch in/out, n, numKeys)                     -> scratch
+//     GroupbyPartition  (keys, vals, n, numKeys, scratch in/out)         -> scratch
+//     GroupbyKeyCount   (scratch in/out, n, numKeys)                     -> scratch
+//     GroupbyKeyScan    (scratch in/out, n, numKeys)                     -> scratch, offsets
+//     GroupbyReshuffle  (scratch in/out, n, numKeys, offsets)            -> scratch, outKeys, outVals
+//     GroupbyTotals     (scratch, n, numKeys)                            -> table, groups
 //   and with --coalesce the reference's CoalesceGroups layout (HostTasks.cpp:131-197):
 //     GroupbyCoalesce   (outVals, outKeys, groups, offsets)              -> coalesced, groupCounts
 // --workload pfxsum: one task pfxsum(pin, pout, N) wired as cpfxsum.cpp:204-226 wires it.
@@ -20,6 +23,7 @@
 // first run uploads.  Prints one JSON object; --dump writes inputs and outputs.
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -47,7 +51,7 @@ struct Options {
     int repeat = 1, warmup = 0;
     std::string dump;
     bool profile = false;
-    int gbGrid = 2048;  // at most this many blocks for the two chunk-parallel groupby kernels
+    int gbGrid = 2048;  // at most this many blocks for GroupbyPartition (also at most two per CU)
     bool coalesce = false;  // append GroupbyCoalesce (the reference's CoalesceGroups layout)
     bool exact = false;     // ... with its outputs allocated to their exact sizes
     std::string serialize;    // write the graph as XML (and stop)
@@ -201,64 +205,89 @@ int main(int argc, char **argv) {
         DatablockTemplate *tGroups = Runtime::GetDatablockTemplate((char *)"gbGroups", sizeof(int64_t), (UINT)groupElems, 1, 1);
         const long long scratchBytes = ptk_gb_scratch_bytes(n, o.numKeys);
         DatablockTemplate *tScratch = Runtime::GetDatablockTemplate((char *)"gbScratch", 16, (UINT)(scratchBytes / 16), 1, 1);
-        const int chunkGrid = (int)std::max(1LL, std::min((long long)o.gbGrid, ptk_gb_chunks(n)));
+        const int chunks = (int)std::max(1LL, ptk_gb_chunks(n));
         const int buckets = (int)ptk_gb_buckets(o.numKeys);
-        // GroupbyAccumulate: per-chunk LDS aggregation into the scratch block
-        Port *aIn[4] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"keys", 0),
-                        Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"vals", 1),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", 2),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 3)};
-        Port *aOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch", 4)};
-        Task *ta = g->AddTask(K("GroupbyAccumulate"), 4, aIn, 1, aOut, "GroupbyAccumulate");
-        geometry(ta, chunkGrid, 512);
-        // GroupbyMerge: one block per (key bucket, merge group) unit; scratch in/out
-        Port *mIn[3] = {Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 0, 0),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", 1),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 2)};
-        Port *mOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch(out)", 0)};
-        Task *tm = g->AddTask(K("GroupbyMerge"), 3, mIn, 1, mOut, "GroupbyMerge");
-        geometry(tm, (int)std::min(65535LL, buckets * ptk_gb_groups(n)), PTK_GB_BUCKET);
-        // GroupbyTotals: one block per key bucket; scratch in/out, table out (every count, sum and
-        // the error count are written)
-        Port *tIn[3] = {Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 0, 0),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", 1),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 2)};
-        Port *tOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch(out)", 0),
-                         Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table", 3)};
-        Task *tt = g->AddTask(K("GroupbyTotals"), 3, tIn, 2, tOut, "GroupbyTotals");
-        geometry(tt, buckets, PTK_GB_BUCKET);
-        Port *oIn[3] = {Runtime::CreatePort(INPUT_PORT, tTable, uid++, (char *)"table", 0),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 1),
-                        Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 2)};
-        Port *oOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tOff, uid++, (char *)"offsets", 3),
+        const int tiles = (int)ptk_gb_tiles_max(n, o.numKeys);
+        int cus = 256;  // GroupbyPartition: two blocks per compute unit
+        if (hip)
+            if (Accelerator *acc = Runtime::GetAccelerator(ACCELERATOR_CLASS_HIP, 0))
+                if (acc->GetCoreCount()) cus = (int)acc->GetCoreCount();
+        const int partGrid = std::max(1, std::min(std::min(chunks, o.gbGrid), 2 * cus));
+        auto nk = [&](UINT i) {  // the sticky n and numKeys ports at formal indices i and i + 1
+            return std::array<Port *, 2>{Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", i),
+                                         Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", i + 1)};
+        };
+        // GroupbyHistogram: rows per bucket per chunk
+        auto hNK = nk(1);
+        Port *hIn[3] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"keys", 0), hNK[0], hNK[1]};
+        Port *hOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch", 3)};
+        Task *th = g->AddTask(K("GroupbyHistogram"), 3, hIn, 1, hOut, "GroupbyHistogram");
+        geometry(th, chunks, PTK_GB_HIST_THREADS);
+        // GroupbyBucketScan: chunk prefixes per bucket (scratch in/out)
+        auto bNK = nk(1);
+        Port *bIn[3] = {Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 0, 0), bNK[0], bNK[1]};
+        Port *bOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch(out)", 0)};
+        Task *tb = g->AddTask(K("GroupbyBucketScan"), 3, bIn, 1, bOut, "GroupbyBucketScan");
+        geometry(tb, buckets, PTK_GB_SCAN_THREADS);
+        // GroupbyPartition: rows by bucket into the scratch (scratch in/out)
+        auto pNK = nk(2);
+        Port *pIn[5] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"keys", 0),
+                        Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"vals", 1), pNK[0], pNK[1],
+                        Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 4, 0)};
+        Port *pOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch(out)", 4)};
+        Task *tp = g->AddTask(K("GroupbyPartition"), 5, pIn, 1, pOut, "GroupbyPartition");
+        geometry(tp, partGrid, PTK_GB_PART_THREADS);
+        // GroupbyKeyCount: rows per key per tile (scratch in/out)
+        auto cNK = nk(1);
+        Port *cIn[3] = {Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 0, 0), cNK[0], cNK[1]};
+        Port *cOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch(out)", 0)};
+        Task *tc = g->AddTask(K("GroupbyKeyCount"), 3, cIn, 1, cOut, "GroupbyKeyCount");
+        geometry(tc, tiles, PTK_GB_COUNT_THREADS);
+        // GroupbyKeyScan: tile prefixes per key, offsets (scratch in/out)
+        auto sNK = nk(1);
+        Port *sIn[3] = {Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 0, 0), sNK[0], sNK[1]};
+        Port *sOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch(out)", 0),
+                         Runtime::CreatePort(OUTPUT_PORT, tOff, uid++, (char *)"offsets", 3)};
+        Task *ts = g->AddTask(K("GroupbyKeyScan"), 3, sIn, 2, sOut, "GroupbyKeyScan");
+        geometry(ts, 2 * buckets, PTK_GB_KEYSCAN_THREADS);
+        // GroupbyReshuffle: rows by key into outKeys / outVals, per-tile sums (scratch in/out)
+        auto rNK = nk(1);
+        Port *rIn[4] = {Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 0, 0), rNK[0], rNK[1],
+                        Runtime::CreatePort(INPUT_PORT, tOff, uid++, (char *)"offsets", 3)};
+        Port *rOut[3] = {Runtime::CreatePort(OUTPUT_PORT, tScratch, uid++, (char *)"scratch(out)", 0),
+                         Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outKeys", 4),
+                         Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outVals", 5)};
+        Task *tr = g->AddTask(K("GroupbyReshuffle"), 4, rIn, 3, rOut, "GroupbyReshuffle");
+        geometry(tr, tiles, PTK_GB_TILE_THREADS);
+        // GroupbyTotals: the table and the group records
+        auto tNK = nk(1);
+        Port *tIn[3] = {Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 0), tNK[0], tNK[1]};
+        Port *tOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tTable, uid++, (char *)"table", 3),
                          Runtime::CreatePort(OUTPUT_PORT, tGroups, uid++, (char *)"groups", 4)};
-        Task *to = g->AddTask(K("GroupbyOffsets"), 3, oIn, 2, oOut, "GroupbyOffsets");
-        geometry(to, buckets, PTK_GB_BUCKET);
-        Port *rIn[6] = {Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"keys", 0),
-                        Runtime::CreatePort(INPUT_PORT, tRows, uid++, (char *)"vals", 1),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"n", 2),
-                        Runtime::CreatePort(STICKY_PORT, tInt, uid++, (char *)"numKeys", 3),
-                        Runtime::CreatePort(INPUT_PORT, tOff, uid++, (char *)"offsets", 4),
-                        Runtime::CreatePort(INPUT_PORT, tScratch, uid++, (char *)"scratch", 5)};
-        Port *rOut[2] = {Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outKeys", 6),
-                         Runtime::CreatePort(OUTPUT_PORT, tRows, uid++, (char *)"outVals", 7)};
-        Task *tr = g->AddTask(K("GroupbyReshuffle"), 6, rIn, 2, rOut, "GroupbyReshuffle");
-        geometry(tr, chunkGrid, PTK_GB_PLACE_THREADS);
-        keyIn = {g->AddInputChannel(aIn[0], "keys->acc"), g->AddInputChannel(rIn[0], "keys->shuffle")};
-        valIn = {g->AddInputChannel(aIn[1], "vals->acc"), g->AddInputChannel(rIn[1], "vals->shuffle")};
-        nIn = {g->AddInputChannel(aIn[2], "n->acc"), g->AddInputChannel(mIn[1], "n->merge"),
-               g->AddInputChannel(tIn[1], "n->totals"), g->AddInputChannel(rIn[2], "n->shuffle")};
-        kIn = {g->AddInputChannel(aIn[3], "nk->acc"), g->AddInputChannel(mIn[2], "nk->merge"),
-               g->AddInputChannel(tIn[2], "nk->totals"), g->AddInputChannel(oIn[1], "nk->offsets"),
-               g->AddInputChannel(rIn[3], "nk->shuffle")};
-        g->AddInternalChannel(aOut[0], mIn[0], "acc->merge");
-        g->AddInternalChannel(mOut[0], tIn[0], "merge->totals");
-        g->AddInternalChannel(tOut[1], oIn[0], "totals->offsets(table)");
-        g->AddInternalChannel(tOut[0], oIn[2], "totals->offsets(scratch)");
-        g->AddInternalChannel(tOut[0], rIn[5], "totals->shuffle(scratch)");
-        g->AddInternalChannel(oOut[0], rIn[4], "offsets->shuffle");
-        outs = {g->AddOutputChannel(oOut[1], "groups"), g->AddOutputChannel(rOut[0], "outKeys"),
-                g->AddOutputChannel(rOut[1], "outVals")};
+        Task *tt = g->AddTask(K("GroupbyTotals"), 3, tIn, 2, tOut, "GroupbyTotals");
+        geometry(tt, 4 * buckets, PTK_GB_TOTALS_THREADS);
+        keyIn = {g->AddInputChannel(hIn[0], "keys->histogram"), g->AddInputChannel(pIn[0], "keys->partition")};
+        valIn = {g->AddInputChannel(pIn[1], "vals->partition")};
+        nIn.clear();
+        kIn.clear();
+        Port *nkPorts[7][2] = {{hNK[0], hNK[1]}, {bNK[0], bNK[1]}, {pNK[0], pNK[1]}, {cNK[0], cNK[1]},
+                               {sNK[0], sNK[1]}, {rNK[0], rNK[1]}, {tNK[0], tNK[1]}};
+        static const char *stage[7] = {"histogram", "bucketscan", "partition", "keycount", "keyscan", "reshuffle", "totals"};
+        for (int i = 0; i < 7; ++i) {
+            nIn.push_back(g->AddInputChannel(nkPorts[i][0], (std::string("n->") + stage[i]).c_str()));
+            kIn.push_back(g->AddInputChannel(nkPorts[i][1], (std::string("nk->") + stage[i]).c_str()));
+        }
+        g->AddInternalChannel(hOut[0], bIn[0], "histogram->bucketscan");
+        g->AddInternalChannel(bOut[0], pIn[4], "bucketscan->partition");
+        g->AddInternalChannel(pOut[0], cIn[0], "partition->keycount");
+        g->AddInternalChannel(cOut[0], sIn[0], "keycount->keyscan");
+        g->AddInternalChannel(sOut[0], rIn[0], "keyscan->reshuffle(scratch)");
+        g->AddInternalChannel(sOut[1], rIn[3], "keyscan->reshuffle(offsets)");
+        g->AddInternalChannel(rOut[0], tIn[0], "reshuffle->totals");
+        Port *oOut[2] = {sOut[1], tOut[1]};  // offsets, groups (for the coalesced form below)
+        Port *rOutKV[2] = {rOut[1], rOut[2]};
+        outs = {g->AddOutputChannel(oOut[1], "groups"), g->AddOutputChannel(rOutKV[0], "outKeys"),
+                g->AddOutputChannel(rOutKV[1], "outVals")};
         outNames = {"groups", "outKeys", "outVals"};
         if (o.coalesce) {
             // GroupbyCoalesce: per group its key then its values (n + numKeys + 1 words at most),
@@ -280,12 +309,12 @@ int main(int argc, char **argv) {
                 // bound to the output it sizes (its in/out routing index), so the pulled blocks hold
                 // exactly rows + G and G words, the reference's wire format
                 DatablockTemplate *tBytes = Runtime::GetDatablockTemplate((char *)"gbBytes", sizeof(int32_t), 1, 1, 1);
-                Port *sIn[1] = {Runtime::CreatePort(INPUT_PORT, tGroups, uid++, (char *)"groups", 0)};
+                Port *sIn2[1] = {Runtime::CreatePort(INPUT_PORT, tGroups, uid++, (char *)"groups", 0)};
                 sOut[0] = Runtime::CreatePort(OUTPUT_PORT, tBytes, uid++, (char *)"coalescedBytes", 1);
                 sOut[1] = Runtime::CreatePort(OUTPUT_PORT, tBytes, uid++, (char *)"groupCountsBytes", 2);
-                Task *ts = g->AddTask(K("GroupbyCoalesceSizes"), 1, sIn, 2, sOut, "GroupbyCoalesceSizes");
-                geometry(ts, 1, 64);
-                g->AddInternalChannel(oOut[1], sIn[0], "offsets->sizes(groups)");
+                Task *tz = g->AddTask(K("GroupbyCoalesceSizes"), 1, sIn2, 2, sOut, "GroupbyCoalesceSizes");
+                geometry(tz, 1, 64);
+                g->AddInternalChannel(oOut[1], sIn2[0], "totals->sizes(groups)");
                 for (int q = 0; q < 2; ++q) {
                     Port *m = Runtime::CreatePort(META_PORT, tBytes, uid++, (char *)(q ? "groupCountsSize" : "coalescedSize"),
                                                   (UINT)-1, (UINT)q);
@@ -294,15 +323,15 @@ int main(int argc, char **argv) {
                 }
             }
             Task *tk = g->AddTask(K("GroupbyCoalesce"), (UINT)kIn4.size(), kIn4.data(), 2, kOut2, "GroupbyCoalesce");
-            geometry(tk, chunkGrid, 512);
+            geometry(tk, (int)std::max(1LL, std::min(2048LL, ((long long)n + 2047) / 2048)), 512);
             if (o.exact) {
                 g->AddInternalChannel(sOut[0], kIn4[4], "sizes->coalesce(coalescedBytes)");
                 g->AddInternalChannel(sOut[1], kIn4[5], "sizes->coalesce(groupCountsBytes)");
             }
-            g->AddInternalChannel(rOut[1], kIn4[0], "shuffle->coalesce(vals)");
-            g->AddInternalChannel(rOut[0], kIn4[1], "shuffle->coalesce(keys)");
-            g->AddInternalChannel(oOut[1], kIn4[2], "offsets->coalesce(groups)");
-            g->AddInternalChannel(oOut[0], kIn4[3], "offsets->coalesce(offsets)");
+            g->AddInternalChannel(rOutKV[1], kIn4[0], "shuffle->coalesce(vals)");
+            g->AddInternalChannel(rOutKV[0], kIn4[1], "shuffle->coalesce(keys)");
+            g->AddInternalChannel(oOut[1], kIn4[2], "totals->coalesce(groups)");
+            g->AddInternalChannel(oOut[0], kIn4[3], "keyscan->coalesce(offsets)");
             outs.push_back(g->AddOutputChannel(kOut2[0], "coalesced"));
             outs.push_back(g->AddOutputChannel(kOut2[1], "groupCounts"));
             outNames.push_back("coalesced");

@@ -7,7 +7,7 @@
 //   * batched: one captured hipGraph of the persistent multi-instance kernels of batch.hsaco,
 //     whose work-list lengths live in device memory so the same graph serves every mix, or
 //   * plain: the baseline, one kernel launch per task per instance (the six Yee tasks of a step,
-//     the five groupby tasks, the two EM tasks) in submission order on one stream.
+//     the seven groupby tasks, the two EM tasks) in submission order on one stream.
 // Every HIP call goes through the C ABI of libptk_hip.so.
 
 #include <algorithm>
@@ -43,9 +43,9 @@ int Fail(int code, const std::string &msg) {
 constexpr unsigned long long kAlign = 256;
 constexpr int kFdtdCap = 16384, kGbCap = 8192, kGmmCap = 8192;
 // device work-list lengths: FDTD tiles, FDTD instances, groupby chunk units, groupby buckets,
-// GMM accumulate units, GMM finalize units, three-step items, three-step blocks, groupby merge
-// units
-constexpr int kCounts = 9;
+// GMM accumulate units, GMM finalize units, three-step items, three-step blocks, groupby tiles,
+// groupby half buckets, groupby quarter buckets
+constexpr int kCounts = 11;
 
 unsigned long long Up(unsigned long long x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
@@ -150,12 +150,12 @@ struct ptk_batch {
     ptk_hip_stream stream = nullptr;
     ptk_hip_event evStart = nullptr, evStop = nullptr;
     ptk_hip_module mBatch = nullptr, mFdtd = nullptr, mGb = nullptr, mGmm = nullptr, mUtil = nullptr;
-    ptk_hip_function fH = nullptr, fE = nullptr, fStep = nullptr, fStep3 = nullptr, fAdv = nullptr, fGbAcc = nullptr,
-                     fGbMerge = nullptr, fGbTot = nullptr, fGbOff = nullptr, fGbRes = nullptr, fGmmAcc = nullptr, fGmmFin = nullptr, fInitFdtd = nullptr,
+    ptk_hip_function fH = nullptr, fE = nullptr, fStep = nullptr, fStep3 = nullptr, fAdv = nullptr, fGbHist = nullptr,
+                     fGbBScan = nullptr, fGbPart = nullptr, fGbKCount = nullptr, fGbKScan = nullptr, fGbRes = nullptr,
+                     fGbTot = nullptr, fGmmAcc = nullptr, fGmmFin = nullptr, fInitFdtd = nullptr,
                      fInitGb = nullptr, fInitGmm = nullptr, fInitGmmModel = nullptr, fSumWords = nullptr,
                      fSumPairs = nullptr, fSumGroups = nullptr, fSumJobs = nullptr, fFill = nullptr;
-    ptk_hip_function fSix[6] = {}, fPGbAcc = nullptr, fPGbMerge = nullptr, fPGbTot = nullptr, fPGbOff = nullptr,
-                     fPGbRes = nullptr,
+    ptk_hip_function fSix[6] = {}, fPGb[7] = {},  // the groupby graph's seven kernels, in order
                      fPGmmAcc = nullptr, fPGmmFin = nullptr;
     void *arenaBase = nullptr;
     Arena arena;
@@ -169,7 +169,7 @@ struct ptk_batch {
     List<int4> tiles, tiles3;  // FDTD tiles of the current form; three-step items (when cur3)
     List<int> t3start;         // three-step items of virtual block b: [t3start[b], t3start[b+1])
     List<int> fdtdActive;
-    List<int2> gbUnits, gbBuckets, gbMerge, gmmAcc, gmmFin;
+    List<int2> gbUnits, gbBuckets, gbTiles, gbHalves, gbQuarters, gmmAcc, gmmFin;
     List<PTK_CSUM_JOB> csumJobs;
     double *dCdf = nullptr;
     int cdfKeys = -1;
@@ -484,7 +484,7 @@ int BuildLists(ptk_batch *B) {
         } else if (in.kind == PTK_INST_GROUPBY) {
             ngu += (size_t)((in.a + in.gb.chunkRows - 1) / in.gb.chunkRows);
             ngi += (size_t)ptk_gb_buckets(in.b);
-            ngm += (size_t)(ptk_gb_buckets(in.b) * ptk_gb_groups(in.a));
+            ngm += (size_t)ptk_gb_tiles_max(in.a, in.b);
         } else {
             nga += (size_t)in.gm.d.blocks;
             ngf += (size_t)in.gm.d.K + 1;
@@ -496,10 +496,13 @@ int BuildLists(ptk_batch *B) {
     if (int r = Reserve(B, B->fdtdActive, nf)) return r;
     if (int r = Reserve(B, B->gbUnits, ngu)) return r;
     if (int r = Reserve(B, B->gbBuckets, ngi)) return r;
-    if (int r = Reserve(B, B->gbMerge, ngm)) return r;
+    if (int r = Reserve(B, B->gbTiles, ngm)) return r;
+    if (int r = Reserve(B, B->gbHalves, 2 * ngi)) return r;
+    if (int r = Reserve(B, B->gbQuarters, 4 * ngi)) return r;
     if (int r = Reserve(B, B->gmmAcc, nga)) return r;
     if (int r = Reserve(B, B->gmmFin, ngf)) return r;
-    B->tiles.n = B->tiles3.n = B->t3start.n = B->fdtdActive.n = B->gbUnits.n = B->gbBuckets.n = B->gbMerge.n = B->gmmAcc.n = B->gmmFin.n = 0;
+    B->tiles.n = B->tiles3.n = B->t3start.n = B->fdtdActive.n = B->gbUnits.n = B->gbBuckets.n = B->gbTiles.n = 0;
+    B->gbHalves.n = B->gbQuarters.n = B->gmmAcc.n = B->gmmFin.n = 0;
     for (int id : B->active) {
         const Inst &in = B->insts[(size_t)id];
         if (in.kind == PTK_INST_FDTD) {
@@ -522,9 +525,12 @@ int BuildLists(ptk_batch *B) {
         } else if (in.kind == PTK_INST_GROUPBY) {
             const long long chunks = (in.a + in.gb.chunkRows - 1) / in.gb.chunkRows;
             for (long long c = 0; c < chunks; ++c) B->gbUnits.host[B->gbUnits.n++] = int2{in.slot, (int)c};
-            for (long long q = 0; q < ptk_gb_buckets(in.b); ++q) B->gbBuckets.host[B->gbBuckets.n++] = int2{in.slot, (int)q};
-            for (long long q = 0; q < ptk_gb_buckets(in.b) * ptk_gb_groups(in.a); ++q)
-                B->gbMerge.host[B->gbMerge.n++] = int2{in.slot, (int)q};
+            const long long nb = ptk_gb_buckets(in.b);
+            for (long long q = 0; q < nb; ++q) B->gbBuckets.host[B->gbBuckets.n++] = int2{in.slot, (int)q};
+            for (long long q = 0; q < 2 * nb; ++q) B->gbHalves.host[B->gbHalves.n++] = int2{in.slot, (int)q};
+            for (long long q = 0; q < 4 * nb; ++q) B->gbQuarters.host[B->gbQuarters.n++] = int2{in.slot, (int)q};
+            // every tile the instance may have; those past its tiles (known on the device) do nothing
+            for (long long q = 0; q < ptk_gb_tiles_max(in.a, in.b); ++q) B->gbTiles.host[B->gbTiles.n++] = int2{in.slot, (int)q};
         } else {
             for (int b = 0; b < in.gm.d.blocks; ++b) B->gmmAcc.host[B->gmmAcc.n++] = int2{in.slot, b};
             for (int k = 0; k <= in.gm.d.K; ++k) B->gmmFin.host[B->gmmFin.n++] = int2{in.slot, k};
@@ -542,7 +548,9 @@ int BuildLists(ptk_batch *B) {
     }
     B->hCounts[6] = (int)B->tiles3.n;
     B->hCounts[7] = three ? B->grid3 : 0;
-    B->hCounts[8] = (int)B->gbMerge.n;
+    B->hCounts[8] = (int)B->gbTiles.n;
+    B->hCounts[9] = (int)B->gbHalves.n;
+    B->hCounts[10] = (int)B->gbQuarters.n;
     auto up = [&](auto &l) -> int {
         if (l.n) HIPCHK(ptk_hip_memcpy_htod_async(l.dev, l.host, l.n * sizeof(*l.host), B->stream));
         return PTK_BATCH_OK;
@@ -553,7 +561,9 @@ int BuildLists(ptk_batch *B) {
     if (int r = up(B->fdtdActive)) return r;
     if (int r = up(B->gbUnits)) return r;
     if (int r = up(B->gbBuckets)) return r;
-    if (int r = up(B->gbMerge)) return r;
+    if (int r = up(B->gbTiles)) return r;
+    if (int r = up(B->gbHalves)) return r;
+    if (int r = up(B->gbQuarters)) return r;
     if (int r = up(B->gmmAcc)) return r;
     if (int r = up(B->gmmFin)) return r;
     HIPCHK(ptk_hip_memcpy_htod_async(B->dCounts, B->hCounts, kCounts * sizeof(int), B->stream));
@@ -598,18 +608,22 @@ int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance, bool 
         if (int r = B->Launch(B->fGmmAcc, grid(768, hc[4]), 1, 1, 256, a)) return r;
         if (int r = B->Launch(B->fGmmFin, grid(512, hc[5]), 1, 1, 256, f)) return r;
     }
-    if (gb) {
-        Args a, m, t, o, s;
-        a.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
-        m.ptr(B->gbMerge.dev).ptr(B->dCounts + 8).ptr(B->dGb);
-        t.ptr(B->gbBuckets.dev).ptr(B->dCounts + 3).ptr(B->dGb);
-        o.ptr(B->gbBuckets.dev).ptr(B->dCounts + 3).ptr(B->dGb);
-        s.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
-        if (int r = B->Launch(B->fGbAcc, grid(1024, hc[2]), 1, 1, 512, a)) return r;
-        if (int r = B->Launch(B->fGbMerge, grid(4096, hc[8]), 1, 1, PTK_GB_BUCKET, m)) return r;
-        if (int r = B->Launch(B->fGbTot, grid(2048, hc[3]), 1, 1, PTK_GB_BUCKET, t)) return r;
-        if (int r = B->Launch(B->fGbOff, grid(2048, hc[3]), 1, 1, PTK_GB_BUCKET, o)) return r;
-        if (int r = B->Launch(B->fGbRes, grid(1024, hc[2]), 1, 1, PTK_GB_PLACE_THREADS, s)) return r;
+    if (gb) {  // the groupby graph's seven passes, each over its work list
+        Args u, b, p, k, h, s, q;
+        u.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
+        b.ptr(B->gbBuckets.dev).ptr(B->dCounts + 3).ptr(B->dGb);
+        p.ptr(B->gbUnits.dev).ptr(B->dCounts + 2).ptr(B->dGb);
+        k.ptr(B->gbTiles.dev).ptr(B->dCounts + 8).ptr(B->dGb);
+        h.ptr(B->gbHalves.dev).ptr(B->dCounts + 9).ptr(B->dGb);
+        s.ptr(B->gbTiles.dev).ptr(B->dCounts + 8).ptr(B->dGb);
+        q.ptr(B->gbQuarters.dev).ptr(B->dCounts + 10).ptr(B->dGb);
+        if (int r = B->Launch(B->fGbHist, grid(2048, hc[2]), 1, 1, PTK_GB_HIST_THREADS, u)) return r;
+        if (int r = B->Launch(B->fGbBScan, grid(2048, hc[3]), 1, 1, PTK_GB_SCAN_THREADS, b)) return r;
+        if (int r = B->Launch(B->fGbPart, grid(2 * B->cus, hc[2]), 1, 1, PTK_GB_PART_THREADS, p)) return r;
+        if (int r = B->Launch(B->fGbKCount, grid(8192, hc[8]), 1, 1, PTK_GB_COUNT_THREADS, k)) return r;
+        if (int r = B->Launch(B->fGbKScan, grid(4096, hc[9]), 1, 1, PTK_GB_KEYSCAN_THREADS, h)) return r;
+        if (int r = B->Launch(B->fGbRes, grid(8192, hc[8]), 1, 1, PTK_GB_TILE_THREADS, s)) return r;
+        if (int r = B->Launch(B->fGbTot, grid(8192, hc[10]), 1, 1, PTK_GB_TOTALS_THREADS, q)) return r;
     }
     return PTK_BATCH_OK;
 }
@@ -632,20 +646,24 @@ int IssuePlain(ptk_batch *B, const Inst &in) {
         }
     } else if (in.kind == PTK_INST_GROUPBY) {
         const PTK_GROUPBY_INST &d = in.gb;
-        const unsigned grid = (unsigned)std::max(1LL, std::min(2048LL, ptk_gb_chunks(d.n)));
-        const unsigned buckets = (unsigned)ptk_gb_buckets(d.numKeys);
-        const unsigned units = (unsigned)std::min(65535LL, ptk_gb_buckets(d.numKeys) * ptk_gb_groups(d.n));
-        Args a, m, t, o, s;
-        a.ptr(d.keys).ptr(d.vals).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.scratch);
-        m.ptr(d.scratch).add<int>((int)d.n).add<int>(d.numKeys);
-        t.ptr(d.scratch).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.table);
-        o.ptr(d.table).add<int>(d.numKeys).ptr(d.scratch).ptr(d.offsets).ptr(d.groups);
-        s.ptr(d.keys).ptr(d.vals).add<int>((int)d.n).add<int>(d.numKeys).ptr(d.offsets).ptr(d.scratch).ptr(d.outKeys).ptr(d.outVals);
-        if (int r = B->Launch(B->fPGbAcc, grid, 1, 1, 512, a)) return r;
-        if (int r = B->Launch(B->fPGbMerge, units, 1, 1, PTK_GB_BUCKET, m)) return r;
-        if (int r = B->Launch(B->fPGbTot, buckets, 1, 1, PTK_GB_BUCKET, t)) return r;
-        if (int r = B->Launch(B->fPGbOff, buckets, 1, 1, PTK_GB_BUCKET, o)) return r;
-        if (int r = B->Launch(B->fPGbRes, grid, 1, 1, PTK_GB_PLACE_THREADS, s)) return r;
+        const unsigned chunks = (unsigned)std::max(1LL, ptk_gb_chunks(d.n)), nb = (unsigned)ptk_gb_buckets(d.numKeys);
+        const unsigned tiles = (unsigned)ptk_gb_tiles_max(d.n, d.numKeys);
+        const int n = (int)d.n, nk = d.numKeys;
+        Args h, b, p, k, s, r6, t;
+        h.ptr(d.keys).add<int>(n).add<int>(nk).ptr(d.scratch);
+        b.ptr(d.scratch).add<int>(n).add<int>(nk);
+        p.ptr(d.keys).ptr(d.vals).add<int>(n).add<int>(nk).ptr(d.scratch);
+        k.ptr(d.scratch).add<int>(n).add<int>(nk);
+        s.ptr(d.scratch).add<int>(n).add<int>(nk).ptr(d.offsets);
+        r6.ptr(d.scratch).add<int>(n).add<int>(nk).ptr(d.offsets).ptr(d.outKeys).ptr(d.outVals);
+        t.ptr(d.scratch).add<int>(n).add<int>(nk).ptr(d.table).ptr(d.groups);
+        if (int r = B->Launch(B->fPGb[0], chunks, 1, 1, PTK_GB_HIST_THREADS, h)) return r;
+        if (int r = B->Launch(B->fPGb[1], nb, 1, 1, PTK_GB_SCAN_THREADS, b)) return r;
+        if (int r = B->Launch(B->fPGb[2], std::min(chunks, (unsigned)(2 * B->cus)), 1, 1, PTK_GB_PART_THREADS, p)) return r;
+        if (int r = B->Launch(B->fPGb[3], tiles, 1, 1, PTK_GB_COUNT_THREADS, k)) return r;
+        if (int r = B->Launch(B->fPGb[4], 2 * nb, 1, 1, PTK_GB_KEYSCAN_THREADS, s)) return r;
+        if (int r = B->Launch(B->fPGb[5], tiles, 1, 1, PTK_GB_TILE_THREADS, r6)) return r;
+        if (int r = B->Launch(B->fPGb[6], 4 * nb, 1, 1, PTK_GB_TOTALS_THREADS, t)) return r;
     } else {
         const PTK_GMM_INST &d = in.gm;
         Args a, f;
@@ -760,8 +778,9 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
     const std::pair<ptk_hip_function *, const char *> bf[] = {
         {&B->fH, "BatchFdtdH"}, {&B->fE, "BatchFdtdE"}, {&B->fStep, "BatchFdtdStep"},
         {&B->fStep3, "BatchFdtdSteps3"}, {&B->fAdv, "BatchFdtdAdvance"},
-        {&B->fGbAcc, "BatchGroupbyAccumulate"}, {&B->fGbMerge, "BatchGroupbyMerge"}, {&B->fGbTot, "BatchGroupbyTotals"},
-        {&B->fGbOff, "BatchGroupbyOffsets"}, {&B->fGbRes, "BatchGroupbyReshuffle"},
+        {&B->fGbHist, "BatchGroupbyHistogram"}, {&B->fGbBScan, "BatchGroupbyBucketScan"},
+        {&B->fGbPart, "BatchGroupbyPartition"}, {&B->fGbKCount, "BatchGroupbyKeyCount"},
+        {&B->fGbKScan, "BatchGroupbyKeyScan"}, {&B->fGbRes, "BatchGroupbyReshuffle"}, {&B->fGbTot, "BatchGroupbyTotals"},
         {&B->fGmmAcc, "BatchGmmAccumulate"}, {&B->fGmmFin, "BatchGmmFinalize"},
         {&B->fInitFdtd, "BatchInitFdtd"}, {&B->fInitGb, "BatchInitGroupby"}, {&B->fInitGmm, "BatchInitGmm"},
         {&B->fInitGmmModel, "BatchInitGmmModel"}, {&B->fSumWords, "BatchChecksumWords"},
@@ -773,10 +792,11 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
                                  "ExComputation", "EyComputation", "EzComputation"};
     for (int t = 0; t < 6; ++t)
         if ((r = Load(B->mFdtd, six[t], &B->fSix[t]))) return fail(r);
-    if ((r = Load(B->mGb, "GroupbyAccumulate", &B->fPGbAcc)) ||
-        (r = Load(B->mGb, "GroupbyMerge", &B->fPGbMerge)) || (r = Load(B->mGb, "GroupbyTotals", &B->fPGbTot)) ||
-        (r = Load(B->mGb, "GroupbyOffsets", &B->fPGbOff)) || (r = Load(B->mGb, "GroupbyReshuffle", &B->fPGbRes)) ||
-        (r = Load(B->mGmm, "GmmAccumulate", &B->fPGmmAcc)) || (r = Load(B->mGmm, "GmmFinalize", &B->fPGmmFin)) ||
+    static const char *gbk[7] = {"GroupbyHistogram", "GroupbyBucketScan", "GroupbyPartition", "GroupbyKeyCount",
+                                 "GroupbyKeyScan", "GroupbyReshuffle", "GroupbyTotals"};
+    for (int i = 0; i < 7; ++i)
+        if ((r = Load(B->mGb, gbk[i], &B->fPGb[i]))) return fail(r);
+    if ((r = Load(B->mGmm, "GmmAccumulate", &B->fPGmmAcc)) || (r = Load(B->mGmm, "GmmFinalize", &B->fPGmmFin)) ||
         (r = Load(B->mUtil, "PtkFillWords", &B->fFill)))
         return fail(r);
     if (ptk_hip_malloc(&B->arenaBase, arena_bytes))
@@ -808,7 +828,9 @@ int ptk_batch_destroy(ptk_batch *B) {
     freeList(B->t3start);
     freeList(B->fdtdActive);
     freeList(B->gbBuckets);
-    freeList(B->gbMerge);
+    freeList(B->gbTiles);
+    freeList(B->gbHalves);
+    freeList(B->gbQuarters);
     freeList(B->gbUnits);
     freeList(B->gmmAcc);
     freeList(B->gmmFin);

@@ -6,6 +6,7 @@
 // application gives its tasks.  Results are therefore bitwise the graph's.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <mutex>
 #include <string>
@@ -35,21 +36,25 @@ bool ShapeOk(long long n, int numKeys) {
     return n >= 1 && n <= INT_MAX && numKeys >= 0 && (long long)numKeys + 1 <= PTK_GB_MAX_KEYS;
 }
 
-// The five launches on `stream`; table and scratch lie in device memory the caller provides.
+// The seven launches on `stream`; table and scratch lie in device memory the caller provides.
 int Launch(const int *keys, const int *vals, int n, int numKeys, long long *table, void *scratch, long long *offsets,
            long long *groups, int *outKeys, int *outVals, hipStream_t stream) {
-    const long long chunks = ptk_gb_chunks(n), buckets = ptk_gb_buckets(numKeys);
-    const unsigned chunkGrid = (unsigned)(chunks < 2048 ? chunks : 2048);
-    const long long units = buckets * ptk_gb_groups(n);
-    const unsigned unitGrid = (unsigned)(units < 65535 ? units : 65535);
-    hipLaunchKernelGGL(GroupbyAccumulate, dim3(chunkGrid), dim3(512), 0, stream, keys, vals, n, numKeys, scratch);
-    hipLaunchKernelGGL(GroupbyMerge, dim3(unitGrid), dim3(PTK_GB_BUCKET), 0, stream, scratch, n, numKeys);
-    hipLaunchKernelGGL(GroupbyTotals, dim3((unsigned)buckets), dim3(PTK_GB_BUCKET), 0, stream, scratch, n, numKeys,
-                       table);
-    hipLaunchKernelGGL(GroupbyOffsets, dim3((unsigned)buckets), dim3(PTK_GB_BUCKET), 0, stream, table, numKeys,
-                       scratch, offsets, groups);
-    hipLaunchKernelGGL(GroupbyReshuffle, dim3(chunkGrid), dim3(PTK_GB_PLACE_THREADS), 0, stream, keys, vals, n,
-                       numKeys, (const long long *)offsets, scratch, outKeys, outVals);
+    const unsigned chunks = (unsigned)ptk_gb_chunks(n), nb = (unsigned)ptk_gb_buckets(numKeys);
+    const unsigned tiles = (unsigned)ptk_gb_tiles_max(n, numKeys);
+    int cus = 256, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    const unsigned partGrid = std::min(chunks, (unsigned)(2 * std::max(cus, 1)));  // two blocks per CU
+    hipLaunchKernelGGL(GroupbyHistogram, dim3(chunks), dim3(PTK_GB_HIST_THREADS), 0, stream, keys, n, numKeys, scratch);
+    hipLaunchKernelGGL(GroupbyBucketScan, dim3(nb), dim3(PTK_GB_SCAN_THREADS), 0, stream, scratch, n, numKeys);
+    hipLaunchKernelGGL(GroupbyPartition, dim3(partGrid), dim3(PTK_GB_PART_THREADS), 0, stream, keys, vals, n, numKeys,
+                       scratch);
+    hipLaunchKernelGGL(GroupbyKeyCount, dim3(tiles), dim3(PTK_GB_COUNT_THREADS), 0, stream, scratch, n, numKeys);
+    hipLaunchKernelGGL(GroupbyKeyScan, dim3(2 * nb), dim3(PTK_GB_KEYSCAN_THREADS), 0, stream, scratch, n, numKeys, offsets);
+    hipLaunchKernelGGL(GroupbyReshuffle, dim3(tiles), dim3(PTK_GB_TILE_THREADS), 0, stream, scratch, n, numKeys,
+                       (const long long *)offsets, outKeys, outVals);
+    hipLaunchKernelGGL(GroupbyTotals, dim3(4 * nb), dim3(PTK_GB_TOTALS_THREADS), 0, stream, scratch, n, numKeys, table,
+                       groups);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? PTK_GB_OK : HipFail(e, "groupby launch");
 }

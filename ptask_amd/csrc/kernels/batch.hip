@@ -17,9 +17,11 @@
 //                             overlapped 62 x 6-column tiles (the form when fdtd_cols == 0)
 //   BatchFdtdSteps3           items {inst, tile x, tile y, k0 | k1 << 16} over 58 x 10 columns in
 //                             per-block ranges: three whole steps per launch
-//   BatchGroupbyAccumulate/Reshuffle          units {inst, unit}: chunkRows rows
-//   BatchGroupbyMerge                         units {inst, merge unit (bucket, merge group)}
-//   BatchGroupbyTotals/Offsets                units {inst, key bucket}
+//   BatchGroupbyHistogram/Partition           units {inst, unit}: chunkRows rows
+//   BatchGroupbyBucketScan                    units {inst, key bucket}
+//   BatchGroupbyKeyCount/Reshuffle            units {inst, tile}
+//   BatchGroupbyKeyScan                       units {inst, half bucket}
+//   BatchGroupbyTotals                        units {inst, quarter bucket}
 //   BatchGmmAccumulate        units {inst, partial row b}
 //   BatchGmmFinalize          units {inst, component k (K = log-likelihood block)}
 //   BatchInit*                device-side input generation at admission
@@ -148,15 +150,14 @@ BatchInitFdtd(const PTK_FDTD_INST *__restrict__ insts, int inst, unsigned long l
 }
 
 // ---- groupby ----------------------------------------------------------------------------------
+// The groupby graph's seven passes over work lists; each pass's units are independent, and the
+// passes run in order (the executor launches them in sequence).
 
-__device__ inline long long gb_chunks(const PTK_GROUPBY_INST &I) {
-    return (I.n + I.chunkRows - 1) / I.chunkRows;
-}
-
-// A unit {inst, u} covers the chunks of rows [u * chunkRows, (u + 1) * chunkRows).
-extern "C" __global__ void __launch_bounds__(1024)
-BatchGroupbyAccumulate(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
-    __shared__ GbChunkLds L;
+// Units {inst, u}: the chunks of rows [u * chunkRows, (u + 1) * chunkRows).
+extern "C" __global__ void __launch_bounds__(PTK_GB_HIST_THREADS)
+BatchGroupbyHistogram(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ GbHistLds L;
+    if (blockDim.x != PTK_GB_HIST_THREADS) return;
     const int n = *count;
     for (int u = blockIdx.x; u < n; u += gridDim.x) {
         const int2 w = units[u];
@@ -164,62 +165,32 @@ BatchGroupbyAccumulate(const int2 *__restrict__ units, const int *__restrict__ c
         if (!gb_keys_ok(I.numKeys) || I.n <= 0) continue;
         const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
         const long long per = I.chunkRows / PTK_GB_CHUNK, c0 = (long long)w.y * per;
-        for (long long c = c0; c < c0 + per && c < S.chunks; ++c) gb_accumulate_chunk(I.keys, I.vals, I.n, I.numKeys, c, S, L);
+        for (long long c = c0; c < c0 + per && c < S.chunks; ++c) gb_histogram_chunk(I.keys, I.n, I.numKeys, c, S, L);
     }
 }
 
-// Units {inst, merge unit} (bucket-major (bucket, merge group) pairs, gb_merge_unit), taken in
-// an XCD-contiguous order (neighbouring units share lines); blocks of PTK_GB_BUCKET threads.
-extern "C" __global__ void __launch_bounds__(PTK_GB_BUCKET)
-BatchGroupbyMerge(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
-    __shared__ GbMergeLds L;
-    if (blockDim.x != PTK_GB_BUCKET) return;
-    const int n = *count;
-    for (int u = (int)gb_xcd_block(blockIdx.x, gridDim.x); u < n; u += gridDim.x) {
-        const int2 w = units[u];
-        const PTK_GROUPBY_INST &I = insts[w.x];
-        if (!gb_keys_ok(I.numKeys)) continue;
-        const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
-        if (w.y < S.buckets * S.groups) gb_merge_unit(S, w.y, L);
-    }
-}
-
-// Units {inst, bucket}; blocks of PTK_GB_BUCKET threads.
-extern "C" __global__ void __launch_bounds__(PTK_GB_BUCKET)
-BatchGroupbyTotals(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
-    __shared__ unsigned long long red[16];
-    if (blockDim.x != PTK_GB_BUCKET) return;
+// Units {inst, bucket}.
+extern "C" __global__ void __launch_bounds__(PTK_GB_SCAN_THREADS)
+BatchGroupbyBucketScan(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ unsigned red[16];
+    if (blockDim.x != PTK_GB_SCAN_THREADS) return;
     const int n = *count;
     for (int u = blockIdx.x; u < n; u += gridDim.x) {
         const int2 w = units[u];
         const PTK_GROUPBY_INST &I = insts[w.x];
-        if (!gb_keys_ok(I.numKeys)) continue;
+        if (!gb_keys_ok(I.numKeys) || I.n < 0) continue;
         const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
-        if (w.y < S.buckets) gb_totals_bucket(S, I.numKeys, w.y, I.table, red);
+        if (w.y < S.buckets) gb_bucketscan(S, w.y, red);
     }
 }
 
-// Units {inst, bucket}; blocks of PTK_GB_BUCKET threads.
-extern "C" __global__ void __launch_bounds__(PTK_GB_BUCKET)
-BatchGroupbyOffsets(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
-    __shared__ GbOffsetsLds L;
-    if (blockDim.x != PTK_GB_BUCKET) return;
-    const int n = *count;
-    for (int u = blockIdx.x; u < n; u += gridDim.x) {
-        const int2 w = units[u];
-        const PTK_GROUPBY_INST &I = insts[w.x];
-        if (!gb_keys_ok(I.numKeys)) continue;
-        const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
-        if (w.y < S.buckets) gb_offsets_bucket(I.table, I.numKeys, S, w.y, I.offsets, I.groups, L);
-    }
-}
-
-extern "C" __global__ void __launch_bounds__(PTK_GB_PLACE_THREADS)
-BatchGroupbyReshuffle(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
-    __shared__ GbPlaceLds L;
-    if (blockDim.x != PTK_GB_PLACE_THREADS) return;
-    // each XCD takes a contiguous run of units (consecutive chunks of an instance): see
-    // GroupbyReshuffle
+// Units {inst, u} as in BatchGroupbyHistogram; unit 0 of an instance also writes its bucket and
+// tile starts and tile table.  Each XCD takes a contiguous run of units (consecutive chunks of
+// an instance): see GroupbyPartition.
+extern "C" __global__ void __launch_bounds__(PTK_GB_PART_THREADS, 4)
+BatchGroupbyPartition(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ GbPartLds L;
+    if (blockDim.x != PTK_GB_PART_THREADS) return;
     const int n = *count, G = (int)gridDim.x < n ? (int)gridDim.x : n;
     if ((int)blockIdx.x >= G) return;
     for (int u = (int)gb_xcd_block(blockIdx.x, (unsigned)G); u < n; u += G) {
@@ -227,9 +198,72 @@ BatchGroupbyReshuffle(const int2 *__restrict__ units, const int *__restrict__ co
         const PTK_GROUPBY_INST &I = insts[w.x];
         if (!gb_keys_ok(I.numKeys) || I.n <= 0) continue;
         const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
+        if (w.y == 0) gb_partition_starts(S, L);
         const long long per = I.chunkRows / PTK_GB_CHUNK, c0 = (long long)w.y * per;
-        for (long long c = c0; c < c0 + per && c < S.chunks; ++c)
-            gb_place_chunk(I.keys, I.vals, I.n, I.numKeys, c, I.offsets, S, I.outKeys, I.outVals, L);
+        gb_partition_chunks(I.keys, I.vals, I.n, I.numKeys, c0, c0 + per < S.chunks ? c0 + per : S.chunks, 1, S, L);
+    }
+}
+
+// Units {inst, tile} for every tile an instance may have (ceil(n / PTK_GB_TILE) + buckets); those
+// past the instance's tiles do nothing.
+extern "C" __global__ void __launch_bounds__(PTK_GB_COUNT_THREADS)
+BatchGroupbyKeyCount(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ GbKeyCountLds L;
+    if (blockDim.x != PTK_GB_COUNT_THREADS) return;
+    const int n = *count;
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const int2 w = units[u];
+        const PTK_GROUPBY_INST &I = insts[w.x];
+        if (!gb_keys_ok(I.numKeys) || I.n <= 0) continue;
+        const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
+        if ((unsigned)w.y < S.tstart[S.buckets]) gb_keycount_tile(S, (unsigned)w.y, L);
+    }
+}
+
+// Units {inst, half bucket}.
+extern "C" __global__ void __launch_bounds__(PTK_GB_KEYSCAN_THREADS)
+BatchGroupbyKeyScan(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ GbKeyScanLds L;
+    if (blockDim.x != PTK_GB_KEYSCAN_THREADS) return;
+    const int n = *count;
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const int2 w = units[u];
+        const PTK_GROUPBY_INST &I = insts[w.x];
+        if (!gb_keys_ok(I.numKeys) || I.n < 0) continue;
+        const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
+        if (w.y < 2 * S.buckets) gb_keyscan_half(S, I.numKeys, w.y / 2, w.y % 2, I.offsets, L);
+    }
+}
+
+// Units {inst, tile} as in BatchGroupbyKeyCount, each XCD a contiguous run (see GroupbyReshuffle).
+extern "C" __global__ void __launch_bounds__(PTK_GB_TILE_THREADS)
+BatchGroupbyReshuffle(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ GbShuffleLds L;
+    if (blockDim.x != PTK_GB_TILE_THREADS) return;
+    const int n = *count, G = (int)gridDim.x < n ? (int)gridDim.x : n;
+    if ((int)blockIdx.x >= G) return;
+    for (int u = (int)gb_xcd_block(blockIdx.x, (unsigned)G); u < n; u += G) {
+        const int2 w = units[u];
+        const PTK_GROUPBY_INST &I = insts[w.x];
+        if (!gb_keys_ok(I.numKeys) || I.n <= 0) continue;
+        const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
+        const unsigned t = (unsigned)w.y;
+        if (t < S.tstart[S.buckets]) gb_reshuffle_tiles(S, I.numKeys, t, t + 1, 1, I.offsets, I.outKeys, I.outVals, I.n, L);
+    }
+}
+
+// Units {inst, quarter bucket}.
+extern "C" __global__ void __launch_bounds__(PTK_GB_TOTALS_THREADS)
+BatchGroupbyTotals(const int2 *__restrict__ units, const int *__restrict__ count, const PTK_GROUPBY_INST *__restrict__ insts) {
+    __shared__ GbTotalsLds L;
+    if (blockDim.x != PTK_GB_TOTALS_THREADS) return;
+    const int n = *count;
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const int2 w = units[u];
+        const PTK_GROUPBY_INST &I = insts[w.x];
+        if (!gb_keys_ok(I.numKeys) || I.n < 0) continue;
+        const PTK_GB_SCRATCH S = ptk_gb_scratch(I.scratch, I.n, I.numKeys);
+        if (w.y < 4 * S.buckets) gb_totals_quarter(S, I.numKeys, w.y / 4, w.y % 4, I.table, I.groups, L);
     }
 }
 

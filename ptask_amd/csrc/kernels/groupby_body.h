@@ -2,46 +2,34 @@
 // kernels (groupby.hip) and the batched multi-instance kernels (batch.hip).  Layouts of the
 // table, groups and scratch arrays: include/ptk_groupby.h.
 //
-// No pass uses a global atomic on the data.  Per-key sums are exact int64 (wrapping) additions,
-// whose result does not depend on their order, and every position is computed in row order, so
-// every output -- including the order of rows inside a group, which is input order -- is
-// bit-reproducible.
+// A stable two-level radix partition of the rows by key.  Level 1 splits the rows into buckets
+// of PTK_GB_BUCKET consecutive keys, level 2 orders each bucket by key.  Every position is
+// computed in row order (ballot ranks inside a wave, wave bases inside a block, chunk or tile
+// prefixes across blocks), so a group holds its rows in input order and every output is
+// bit-reproducible; per-key sums are exact int64 (wrapping) additions.  No pass uses a global
+// atomic or waits on another block.
 //
-//   accumulate  a block aggregates one chunk of PTK_GB_CHUNK rows in an LDS hash table of
-//               GB_SLOTS = 2 x PTK_GB_CHUNK slots (a chunk has at most PTK_GB_CHUNK distinct
-//               keys, so the table never fills and every probe sequence ends), then writes the
-//               chunk's distinct keys with their count and sum, grouped by key bucket.
-//   merge       a block owns one (bucket, merge group) unit: PTK_GB_BUCKET keys over
-//               PTK_GB_GROUP consecutive chunks.  It stages the chunks' entries of the bucket
-//               in LDS, 64 chunks at a time, as a per-key mask of the chunks holding the key;
-//               the thread of a key walks the staged chunks in chunk order, so the base it
-//               writes into each entry (the key's rows in the group's earlier chunks) follows
-//               row order.  The unit's per-key totals go to gcnt / gsum.
-//   totals      a block per bucket turns the groups' per-key totals into per-group bases (in
-//               group order) and writes the bucket's counts, sums and totals.
-//   offsets     a block per bucket: the bucket totals of the lower buckets give its starting row
-//               and group index; a block scan over its keys gives offsets and group records.
-//   reshuffle   a block re-reads one chunk: each distinct key's first output row is offsets[key]
-//               + its group base + its merge base; a row's rank among the chunk's rows of its key
-//               is counted in row order (per wave with ballots, then wave bases); rows are staged
-//               in LDS in output order and stored in that order.
+//   histogram   a block counts one chunk of PTK_GB_CHUNK rows per bucket (and its invalid keys)
+//   bucketscan  a block per bucket: the exclusive prefix of its chunk counts over the chunks,
+//               in place, and the bucket's rows
+//   partition   a block re-reads one chunk, ranks its rows by bucket in row order, stages them
+//               in LDS in that order and writes each bucket's run of rows behind the earlier
+//               chunks' runs of the bucket: the key's low byte and the value
+//   keycount    a block per tile of PTK_GB_TILE partitioned rows (tiles never straddle a bucket)
+//               counts its rows per key
+//   keyscan     a block per bucket: the exclusive prefix of its tiles' key counts over the
+//               tiles, in place; the keys' counts and offsets
+//   reshuffle   a block re-reads one tile, ranks its rows by key in row order, stages them in
+//               LDS in that order, writes each key's run behind the earlier tiles' rows of the
+//               key and adds up its values per key
+//   totals      a block per bucket: the keys' sums over the tiles, the table, the group records
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "ptk_groupby.h"
 
-#define GB_SLOTS (2 * PTK_GB_CHUNK)  // LDS hash slots per chunk (power of two)
-#define GB_EMPTY 0xFFFFFFFFu
-
 namespace ptk_groupby {
-
-__device__ inline unsigned gb_hash(unsigned k) {
-    k ^= k >> 16;
-    k *= 0x7feb352du;
-    k ^= k >> 15;
-    return k;
-}
 
 __device__ inline unsigned long long *gb_count(long long *t, int numKeys) { (void)numKeys; return (unsigned long long *)t; }
 __device__ inline unsigned long long *gb_sum(long long *t, int numKeys) { return (unsigned long long *)t + (numKeys + 1); }
@@ -78,6 +66,42 @@ __device__ inline unsigned long long block_sum(unsigned long long v, unsigned lo
     return t;
 }
 
+__device__ inline unsigned wave_incl32(unsigned v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+// Exclusive scan of a[0, len) in LDS, in place (each thread a run of consecutive elements; the
+// total is below 2^32); returns the total to every thread.  blockDim.x a multiple of 64, at most
+// 1024; begins and ends with a barrier.
+__device__ inline unsigned gb_lds_exscan(unsigned *a, int len, unsigned *red) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
+    const int per = (len + (int)blockDim.x - 1) / (int)blockDim.x, i0 = tid * per;
+    __syncthreads();
+    unsigned mine = 0;
+    for (int i = i0; i < i0 + per && i < len; ++i) mine += a[i];
+    const unsigned incl = wave_incl32(mine);
+    if (lane == 63) red[w] = incl;
+    __syncthreads();
+    unsigned run = incl - mine, tot = 0;
+    for (int q = 0; q < nw; ++q) {
+        if (q < w) run += red[q];
+        tot += red[q];
+    }
+    for (int i = i0; i < i0 + per && i < len; ++i) {
+        const unsigned v = a[i];
+        a[i] = run;
+        run += v;
+    }
+    __syncthreads();
+    return tot;
+}
+
 // Bijective renumbering of block b of n: blocks are dealt to the 8 XCDs round-robin, and this
 // gives each XCD a contiguous run of the numbering.
 __device__ inline unsigned gb_xcd_block(unsigned b, unsigned n) {
@@ -86,482 +110,658 @@ __device__ inline unsigned gb_xcd_block(unsigned b, unsigned n) {
     return xcd * per + (xcd < rem ? xcd : rem) + local;
 }
 
-// Calls f(key, val) for the rows [r0, r1) that this thread owns: 4 consecutive rows per 16-byte
-// load when the chunk starts on a multiple of 4.
-template <class F>
-__device__ inline void gb_for_rows(const int *__restrict__ keys, const int *__restrict__ vals, long long r0,
-                                   long long r1, F f) {
-    for (long long r = r0 + 4ll * threadIdx.x; r < r1; r += 4ll * blockDim.x) {
-        if (r + 4 <= r1 && (r & 3) == 0) {
-            const int4 k = *(const int4 *)&keys[r];
-            const int4 v = *(const int4 *)&vals[r];
-            f(k.x, v.x);
-            f(k.y, v.y);
-            f(k.z, v.z);
-            f(k.w, v.w);
-        } else {
-            for (long long q = r; q < r1 && q < r + 4; ++q) f(keys[q], vals[q]);
-        }
+// Lanes of the wave holding the same digit d (of BITS bits) as this lane, among the lanes with
+// use set.
+template <int BITS>
+__device__ inline unsigned long long gb_match(unsigned d, bool use) {
+    const unsigned long long u = __ballot(use);
+    unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+        // t: all ones where the bit is set; a lane keeps the lanes whose bit equals its own
+        const unsigned t = (unsigned)__builtin_amdgcn_sbfe((int)d, b, 1);
+        const unsigned long long m = __ballot(t != 0u);
+        lo &= ~((unsigned)m ^ t);
+        hi &= ~((unsigned)(m >> 32) ^ t);
+    }
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+#define GB_HIST_WAVES (PTK_GB_HIST_THREADS / 64)
+#define GB_HIST_ROWS (PTK_GB_CHUNK / PTK_GB_HIST_THREADS)  // rows per thread (16)
+
+struct GbHistLds {
+    unsigned cnt[PTK_GB_MAX_BUCKETS];  // per bucket: rows of the chunk
+    unsigned long long red[16];
+};
+
+// Loads rows [wr0 + 64 s + lane] (s < R) of a (wave-uniform wr0): out-of-range rows read the
+// last row and are flagged by row >= n.  base + wr0 is uniform, so the loads take a scalar base
+// and a 32-bit lane offset.
+template <int R, class T>
+__device__ inline void gb_load_rows(const T *__restrict__ a, long long wr0, long long n, T (&v)[R]) {
+    if (wr0 >= n) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) v[s] = T(0);
+        return;
+    }
+    const T *b = a + wr0;
+    const unsigned lane = threadIdx.x & 63;
+    if (wr0 + 64 * R <= n) {  // every row in range: one lane offset, immediate step offsets
+#pragma unroll
+        for (int s = 0; s < R; ++s) v[s] = b[64u * s + lane];
+        return;
+    }
+    const long long lim = n - 1 - wr0;
+    const unsigned last = lim < 64 * R - 1 ? (unsigned)lim : 64u * R - 1u;
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const unsigned i = 64u * s + lane;
+        v[s] = b[i < last ? i : last];
     }
 }
 
-struct GbChunkLds {
-    unsigned key[GB_SLOTS];
-    unsigned cnt[GB_SLOTS];
-    unsigned long long sum[GB_SLOTS];
-    unsigned hist[PTK_GB_MAX_BUCKETS + 1];  // entries per bucket, then running positions
-    unsigned err;
+// Chunk c: its rows per bucket into S.hist[c], its invalid keys into S.berr[c].  Each thread
+// counts 4 consecutive rows per 16-byte load with LDS atomics that return nothing.
+// blockDim.x == PTK_GB_HIST_THREADS; begins and ends with a barrier.
+__device__ inline void gb_histogram_chunk(const int *__restrict__ keys, long long n, int numKeys, long long c,
+                                          const PTK_GB_SCRATCH &S, GbHistLds &L) {
+    const int NB = (int)S.buckets, tid = threadIdx.x;
+    const long long r0 = c * PTK_GB_CHUNK, nr = n - r0 < PTK_GB_CHUNK ? n - r0 : PTK_GB_CHUNK;
+    __syncthreads();
+    for (int b = tid; b < NB; b += PTK_GB_HIST_THREADS) L.cnt[b] = 0;
+    int4 kv[GB_HIST_ROWS / 4];
+    const int4 *k4 = (const int4 *)(keys + r0);  // r0 is a multiple of 4: 16-byte aligned
+    unsigned inRange = GB_HIST_ROWS;
+    if (nr == PTK_GB_CHUNK) {
+#pragma unroll
+        for (int q = 0; q < GB_HIST_ROWS / 4; ++q) kv[q] = k4[q * PTK_GB_HIST_THREADS + tid];
+    } else {  // the last chunk: rows past n read as -1 and are not counted
+        inRange = 0;
+        for (int q = 0; q < GB_HIST_ROWS / 4; ++q) {
+            const long long i = 4ll * (q * PTK_GB_HIST_THREADS + tid);
+            kv[q].x = i < nr ? keys[r0 + i] : -1;
+            kv[q].y = i + 1 < nr ? keys[r0 + i + 1] : -1;
+            kv[q].z = i + 2 < nr ? keys[r0 + i + 2] : -1;
+            kv[q].w = i + 3 < nr ? keys[r0 + i + 3] : -1;
+            const long long left = nr - i;
+            inRange += left <= 0 ? 0u : (left >= 4 ? 4u : (unsigned)left);
+        }
+    }
+    __syncthreads();
+    unsigned good = 0;
+    auto add = [&](int k) {
+        if (k >= 0 && k <= numKeys) {
+            atomicAdd(&L.cnt[(unsigned)k / PTK_GB_BUCKET], 1u);
+            ++good;
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < GB_HIST_ROWS / 4; ++q) {
+        add(kv[q].x);
+        add(kv[q].y);
+        add(kv[q].z);
+        add(kv[q].w);
+    }
+    const unsigned err = inRange - good;
+    __syncthreads();
+    unsigned *h = S.hist + c * NB;
+    for (int b = tid; b < NB; b += PTK_GB_HIST_THREADS) h[b] = L.cnt[b];
+    const unsigned long long e = block_sum(err, L.red);
+    if (tid == 0) S.berr[c] = (unsigned)e;
+    __syncthreads();
+}
+
+// Bucket b (any blockDim.x, a multiple of 64): S.hist[.][b] becomes the exclusive prefix over
+// the chunks, S.btot[b] the bucket's rows; bucket 0 also adds up the invalid keys.
+__device__ inline void gb_bucketscan(const PTK_GB_SCRATCH &S, int b, unsigned *red) {
+    const int NB = (int)S.buckets, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const long long C = S.chunks, per = (C + blockDim.x - 1) / blockDim.x, c0 = (long long)tid * per;
+    unsigned mine = 0;
+    for (long long c = c0; c < c0 + per && c < C; ++c) mine += S.hist[c * NB + b];
+    const unsigned incl = wave_incl32(mine);
+    __syncthreads();
+    if (lane == 63) red[w] = incl;
+    __syncthreads();
+    unsigned run = incl - mine, tot = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+        if (q < w) run += red[q];
+        tot += red[q];
+    }
+    for (long long c = c0; c < c0 + per && c < C; ++c) {
+        const unsigned v = S.hist[c * NB + b];
+        S.hist[c * NB + b] = run;
+        run += v;
+    }
+    if (tid == 0) S.btot[b] = tot;
+    __syncthreads();
+}
+
+#define GB_PART_WAVES (PTK_GB_PART_THREADS / 64)
+#define GB_PART_ROWS (PTK_GB_SUB / PTK_GB_PART_THREADS)  // rows per thread (8)
+#define GB_PART_SUBS (PTK_GB_CHUNK / PTK_GB_SUB)         // quarter chunks per chunk
+
+struct GbPartLds {
+    union {
+        unsigned short wcnt[GB_PART_WAVES][PTK_GB_MAX_BUCKETS];  // per wave and bucket: rows, then the wave's base
+        int2 stage[PTK_GB_SUB];  // once the rows are ranked: {key, value} in partitioned order
+    } u;
+    unsigned dest[PTK_GB_MAX_BUCKETS];  // per bucket: the partitioned row of the chunk's next row of the bucket
+    unsigned cnt[PTK_GB_MAX_BUCKETS];   // per bucket: the quarter's rows
+    unsigned base[PTK_GB_MAX_BUCKETS];  // per bucket: local start, then the partitioned row of
+                                        // local position 0 of the bucket
+    unsigned bst[PTK_GB_MAX_BUCKETS];   // bucket starts (for the block's lifetime)
+    unsigned red[16];
+    unsigned long long red64[16];
 };
 
-// Chunk c of n rows: its distinct keys with count and sum, grouped by bucket, into S.entry, the
-// per-bucket starts into S.boff[c], the invalid keys into S.berr[c].  Called by every thread of
-// the block (blockDim.x a multiple of 64); begins and ends with a barrier-separated LDS phase.
-__device__ inline void gb_accumulate_chunk(const int *__restrict__ keys, const int *__restrict__ vals, long long n,
-                                           int numKeys, long long c, const PTK_GB_SCRATCH &S, GbChunkLds &L) {
-    const long long r0 = c * PTK_GB_CHUNK, r1 = r0 + PTK_GB_CHUNK < n ? r0 + PTK_GB_CHUNK : n;
-    const int NB = (int)S.buckets;
-    for (int s = threadIdx.x; s < GB_SLOTS; s += blockDim.x) {
-        L.key[s] = GB_EMPTY;
-        L.cnt[s] = 0;
-        L.sum[s] = 0;
-    }
-    for (int b = threadIdx.x; b <= NB; b += blockDim.x) L.hist[b] = 0;
-    if (threadIdx.x == 0) L.err = 0;
+// After the bucket scan, by one block of the partition launch: the bucket starts (S.bstart) and
+// the first tile of each bucket (S.tstart), which the later passes read.  Any blockDim.x (a
+// multiple of 64); begins and ends with a barrier.
+__device__ inline void gb_partition_starts(const PTK_GB_SCRATCH &S, GbPartLds &L) {
+    const int NB = (int)S.buckets, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    unsigned *ts = (unsigned *)&L.u.wcnt[0][0];  // tile starts (NB + 1 <= PTK_GB_MAX_BUCKETS + 1 entries)
     __syncthreads();
-    gb_for_rows(keys, vals, r0, r1, [&](int k, int v) {
-        if (k < 0 || k > numKeys) {
-            atomicAdd(&L.err, 1u);
-            return;
-        }
-        const unsigned uk = (unsigned)k;
-        unsigned h = gb_hash(uk) & (GB_SLOTS - 1);
-        for (int p = 0; p < GB_SLOTS; ++p) {  // at most PTK_GB_CHUNK slots are taken: ends early
-            unsigned cur = L.key[h];
-            if (cur == GB_EMPTY) cur = atomicCAS(&L.key[h], GB_EMPTY, uk);
-            if (cur == GB_EMPTY || cur == uk) {
-                atomicAdd(&L.cnt[h], 1u);
-                atomicAdd(&L.sum[h], (unsigned long long)(long long)v);
-                return;
-            }
-            h = (h + 1) & (GB_SLOTS - 1);
-        }
-    });
-    __syncthreads();
-    for (int s = threadIdx.x; s < GB_SLOTS; s += blockDim.x)
-        if (L.key[s] != GB_EMPTY) atomicAdd(&L.hist[L.key[s] / PTK_GB_BUCKET], 1u);
-    __syncthreads();
-    unsigned *boff = S.boff + c * (NB + 1);
-    if (threadIdx.x < 64) {  // one wave: exclusive scan of the bucket histogram
-        const int lane = threadIdx.x;
-        unsigned long long carry = 0;
+    for (int b = tid; b < NB; b += blockDim.x) L.base[b] = S.btot[b];
+    const unsigned rows = gb_lds_exscan(L.base, NB, L.red);
+    for (int b = tid; b < NB; b += blockDim.x) S.bstart[b] = L.base[b];
+    if (tid == 0) S.bstart[NB] = rows;
+    if (w == 0) {  // tiles of PTK_GB_TILE rows per bucket: exclusive scan of their counts
+        unsigned carry = 0;
         for (int b0 = 0; b0 < NB; b0 += 64) {
             const int b = b0 + lane;
-            const unsigned long long v = b < NB ? L.hist[b] : 0;
-            const unsigned long long incl = wave_incl(v);
-            if (b < NB) {
-                L.hist[b] = (unsigned)(carry + incl - v);
-                boff[b] = (unsigned)(carry + incl - v);
-            }
+            const unsigned v = b < NB ? (S.btot[b] + PTK_GB_TILE - 1u) / PTK_GB_TILE : 0u;
+            const unsigned incl = wave_incl32(v);
+            if (b < NB) ts[b] = S.tstart[b] = carry + incl - v;
             carry += __shfl(incl, 63, 64);
         }
-        if (lane == 0) {
-            boff[NB] = (unsigned)carry;
-            S.berr[c] = L.err;
-        }
+        if (lane == 0) ts[NB] = S.tstart[NB] = carry;
     }
     __syncthreads();
-    PTK_GB_ENTRY *out = S.entry + c * PTK_GB_CHUNK;
-    for (int s = threadIdx.x; s < GB_SLOTS; s += blockDim.x) {
-        const unsigned k = L.key[s];
-        if (k == GB_EMPTY) continue;
-        const unsigned pos = atomicAdd(&L.hist[k / PTK_GB_BUCKET], 1u);
-        PTK_GB_ENTRY e;
-        e.key = (int)k;
-        e.count = L.cnt[s];
-        e.sum = (long long)L.sum[s];
-        out[pos] = e;
+    unsigned long long err = 0;  // invalid keys
+    for (long long c = tid; c < S.chunks; c += blockDim.x) err += S.berr[c];
+    err = block_sum(err, L.red64);
+    if (tid == 0) S.head[0] = err;
+    for (int b = tid; b < NB; b += blockDim.x) {  // the tile table: bucket, first and end row
+        const unsigned r0 = L.base[b], end = r0 + S.btot[b], t0 = ts[b], t1 = ts[b + 1];
+        for (unsigned t = t0; t < t1; ++t) {
+            const unsigned a = r0 + (t - t0) * PTK_GB_TILE;
+            *(uint4 *)(S.tinfo + 4ull * t) = make_uint4((unsigned)b, a, a + PTK_GB_TILE < end ? a + PTK_GB_TILE : end, 0u);
+        }
     }
     __syncthreads();
 }
 
-#define GB_MERGE_BATCH 16  // chunks a merge unit stages at once: one bit of a 64-bit mask each
-#define GB_MERGE_LOADS 4   // entries a thread loads before it stages them
+#define GB_PART_HX ((PTK_GB_MAX_BUCKETS + PTK_GB_PART_THREADS - 1) / PTK_GB_PART_THREADS)
 
-struct GbMergeLds {
-    unsigned tab[GB_MERGE_BATCH][PTK_GB_BUCKET];  // per staged chunk and key: index of the key's
-                                                  // entry in the chunk's bucket segment << 16 | count
-    unsigned long long mask[PTK_GB_BUCKET];       // per key: the staged chunks holding it
-    unsigned long long sum[PTK_GB_BUCKET];
-    unsigned segS[PTK_GB_GROUP];                  // per chunk of the group: its bucket segment
-    unsigned ebase[GB_MERGE_BATCH];               // per staged chunk: index of its segment's first entry
-    unsigned pre[PTK_GB_GROUP + 1];               // and the exclusive scan of the segment sizes
-    unsigned long long red[16];
-};
-
-// Merge unit u = r * S.groups + g (bucket r, merge group g); blockDim.x == PTK_GB_BUCKET, thread k
-// owns key r * PTK_GB_BUCKET + k.  Writes each entry's base inside the group and the group's
-// per-key totals (gcnt, gsum).  The entries of a batch of chunks are loaded by all threads at
-// once (flat over the batch's segments), so a unit waits for memory about once per batch; a
-// thread then walks the staged chunks holding its key in chunk order.
-__device__ inline void gb_merge_unit(const PTK_GB_SCRATCH &S, long long u, GbMergeLds &L) {
+// Chunks c, c + step, ... below cend, after the bucket scan, by one block, a quarter chunk at a
+// time: writes each row's low key byte and value to S.tlow / S.tval at bstart[bucket] + the
+// earlier chunks' rows of the bucket + the row's rank among the chunk's rows of the bucket (row
+// order).  The next quarter's rows are loaded while the current quarter's are written.
+// blockDim.x == PTK_GB_PART_THREADS; begins and ends with a barrier.
+template <int BITS>
+__device__ inline void gb_partition_chunks_t(const int *__restrict__ keys, const int *__restrict__ vals, long long n,
+                                             int numKeys, long long c, long long cend, long long step,
+                                             const PTK_GB_SCRATCH &S, GbPartLds &L) {
     const int NB = (int)S.buckets;
-    const long long NG = S.groups;
-    const int r = (int)(u / NG);
-    const long long g = u % NG;
-    const int k = threadIdx.x, lane = k & 63, w = k >> 6;
-    const unsigned key0 = (unsigned)r * PTK_GB_BUCKET;
-    const long long KP = (long long)NB * PTK_GB_BUCKET;
-    const long long cg0 = g * PTK_GB_GROUP;
-    const int nc = (int)(cg0 + PTK_GB_GROUP < S.chunks ? PTK_GB_GROUP : S.chunks - cg0);
-    // the group's bucket segments (thread k: chunk cg0 + k) and their flat positions
-    unsigned len = 0;
-    if (k < nc) {
-        const unsigned *bo = S.boff + (cg0 + k) * (NB + 1) + r;
-        const unsigned s0 = bo[0] < PTK_GB_CHUNK ? bo[0] : PTK_GB_CHUNK;
-        const unsigned s1 = bo[1] < PTK_GB_CHUNK ? bo[1] : PTK_GB_CHUNK;
-        L.segS[k] = s0;
-        len = s1 > s0 ? s1 - s0 : 0;
-        len = len < PTK_GB_BUCKET ? len : PTK_GB_BUCKET;  // a segment holds distinct keys of one bucket
-    }
-    const unsigned long long incl = wave_incl(len);
-    L.sum[k] = 0;
-    L.mask[k] = 0;
-    __syncthreads();  // the previous unit is done with red
-    if (lane == 63) L.red[w] = incl;
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     __syncthreads();
-    unsigned long long pre = incl - len;
-    for (int q = 0; q < w; ++q) pre += L.red[q];
-    L.pre[k] = (unsigned)pre;
-    if (k == PTK_GB_BUCKET - 1) L.pre[PTK_GB_BUCKET] = (unsigned)(pre + len);
-    __syncthreads();
-    unsigned run = 0;  // rows of the key in the chunks walked so far
-    for (int j0 = 0; j0 < nc; j0 += GB_MERGE_BATCH) {
-        const int nb = nc - j0 < GB_MERGE_BATCH ? nc - j0 : GB_MERGE_BATCH;
-        const unsigned F0 = L.pre[j0], F1 = L.pre[j0 + nb];
-        for (unsigned f0 = F0 + k; f0 < F1; f0 += GB_MERGE_LOADS * PTK_GB_BUCKET) {
-            PTK_GB_ENTRY e[GB_MERGE_LOADS];
-            int jj[GB_MERGE_LOADS];
-            unsigned ii[GB_MERGE_LOADS];
+    for (int b = tid; b < NB; b += PTK_GB_PART_THREADS) L.bst[b] = S.btot[b];
+    gb_lds_exscan(L.bst, NB, L.red);  // bucket starts
+    int kr[GB_PART_ROWS], vr[GB_PART_ROWS];
+    auto fetch = [&](long long row0) {
+        const long long wr0 = row0 + (long long)w * (PTK_GB_SUB / GB_PART_WAVES);
+        gb_load_rows<GB_PART_ROWS>(keys, wr0, n, kr);
+        gb_load_rows<GB_PART_ROWS>(vals, wr0, n, vr);
+    };
+    if (c < cend) fetch(c * PTK_GB_CHUNK);
+    // one quarter chunk per iteration: (c, q), q = 0..GB_PART_SUBS - 1, then c += step
+    for (int q = 0; c < cend; q = q + 1 < GB_PART_SUBS ? q + 1 : 0, c += q == 0 ? step : 0) {
+        if (q == 0)
+            for (int b = tid; b < NB; b += PTK_GB_PART_THREADS) L.dest[b] = L.bst[b] + S.hist[c * NB + b];
+        {
+            const long long row0 = c * PTK_GB_CHUNK + (long long)q * PTK_GB_SUB;
+            const long long wr0 = row0 + (long long)w * (PTK_GB_SUB / GB_PART_WAVES);
+            const int nin = wr0 >= n ? 0 : (n - wr0 < 64 * GB_PART_ROWS ? (int)(n - wr0) : 64 * GB_PART_ROWS);
+            for (int v = 0; v < GB_PART_WAVES; ++v)
+                for (int b = tid; b < NB; b += PTK_GB_PART_THREADS) L.u.wcnt[v][b] = 0;
+            __syncthreads();
+            // ranks in row order: wave w, step s covers 64 consecutive rows; rk: the rank among the
+            // wave's rows of the bucket, then the row's position in the quarter's partitioned order
+            const unsigned long long below = (1ull << lane) - 1ull;
+            unsigned rk[GB_PART_ROWS];
 #pragma unroll
-            for (int q = 0; q < GB_MERGE_LOADS; ++q) {
-                const unsigned f = f0 + q * PTK_GB_BUCKET;
-                jj[q] = -1;
-                if (f < F1) {
-                    int lo = j0, hi = j0 + nb;  // the last chunk whose segment starts at or before f
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (L.pre[mid] <= f) lo = mid;
-                        else hi = mid;
+            for (int s = 0; s < GB_PART_ROWS; ++s) {
+                const bool use = 64 * s + lane < nin && kr[s] >= 0 && kr[s] <= numKeys;
+                const unsigned d = use ? (unsigned)kr[s] / PTK_GB_BUCKET : 0u;
+                const unsigned long long peers = gb_match<BITS>(d, use);
+                unsigned old = 0;
+                if (use) old = L.u.wcnt[w][d];
+                if (use && (peers & below) == 0) L.u.wcnt[w][d] = (unsigned short)(old + __popcll(peers));
+                rk[s] = use ? old + (unsigned)__popcll(peers & below) : 0xFFFFFFFFu;
+                __builtin_amdgcn_sched_barrier(0);  // one step at a time: keeps the ballots' masks few
+            }
+            __syncthreads();
+            for (int b = tid; b < NB; b += PTK_GB_PART_THREADS) {  // wave bases and the quarter's rows per bucket
+                unsigned t = 0;
+#pragma unroll
+                for (int v = 0; v < GB_PART_WAVES; ++v) {
+                    const unsigned x = L.u.wcnt[v][b];
+                    L.u.wcnt[v][b] = (unsigned short)t;
+                    t += x;
+                }
+                L.base[b] = t;
+                L.cnt[b] = t;
+            }
+            const unsigned valid = gb_lds_exscan(L.base, NB, L.red);  // local bucket starts
+#pragma unroll
+            for (int s = 0; s < GB_PART_ROWS; ++s) {
+                if (rk[s] == 0xFFFFFFFFu) continue;
+                const unsigned d = (unsigned)kr[s] / PTK_GB_BUCKET;
+                rk[s] += L.base[d] + L.u.wcnt[w][d];
+            }
+            __syncthreads();
+            // per bucket: the partitioned row of local position 0, and the next quarter's start
+            for (int b = tid; b < NB; b += PTK_GB_PART_THREADS) {
+                L.base[b] = L.dest[b] - L.base[b];
+                L.dest[b] += L.cnt[b];
+            }
+#pragma unroll
+            for (int s = 0; s < GB_PART_ROWS; ++s)
+                if (rk[s] < PTK_GB_SUB) L.u.stage[rk[s]] = make_int2(kr[s], vr[s]);  // every valid row (rk < PTK_GB_SUB)
+            __syncthreads();
+            if (q + 1 < GB_PART_SUBS) fetch(row0 + PTK_GB_SUB);  // in flight while this quarter is written
+            else if (c + step < cend) fetch((c + step) * PTK_GB_CHUNK);
+            __builtin_amdgcn_sched_barrier(0);
+            const unsigned nv = valid < PTK_GB_SUB ? valid : PTK_GB_SUB;
+#pragma unroll
+            for (int h = 0; h < GB_PART_ROWS; h += 4) {  // four staged rows per thread at a time
+                int2 kv[4];
+                unsigned g[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) kv[i] = L.u.stage[tid + (h + i) * PTK_GB_PART_THREADS];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const unsigned b = (unsigned)kv[i].x / PTK_GB_BUCKET;
+                    g[i] = b < (unsigned)NB ? L.base[b] : 0u;
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const unsigned p = tid + (h + i) * PTK_GB_PART_THREADS, qq = g[i] + p;
+                    if (p < nv && qq < (unsigned)n) {  // qq < n always, for a consistent scratch
+                        S.tlow[qq] = (unsigned char)(kv[i].x & (PTK_GB_BUCKET - 1));
+                        S.tval[qq] = kv[i].y;
                     }
-                    jj[q] = lo;
-                    ii[q] = f - L.pre[lo];
-                    e[q] = S.entry[(cg0 + lo) * PTK_GB_CHUNK + L.segS[lo] + ii[q]];
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
-#pragma unroll
-            for (int q = 0; q < GB_MERGE_LOADS; ++q) {
-                if (jj[q] < 0) continue;
-                const unsigned kk = (unsigned)e[q].key - key0;
-                if (kk >= PTK_GB_BUCKET) continue;  // only for an inconsistent scratch
-                const int j = jj[q] - j0;
-                atomicOr(&L.mask[kk], 1ull << j);
-                L.tab[j][kk] = (ii[q] << 16) | (e[q].count & 0xFFFFu);
-                atomicAdd(&L.sum[kk], (unsigned long long)e[q].sum);
-            }
+            __syncthreads();
         }
-        if (k < nb) L.ebase[k] = (unsigned)((cg0 + j0 + k) * PTK_GB_CHUNK + L.segS[j0 + k]);
-        __syncthreads();
-        unsigned long long m = L.mask[k];
-        L.mask[k] = 0;
-        while (m) {  // the key's chunks in chunk order, up to four per round (their reads overlap)
-            int j[4];
-            unsigned v[4], eb[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                j[q] = m ? __builtin_ctzll(m) : -1;
-                m &= m - 1;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (j[q] >= 0) {
-                    v[q] = L.tab[j[q]][k];
-                    eb[q] = L.ebase[j[q]];
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (j[q] >= 0) {
-                    S.entry[eb[q] + (v[q] >> 16)].sum = (long long)run;
-                    run += v[q] & 0xFFFFu;
-                }
-            }
-        }
-        __syncthreads();
     }
-    S.gsum[g * KP + key0 + k] = L.sum[k];
-    S.gcnt[g * KP + key0 + k] = run;
 }
 
-// Bucket r (blockDim.x == PTK_GB_BUCKET, thread k owns key r * PTK_GB_BUCKET + k), after every
-// merge unit: turns the groups' per-key totals into per-group bases in group order, writes the
-// keys' counts and sums to the table, the bucket's rows and non-empty keys to S.btot and (bucket
-// 0) the error count.
-__device__ inline void gb_totals_bucket(const PTK_GB_SCRATCH &S, int numKeys, int r, long long *table,
-                                        unsigned long long *red) {
-    const int NB = (int)S.buckets, k = threadIdx.x;
-    const long long NG = S.groups, KP = (long long)NB * PTK_GB_BUCKET;
-    const long long key = (long long)r * PTK_GB_BUCKET + k;
-    unsigned long long tot = 0, sum = 0;
-    constexpr int kT = 8;  // groups whose totals are loaded together
-    for (long long q0 = 0; q0 < NG; q0 += kT) {
-        unsigned c[kT];
-        unsigned long long v[kT];
+// The two bucket-number widths: up to 256 buckets (65536 keys) and up to PTK_GB_MAX_BUCKETS.
+__device__ inline void gb_partition_chunks(const int *__restrict__ keys, const int *__restrict__ vals, long long n,
+                                           int numKeys, long long c, long long cend, long long step,
+                                           const PTK_GB_SCRATCH &S, GbPartLds &L) {
+    if (S.buckets <= 256) gb_partition_chunks_t<8>(keys, vals, n, numKeys, c, cend, step, S, L);
+    else gb_partition_chunks_t<11>(keys, vals, n, numKeys, c, cend, step, S, L);
+}
+
+// Tile t of the partitioned rows (t < S.tstart[NB]): its bucket and row range.
+struct GbTile {
+    int bucket;
+    unsigned r0, r1;
+};
+
+__device__ inline GbTile gb_tile(const PTK_GB_SCRATCH &S, unsigned t) {
+    const uint4 v = *(const uint4 *)(S.tinfo + 4ull * t);
+    GbTile T;
+    T.bucket = (int)v.x;
+    T.r0 = v.y;
+    T.r1 = v.z;
+    return T;
+}
+
+struct GbKeyCountLds {
+    unsigned cnt[PTK_GB_BUCKET];
+};
+
+// Tile t: its rows per key into S.c2[t].  Each thread counts 16 consecutive rows (one 16-byte
+// load of low key bytes) and adds runs of equal keys at once.  Any blockDim.x (a multiple of 64)
+// of at least PTK_GB_BUCKET threads (PTK_GB_COUNT_THREADS in the kernels); begins and ends with
+// a barrier.
+__device__ inline void gb_keycount_tile(const PTK_GB_SCRATCH &S, unsigned t, GbKeyCountLds &L) {
+    const GbTile T = gb_tile(S, t);
+    const int tid = threadIdx.x;
+    __syncthreads();
+    if (tid < PTK_GB_BUCKET) L.cnt[tid] = 0;
+    __syncthreads();
+    const unsigned a0 = T.r0 & ~15u;
+    for (unsigned q = a0 + 16u * tid; q < T.r1; q += 16u * blockDim.x) {
+        const uint4 v = *(const uint4 *)(S.tlow + q);
+        const unsigned wds[4] = {v.x, v.y, v.z, v.w};
+        unsigned prev = 0xFFFFFFFFu, run = 0;
 #pragma unroll
-        for (int q = 0; q < kT; ++q) {
-            c[q] = q0 + q < NG ? S.gcnt[(q0 + q) * KP + key] : 0u;
-            v[q] = q0 + q < NG ? S.gsum[(q0 + q) * KP + key] : 0ull;
+        for (int i = 0; i < 16; ++i) {
+            const unsigned r = q + i;
+            if (r < T.r0 || r >= T.r1) continue;
+            const unsigned k = (wds[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+            if (k != prev) {
+                if (run) atomicAdd(&L.cnt[prev], run);
+                prev = k;
+                run = 0;
+            }
+            ++run;
         }
-#pragma unroll
-        for (int q = 0; q < kT; ++q) {
-            if (q0 + q < NG) S.gcnt[(q0 + q) * KP + key] = (unsigned)tot;
-            tot += c[q];
-            sum += v[q];
-        }
+        if (run) atomicAdd(&L.cnt[prev], run);
     }
-    const bool in = key <= numKeys;
-    if (in) {
-        gb_count(table, numKeys)[key] = tot;
-        gb_sum(table, numKeys)[key] = sum;
-    }
-    const unsigned long long rows = block_sum(in ? tot : 0, red);
-    const unsigned long long nz = block_sum(in && tot ? 1 : 0, red);
-    if (k == 0) {
-        S.btot[2 * r] = (long long)rows;
-        S.btot[2 * r + 1] = (long long)nz;
-    }
-    if (r == 0) {
-        unsigned long long err = 0;
-        for (long long c = k; c < S.chunks; c += blockDim.x) err += S.berr[c];
-        err = block_sum(err, red);
-        if (k == 0) *gb_errors(table, numKeys) = err;
-    }
+    __syncthreads();
+    if (tid < PTK_GB_BUCKET) S.c2[(unsigned long long)t * PTK_GB_BUCKET + tid] = L.cnt[tid];
     __syncthreads();
 }
 
-struct GbOffsetsLds {
-    unsigned long long red[16];
-    unsigned long long wrows[16], wnz[16];
+#define GB_KS_KEYS (PTK_GB_BUCKET / 2)                    // keys per keyscan block (half a bucket)
+#define GB_KS_PARTS (PTK_GB_KEYSCAN_THREADS / GB_KS_KEYS)    // threads per key in keyscan
+#define GB_TS_KEYS (PTK_GB_BUCKET / 4)                    // keys per totals block (a quarter bucket)
+#define GB_TS_PARTS (PTK_GB_TOTALS_THREADS / GB_TS_KEYS)     // threads per key in totals
+
+struct GbKeyScanLds {
+    unsigned part[GB_KS_PARTS][GB_KS_KEYS];
+    unsigned wrows[16], wnz[16];
 };
 
-// Bucket r (blockDim.x == PTK_GB_BUCKET): offsets of its keys and their group records; bucket 0
-// also writes the groups header {G, rows, errors}.
-__device__ inline void gb_offsets_bucket(long long *table, int numKeys, const PTK_GB_SCRATCH &S, int r,
-                                         long long *offsets, long long *groups, GbOffsetsLds &L) {
-    const int NB = (int)S.buckets;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    unsigned long long rowsBefore = 0, nzBefore = 0, rowsAll = 0, nzAll = 0;
-    for (int q = threadIdx.x; q < NB; q += blockDim.x) {
-        const unsigned long long a = (unsigned long long)S.btot[2 * q], b = (unsigned long long)S.btot[2 * q + 1];
-        if (q < r) {
-            rowsBefore += a;
-            nzBefore += b;
-        }
-        rowsAll += a;
-        nzAll += b;
+// Half h of bucket b (blockDim.x == PTK_GB_KEYSCAN_THREADS; thread q * GB_KS_KEYS + k takes part q of
+// the bucket's tiles for key b * PTK_GB_BUCKET + h * GB_KS_KEYS + k): S.c2 of those keys over the
+// bucket's tiles becomes the exclusive prefix over the tiles, in place; each key's count goes to
+// S.kcnt, its first output row to offsets, and the half's non-empty keys to S.nz[2 b + h].  The
+// upper half's first row is the bucket's first row plus its rows less the upper half's own.
+__device__ inline void gb_keyscan_half(const PTK_GB_SCRATCH &S, int numKeys, int b, int h, long long *offsets,
+                                       GbKeyScanLds &L) {
+    const int tid = threadIdx.x, k = tid % GB_KS_KEYS, q = tid / GB_KS_KEYS, lane = tid & 63, w = tid >> 6;
+    const unsigned t0 = S.tstart[b], t1 = S.tstart[b + 1], nt = t1 - t0;
+    const unsigned per = (nt + GB_KS_PARTS - 1) / GB_KS_PARTS;
+    const unsigned p0 = t0 + q * per, p1 = p0 + per < t1 ? p0 + per : t1;
+    const unsigned kk = (unsigned)(h * GB_KS_KEYS + k);  // key within the bucket
+    constexpr int kB = 32;  // tiles whose counts are loaded together
+    unsigned v[kB];
+    unsigned mine = 0;
+    for (unsigned t = p0; t < p1; t += kB) {
+#pragma unroll
+        for (int i = 0; i < kB; ++i) v[i] = t + i < p1 ? S.c2[(unsigned long long)(t + i) * PTK_GB_BUCKET + kk] : 0u;
+#pragma unroll
+        for (int i = 0; i < kB; ++i) mine += v[i];
     }
-    rowsBefore = block_sum(rowsBefore, L.red);
-    nzBefore = block_sum(nzBefore, L.red);
-    if (r == 0) {
-        rowsAll = block_sum(rowsAll, L.red);
-        nzAll = block_sum(nzAll, L.red);
-        if (threadIdx.x == 0) {
-            groups[0] = (long long)nzAll;
-            groups[1] = (long long)rowsAll;
-            groups[2] = (long long)*gb_errors(table, numKeys);
+    __syncthreads();
+    L.part[q][k] = mine;
+    __syncthreads();
+    unsigned run = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < GB_KS_PARTS; ++i) {
+        if (i < q) run += L.part[i][k];
+        tot += L.part[i][k];
+    }
+    if (p1 - p0 <= (unsigned)kB && p1 > p0) {  // the counts are still in v
+#pragma unroll
+        for (int i = 0; i < kB; ++i) {
+            if (p0 + i < p1) S.c2[(unsigned long long)(p0 + i) * PTK_GB_BUCKET + kk] = run;
+            run += v[i];
+        }
+    } else {
+        for (unsigned t = p0; t < p1; t += kB) {
+#pragma unroll
+            for (int i = 0; i < kB; ++i) v[i] = t + i < p1 ? S.c2[(unsigned long long)(t + i) * PTK_GB_BUCKET + kk] : 0u;
+#pragma unroll
+            for (int i = 0; i < kB; ++i) {
+                if (t + i < p1) S.c2[(unsigned long long)(t + i) * PTK_GB_BUCKET + kk] = run;
+                run += v[i];
+            }
         }
     }
-    const long long key = (long long)r * PTK_GB_BUCKET + threadIdx.x;
+    // part 0 of each key (the first GB_KS_KEYS / 64 waves): count, offsets, non-empty keys
+    const long long key = (long long)b * PTK_GB_BUCKET + kk;
     const bool in = key <= numKeys;
-    const unsigned long long c = in ? gb_count(table, numKeys)[key] : 0ull;
-    const unsigned long long incl = wave_incl(c);
+    const unsigned c = (q == 0 && in) ? tot : 0u;
+    const unsigned incl = wave_incl32(c);
     const unsigned long long has = __ballot(c != 0);
-    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
     if (lane == 63) {
         L.wrows[w] = incl;
-        L.wnz[w] = (unsigned long long)__popcll(has);
+        L.wnz[w] = (unsigned)__popcll(has);
     }
     __syncthreads();
-    unsigned long long rowBase = rowsBefore, nzBase = nzBefore;
-    for (int i = 0; i < w && i < nw; ++i) {
-        rowBase += L.wrows[i];
-        nzBase += L.wnz[i];
-    }
-    if (in) {
-        offsets[key] = (long long)(rowBase + incl - c);
-        if (c) {
-            long long *g = groups + 3 + 3 * (long long)(nzBase + __popcll(has & below));
-            g[0] = key;
-            g[1] = (long long)c;
-            g[2] = (long long)gb_sum(table, numKeys)[key];
+    if (q == 0) {
+        unsigned before = 0, nz = 0, half = 0;
+        for (int i = 0; i < GB_KS_KEYS / 64; ++i) {
+            if (i < w) before += L.wrows[i];
+            nz += L.wnz[i];
+            half += L.wrows[i];
         }
+        const unsigned lower = h == 0 ? 0u : S.btot[b] - half;  // the rows of the lower half
+        S.kcnt[key] = c;
+        if (in) offsets[key] = (long long)(S.bstart[b] + lower + before + incl - c);
+        if (tid == 0) S.nz[2 * b + h] = nz;
     }
     __syncthreads();
 }
 
-#define GB_PLACE_WAVES (PTK_GB_PLACE_THREADS / 64)
-#define GB_PLACE_ROWS (PTK_GB_CHUNK / PTK_GB_PLACE_THREADS)  // rows per thread
-#define GB_ENTRY_BITS 11                                     // entry indices < PTK_GB_CHUNK
+#define GB_TILE_WAVES (PTK_GB_TILE_THREADS / 64)
+#define GB_TILE_ROWS (PTK_GB_TILE / PTK_GB_TILE_THREADS)  // rows per thread (8)
 
-#define GB_KEY_MASK 0x7FFFFu  // keys < PTK_GB_MAX_KEYS = 2^19; a hash slot holds key | entry << 19
+struct GbShuffleLds {
+    unsigned short wcnt[GB_TILE_WAVES][PTK_GB_BUCKET];  // per wave and key: rows, then the wave's base
+    unsigned start[PTK_GB_BUCKET];                      // per key: local start
+    int base[PTK_GB_BUCKET];                            // per key: output row of local position 0
+    unsigned long long sum[PTK_GB_BUCKET];
+    alignas(16) int val[PTK_GB_TILE];                   // the tile's rows in key order
+    alignas(16) unsigned char low[PTK_GB_TILE];
+    unsigned red[16];
+};
 
-struct GbPlaceLds {
-    union {
-        unsigned slot[GB_SLOTS];  // hash of the chunk's distinct keys: key | entry index << 19
-        struct {                  // once the rows are ranked: the rows in output order
-            int row[PTK_GB_CHUNK], key[PTK_GB_CHUNK], val[PTK_GB_CHUNK];
-        } stage;
-    } u;
-    int out0[PTK_GB_CHUNK];                             // per entry: output row of its first row
-    unsigned short start[PTK_GB_CHUNK];                 // per entry: chunk-local position of its first row
-    unsigned short wcnt[GB_PLACE_WAVES][PTK_GB_CHUNK];  // per wave and entry: rows, then the wave's base
+// Tiles t, t + step, ... below tend, after the key scan, by one block: places each tile's rows
+// at offsets[key] + the earlier tiles' rows of the key + the row's rank among the tile's rows of
+// the key (row order) and writes the tile's per-key sums to S.s2[t].  The next tile's rows are
+// loaded while the current tile's are written.  blockDim.x == PTK_GB_TILE_THREADS; begins and
+// ends with a barrier.
+__device__ inline void gb_reshuffle_tiles(const PTK_GB_SCRATCH &S, int numKeys, unsigned t, unsigned tend,
+                                          unsigned step, const long long *__restrict__ offsets,
+                                          int *__restrict__ outKeys, int *__restrict__ outVals, long long n,
+                                          GbShuffleLds &L) {
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    unsigned char kb[GB_TILE_ROWS];
+    int vr[GB_TILE_ROWS];
+    auto fetch = [&](const GbTile &T) {
+        const long long wr0 = (long long)T.r0 + (long long)w * (PTK_GB_TILE / GB_TILE_WAVES);
+        gb_load_rows<GB_TILE_ROWS>(S.tlow, wr0, (long long)T.r1, kb);
+        gb_load_rows<GB_TILE_ROWS>(S.tval, wr0, (long long)T.r1, vr);
+    };
+    if (t >= tend) return;
+    GbTile T = gb_tile(S, t);
+    fetch(T);
+    for (;;) {
+        const unsigned rows = T.r1 - T.r0;
+        const long long wr0 = (long long)T.r0 + (long long)w * (PTK_GB_TILE / GB_TILE_WAVES);
+        const int nin = wr0 >= T.r1 ? 0 : (T.r1 - wr0 < 64 * GB_TILE_ROWS ? (int)(T.r1 - wr0) : 64 * GB_TILE_ROWS);
+        __syncthreads();
+        for (int q = 0; q < GB_TILE_WAVES; ++q)
+            for (int k = tid; k < PTK_GB_BUCKET; k += PTK_GB_TILE_THREADS) L.wcnt[q][k] = 0;
+        for (int k = tid; k < PTK_GB_BUCKET; k += PTK_GB_TILE_THREADS) L.sum[k] = 0;
+        __syncthreads();
+        const unsigned long long below = (1ull << lane) - 1ull;
+        unsigned rk[GB_TILE_ROWS];
+#pragma unroll
+        for (int s = 0; s < GB_TILE_ROWS; ++s) {
+            const bool use = 64 * s + lane < nin;
+            const unsigned d = kb[s];
+            const unsigned long long peers = gb_match<8>(d, use);
+            unsigned old = 0;
+            if (use) old = L.wcnt[w][d];
+            if (use && (peers & below) == 0) L.wcnt[w][d] = (unsigned short)(old + __popcll(peers));
+            rk[s] = use ? old + (unsigned)__popcll(peers & below) : 0xFFFFFFFFu;
+            __builtin_amdgcn_sched_barrier(0);  // one step at a time: keeps the ballots' masks few
+        }
+        __syncthreads();
+        for (int k = tid; k < PTK_GB_BUCKET; k += PTK_GB_TILE_THREADS) {
+            unsigned tt = 0;
+#pragma unroll
+            for (int q = 0; q < GB_TILE_WAVES; ++q) {
+                const unsigned v = L.wcnt[q][k];
+                L.wcnt[q][k] = (unsigned short)tt;
+                tt += v;
+            }
+            L.start[k] = tt;
+        }
+        gb_lds_exscan(L.start, PTK_GB_BUCKET, L.red);
+        const long long key0 = (long long)T.bucket * PTK_GB_BUCKET;
+        const unsigned *c2 = S.c2 + (unsigned long long)t * PTK_GB_BUCKET;
+        for (int k = tid; k < PTK_GB_BUCKET; k += PTK_GB_TILE_THREADS) {
+            const long long o = key0 + k <= numKeys ? offsets[key0 + k] : 0;  // keys past numKeys have no rows
+            L.base[k] = (int)(o + (long long)c2[k] - (long long)L.start[k]);
+        }
+#pragma unroll
+        for (int s = 0; s < GB_TILE_ROWS; ++s) {
+            if (rk[s] == 0xFFFFFFFFu) continue;
+            const unsigned p = L.start[kb[s]] + L.wcnt[w][kb[s]] + rk[s];
+            if (p < PTK_GB_TILE) {  // always
+                L.val[p] = vr[s];
+                L.low[p] = kb[s];
+            }
+        }
+        __syncthreads();
+        const unsigned tn = t + step;
+        if (tn < tend) {  // in flight while this tile is written
+            T = gb_tile(S, tn);
+            fetch(T);
+        }
+        // per-key sums: each thread adds up the runs of its GB_TILE_ROWS consecutive staged rows
+        if ((unsigned)tid * GB_TILE_ROWS < rows) {
+            const unsigned p0 = (unsigned)tid * GB_TILE_ROWS;
+            unsigned lw[GB_TILE_ROWS / 4];
+            int v[GB_TILE_ROWS];
+#pragma unroll
+            for (int q = 0; q < GB_TILE_ROWS / 4; ++q) {
+                lw[q] = *(const unsigned *)&L.low[p0 + 4 * q];
+                const int4 x = *(const int4 *)&L.val[p0 + 4 * q];
+                v[4 * q] = x.x;
+                v[4 * q + 1] = x.y;
+                v[4 * q + 2] = x.z;
+                v[4 * q + 3] = x.w;
+            }
+            const unsigned m = rows - p0 < GB_TILE_ROWS ? rows - p0 : GB_TILE_ROWS;
+            unsigned prev = lw[0] & 0xFFu;
+            unsigned long long acc = 0;
+#pragma unroll
+            for (int i = 0; i < GB_TILE_ROWS; ++i) {
+                if ((unsigned)i >= m) break;
+                const unsigned k = (lw[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+                if (k != prev) {
+                    atomicAdd(&L.sum[prev], acc);
+                    prev = k;
+                    acc = 0;
+                }
+                acc += (unsigned long long)(long long)v[i];
+            }
+            atomicAdd(&L.sum[prev], acc);
+        }
+        unsigned kk[GB_TILE_ROWS];
+        int vv[GB_TILE_ROWS];
+#pragma unroll
+        for (int i = 0; i < GB_TILE_ROWS; ++i) {
+            kk[i] = L.low[tid + i * PTK_GB_TILE_THREADS];
+            vv[i] = L.val[tid + i * PTK_GB_TILE_THREADS];
+        }
+        int gb[GB_TILE_ROWS];
+#pragma unroll
+        for (int i = 0; i < GB_TILE_ROWS; ++i) gb[i] = L.base[kk[i]];
+#pragma unroll
+        for (int i = 0; i < GB_TILE_ROWS; ++i) {
+            const unsigned p = tid + i * PTK_GB_TILE_THREADS, q = (unsigned)(gb[i] + (int)p);
+            if (p < rows && q < (unsigned)n) {  // q < n always, for a consistent scratch
+                outKeys[q] = (int)key0 + (int)kk[i];
+                outVals[q] = vv[i];
+            }
+        }
+        __syncthreads();
+        unsigned long long *s2 = S.s2 + (unsigned long long)t * PTK_GB_BUCKET;
+        for (int k = tid; k < PTK_GB_BUCKET; k += PTK_GB_TILE_THREADS) s2[k] = L.sum[k];
+        if (tn >= tend) break;
+        t = tn;
+    }
+    __syncthreads();
+}
+
+struct GbTotalsLds {
+    unsigned long long part[GB_TS_PARTS][GB_TS_KEYS];
     unsigned long long red[16];
 };
 
-// Chunk c: places its rows at offsets[key] + group base + merge base + (rank among the chunk's
-// rows of the key, in row order), so a group holds its rows in input order.  Wave w counts the
-// rows [w, w + 1) * PTK_GB_CHUNK / GB_PLACE_WAVES of the chunk in row order: a row's rank in its
-// wave is the wave's running count of its entry plus the lower lanes of the same step holding the
-// same entry (ballots over the entry index bits); the lower waves' counts are added afterwards.
-// The chunk's rows are staged in LDS in output order and stored in that order, so a key's run
-// of rows is written with consecutive stores.  blockDim.x == PTK_GB_PLACE_THREADS; begins and
-// ends with a barrier.
-__device__ inline void gb_place_chunk(const int *__restrict__ keys, const int *__restrict__ vals, long long n,
-                                      int numKeys, long long c, const long long *__restrict__ offsets,
-                                      const PTK_GB_SCRATCH &S, int *__restrict__ outKeys, int *__restrict__ outVals,
-                                      GbPlaceLds &L) {
-    const long long r0 = c * PTK_GB_CHUNK, r1 = r0 + PTK_GB_CHUNK < n ? r0 + PTK_GB_CHUNK : n;
+// Quarter r of bucket b (blockDim.x == PTK_GB_TOTALS_THREADS; thread q * GB_TS_KEYS + k takes part
+// q of the bucket's tiles for key b * PTK_GB_BUCKET + r * GB_TS_KEYS + k), after the reshuffle:
+// the keys' sums over the bucket's tiles (tile order), the table's counts and sums and the keys'
+// group records; the last quarter of the last bucket also writes the error count and the groups
+// header {G, rows, errors}.
+__device__ inline void gb_totals_quarter(const PTK_GB_SCRATCH &S, int numKeys, int b, int r, long long *table,
+                                         long long *groups, GbTotalsLds &L) {
     const int NB = (int)S.buckets;
-    const unsigned *gbase = S.gcnt + (c / PTK_GB_GROUP) * ((long long)NB * PTK_GB_BUCKET);
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int s = tid; s < GB_SLOTS; s += PTK_GB_PLACE_THREADS) L.u.slot[s] = GB_EMPTY;
-    for (int i = tid; i < GB_PLACE_WAVES * PTK_GB_CHUNK / 2; i += PTK_GB_PLACE_THREADS)
-        ((unsigned *)&L.wcnt[0][0])[i] = 0;
-    // the rows of this thread (row order: step s of wave w covers 64 consecutive rows)
-    const long long wr0 = r0 + (long long)w * (PTK_GB_CHUNK / GB_PLACE_WAVES);
-    int kr[GB_PLACE_ROWS], vr[GB_PLACE_ROWS];
-#pragma unroll
-    for (int s = 0; s < GB_PLACE_ROWS; ++s) {
-        const long long row = wr0 + 64 * s + lane;
-        kr[s] = row < r1 ? keys[row] : -1;
-        vr[s] = row < r1 ? vals[row] : 0;
-    }
-    unsigned D = S.boff[c * (NB + 1) + NB];  // distinct keys of the chunk (<= PTK_GB_CHUNK)
-    D = D < PTK_GB_CHUNK ? D : PTK_GB_CHUNK;
-    // chunk-local start of each entry's rows: exclusive scan of the counts in entry order, each
-    // thread taking a run of consecutive entries
-    const unsigned per = (D + PTK_GB_PLACE_THREADS - 1) / PTK_GB_PLACE_THREADS, i0 = tid * per;
-    const PTK_GB_ENTRY *ent = S.entry + c * PTK_GB_CHUNK;
+    const int tid = threadIdx.x, k = tid % GB_TS_KEYS, q = tid / GB_TS_KEYS;
+    const unsigned t0 = S.tstart[b], t1 = S.tstart[b + 1], nt = t1 - t0;
+    const unsigned per = (nt + GB_TS_PARTS - 1) / GB_TS_PARTS;
+    const unsigned p0 = t0 + q * per, p1 = p0 + per < t1 ? p0 + per : t1;
+    const unsigned kk = (unsigned)(r * GB_TS_KEYS + k);  // key within the bucket
+    constexpr int kB = 24;
     unsigned long long mine = 0;
-    for (unsigned i = i0; i < i0 + per && i < D; ++i) mine += ent[i].count;
-    const unsigned long long incl = wave_incl(mine);
-    __syncthreads();  // the slots and counters are cleared; red is free
-    if (lane == 63) L.red[w] = incl;
-    __syncthreads();
-    unsigned long long start = incl - mine;
-    for (int q = 0; q < w; ++q) start += L.red[q];
-    for (unsigned i = i0; i < i0 + per && i < D; ++i) {
-        const int k = ent[i].key;
-        if (k >= 0 && k <= numKeys) {
-            const long long out0 = offsets[k] + (long long)gbase[k] + ent[i].sum;  // sum holds the merge base
-            unsigned h = gb_hash((unsigned)k) & (GB_SLOTS - 1);
-            for (int p = 0; p < GB_SLOTS; ++p) {  // distinct keys: the first empty slot is ours
-                if (atomicCAS(&L.u.slot[h], GB_EMPTY, (unsigned)k | (i << 19)) == GB_EMPTY) break;
-                h = (h + 1) & (GB_SLOTS - 1);
-            }
-            L.start[i] = (unsigned short)(start < PTK_GB_CHUNK ? start : PTK_GB_CHUNK);
-            L.out0[i] = (int)out0;
-        }
-        start += ent[i].count;
-    }
-    __syncthreads();
-    unsigned er[GB_PLACE_ROWS], rk[GB_PLACE_ROWS];
-    const unsigned long long below = (1ull << lane) - 1ull;
+    for (unsigned t = p0; t < p1; t += kB) {
+        unsigned long long v[kB];
 #pragma unroll
-    for (int s = 0; s < GB_PLACE_ROWS; ++s) {
-        const int k = kr[s];
-        unsigned e = 0;
-        bool use = false;
-        if (k >= 0 && k <= numKeys) {
-            unsigned h = gb_hash((unsigned)k) & (GB_SLOTS - 1);
-            for (int p = 0; p < GB_SLOTS; ++p) {
-                const unsigned cur = L.u.slot[h];
-                if (cur == GB_EMPTY) break;  // only for an inconsistent scratch
-                if ((cur & GB_KEY_MASK) == (unsigned)k) {
-                    e = cur >> 19;
-                    use = true;
-                    break;
-                }
-                h = (h + 1) & (GB_SLOTS - 1);
+        for (int i = 0; i < kB; ++i) v[i] = t + i < p1 ? S.s2[(unsigned long long)(t + i) * PTK_GB_BUCKET + kk] : 0ull;
+#pragma unroll
+        for (int i = 0; i < kB; ++i) mine += v[i];
+    }
+    // group records before this quarter's keys: the non-empty keys of the lower buckets, then of
+    // this bucket's lower keys
+    unsigned long long nzb = 0;
+    for (int i = tid; i < 2 * b; i += blockDim.x) nzb += S.nz[i];
+    for (int i = tid; i < r * GB_TS_KEYS; i += blockDim.x) nzb += S.kcnt[(long long)b * PTK_GB_BUCKET + i] ? 1u : 0u;
+    nzb = block_sum(nzb, L.red);
+    L.part[q][k] = mine;
+    const long long key = (long long)b * PTK_GB_BUCKET + kk;
+    const bool in = key <= numKeys;
+    const unsigned c = (q == 0 && in) ? S.kcnt[key] : 0u;
+    const unsigned long long has = __ballot(c != 0);  // wave 0 holds the quarter's GB_TS_KEYS keys
+    __syncthreads();
+    if (q == 0) {
+        unsigned long long sum = 0;
+#pragma unroll
+        for (int i = 0; i < GB_TS_PARTS; ++i) sum += L.part[i][k];
+        const unsigned long long g = nzb + (unsigned long long)__popcll(has & ((1ull << k) - 1ull));
+        if (in) {
+            gb_count(table, numKeys)[key] = c;
+            gb_sum(table, numKeys)[key] = sum;
+            if (c) {
+                long long *e = groups + 3 + 3 * (long long)g;
+                e[0] = key;
+                e[1] = (long long)c;
+                e[2] = (long long)sum;
             }
         }
-        // lanes of this step holding the same entry
-        unsigned long long peers = __ballot(use);
-#pragma unroll
-        for (int b = 0; b < GB_ENTRY_BITS; ++b) {
-            const bool bit = (e >> b) & 1u;
-            const unsigned long long m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
-        const unsigned rank = (unsigned)__popcll(peers & below);
-        unsigned old = 0;
-        if (use) old = L.wcnt[w][e];
-        if (use && rank == 0) L.wcnt[w][e] = (unsigned short)(old + (unsigned)__popcll(peers));
-        er[s] = use ? e : 0xFFFFFFFFu;
-        rk[s] = old + rank;
-    }
-    __syncthreads();  // the hash slots are dead from here on: their memory stages the rows
-    for (unsigned e = tid; e < D; e += PTK_GB_PLACE_THREADS) {  // the lower waves' rows of each entry
-        unsigned b = 0;
-#pragma unroll
-        for (int q = 0; q < GB_PLACE_WAVES; ++q) {
-            const unsigned t = L.wcnt[q][e];
-            L.wcnt[q][e] = (unsigned short)b;
-            b += t;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < GB_PLACE_ROWS; ++s) {
-        if (er[s] == 0xFFFFFFFFu) continue;
-        const unsigned e = er[s];
-        const unsigned d = L.wcnt[w][e] + rk[s];  // the row's rank among the chunk's rows of its key
-        const unsigned pos = (unsigned)L.start[e] + d;
-        if (pos < PTK_GB_CHUNK) {  // always, for a consistent scratch
-            L.u.stage.row[pos] = L.out0[e] + (int)d;
-            L.u.stage.key[pos] = kr[s];
-            L.u.stage.val[pos] = vr[s];
-        }
-    }
-    __syncthreads();
-    unsigned long long rows = 0;  // valid rows of the chunk: every local position below is filled
-    for (int q = 0; q < GB_PLACE_WAVES; ++q) rows += L.red[q];
-    rows = rows < PTK_GB_CHUNK ? rows : PTK_GB_CHUNK;
-    for (unsigned p = tid; p < rows; p += PTK_GB_PLACE_THREADS) {
-        const int row = L.u.stage.row[p];
-        if (row >= 0 && row < n) {  // always, for a consistent scratch
-            outKeys[row] = L.u.stage.key[p];
-            outVals[row] = L.u.stage.val[p];
+        if (b == NB - 1 && r == 3 && tid == 0) {  // the groups header and the error count
+            const unsigned long long errors = S.head[0];
+            *gb_errors(table, numKeys) = errors;
+            groups[0] = (long long)(nzb + (unsigned long long)__popcll(has));
+            groups[1] = (long long)S.bstart[NB];
+            groups[2] = (long long)errors;
         }
     }
     __syncthreads();
 }
+
+#define GB_COALESCE_ROWS 2048  // grouped rows per GroupbyCoalesce unit
 
 struct GbCoalesceLds {
     long long g0;
     unsigned long long wtot[16];
 };
 
-// The coalesced layout of HostTasks/HostTasks.cpp:131-197 for rows [i0, i0 + PTK_GB_CHUNK) of a
+// The coalesced layout of HostTasks/HostTasks.cpp:131-197 for rows [i0, i0 + GB_COALESCE_ROWS) of a
 // grouped output: group g (in key order) becomes the words key, then its rows' values, so row i
 // of group g lands at word i + g + 1 and the key of g at start(g) + g; meta[g] = count of g.
 // The block finds the group of its first row by binary search over the group starts
@@ -571,7 +771,7 @@ __device__ inline void gb_coalesce_chunk(const int *__restrict__ vals, const int
                                          const long long *__restrict__ groups, const long long *__restrict__ offsets,
                                          long long c, int *__restrict__ out, int *__restrict__ meta, GbCoalesceLds &L) {
     const long long G = groups[0], rows = groups[1];
-    const long long i0 = c * PTK_GB_CHUNK, i1 = i0 + PTK_GB_CHUNK < rows ? i0 + PTK_GB_CHUNK : rows;
+    const long long i0 = c * GB_COALESCE_ROWS, i1 = i0 + GB_COALESCE_ROWS < rows ? i0 + GB_COALESCE_ROWS : rows;
     if (i0 >= i1 || G <= 0) return;  // uniform over the block
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     if (threadIdx.x == 0) {  // the last group whose start is <= i0
@@ -583,8 +783,8 @@ __device__ inline void gb_coalesce_chunk(const int *__restrict__ vals, const int
         }
         L.g0 = lo;
     }
-    // each thread takes PTK_GB_CHUNK / blockDim.x consecutive rows
-    const int per = (PTK_GB_CHUNK + blockDim.x - 1) / blockDim.x;
+    // each thread takes GB_COALESCE_ROWS / blockDim.x consecutive rows
+    const int per = (GB_COALESCE_ROWS + blockDim.x - 1) / blockDim.x;
     const long long a = i0 + (long long)threadIdx.x * per, b = a + per < i1 ? a + per : i1;
     unsigned long long mine = 0;
     for (long long i = a; i < b; ++i) mine += (i > i0 && keys[i] != keys[i - 1]) ? 1 : 0;

@@ -1,10 +1,17 @@
 """Groupby scratch layout (include/ptk_groupby.h) for host-side sizing and tests."""
 from __future__ import annotations
 
-CHUNK = 2048          # rows per chunk (PTK_GB_CHUNK)
+CHUNK = 16384         # rows per chunk (PTK_GB_CHUNK)
+SUB = 4096            # rows a GroupbyPartition block ranks at once (PTK_GB_SUB)
+TILE = 4096           # partitioned rows per tile (PTK_GB_TILE)
 BUCKET = 256          # keys per bucket (PTK_GB_BUCKET)
-GROUP = 256           # chunks per merge group (PTK_GB_GROUP)
-PLACE_THREADS = 512   # threads of a GroupbyReshuffle block (PTK_GB_PLACE_THREADS)
+HIST_THREADS = 512    # PTK_GB_HIST_THREADS
+COUNT_THREADS = 256   # PTK_GB_COUNT_THREADS
+PART_THREADS = 512    # PTK_GB_PART_THREADS
+TILE_THREADS = 512    # PTK_GB_TILE_THREADS
+SCAN_THREADS = 1024   # PTK_GB_SCAN_THREADS
+KEYSCAN_THREADS = 512   # PTK_GB_KEYSCAN_THREADS
+TOTALS_THREADS = 256    # PTK_GB_TOTALS_THREADS
 MAX_BUCKETS = 2048    # PTK_GB_MAX_BUCKETS
 MAX_KEYS = BUCKET * MAX_BUCKETS   # numKeys + 1 <= MAX_KEYS
 
@@ -17,25 +24,30 @@ def buckets(num_keys: int) -> int:
     return (num_keys + 1 + BUCKET - 1) // BUCKET
 
 
-def groups(n: int) -> int:
-    return (chunks(n) + GROUP - 1) // GROUP
+def tiles_max(n: int, num_keys: int) -> int:
+    return (n + TILE - 1) // TILE + buckets(num_keys)
 
 
-def _a16(b: int) -> int:
-    return (b + 15) // 16 * 16
+def _a256(b: int) -> int:
+    return (b + 255) // 256 * 256
+
+
+_ORDER = ["head", "btot", "bstart", "tstart", "tinfo", "nz", "berr", "hist", "kcnt", "c2", "s2", "tval",
+          "tlow", "total"]
 
 
 def scratch_offsets(n: int, num_keys: int) -> dict:
-    """Byte offsets of btot, berr, boff, gsum, gcnt, entry and the total size."""
-    B, NB, NG = chunks(n), buckets(num_keys), groups(n)
+    """Byte offsets of the scratch arrays (include/ptk_groupby.h) and the total size."""
+    C, NB, TM = chunks(n), buckets(num_keys), tiles_max(n, num_keys)
     KP = NB * BUCKET
-    o = {"btot": 0}
-    o["berr"] = _a16(16 * NB)
-    o["boff"] = _a16(o["berr"] + 4 * B)
-    o["gsum"] = _a16(o["boff"] + 4 * B * (NB + 1))
-    o["gcnt"] = _a16(o["gsum"] + 8 * NG * KP)
-    o["entry"] = _a16(o["gcnt"] + 4 * NG * KP)
-    o["total"] = o["entry"] + 16 * B * CHUNK
+    size = {"head": 256, "btot": 4 * NB, "bstart": 4 * (NB + 1), "tstart": 4 * (NB + 1), "tinfo": 16 * TM,
+            "nz": 8 * NB, "berr": 4 * C, "hist": 4 * C * NB, "kcnt": 4 * KP, "c2": 4 * TM * BUCKET,
+            "s2": 8 * TM * BUCKET, "tval": 4 * n, "tlow": n + 16}
+    o, at = {}, 0
+    for name in _ORDER[:-1]:
+        o[name] = at
+        at = _a256(at + size[name])
+    o["total"] = at
     return o
 
 

@@ -20,8 +20,9 @@ def test_footprints():
     assert batch.footprint(batch.FDTD, nx, ny, nz) == want
     rows, nk = 1000, 99
     K = nk + 1
-    # btot 16, berr 16, boff 16, gsum 8 x 256, gcnt 4 x 256, entries 32768
-    assert groupby.scratch_bytes(rows, nk) == 35888
+    # one bucket, one chunk, at most two tiles; each array 256-aligned: head 256, btot, bstart,
+    # tstart, tinfo, nz, berr, hist 256 each, kcnt 1024, c2 2048, s2 4096, tval 4096, tlow 1024
+    assert groupby.scratch_bytes(rows, nk) == 14336
     assert batch.footprint(batch.GROUPBY, rows, nk) == (4 * up(4 * rows) + up(8 * (3 * K + 1)) + up(8 * K) +
                                                         up(8 * (3 + 3 * K)) + up(groupby.scratch_bytes(rows, nk)))
     n, d, k, steps = 5000, 7, 5, 9

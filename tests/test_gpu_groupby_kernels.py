@@ -1,10 +1,9 @@
-"""The five groupby task kernels launched directly through the C ABI, against the oracle, over
-key ranges, skews, row counts and launch grids (including grids smaller than the work, so blocks
-stride over chunks and buckets).
+"""The seven groupby task kernels launched directly through the C ABI, against the oracle, over
+key ranges (up to 2^19 keys: 2048 buckets), skews, row counts and launch grids (including grids
+smaller than the work, so blocks stride over chunks, buckets and tiles).
 
 Bar: bit-exact for every integer output -- per-key int64 count and sum, the groups header and
-records, offsets, the reshuffled keys and values (rows of a group in input order).  Merge and
-Reshuffle take blocks of exactly 256 threads; the accumulate block size varies."""
+records, offsets, the reshuffled keys and values (rows of a group in input order)."""
 import numpy as np
 import pytest
 
@@ -47,12 +46,13 @@ def dev(hip, a):
     return b
 
 
-def run_kernels(hip, gmod, keys, vals, nk, chunk_grid, merge_grid, off_grid, block):
+def run_kernels(hip, gmod, keys, vals, nk, grids):
+    """The seven launches in the graph's order; grids[i] caps launch i's grid (None: full)."""
     n = keys.size
     K = nk + 1
     NB = GB.buckets(nk)
-    units = NB * GB.groups(n)
-    table = dev(hip, np.full(3 * K + 1, 0x5A5A5A5A5A5A5A5A, np.int64))  # merge writes what it owns
+    TM = GB.tiles_max(n, nk)
+    table = dev(hip, np.full(3 * K + 1, 0x5A5A5A5A5A5A5A5A, np.int64))  # totals writes what it owns
     scratch = hip.DeviceBuffer(GB.scratch_bytes(n, nk))
     offsets = dev(hip, np.zeros(K, np.int64))
     groups = dev(hip, np.zeros(3 + 3 * K, np.int64))
@@ -60,17 +60,19 @@ def run_kernels(hip, gmod, keys, vals, nk, chunk_grid, merge_grid, off_grid, blo
     ok = dev(hip, np.zeros(n, np.int32))
     ov = dev(hip, np.zeros(n, np.int32))
     f = gmod.function
-    hip.launch(f("GroupbyAccumulate"), (min(chunk_grid, GB.chunks(n)), 1, 1), (block, 1, 1),
+    full = [max(1, GB.chunks(n)), NB, max(1, min(GB.chunks(n), 512)), TM, 2 * NB, TM, 4 * NB]
+    g = [full[i] if grids[i] is None else max(1, min(grids[i], full[i])) for i in range(7)]
+    sc = lambda: hip.ArgPack().ptr(scratch.ptr).i32(n).i32(nk)
+    hip.launch(f("GroupbyHistogram"), (g[0], 1, 1), (GB.HIST_THREADS, 1, 1),
+               hip.ArgPack().ptr(dk.ptr).i32(n).i32(nk).ptr(scratch.ptr))
+    hip.launch(f("GroupbyBucketScan"), (g[1], 1, 1), (GB.SCAN_THREADS, 1, 1), sc())
+    hip.launch(f("GroupbyPartition"), (g[2], 1, 1), (GB.PART_THREADS, 1, 1),
                hip.ArgPack().ptr(dk.ptr).ptr(dv.ptr).i32(n).i32(nk).ptr(scratch.ptr))
-    hip.launch(f("GroupbyMerge"), (min(merge_grid, units), 1, 1), (GB.BUCKET, 1, 1),
-               hip.ArgPack().ptr(scratch.ptr).i32(n).i32(nk))
-    hip.launch(f("GroupbyTotals"), (min(off_grid, NB), 1, 1), (GB.BUCKET, 1, 1),
-               hip.ArgPack().ptr(scratch.ptr).i32(n).i32(nk).ptr(table.ptr))
-    hip.launch(f("GroupbyOffsets"), (min(off_grid, NB), 1, 1), (GB.BUCKET, 1, 1),
-               hip.ArgPack().ptr(table.ptr).i32(nk).ptr(scratch.ptr).ptr(offsets.ptr).ptr(groups.ptr))
-    hip.launch(f("GroupbyReshuffle"), (min(chunk_grid, GB.chunks(n)), 1, 1), (GB.PLACE_THREADS, 1, 1),
-               hip.ArgPack().ptr(dk.ptr).ptr(dv.ptr).i32(n).i32(nk).ptr(offsets.ptr).ptr(scratch.ptr)
-               .ptr(ok.ptr).ptr(ov.ptr))
+    hip.launch(f("GroupbyKeyCount"), (g[3], 1, 1), (GB.COUNT_THREADS, 1, 1), sc())
+    hip.launch(f("GroupbyKeyScan"), (g[4], 1, 1), (GB.KEYSCAN_THREADS, 1, 1), sc().ptr(offsets.ptr))
+    hip.launch(f("GroupbyReshuffle"), (g[5], 1, 1), (GB.TILE_THREADS, 1, 1),
+               sc().ptr(offsets.ptr).ptr(ok.ptr).ptr(ov.ptr))
+    hip.launch(f("GroupbyTotals"), (g[6], 1, 1), (GB.TOTALS_THREADS, 1, 1), sc().ptr(table.ptr).ptr(groups.ptr))
     hip.synchronize()
     out = {}
     for name, buf, a in (("table", table, np.zeros(3 * K + 1, np.int64)),
@@ -82,12 +84,13 @@ def run_kernels(hip, gmod, keys, vals, nk, chunk_grid, merge_grid, off_grid, blo
     return out
 
 
-# 1M rows span two merge groups of 256 chunks, 2.1M rows five (the last one short)
+# 2.1M rows: 129 chunks of 16384 rows (the last one short), hundreds of tiles; one_key puts a whole
+# input into one bucket (hundreds of tiles of one key); 2^19 keys use 2048 buckets
 CASES = [("zipf", 200_003, 65535), ("zipf", 1_000_000, GB.MAX_KEYS - 1), ("uniform", 4097, 0),
          ("invalid", 10_000, 1000), ("one_key", 50_001, 300), ("uniform", 2049, 513),
          ("uniform", 3, 7), ("uniform", 300_000, 299_999), ("zipf", 2048, 255),
-         ("zipf", 2_100_000, 65535), ("one_key", 1_050_000, 0)]
-GRIDS = {"full": (2048, 1 << 20, 1 << 20, 512), "strided": (3, 2, 2, 256), "wide": (2048, 7, 5, 1024)}
+         ("zipf", 2_100_000, 65535), ("one_key", 1_050_000, 0), ("invalid", 70_000, GB.MAX_KEYS - 2)]
+GRIDS = {"full": [None] * 7, "strided": [3, 2, 1, 3, 2, 5, 3], "wide": [None, None, 7, None, 5, 11, 6]}
 
 
 @pytest.mark.parametrize("grid", sorted(GRIDS))
@@ -95,7 +98,7 @@ GRIDS = {"full": (2048, 1 << 20, 1 << 20, 512), "strided": (3, 2, 2, 256), "wide
 def test_groupby_kernels_equal_oracle(hip, gmod, kind, n, nk, grid):
     print(f"case groupby kernels {kind} n={n} nk={nk} grid={grid}", flush=True)
     keys, vals = make_keys(kind, n, nk, seed=n + nk)
-    got = run_kernels(hip, gmod, keys, vals, nk, *GRIDS[grid])
+    got = run_kernels(hip, gmod, keys, vals, nk, GRIDS[grid])
     g, head, offsets, eok, eov = oracle.groupby(keys, vals, nk)
     K = nk + 1
     cnt = np.zeros(K, np.int64)

@@ -22,6 +22,19 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), n
 
 
+def test_python_layout_constants_equal_the_header():
+    """ptask_amd/groupby.py mirrors the PTK_GB_* constants of include/ptk_groupby.h."""
+    import os
+    import re
+
+    from ptask_amd import paths
+    text = open(os.path.join(paths.include_dir(), "ptk_groupby.h")).read()
+    macros = dict(re.findall(r"^#define PTK_GB_(\w+) (\d+)", text, re.M))
+    for name in ("CHUNK", "SUB", "TILE", "BUCKET", "HIST_THREADS", "COUNT_THREADS", "PART_THREADS",
+                 "TILE_THREADS", "SCAN_THREADS", "KEYSCAN_THREADS", "TOTALS_THREADS", "MAX_BUCKETS"):
+        assert int(macros[name]) == getattr(GB, name), name
+
+
 @pytest.mark.parametrize("n,nk", [(1, 0), (1000, 99), (2_100_000, 65535), (300_000, GB.MAX_KEYS - 2)])
 def test_workspace_is_table_plus_scratch(n, nk):
     up = lambda x: (x + 255) // 256 * 256
@@ -101,7 +114,7 @@ def test_noalloc_rejects_inputs_beyond_its_static_workspace():
     cap = L.ptk_groupby_noalloc_capacity()
     assert cap == 64 << 20
     p = C.c_void_p(0x1000)  # never dereferenced: the size check fails first
-    n = 8_000_000
+    n = 12_000_000
     assert L.ptk_groupby_workspace_bytes(n, 65535) > cap
     assert L.ptk_groupby_run_noalloc(p, p, n, 65535, p, p, p, p, None) == GB.ESPACE
     assert b"static workspace" in L.ptk_groupby_last_error()

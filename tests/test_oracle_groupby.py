@@ -136,12 +136,12 @@ def test_host_pfxsum_graph_equals_oracle(tmp_path):
         assert np.array_equal(out, oracle.pfxsum(keys)), n
 
 
-@pytest.mark.parametrize("n,nk", [(1, 0), (2049, 513), (10_000, 1000), (6000, 255), (1_100_000, 4095)])
+@pytest.mark.parametrize("n,nk", [(1, 0), (2049, 513), (10_000, 1000), (6000, 255), (1_100_000, 4095),
+                                  (70_000, 2**19 - 2)])
 def test_host_task_forms_pipeline(n, nk):
-    """The host-task forms of the five groupby tasks (the chunk/bucket scratch layout of
-    include/ptk_groupby.h restated on the host) chained as the graph chains them, with invalid
-    keys, equal the oracle's pipeline exactly, values in input order included (1.1M rows span three
-    merge groups)."""
+    """The host-task forms of the seven groupby tasks (the two-level radix partition and scratch
+    layout of include/ptk_groupby.h restated on the host) chained as the graph chains them, with
+    invalid keys, equal the oracle's pipeline exactly, values in input order included."""
     import ctypes as C
 
     from ptask_amd import groupby as GB
@@ -152,7 +152,7 @@ def test_host_task_forms_pipeline(n, nk):
     bad = rng.random(n) < 0.05
     keys[bad] = rng.choice([-1, K, 2**31 - 1], bad.sum()).astype(np.int32)
     vals = rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32)
-    table = np.full(3 * K + 1, 0x5A5A5A5A5A5A5A5A, np.int64)  # merge writes what it owns
+    table = np.full(3 * K + 1, 0x5A5A5A5A5A5A5A5A, np.int64)  # totals writes what it owns
     scratch = np.zeros(GB.scratch_bytes(n, nk) // 8 + 2, np.int64)
     offsets = np.zeros(K, np.int64)
     groups = np.zeros(3 + 3 * K, np.int64)
@@ -163,11 +163,13 @@ def test_host_task_forms_pipeline(n, nk):
         getattr(L, name)(C.c_uint(len(args)), arr)
 
     p = lambda a: a.ctypes.data
-    call("GroupbyAccumulate", p(keys), p(vals), n, nk, p(scratch))
-    call("GroupbyMerge", p(scratch), n, nk)
-    call("GroupbyTotals", p(scratch), n, nk, p(table))
-    call("GroupbyOffsets", p(table), nk, p(scratch), p(offsets), p(groups))
-    call("GroupbyReshuffle", p(keys), p(vals), n, nk, p(offsets), p(scratch), p(ok), p(ov))
+    call("GroupbyHistogram", p(keys), n, nk, p(scratch))
+    call("GroupbyBucketScan", p(scratch), n, nk)
+    call("GroupbyPartition", p(keys), p(vals), n, nk, p(scratch))
+    call("GroupbyKeyCount", p(scratch), n, nk)
+    call("GroupbyKeyScan", p(scratch), n, nk, p(offsets))
+    call("GroupbyReshuffle", p(scratch), n, nk, p(offsets), p(ok), p(ov))
+    call("GroupbyTotals", p(scratch), n, nk, p(table), p(groups))
     g, head, eoff, eok, eov = oracle.groupby(keys, vals, nk)
     assert np.array_equal(groups[:3], head) and head[2] == bad.sum()
     assert np.array_equal(groups[3:3 + g.size], g.reshape(-1))
