@@ -58,6 +58,12 @@ public:
     /* A sticky or in/out block that survives across dispatches until replaced. */
     void SetPermanentBlock(BOOL b) { m_bPermanentBlock = b; }
     BOOL IsPermanentBlock() { return m_bPermanentBlock; }
+    /* An output port whose block goes to a reader and a writer (an in/out consumer or a graph
+     * output channel), or to two writers, pushes copies to the writers; TRUE here pushes the one
+     * block to all of them, for graphs whose structure already orders the writes after the reads
+     * (reference: Port::SetSuppressClones, port.h:1274, OutputPort.cpp:1760). */
+    virtual void SetSuppressClones(BOOL b) { m_bSuppressClones = b; }
+    virtual BOOL GetSuppressClones() { return m_bSuppressClones; }
 
     virtual void BindChannel(Channel *pChannel);
     virtual void BindControlChannel(Channel *pChannel);
@@ -108,6 +114,7 @@ protected:
     Port *m_pDescribedPort;
     std::vector<Port *> m_vIterationTargets;
     BOOL m_bDependentBinding = FALSE;
+    BOOL m_bSuppressClones = FALSE;
     ACCELERATOR_CLASS m_eDependentClass = ACCELERATOR_CLASS_UNKNOWN;
     int m_nDependentIndex = 0;
 };

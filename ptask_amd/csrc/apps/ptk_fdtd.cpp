@@ -275,9 +275,13 @@ struct FdtdGraph {
             meta->SetMetaFunction(MF_GENERAL_ITERATOR);
             ins.push_back(meta);
         }
-        for (size_t b = 0; b < ts.out.size(); ++b)
+        // A field block goes to tasks that only read it and to this application, which only reads
+        // what it pulls: no consumer writes it in place, so the consumers share it uncopied.
+        for (size_t b = 0; b < ts.out.size(); ++b) {
             outs.push_back(Runtime::CreatePort(OUTPUT_PORT, field[ts.out[b]], uid++,
                                                (char *)kFieldNames[ts.out[b]], n + 1 + (UINT)b));
+            outs.back()->SetSuppressClones(TRUE);
+        }
         std::string name = std::string(ts.kernel) + (step >= 0 ? "_" + std::to_string(step) : step == -2 ? "_tail" : "");
         Task *task = g->AddTask(k, (UINT)ins.size(), ins.data(), (UINT)outs.size(), outs.data(), name.c_str());
         if (task) Geometry(task, ts);
@@ -323,6 +327,8 @@ struct FdtdGraph {
                     if (t == field[f]) {
                         if (outputs[f]) return Runtime::HandleError("two graph outputs carry %s", kFieldNames[f]);
                         outputs[f] = (GraphOutputChannel *)c;
+                        // as when built here: the consumers of a field only read it
+                        if (Port *src = c->GetBoundPort(CE_SRC)) src->SetSuppressClones(TRUE);
                     }
             }
         }

@@ -216,6 +216,10 @@ int main(int argc, char **argv) {
                         Runtime::CreatePort(INPUT_PORT, tPart, uid++, (char *)"partials", 1),
                         Runtime::CreatePort(STICKY_PORT, tDims, uid++, (char *)"dims", 2), nullptr};
         Port *fOut[1] = {Runtime::CreatePort(OUTPUT_PORT, tModel, uid++, (char *)"model(out)", 0)};
+        // The next finalize writes the model in place while the next accumulate reads it, but the
+        // finalize also consumes that accumulate's partials, so it cannot start before the read
+        // ends: the two share the block and the runtime makes no copy of it.
+        fOut[0]->SetSuppressClones(TRUE);
         UINT na = 3, nf = 3;
         if (looped) {
             aIn[3] = Runtime::CreatePort(META_PORT, tIter, uid++, (char *)"iterations");

@@ -8,6 +8,8 @@
 //                                         dispatch for about 2 ms, copies the tag out
 //   RecordLog(int* out, int max) -> n     the log so far (called here through dlsym)
 //   ScaleVector(float* in, float s, int n, float* out)   out = s * in
+//   ScaleInPlace(float* A, float s, int n)               A = s * A (an in/out port)
+//   AddVectors(float* A, float* B, int n, float* out)    out = A + B
 //
 // Scenarios (--scenario):
 //   contend     G graphs of K independent RecordTask tasks each, graph g's tasks at priority
@@ -22,6 +24,10 @@
 //               reports the stall, then the missing input arrives and the task runs.
 //   concurrent  G graphs, each one ScaleVector task whose MF_SIMPLE_ITERATOR meta port repeats
 //               its dispatch --iterations times, all running at once; every output is checked.
+//   fanout      one output port feeding both an in/out consumer and a reader (reference
+//               OutputPort.cpp:370-440): P = s1*A goes to ScaleInPlace (W = s2*P, in place) and
+//               to AddVectors, whose second input is W, so the reader runs after the writer and
+//               must still see P: out = P + W.  Repeated --iterations times on one graph.
 // Prints one JSON object.
 #include <dlfcn.h>
 
@@ -340,6 +346,90 @@ int Stall(const Options &o) {
     return ok ? 0 : 1;
 }
 
+// fanout: P = ScaleVector(A, s1) fans out to W = ScaleInPlace(P, s2) (in/out: port 0 in and out)
+// and to R = AddVectors(P, W).  R cannot dispatch before W has produced W, so a writer that
+// scaled the shared block in place would hand R s2*P for P; the runtime must give the writer its
+// own copy instead.  out = s1*A + s2*s1*A, exact in binary32 for the values used.
+int Fanout(const Options &o) {
+    DatablockTemplate *tVec = Runtime::GetDatablockTemplate((char *)"vec", sizeof(float), (UINT)o.elems, 1, 1);
+    DatablockTemplate *tScale = Runtime::GetDatablockTemplate((char *)"scale", sizeof(float), PTPARM_FLOAT);
+    DatablockTemplate *tN = Runtime::GetDatablockTemplate((char *)"n", sizeof(int), PTPARM_INT);
+    CompiledKernel *kP = Runtime::GetCompiledKernel((char *)o.kernels.c_str(), (char *)"ScaleVector");
+    CompiledKernel *kW = Runtime::GetCompiledKernel((char *)o.kernels.c_str(), (char *)"ScaleInPlace");
+    CompiledKernel *kR = Runtime::GetCompiledKernel((char *)o.kernels.c_str(), (char *)"AddVectors");
+    if (!kP || !kW || !kR) return 1;
+    UINT uid = 0;
+    Graph *g = new Graph("fanout");
+    Port *pin[3] = {Runtime::CreatePort(INPUT_PORT, tVec, uid++, (char *)"A", 0),
+                    Runtime::CreatePort(STICKY_PORT, tScale, uid++, (char *)"s1", 1),
+                    Runtime::CreatePort(STICKY_PORT, tN, uid++, (char *)"N", 2)};
+    Port *pout = Runtime::CreatePort(OUTPUT_PORT, tVec, uid++, (char *)"P", 3);
+    Port *win[3] = {Runtime::CreatePort(INPUT_PORT, tVec, uid++, (char *)"W", 0, 0),
+                    Runtime::CreatePort(STICKY_PORT, tScale, uid++, (char *)"s2", 1),
+                    Runtime::CreatePort(STICKY_PORT, tN, uid++, (char *)"N", 2)};
+    Port *wout = Runtime::CreatePort(OUTPUT_PORT, tVec, uid++, (char *)"W(out)", 0);
+    Port *rin[3] = {Runtime::CreatePort(INPUT_PORT, tVec, uid++, (char *)"X", 0),
+                    Runtime::CreatePort(INPUT_PORT, tVec, uid++, (char *)"Y", 1),
+                    Runtime::CreatePort(STICKY_PORT, tN, uid++, (char *)"N", 2)};
+    Port *rout = Runtime::CreatePort(OUTPUT_PORT, tVec, uid++, (char *)"out", 3);
+    Task *tP = g->AddTask(kP, 3, pin, 1, &pout, "P");
+    Task *tW = g->AddTask(kW, 3, win, 1, &wout, "W");
+    Task *tR = g->AddTask(kR, 3, rin, 1, &rout, "R");
+    if (!tP || !tW || !tR) return 1;
+    for (Task *t : {tP, tW, tR}) t->SetComputeGeometry(1, 1, 1);
+    GraphInputChannel *ca = g->AddInputChannel(pin[0], "A"), *cs1 = g->AddInputChannel(pin[1], "s1");
+    GraphInputChannel *cs2 = g->AddInputChannel(win[1], "s2");
+    GraphInputChannel *cn[3] = {g->AddInputChannel(pin[2], "Np"), g->AddInputChannel(win[2], "Nw"),
+                                g->AddInputChannel(rin[2], "Nr")};
+    g->AddInternalChannel(pout, win[0], "P->W");
+    g->AddInternalChannel(pout, rin[0], "P->R");
+    g->AddInternalChannel(wout, rin[1], "W->R");
+    GraphOutputChannel *co = g->AddOutputChannel(rout, "out");
+    g->Run();
+    const float s1 = 1.5f, s2 = 4.0f;
+    const int n = o.elems;
+    Datablock *b1 = Runtime::AllocateDatablock(tScale, (void *)&s1, sizeof(s1), nullptr);
+    Datablock *b2 = Runtime::AllocateDatablock(tScale, (void *)&s2, sizeof(s2), nullptr);
+    cs1->Push(b1);
+    cs2->Push(b2);
+    b1->Release();
+    b2->Release();
+    for (GraphInputChannel *c : cn) {
+        Datablock *bn = Runtime::AllocateDatablock(tN, (void *)&n, sizeof(n), nullptr);
+        c->Push(bn);
+        bn->Release();
+    }
+    bool ok = true;
+    int bad = 0;
+    for (int it = 0; it < o.iterations; ++it) {
+        std::vector<float> A((size_t)n);
+        for (int e = 0; e < n; ++e) A[(size_t)e] = (float)(it + 1) + 0.25f * (float)(e % 1024);
+        Datablock *ba = Runtime::AllocateDatablock(tVec, A.data(), (UINT)(n * sizeof(float)), nullptr);
+        ca->Push(ba);
+        ba->Release();
+        Datablock *r = co->Pull(60000);
+        if (!r) {
+            ok = false;
+            break;
+        }
+        r->Lock();
+        const float *p = (const float *)r->GetDataPointer(FALSE);
+        for (int e = 0; e < n; ++e) {
+            const float P = s1 * A[(size_t)e];
+            if (p[e] != P + s2 * P) ++bad;
+        }
+        r->Unlock();
+        r->Release();
+    }
+    g->Stop();
+    g->Teardown();
+    Graph::DestroyGraph(g);
+    ok = ok && bad == 0;
+    printf("{\"scenario\": \"fanout\", \"ok\": %s, \"iterations\": %d, \"elems\": %d, \"mismatches\": %d}\n",
+           ok ? "true" : "false", o.iterations, n, bad);
+    return ok ? 0 : 1;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -370,7 +460,7 @@ int main(int argc, char **argv) {
     }
     if (o.kernels.empty() || o.graphs < 1 || o.tasks < 1 || o.hostSlots < 1 || o.iterations < 1 || o.elems < 1 ||
         (o.mode != "priority" && o.mode != "fifo")) {
-        fprintf(stderr, "usage: ptk_multigraph --kernels <hosttasks.so> [--scenario contend|pause|concurrent|stall] "
+        fprintf(stderr, "usage: ptk_multigraph --kernels <hosttasks.so> [--scenario contend|pause|concurrent|stall|fanout] "
                         "[--mode priority|fifo] [--graphs G] [--tasks K] [--priorities p0,p1,...] "
                         "[--host-slots S] [--iterations I] [--elems N] [--watchdog-ms T]\n");
         return 2;
@@ -394,6 +484,7 @@ int main(int argc, char **argv) {
     else if (o.scenario == "pause") r = Pause(o);
     else if (o.scenario == "concurrent") r = Concurrent(o);
     else if (o.scenario == "stall") r = Stall(o);
+    else if (o.scenario == "fanout") r = Fanout(o);
     else fprintf(stderr, "unknown scenario %s\n", o.scenario.c_str());
     Runtime::Terminate();
     return r;

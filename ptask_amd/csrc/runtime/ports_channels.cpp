@@ -285,11 +285,43 @@ Datablock *OutputPort::GetDestinationBlock(UINT cbBytes) {
     return b;
 }
 
+// reference: OutputPort::Push (OutputPort.cpp:340-440), GetDownstreamWriterPortCount and
+// GetDownstreamReadonlyPortCount (OutputPort.cpp:1641-1750).  Of the channels a block passes, a
+// graph output channel (the application may write what it pulls) and a channel into an in/out
+// port are writers, the rest readers.  Readers share the block.  When a reader and a writer, or
+// two writers, share it, every writer gets its own copy -- but the first writer keeps the block
+// when there is no reader -- so no consumer sees another's writes.  SetSuppressClones(TRUE) on
+// the port pushes the one block to all.
 void OutputPort::Push(Datablock *pBlock) {
     if (!pBlock) return;
     if (IsGateOpen()) {
+        auto writer = [](Channel *c) {
+            if (c->GetType() == CT_GRAPH_OUTPUT) return true;
+            Port *d = c->GetBoundPort(CE_DST);
+            return d && d->GetPortType() == INPUT_PORT && static_cast<InputPort *>(d)->GetInOutConsumer() != nullptr;
+        };
+        UINT nReaders = 0, nWriters = 0;
         for (Channel *c : m_vChannels) {
             if (!c->PassesPredicate(CE_SRC, pBlock)) continue;
+            if (writer(c)) ++nWriters;
+            else ++nReaders;
+        }
+        const bool clones = !m_bSuppressClones && ((nReaders && nWriters) || nWriters > 1);
+        UINT writersSeen = 0;
+        for (Channel *c : m_vChannels) {
+            if (!c->PassesPredicate(CE_SRC, pBlock)) continue;
+            if (clones && writer(c) && (nReaders > 0 || writersSeen++ > 0)) {
+                Graph *g = m_pTask ? m_pTask->GetGraph() : nullptr;
+                if (g && g->GetReplay()) g->GetReplay()->OnBlockCopied();
+                Datablock *copy = pBlock->CloneForWriter();
+                if (!copy) {
+                    Runtime::HandleError("%s: no memory for the copy a writing consumer needs", GetVariableBinding());
+                    continue;
+                }
+                c->Enqueue(copy);
+                copy->Release();
+                continue;
+            }
             c->Enqueue(pBlock);
         }
     }

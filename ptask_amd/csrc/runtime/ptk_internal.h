@@ -255,6 +255,15 @@ private:
     };
     void Fail(const char *why);
     void Seal();
+
+public:
+    // A runtime-side copy of a block for an in/out consumer (OutputPort::Push) is not a launch,
+    // so a run that makes one cannot be replayed from its recorded launches.
+    void OnBlockCopied() {
+        if (m_eState == RECORDING) Fail("a run copied a block for an in/out consumer");
+    }
+
+private:
     bool CaptureLaunches();
 
     Graph *m_pGraph;

@@ -11,7 +11,7 @@ TSANFLAGS := -O1 -g -fPIC -std=c++17 -fsanitize=thread -fno-omit-frame-pointer -
              -I$(ROCM)/include -I$(ROOT)/include -I$(ROOT)/include/ptask
 RT_SRCS   := $(wildcard runtime/*.cpp)
 RT_OBJS   := $(patsubst runtime/%.cpp,$(OUT)/obj/%.o,$(RT_SRCS))
-TSAN_APPS := ptk_fdtd ptk_multigraph ptk_iteration
+TSAN_APPS := ptk_fdtd ptk_multigraph ptk_iteration ptk_cupca
 
 all: $(OUT)/libptask.so $(addprefix $(OUT)/bin/,$(TSAN_APPS))
 

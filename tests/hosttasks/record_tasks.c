@@ -57,3 +57,13 @@ void ScaleInPlace(unsigned int nArgs, void **a) {
     const int n = (int)(intptr_t)a[2];
     for (int i = 0; i < n; ++i) A[i] = s * A[i];
 }
+
+/* AddVectors(float* A, float* B, int n, float* out): out = A + B. */
+void AddVectors(unsigned int nArgs, void **a) {
+    (void)nArgs;
+    const float *A = (const float *)a[0];
+    const float *B = (const float *)a[1];
+    const int n = (int)(intptr_t)a[2];
+    float *out = (float *)a[3];
+    for (int i = 0; i < n; ++i) out[i] = A[i] + B[i];
+}

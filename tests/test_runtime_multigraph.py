@@ -4,7 +4,8 @@ The reference runs many graphs at once from one global scheduler: a ready queue 
 task priorities in SCHEDMODE_PRIORITY, quiescence (rttests.cpp:936-1333, graphbench.cpp:752-845,
 Scheduler.cpp:526-700, 4551-4673).  Checked here: admission order across graphs by priority and
 by arrival, global quiescence, a stopped graph that dispatches nothing until it runs again, and
-many concurrent graphs with iterated tasks computing the right values.
+many concurrent graphs with iterated tasks computing the right values, and a block fanned out
+to an in/out writer and a reader (the writer gets its own copy).
 """
 import json
 import os
@@ -83,3 +84,12 @@ def test_watchdog_quiet_while_graphs_progress(hostlib):
                         "--host-slots", "2", "--watchdog-ms", "200"], capture_output=True, text=True, timeout=120)
     assert p.returncode == 0 and json.loads(p.stdout.strip().splitlines()[-1])["ok"]
     assert "graph monitor" not in p.stderr
+
+
+@pytest.mark.parametrize("slots", [1, 4])
+def test_in_out_consumer_of_a_fanned_out_block_gets_its_own_copy(hostlib, slots):
+    # OutputPort.cpp:370-440: a block pushed to several consumers, one of which writes it in place
+    # (an in/out port), is copied for that writer; the reader here runs after the writer (it also
+    # consumes the writer's output) and must still see the producer's values
+    r = run(hostlib, "--scenario", "fanout", "--iterations", 12, "--elems", 50_000, "--host-slots", slots)
+    assert r["ok"] and r["mismatches"] == 0

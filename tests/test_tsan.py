@@ -2,7 +2,9 @@
 built with -fsanitize=thread).  Scenarios that drive its threads: runner threads over looped and
 unrolled host-task graphs, several graphs at once through the dispatch arbiter with priorities,
 pool sweeps after forced allocation failures while runners recycle blocks, global quiescence and
-stopped graphs (ptk_multigraph), and the Runtime:: helper sequence (ptk_iteration).  A run passes
+stopped graphs, a block fanned out to an in/out writer and a reader (ptk_multigraph), the
+Runtime:: helper sequence (ptk_iteration), and the BLAS-task PCA graph, whose buffers fan out to
+readers and in-place writers at once (ptk_cupca).  A run passes
 when ThreadSanitizer reports nothing beyond the suppressions of ptask_amd/csrc/tsan.supp (the
 internal mutex of libstdc++'s condition_variable_any) and the results are right.
 """
@@ -13,6 +15,7 @@ import subprocess
 import pytest
 
 from ptask_amd import paths
+from test_pca_graph import HOSTLIB as PCA_HOSTLIB
 from test_runtime_fdtd_graph import HOSTLIB
 
 TSAN_BIN = os.path.join(paths.LIB_DIR, "tsan", "bin")
@@ -67,7 +70,8 @@ def test_fdtd_tsan_results_equal_regular_build():
     ["--scenario", "pause"],
     ["--scenario", "concurrent", "--graphs", 6, "--iterations", 5, "--host-slots", 3],
     ["--scenario", "stall", "--watchdog-ms", 50],
-], ids=["contend", "pause", "concurrent", "watchdog"])
+    ["--scenario", "fanout", "--iterations", 6, "--elems", 4096, "--host-slots", 4],
+], ids=["contend", "pause", "concurrent", "watchdog", "fanout"])
 def test_multigraph_race_free(recordlib, args):
     p = run_tsan("ptk_multigraph", "--kernels", recordlib, *args)
     assert p.returncode == 0
@@ -76,5 +80,12 @@ def test_multigraph_race_free(recordlib, args):
 
 def test_runtime_helpers_race_free(recordlib):
     p = run_tsan("ptk_iteration", "--kernels", recordlib, "--iterations", 7)
+    assert p.returncode == 0
+    assert json.loads(p.stdout.strip().splitlines()[-1])["ok"]
+
+
+def test_blas_pca_host_graph_race_free(tmp_path):
+    p = run_tsan("ptk_cupca", "--kernels", PCA_HOSTLIB, "--m", 40, "--n", 24, "--k", 2, "--j", 3, "--runs", 2,
+                 "--dump", tmp_path / "cupca.bin")
     assert p.returncode == 0
     assert json.loads(p.stdout.strip().splitlines()[-1])["ok"]
