@@ -9,7 +9,8 @@
 // values, partial sums) are datablocks chained through in/out port pairs, so every task runs on
 // the device-resident copy; each task has its own sticky PCA_DIMS carrying its component index.
 // X (column-major M x N binary32) is N(0,1) or U[0,1) from the shared generator, or --data.
-// Prints one JSON object; --dump writes X, then T (M x K), P (N x K), R (M x N) column-major
+// --capture records the first run's launches and replays each later run as one hipGraph launch
+// (Graph::EnableLaunchCapture).  Prints one JSON object; --dump writes X, then T (M x K), P (N x K), R (M x N) column-major
 // binary32 and L (K binary64).
 
 #include <algorithm>
@@ -40,6 +41,7 @@ struct Options {
     // needs no cross-stream event waits; 13% faster per run at 65536 x 2048, K=16, J=10), or one
     // per task
     std::string streams = "shared";
+    bool capture = false;  // record the first run's launches and replay later runs as one hipGraph
     int threads = 0;               // graph runner threads (0: the runtime default)
 };
 
@@ -123,6 +125,7 @@ int main(int argc, char **argv) {
         else if (a == "--profile") o.profile = true;
         else if (a == "--streams") o.streams = next();
         else if (a == "--threads") o.threads = atoi(next());
+        else if (a == "--capture") o.capture = true;
         else { fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
     PCA_DIMS d{};
@@ -140,7 +143,7 @@ int main(int argc, char **argv) {
         fprintf(stderr, "usage: ptk_pca --kernels <pca.hsaco|hosttasks.so> [--m M] [--n N<=4096] [--k K<=64] "
                         "[--j J] [--no-center] [--blocks G] [--gr GR] [--repeat R] [--warmup W] [--seed S] "
                         "[--dist normal|uniform] [--data x.f32] [--dump file] [--profile] [--streams shared|task] "
-                        "[--threads T]\n");
+                        "[--threads T] [--capture]\n");
         return 2;
     }
     const int rb = (d.ld / 4 + 255) / 256 <= 2 ? 8 : 4;  // rows per slab of the row passes
@@ -226,6 +229,11 @@ int main(int argc, char **argv) {
     GraphOutputChannel *outR = G.g->AddOutputChannel(G.cur[BR], "R.out");
     GraphOutputChannel *outT = G.g->AddOutputChannel(G.cur[BT], "T.out");
     GraphOutputChannel *outS = G.g->AddOutputChannel(G.cur[BS], "S.out");
+    if (o.capture && !G.g->EnableLaunchCapture(TRUE)) {
+        fprintf(stderr, "launch capture: %s\n", Runtime::GetLastErrorText());
+        Runtime::Terminate();
+        return 1;
+    }
     G.g->Run();
     for (auto &c : G.dimsIn) {
         Datablock *bd = Runtime::AllocateDatablock(G.tDims, &c.second, sizeof(PCA_DIMS), nullptr);
@@ -323,6 +331,8 @@ int main(int argc, char **argv) {
     }
     bX->Release();
     const unsigned tasks = G.g->GetTaskCount();
+    const bool captureArmed = G.g->IsLaunchCaptureArmed();
+    const unsigned captureReplays = G.g->GetLaunchCaptureReplayCount();
     G.g->Stop();
     G.g->Teardown();
     Graph::DestroyGraph(G.g);
@@ -363,11 +373,13 @@ int main(int argc, char **argv) {
            "\"G\": %d, \"GR\": %d, \"tasks\": %u, \"runs\": %d, \"timed_runs\": %d, \"timed_ms\": %.4f, "
            "\"ms_per_run\": %.4f, \"runs_per_s\": %.3f, \"run_bytes\": %.0f, \"gram_ms\": %.4f, \"gram_dispatches\": %u, "
            "\"gram_bytes\": %lld, \"h2d_count_timed\": %llu, \"d2h_count_timed\": %llu, \"kernels\": [%s], "
-           "\"L\": [%s], \"T_fnv1a\": \"%016llx\", \"S_fnv1a\": \"%016llx\"}\n",
+           "\"L\": [%s], \"T_fnv1a\": \"%016llx\", \"S_fnv1a\": \"%016llx\", "
+           "\"capture\": {\"requested\": %s, \"armed\": %s, \"replays\": %u}}\n",
            ok ? "true" : "false", G.hip ? "hip" : "host", o.m, o.n, o.k, o.j, o.center, d.G, d.GR, tasks, runs,
            o.repeat, timedMs, msRun, msRun > 0 ? 1e3 / msRun : 0.0, runBytes, gramMs, gramN, rBytes, h2d, d2h,
            taskJson.c_str(), lj.c_str(), (unsigned long long)Fnv1a(res[BT].data(), res[BT].size()),
-           (unsigned long long)Fnv1a(res[BS].data(), res[BS].size()));
+           (unsigned long long)Fnv1a(res[BS].data(), res[BS].size()), o.capture ? "true" : "false",
+           captureArmed ? "true" : "false", captureReplays);
     Runtime::Terminate();
     return ok ? 0 : 1;
 }

@@ -44,3 +44,16 @@ def test_block_counts_agree_and_reruns_are_bitwise(tmp_path):
     assert b[0]["h2d_count_timed"] == 0
     assert np.allclose(a[5], c[5], rtol=1e-5)
     compare(c[1], c[2], c[3], c[4], c[5], 3, 4, True, GPU_TOL)
+
+
+def test_captured_runs_equal_per_task_dispatch(tmp_path):
+    # Graph::EnableLaunchCapture: the first run is recorded, the later ones replay its launches
+    # as one hipGraph; the last run's T, S and R must equal per-task dispatch bit for bit
+    print("case pca capture", flush=True)
+    args = ("--repeat", "3", "--warmup", "1")
+    a = run_pca(KERNELS, tmp_path, 3000, 700, 3, 4, *args)
+    b = run_pca(KERNELS, tmp_path, 3000, 700, 3, 4, *args, "--capture")
+    assert not a[0]["capture"]["armed"]
+    assert b[0]["capture"]["armed"] and b[0]["capture"]["replays"] == 3
+    assert a[0]["T_fnv1a"] == b[0]["T_fnv1a"] and a[0]["S_fnv1a"] == b[0]["S_fnv1a"]
+    assert np.array_equal(a[4], b[4])

@@ -89,3 +89,12 @@ def test_host_graph_reruns_and_data_file(tmp_path):
 def test_app_rejects_bad_shapes(args):
     p = subprocess.run([APP, "--kernels", HOSTLIB, *args], capture_output=True, text=True, timeout=60)
     assert p.returncode == 2 and "usage" in p.stderr
+
+
+def test_host_graph_ignores_launch_capture(tmp_path):
+    # capture records HIP launches only: on host tasks the recording gives up and every run is
+    # dispatched task by task, with the same results
+    js, X, T, P, R, L = run_pca(HOSTLIB, tmp_path, 120, 30, 2, 3, "--repeat", "2", "--warmup", "1", "--capture")
+    assert js["ok"] and js["capture"]["requested"] and not js["capture"]["armed"]
+    assert js["capture"]["replays"] == 0
+    compare(X, T, P, R, L, 2, 3, True, HOST_TOL)
