@@ -238,60 +238,93 @@ PcaDeflate(float *__restrict__ R, float *__restrict__ T, float *__restrict__ S, 
 }
 
 // ---- PcaReduce: red[g][n] = sum of part[b][n] over the g-th of GR contiguous groups of the G
-// rows.  Grid (ceil(ld/64), GR) blocks of 256 (64 columns x 4 interleaved row subsets).
+// rows.  Grid (ceil(ld/64), GR) blocks of 256 (64 columns x 4 interleaved row subsets).  The
+// loads read clamped, always valid rows unconditionally, eight at a time, so they issue together;
+// the sums take the valid terms in row order.
 extern "C" __global__ void __launch_bounds__(256)
 PcaReduce(float *__restrict__ S, PCA_DIMS d) {
     __shared__ double sh[4][64];
     if (!pca_ok(d) || blockDim.x != 256 || (int)gridDim.x != (d.ld + 63) / 64 || (int)gridDim.y != d.GR) return;
     const int tid = threadIdx.x, c = tid & 63, sub = tid >> 6, g = blockIdx.y, ld = d.ld;
-    const int col = blockIdx.x * 64 + c;
+    const int col = blockIdx.x * 64 + c, cc = col < ld ? col : ld - 1;
     const int lo = (int)((long long)g * d.G / d.GR), hi = (int)((long long)(g + 1) * d.G / d.GR);
     const float *part = pca_part(S, d);
     double s = 0;
-    if (col < ld)
-        for (int b = lo + sub; b < hi; b += 4) s += (double)part[(long long)b * ld + col];
-    sh[sub][c] = s;
+    for (int b0 = lo + sub; b0 < hi; b0 += 32) {
+        float x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int b = b0 + 4 * u < hi ? b0 + 4 * u : hi - 1;
+            x[u] = part[(long long)b * ld + cc];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (b0 + 4 * u < hi) s += (double)x[u];
+    }
+    sh[sub][c] = col < ld ? s : 0.0;
     __syncthreads();
     if (sub == 0 && col < ld) pca_red(S, d)[(long long)g * ld + col] = sh[0][c] + sh[1][c] + sh[2][c] + sh[3][c];
 }
 
 // ---- PcaOrthonormalize: one block of 1024.  p = sum of the GR rows of red; classical
 // Gram-Schmidt against the components before k (U = P_<k^T p, then p -= P_<k U: pca_cpu.cpp:356-372);
-// p *= 1/|p| (dnrm2 + dscal: pca_cpu.cpp:374-376); P[k] = p.
-extern "C" __global__ void __launch_bounds__(1024)
-PcaOrthonormalize(float *__restrict__ S, PCA_DIMS d) {
-    __shared__ double sW[16][PTK_PCA_MAX_K];
-    __shared__ double sU[PTK_PCA_MAX_K];
-    if (!pca_ok(d) || blockDim.x != 1024 || gridDim.x != 1) return;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, ld = d.ld, k = d.k;
+// p *= 1/|p| (dnrm2 + dscal: pca_cpu.cpp:374-376); P[k] = p.  EPT elements per thread (ld <=
+// 1024 * EPT).  Loads read clamped, always valid addresses unconditionally, in groups of eight
+// rows, so a group's loads issue together; each sum then takes its valid terms in order.
+template <int EPT>
+__device__ inline void pca_orthonormalize(float *__restrict__ S, const PCA_DIMS &d, double (&sW)[16][PTK_PCA_MAX_K],
+                                          double (&sU)[PTK_PCA_MAX_K]) {
+    constexpr int kGroup = 8;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, ld = d.ld, k = d.k, GR = d.GR;
     const double *red = pca_red(S, d);
     float *P = pca_P(S, d);
-    double p[4];
+    int nc[EPT];
+    bool nv[EPT];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < EPT; ++i) {
         const int n = tid + 1024 * i;
-        double s = 0;
-        if (n < ld)
-            for (int g = 0; g < d.GR; ++g) s += red[(long long)g * ld + n];
-        p[i] = s;
+        nv[i] = n < ld;
+        nc[i] = n < ld ? n : ld - 1;
     }
+    double p[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) p[i] = 0;
+    for (int g0 = 0; g0 < GR; g0 += kGroup) {
+        double rv[kGroup][EPT];
+#pragma unroll
+        for (int gg = 0; gg < kGroup; ++gg) {
+            const int g = g0 + gg < GR ? g0 + gg : GR - 1;
+#pragma unroll
+            for (int i = 0; i < EPT; ++i) rv[gg][i] = red[(long long)g * ld + nc[i]];
+        }
+#pragma unroll
+        for (int gg = 0; gg < kGroup; ++gg)
+            if (g0 + gg < GR)
+#pragma unroll
+                for (int i = 0; i < EPT; ++i)
+                    if (nv[i]) p[i] += rv[gg][i];
+    }
+    auto loadP = [&](float (&pv)[kGroup][EPT], int j0) {
+#pragma unroll
+        for (int jj = 0; jj < kGroup; ++jj) {
+            const int j = j0 + jj < k ? j0 + jj : k - 1;
+#pragma unroll
+            for (int i = 0; i < EPT; ++i) pv[jj][i] = P[(long long)j * ld + nc[i]];
+        }
+    };
     if (k > 0) {
-        // the k dot products eight at a time: the loads of P[j] are independent across j, and the
-        // eight wave reductions run together
-        constexpr int kGroup = 8;
+        // the k dot products eight at a time; the eight wave reductions run together
         for (int j0 = 0; j0 < k; j0 += kGroup) {
+            float pv[kGroup][EPT];
+            loadP(pv, j0);
             double v[kGroup];
 #pragma unroll
             for (int jj = 0; jj < kGroup; ++jj) {
-                const int j = j0 + jj;
                 v[jj] = 0;
-                if (j < k) {
+                if (j0 + jj < k)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int n = tid + 1024 * i;
-                        if (n < ld) v[jj] += (double)P[(long long)j * ld + n] * p[i];
-                    }
-                }
+                    for (int i = 0; i < EPT; ++i)
+                        if (nv[i]) v[jj] += (double)pv[jj][i] * p[i];
             }
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1)
@@ -309,17 +342,27 @@ PcaOrthonormalize(float *__restrict__ S, PCA_DIMS d) {
             sU[tid] = u;
         }
         __syncthreads();
+        // p -= P_<k U: each element's sum over j in order
+        double s[EPT];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int n = tid + 1024 * i;
-            if (n >= ld) continue;
-            double s = 0;
-            for (int j = 0; j < k; ++j) s += (double)P[(long long)j * ld + n] * sU[j];
-            p[i] -= s;
+        for (int i = 0; i < EPT; ++i) s[i] = 0;
+        for (int j0 = 0; j0 < k; j0 += kGroup) {
+            float pv[kGroup][EPT];
+            loadP(pv, j0);
+#pragma unroll
+            for (int jj = 0; jj < kGroup; ++jj)
+                if (j0 + jj < k)
+#pragma unroll
+                    for (int i = 0; i < EPT; ++i) s[i] += (double)pv[jj][i] * sU[j0 + jj];
         }
+#pragma unroll
+        for (int i = 0; i < EPT; ++i)
+            if (nv[i]) p[i] -= s[i];
         __syncthreads();
     }
-    double v = p[0] * p[0] + p[1] * p[1] + p[2] * p[2] + p[3] * p[3];
+    double v = 0;  // sum of squares in element order (the terms beyond ld are zero)
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) v = i == 0 ? p[0] * p[0] : v + p[i] * p[i];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane == 0) sW[wv][0] = v;
@@ -328,10 +371,17 @@ PcaOrthonormalize(float *__restrict__ S, PCA_DIMS d) {
     for (int w = 0; w < 16; ++w) nrm2 += sW[w][0];
     const double inv = 1.0 / sqrt(nrm2);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int n = tid + 1024 * i;
-        if (n < ld) P[(long long)k * ld + n] = (float)(p[i] * inv);
-    }
+    for (int i = 0; i < EPT; ++i)
+        if (nv[i]) P[(long long)k * ld + tid + 1024 * i] = (float)(p[i] * inv);
+}
+
+extern "C" __global__ void __launch_bounds__(1024)
+PcaOrthonormalize(float *__restrict__ S, PCA_DIMS d) {
+    __shared__ double sW[16][PTK_PCA_MAX_K];
+    __shared__ double sU[PTK_PCA_MAX_K];
+    if (!pca_ok(d) || blockDim.x != 1024 || gridDim.x != 1) return;
+    if (d.ld <= 2048) pca_orthonormalize<2>(S, d, sW, sU);
+    else pca_orthonormalize<4>(S, d, sW, sU);
 }
 
 // ---- PcaFinish: L_k = sqrt(sum of tn) (pca_cpu.cpp:385), T[:,k] *= 1/L_k (:386).  Grid
