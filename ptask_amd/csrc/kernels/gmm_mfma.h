@@ -48,6 +48,38 @@ struct GmmMfmaShared {
     double sLL[4];
 };
 
+// E-step products of component tile kt with point tiles IL pg .. IL pg + IL - 1 of the tile in
+// sX: IL independent MFMA chains, interleaved, sharing each B operand load; log responsibilities
+// C_k - d/2 into sR.
+template <int IL, int NC>
+__device__ __forceinline__ void gmm_mfma_estep(GmmMfmaShared &sh, const float *__restrict__ sX, float *__restrict__ sR,
+                                               int kt, int pg, int K, int l16, int l4) {
+#pragma clang fp contract(fast)
+    constexpr int Dp = 4 * NC, D2 = 2 * Dp;
+    const float *bb = sh.sB + kt * 16 + l16;
+    gmm_f4 acc[IL];
+#pragma unroll
+    for (int u = 0; u < IL; ++u) acc[u] = gmm_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < D2 / 4; ++s) {
+        const int c = 4 * s + l4, cx = c < Dp ? c : c - Dp;
+        const float bv = bb[c * MBS];
+#pragma unroll
+        for (int u = 0; u < IL; ++u) {
+            const float xv = sX[((IL * pg + u) * 16 + l16) * MXS + cx];
+            acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(c < Dp ? xv * xv : xv, bv, acc[u], 0, 0, 0);
+        }
+    }
+    const int k = kt * 16 + l16;
+    if (k < K) {
+        const float e = sh.sE[k];
+#pragma unroll
+        for (int u = 0; u < IL; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sR[((IL * pg + u) * 16 + 4 * l4 + r) * MRS + k] = fmaf(-0.5f, acc[u][r], e);
+    }
+}
+
 template <int NC>
 __device__ __forceinline__ void gmm_mfma_tiles(const float *__restrict__ X, const GMM_DIMS &d,
                                                const double *__restrict__ model, double *__restrict__ row,
@@ -119,24 +151,9 @@ __device__ __forceinline__ void gmm_mfma_tiles(const float *__restrict__ X, cons
 
         // ---- E-step: log responsibilities into sR[p][k]
         if (mfmaE) {
-            for (int tt = wv; tt < 8 * KT; tt += 4) {
-                const int pt = tt / KT, kt = tt - pt * KT;
-                const float *xa = sX + (pt * 16 + l16) * MXS;
-                const float *bb = sh.sB + kt * 16 + l16;
-                gmm_f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int s = 0; s < D2 / 4; ++s) {
-                    const int c = 4 * s + l4;
-                    const float xv = xa[c < Dp ? c : c - Dp];
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(c < Dp ? xv * xv : xv, bb[c * MBS], acc, 0, 0, 0);
-                }
-                const int k = kt * 16 + l16;
-                if (k < K) {
-                    const float e = sh.sE[k];
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) sR[(pt * 16 + 4 * l4 + r) * MRS + k] = fmaf(-0.5f, acc[r], e);
-                }
-            }
+            // job jb: component tile jb % KT against point tiles 4 (jb / KT) .. +3, four
+            // independent MFMA chains interleaved (enough to cover the product's latency)
+            for (int jb = wv; jb < 2 * KT; jb += 4) gmm_mfma_estep<4, NC>(sh, sX, sR, jb % KT, jb / KT, K, l16, l4);
         } else {
             // differences to each component mean from the point itself, as GmmAccumulate
             float x[4 * NC];
@@ -180,27 +197,43 @@ __device__ __forceinline__ void gmm_mfma_tiles(const float *__restrict__ X, cons
         if (valid && eh == 0) ll += (double)tmp;
         __syncthreads();
 
-        // ---- M-step: R^T [x' | x'^2] on the matrix cores, W on the VALU
+        // ---- M-step: R^T [x' | x'^2] on the matrix cores, W on the VALU.  The wave's four
+        // output tiles (tt = wv + 4q) are accumulated together, their products interleaved; a
+        // tile past the last (tt >= KT JT) multiplies zeros and is not stored.
+        {
+            int ki[4], jx[4];
+            bool rowok[4], jlo[4], jhi[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int tt = wv + 4 * q;
-            if (tt < KT * JT) {
-                const int kt = tt / JT, jt = tt - kt * JT;
-                const int ki = kt * 16 + l16, j = jt * 16 + l16;
-                const bool rowok = ki < K;
-                const int jx = j < Dp ? j : (j < D2 ? j - Dp : 0);
-                gmm_f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 8
-                for (int s = 0; s < TILE / 4; ++s) {
-                    const int p = 4 * s + l4;
-                    const float a = rowok ? sR[p * MRS + ki] : 0.f;
-                    const float xv = sX[p * MXS + jx];
-                    const float b = j < Dp ? xv : (j < D2 ? xv * xv : 0.f);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; ++r) dacc[q][r] += (double)acc[r];
+            for (int q = 0; q < 4; ++q) {
+                const int tt = wv + 4 * q;
+                const bool on = tt < KT * JT;
+                const int kt = on ? tt / JT : 0, jt = on ? tt - kt * JT : 0;
+                const int j = jt * 16 + l16;
+                ki[q] = kt * 16 + l16;
+                rowok[q] = on && ki[q] < K;
+                jlo[q] = on && j < Dp;
+                jhi[q] = on && j >= Dp && j < D2;
+                jx[q] = j < Dp ? j : (j < D2 ? j - Dp : 0);
             }
+            gmm_f4 acc[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = gmm_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+            for (int s = 0; s < TILE / 4; ++s) {
+                const int p = 4 * s + l4;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float a = rowok[q] ? sR[p * MRS + ki[q]] : 0.f;
+                    const float xv = sX[p * MXS + jx[q]];
+                    const float b = jlo[q] ? xv : (jhi[q] ? xv * xv : 0.f);
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[q], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (wv + 4 * q < KT * JT)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) dacc[q][r] += (double)acc[q][r];
         }
         {   // W: wave wv sums points [32 wv, 32 wv + 32) of component `lane`; the four wave sums
             // are added in wave order below, after the barrier
