@@ -605,7 +605,8 @@ int IssueBatched(ptk_batch *B, bool fdtd, bool gb, bool gmm, bool advance, bool 
         Args a, f;
         a.ptr(B->gmmAcc.dev).ptr(B->dCounts + 4).ptr(B->dGmm);
         f.ptr(B->gmmFin.dev).ptr(B->dCounts + 5).ptr(B->dGmm);
-        if (int r = B->Launch(B->fGmmAcc, grid(768, hc[4]), 1, 1, 256, a)) return r;
+        // the matrix-core accumulate holds two blocks per CU: a 512-block grid is resident at once
+        if (int r = B->Launch(B->fGmmAcc, grid(512, hc[4]), 1, 1, 256, a)) return r;
         if (int r = B->Launch(B->fGmmFin, grid(512, hc[5]), 1, 1, 256, f)) return r;
     }
     if (gb) {  // the groupby graph's seven passes, each over its work list
@@ -781,7 +782,7 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
         {&B->fGbHist, "BatchGroupbyHistogram"}, {&B->fGbBScan, "BatchGroupbyBucketScan"},
         {&B->fGbPart, "BatchGroupbyPartition"}, {&B->fGbKCount, "BatchGroupbyKeyCount"},
         {&B->fGbKScan, "BatchGroupbyKeyScan"}, {&B->fGbRes, "BatchGroupbyReshuffle"}, {&B->fGbTot, "BatchGroupbyTotals"},
-        {&B->fGmmAcc, "BatchGmmAccumulate"}, {&B->fGmmFin, "BatchGmmFinalize"},
+        {&B->fGmmAcc, "BatchGmmAccumulateMfma"}, {&B->fGmmFin, "BatchGmmFinalize"},
         {&B->fInitFdtd, "BatchInitFdtd"}, {&B->fInitGb, "BatchInitGroupby"}, {&B->fInitGmm, "BatchInitGmm"},
         {&B->fInitGmmModel, "BatchInitGmmModel"}, {&B->fSumWords, "BatchChecksumWords"},
         {&B->fSumPairs, "BatchChecksumPairs"}, {&B->fSumGroups, "BatchChecksumGroups"},
@@ -796,7 +797,7 @@ int ptk_batch_create(const char *kernel_dir, unsigned long long arena_bytes, int
                                  "GroupbyKeyScan", "GroupbyReshuffle", "GroupbyTotals"};
     for (int i = 0; i < 7; ++i)
         if ((r = Load(B->mGb, gbk[i], &B->fPGb[i]))) return fail(r);
-    if ((r = Load(B->mGmm, "GmmAccumulate", &B->fPGmmAcc)) || (r = Load(B->mGmm, "GmmFinalize", &B->fPGmmFin)) ||
+    if ((r = Load(B->mGmm, "GmmAccumulateMfma", &B->fPGmmAcc)) || (r = Load(B->mGmm, "GmmFinalize", &B->fPGmmFin)) ||
         (r = Load(B->mUtil, "PtkFillWords", &B->fFill)))
         return fail(r);
     if (ptk_hip_malloc(&B->arenaBase, arena_bytes))

@@ -6,7 +6,7 @@
 // grid it was launched with.  Because lengths live on the device, one captured hipGraph of these
 // launches stays valid while instances are admitted and retired: the host only rewrites the
 // lists between iterations.  The per-cell / per-row / per-tile arithmetic is the single-graph
-// kernels' own (fdtd_halves.h, groupby_body.h, gmm_body.h), so an instance computes bitwise what
+// kernels' own (fdtd_halves.h, groupby_body.h, gmm_body.h, gmm_mfma.h), so an instance computes bitwise what
 // its graph computes alone.
 //
 //   BatchFdtdH / BatchFdtdE   tiles {inst, col0, ncols, k0 | k1 << 16}: H then E half-step of
@@ -23,6 +23,7 @@
 //   BatchGroupbyKeyScan                       units {inst, half bucket}
 //   BatchGroupbyTotals                        units {inst, quarter bucket}
 //   BatchGmmAccumulate        units {inst, partial row b}
+//   BatchGmmAccumulateMfma    the same units on the matrix cores (gmm_mfma.h; the executor's)
 //   BatchGmmFinalize          units {inst, component k (K = log-likelihood block)}
 //   BatchInit*                device-side input generation at admission
 //   BatchChecksum*            64-bit checksums of results (sums of mixed words: the order in
@@ -32,6 +33,7 @@
 #include "fdtd_common.h"
 #include "fdtd_halves.h"
 #include "gmm_body.h"
+#include "gmm_mfma.h"
 #include "groupby_body.h"
 #include "ptk_batch.h"
 #include "ptk_prng.h"
@@ -304,6 +306,20 @@ BatchGmmAccumulate(const int2 *__restrict__ units, const int *__restrict__ count
         const int2 w = units[u];
         const PTK_GMM_INST &I = insts[w.x];
         gmm_accumulate_unit(I.X, I.d, I.model, I.partials, w.y, sh);
+    }
+}
+
+// The same units with the products on the matrix cores (gmm_mfma.h), as the executor's plain
+// path and the GMM graph launch GmmAccumulateMfma: two blocks per CU by their LDS.
+extern "C" __global__ void __launch_bounds__(256, 2)
+BatchGmmAccumulateMfma(const int2 *__restrict__ units, const int *__restrict__ count,
+                       const PTK_GMM_INST *__restrict__ insts) {
+    __shared__ __align__(16) GmmMfmaShared sh;
+    const int n = *count;
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const int2 w = units[u];
+        const PTK_GMM_INST &I = insts[w.x];
+        gmm_mfma_unit(I.X, I.d, I.model, I.partials, w.y, sh);
     }
 }
 
