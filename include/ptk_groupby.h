@@ -75,8 +75,8 @@ extern "C" {
 #define PTK_GB_PART_THREADS 512    /* threads of a GroupbyPartition block */
 #define PTK_GB_TILE_THREADS 512    /* threads of a GroupbyReshuffle block */
 #define PTK_GB_SCAN_THREADS 1024   /* threads of a GroupbyBucketScan block */
-#define PTK_GB_KEYSCAN_THREADS 512 /* threads of a GroupbyKeyScan block (half a bucket's keys) */
-#define PTK_GB_TOTALS_THREADS 256  /* threads of a GroupbyTotals block (a quarter bucket's keys) */
+#define PTK_GB_KEYSCAN_THREADS 1024 /* threads of a GroupbyKeyScan block (half a bucket's keys) */
+#define PTK_GB_TOTALS_THREADS 256 /* threads of a GroupbyTotals block (a quarter bucket's keys) */
 #define PTK_GB_MAX_BUCKETS 2048    /* bounds the per-chunk bucket counts in LDS */
 #define PTK_GB_MAX_KEYS (PTK_GB_BUCKET * PTK_GB_MAX_BUCKETS)  /* numKeys + 1 <= this */
 

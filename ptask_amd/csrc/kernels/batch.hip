@@ -34,6 +34,8 @@
 #include "fdtd_halves.h"
 #include "gmm_body.h"
 #include "gmm_mfma.h"
+// the work-list loop costs the keyscan a few registers: 32 tiles per load batch keep it unspilled
+#define GB_KS_BATCH 32
 #include "groupby_body.h"
 #include "ptk_batch.h"
 #include "ptk_prng.h"
@@ -309,6 +311,32 @@ BatchGmmAccumulate(const int2 *__restrict__ units, const int *__restrict__ count
     }
 }
 
+// One work unit, called rather than inlined: inlined into the work-list loop, values the loop
+// keeps live push the tile bodies (250 of 256 registers on their own) into spills inside the
+// tile loop; called, the only scratch traffic is the callee-saved registers at entry and exit,
+// once per unit.  The instance's dimensions and pointers are the block's: scalar registers.
+__device__ __attribute__((noinline)) void gmm_mfma_unit_call(const PTK_GMM_INST *Ip, int b, GmmMfmaShared &sh) {
+    const PTK_GMM_INST &I = *Ip;
+    GMM_DIMS d;
+    d.N = __builtin_amdgcn_readfirstlane(I.d.N);
+    d.D = __builtin_amdgcn_readfirstlane(I.d.D);
+    d.K = __builtin_amdgcn_readfirstlane(I.d.K);
+    d.blocks = __builtin_amdgcn_readfirstlane(I.d.blocks);
+    d.maxIter = __builtin_amdgcn_readfirstlane(I.d.maxIter);
+    d.pad[0] = d.pad[1] = d.pad[2] = 0;
+    // readfirstlane is 32-bit: a pointer goes through it in two halves
+    auto uni = [](const void *p) {
+        const unsigned long long v = (unsigned long long)p;
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+        return ((unsigned long long)hi << 32) | lo;
+    };
+    const float *X = (const float *)uni(I.X);
+    const double *model = (const double *)uni(I.model);
+    double *partials = (double *)uni(I.partials);
+    gmm_mfma_unit(X, d, model, partials, __builtin_amdgcn_readfirstlane(b), sh);
+}
+
 // The same units with the products on the matrix cores (gmm_mfma.h), as the executor's plain
 // path and the GMM graph launch GmmAccumulateMfma: two blocks per CU by their LDS.
 extern "C" __global__ void __launch_bounds__(256, 2)
@@ -319,7 +347,7 @@ BatchGmmAccumulateMfma(const int2 *__restrict__ units, const int *__restrict__ c
     for (int u = blockIdx.x; u < n; u += gridDim.x) {
         const int2 w = units[u];
         const PTK_GMM_INST &I = insts[w.x];
-        gmm_mfma_unit(I.X, I.d, I.model, I.partials, w.y, sh);
+        gmm_mfma_unit_call(&I, w.y, sh);
     }
 }
 

@@ -463,6 +463,9 @@ __device__ inline void gb_keycount_tile(const PTK_GB_SCRATCH &S, unsigned t, GbK
 
 #define GB_KS_KEYS (PTK_GB_BUCKET / 2)                    // keys per keyscan block (half a bucket)
 #define GB_KS_PARTS (PTK_GB_KEYSCAN_THREADS / GB_KS_KEYS)    // threads per key in keyscan
+#ifndef GB_KS_BATCH
+#define GB_KS_BATCH 40  // tiles whose counts a keyscan thread loads at once
+#endif
 #define GB_TS_KEYS (PTK_GB_BUCKET / 4)                    // keys per totals block (a quarter bucket)
 #define GB_TS_PARTS (PTK_GB_TOTALS_THREADS / GB_TS_KEYS)     // threads per key in totals
 
@@ -478,17 +481,21 @@ struct GbKeyScanLds {
 // upper half's first row is the bucket's first row plus its rows less the upper half's own.
 __device__ inline void gb_keyscan_half(const PTK_GB_SCRATCH &S, int numKeys, int b, int h, long long *offsets,
                                        GbKeyScanLds &L) {
-    const int tid = threadIdx.x, k = tid % GB_KS_KEYS, q = tid / GB_KS_KEYS, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, k = tid % GB_KS_KEYS, lane = tid & 63, w = tid >> 6;
+    // the part is the same for a whole wave (GB_KS_KEYS is a multiple of 64): its tile range is
+    // scalar, so each tile's counts load from a scalar row address plus the lane's key offset
+    const int q = __builtin_amdgcn_readfirstlane(tid / GB_KS_KEYS);
     const unsigned t0 = S.tstart[b], t1 = S.tstart[b + 1], nt = t1 - t0;
     const unsigned per = (nt + GB_KS_PARTS - 1) / GB_KS_PARTS;
     const unsigned p0 = t0 + q * per, p1 = p0 + per < t1 ? p0 + per : t1;
     const unsigned kk = (unsigned)(h * GB_KS_KEYS + k);  // key within the bucket
-    constexpr int kB = 32;  // tiles whose counts are loaded together
+    constexpr int kB = GB_KS_BATCH;  // tiles whose counts are loaded together
     unsigned v[kB];
     unsigned mine = 0;
+    auto cnt = [&](unsigned t) -> unsigned & { return S.c2[(unsigned long long)t * PTK_GB_BUCKET + kk]; };
     for (unsigned t = p0; t < p1; t += kB) {
 #pragma unroll
-        for (int i = 0; i < kB; ++i) v[i] = t + i < p1 ? S.c2[(unsigned long long)(t + i) * PTK_GB_BUCKET + kk] : 0u;
+        for (int i = 0; i < kB; ++i) v[i] = t + i < p1 ? cnt(t + i) : 0u;
 #pragma unroll
         for (int i = 0; i < kB; ++i) mine += v[i];
     }
@@ -504,16 +511,16 @@ __device__ inline void gb_keyscan_half(const PTK_GB_SCRATCH &S, int numKeys, int
     if (p1 - p0 <= (unsigned)kB && p1 > p0) {  // the counts are still in v
 #pragma unroll
         for (int i = 0; i < kB; ++i) {
-            if (p0 + i < p1) S.c2[(unsigned long long)(p0 + i) * PTK_GB_BUCKET + kk] = run;
+            if (p0 + i < p1) cnt(p0 + i) = run;
             run += v[i];
         }
     } else {
         for (unsigned t = p0; t < p1; t += kB) {
 #pragma unroll
-            for (int i = 0; i < kB; ++i) v[i] = t + i < p1 ? S.c2[(unsigned long long)(t + i) * PTK_GB_BUCKET + kk] : 0u;
+            for (int i = 0; i < kB; ++i) v[i] = t + i < p1 ? cnt(t + i) : 0u;
 #pragma unroll
             for (int i = 0; i < kB; ++i) {
-                if (t + i < p1) S.c2[(unsigned long long)(t + i) * PTK_GB_BUCKET + kk] = run;
+                if (t + i < p1) cnt(t + i) = run;
                 run += v[i];
             }
         }
@@ -702,7 +709,8 @@ struct GbTotalsLds {
 __device__ inline void gb_totals_quarter(const PTK_GB_SCRATCH &S, int numKeys, int b, int r, long long *table,
                                          long long *groups, GbTotalsLds &L) {
     const int NB = (int)S.buckets;
-    const int tid = threadIdx.x, k = tid % GB_TS_KEYS, q = tid / GB_TS_KEYS;
+    const int tid = threadIdx.x, k = tid % GB_TS_KEYS;
+    const int q = __builtin_amdgcn_readfirstlane(tid / GB_TS_KEYS);  // one part per wave
     const unsigned t0 = S.tstart[b], t1 = S.tstart[b + 1], nt = t1 - t0;
     const unsigned per = (nt + GB_TS_PARTS - 1) / GB_TS_PARTS;
     const unsigned p0 = t0 + q * per, p1 = p0 + per < t1 ? p0 + per : t1;

@@ -10,7 +10,7 @@ COUNT_THREADS = 256   # PTK_GB_COUNT_THREADS
 PART_THREADS = 512    # PTK_GB_PART_THREADS
 TILE_THREADS = 512    # PTK_GB_TILE_THREADS
 SCAN_THREADS = 1024   # PTK_GB_SCAN_THREADS
-KEYSCAN_THREADS = 512   # PTK_GB_KEYSCAN_THREADS
+KEYSCAN_THREADS = 1024  # PTK_GB_KEYSCAN_THREADS
 TOTALS_THREADS = 256    # PTK_GB_TOTALS_THREADS
 MAX_BUCKETS = 2048    # PTK_GB_MAX_BUCKETS
 MAX_KEYS = BUCKET * MAX_BUCKETS   # numKeys + 1 <= MAX_KEYS
