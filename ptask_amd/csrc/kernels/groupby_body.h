@@ -468,6 +468,7 @@ __device__ inline void gb_keycount_tile(const PTK_GB_SCRATCH &S, unsigned t, GbK
 #endif
 #define GB_TS_KEYS (PTK_GB_BUCKET / 4)                    // keys per totals block (a quarter bucket)
 #define GB_TS_PARTS (PTK_GB_TOTALS_THREADS / GB_TS_KEYS)     // threads per key in totals
+static_assert(GB_KS_KEYS % 64 == 0 && GB_TS_KEYS % 64 == 0, "a keyscan or totals part is whole waves");
 
 struct GbKeyScanLds {
     unsigned part[GB_KS_PARTS][GB_KS_KEYS];
