@@ -148,7 +148,11 @@ int main(int argc, char **argv) {
     }
     const int rb = (d.ld / 4 + 255) / 256 <= 2 ? 8 : 4;  // rows per slab of the row passes
     const int slabs = d.Mp / rb;
-    d.G = o.blocks > 0 ? o.blocks : std::min(slabs, 512);
+    // Row-pass blocks: one and a half per compute unit of the device (set once the runtime
+    // knows it; 384 on the host).  Measured at 65536 x 2048, K=16, J=10, captured runs on 256
+    // CUs: 256 blocks 27.6 ms, 320-384 26.3-26.7, 448 27.0, 512 28.0, 1024 29.5 per run.
+    auto rowBlocks = [&](int cus) { return o.blocks > 0 ? o.blocks : std::min(slabs, cus > 0 ? 3 * cus / 2 : 384); };
+    d.G = rowBlocks(0);
     d.GR = o.gr > 0 ? std::min(o.gr, d.G) : std::min(d.G, 8);  // 8: measured best at 65536 x 2048
 
     std::vector<float> X((size_t)o.m * o.n);
@@ -178,6 +182,11 @@ int main(int argc, char **argv) {
     PcaGraph G;
     G.kernels = o.kernelFile;
     G.hip = ptaskutils::SelectAcceleratorClass(o.kernelFile.c_str()) == ACCELERATOR_CLASS_HIP;
+    if (G.hip) {
+        Accelerator *acc = Runtime::GetAccelerator(ACCELERATOR_CLASS_HIP, 0);
+        d.G = rowBlocks(acc ? (int)acc->GetCoreCount() : 0);
+        d.GR = o.gr > 0 ? std::min(o.gr, d.G) : std::min(d.G, 8);
+    }
     G.g = new Graph("pca");
     DatablockTemplate *tX = Runtime::GetDatablockTemplate((char *)"pcaX", sizeof(float), (UINT)o.m, (UINT)o.n, 1);
     G.tmpl[BR] = Runtime::GetDatablockTemplate((char *)"pcaR", sizeof(float) * d.ld, (UINT)d.Mp, 1, 1);
