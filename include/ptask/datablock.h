@@ -78,8 +78,9 @@ public:
     size_t GetFootprintBytes();
     /* A new block holding a copy of this one (contents, record count, metadata, control
      * signals), for a consumer that writes its input in place while other consumers still read
-     * this block.  The copy is made where the current data lives: device to device on that
-     * accelerator's transfer stream (ordered after the block's pending writes), else on the host.
+     * this block.  The copy is made where the current data lives, a device first: device to
+     * device on that accelerator's transfer stream (ordered after the block's pending writes),
+     * else on the host.
      * NULL when the storage could not be allocated. */
     Datablock *CloneForWriter();
 

@@ -323,33 +323,34 @@ Datablock *Datablock::CloneForWriter() {
     c->m_bSealed = m_bSealed;
     c->m_eFlags = m_eFlags;
     c->m_vMeta = m_vMeta;
-    BOOL ok = TRUE;
+    // a valid device copy first (the copy stays on the device, where a writing task of the
+    // device class reads it), else the host copy
+    BOOL ok = TRUE, copied = FALSE;
+    for (UINT ms = 1; ms < PTK_MAX_MEMORY_SPACES && !copied; ++ms) {
+        if ((m_eState[ms] != BSTATE_SHARED && m_eState[ms] != BSTATE_EXCLUSIVE) || !m_pBuffers[ms]) continue;
+        Accelerator *acc = AcceleratorForMemorySpace(ms);
+        AsyncContext *ctx = acc ? acc->GetTransferContext() : nullptr;
+        PBuffer *dst = acc && ctx ? AllocDeviceStorage(acc, m_cbAllocated ? m_cbAllocated : 4) : nullptr;
+        copied = TRUE;
+        if (!dst) {
+            ok = FALSE;
+            break;
+        }
+        acc->MakeDeviceContextCurrent();
+        m_pBuffers[ms]->WaitOutstanding(ctx, FALSE);  // after the block's last write
+        if (m_cbLogical) ok = acc->CopyDeviceToDevice(dst, m_pBuffers[ms], m_cbLogical, ctx);
+        SyncPoint *sp = ctx->CreateSyncPoint();
+        dst->AddOutstanding(sp, TRUE);
+        m_pBuffers[ms]->AddOutstanding(sp, FALSE);  // a later writer of this block waits for the copy
+        sp->Release();
+        c->m_pBuffers[ms] = dst;
+        c->m_eState[ms] = BSTATE_EXCLUSIVE;
+    }
     const BUFFER_COHERENCE_STATE hs = m_eState[HOST_MEMORY_SPACE_ID];
-    if ((hs == BSTATE_SHARED || hs == BSTATE_EXCLUSIVE) && m_pHost) {
+    if (!copied && (hs == BSTATE_SHARED || hs == BSTATE_EXCLUSIVE) && m_pHost) {
         ok = c->EnsureHostBuffer();
         if (ok && m_cbLogical) memcpy(c->m_pHost, m_pHost, m_cbLogical);
         c->m_eState[HOST_MEMORY_SPACE_ID] = BSTATE_EXCLUSIVE;
-    } else {
-        for (UINT ms = 1; ms < PTK_MAX_MEMORY_SPACES; ++ms) {
-            if ((m_eState[ms] != BSTATE_SHARED && m_eState[ms] != BSTATE_EXCLUSIVE) || !m_pBuffers[ms]) continue;
-            Accelerator *acc = AcceleratorForMemorySpace(ms);
-            AsyncContext *ctx = acc ? acc->GetTransferContext() : nullptr;
-            PBuffer *dst = acc && ctx ? AllocDeviceStorage(acc, m_cbAllocated ? m_cbAllocated : 4) : nullptr;
-            if (!dst) {
-                ok = FALSE;
-                break;
-            }
-            acc->MakeDeviceContextCurrent();
-            m_pBuffers[ms]->WaitOutstanding(ctx, FALSE);  // after the block's last write
-            if (m_cbLogical) ok = acc->CopyDeviceToDevice(dst, m_pBuffers[ms], m_cbLogical, ctx);
-            SyncPoint *sp = ctx->CreateSyncPoint();
-            dst->AddOutstanding(sp, TRUE);
-            m_pBuffers[ms]->AddOutstanding(sp, FALSE);  // a later writer of this block waits for the copy
-            sp->Release();
-            c->m_pBuffers[ms] = dst;
-            c->m_eState[ms] = BSTATE_EXCLUSIVE;
-            break;
-        }
     }
     Unlock();
     if (!ok) {
