@@ -88,6 +88,11 @@ __device__ __forceinline__ void gmm_mfma_tiles(const float *__restrict__ X, cons
     float *sX = sh.u.t.sX, *sR = sh.u.t.sR;
     const int D = d.D, K = d.K, Dp = 4 * NC, D2 = 2 * Dp, Kp = (K + 15) & ~15;
     const int KT = Kp >> 4, JT = (D2 + 15) >> 4;
+    // M-step on 32 x 32 x 2 products when the moments fill four 32 x 32 tiles (K and 2 Dp in
+    // (32, 64]): one tile per wave, half the operand reads per flop of the 16 x 16 x 4 form
+    const int JT2 = (D2 + 31) >> 5;
+    const bool m32 = ((K + 31) >> 5) * JT2 == 4;
+    typedef float gmm_f16 __attribute__((ext_vector_type(16)));
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int l16 = lane & 15, l4 = lane >> 4;
@@ -197,10 +202,32 @@ __device__ __forceinline__ void gmm_mfma_tiles(const float *__restrict__ X, cons
         if (valid && eh == 0) ll += (double)tmp;
         __syncthreads();
 
-        // ---- M-step: R^T [x' | x'^2] on the matrix cores, W on the VALU.  The wave's four
-        // output tiles (tt = wv + 4q) are accumulated together, their products interleaved; a
-        // tile past the last (tt >= KT JT) multiplies zeros and is not stored.
-        {
+        // ---- M-step: R^T [x' | x'^2] on the matrix cores, W on the VALU.
+        if (m32) {
+            // wave wv: components 32 (wv / JT2) .., features 32 (wv % JT2) ..; lane l supplies
+            // A[l % 32][l / 32] = R[p][k] and B[l / 32][l % 32] = [x' | x'^2][p][j] for the two
+            // points p = 2s + l / 32 of step s; D[8(r/4) + 4(l/32) + r%4][l % 32] in acc[r]
+            const int kt2 = wv / JT2, jt2 = wv - kt2 * JT2, l32 = lane & 31, h = lane >> 5;
+            const int ki = kt2 * 32 + l32, j = jt2 * 32 + l32;
+            const bool rowok = ki < K, jlo = j < Dp, jhi = j >= Dp && j < D2;
+            const int jx = j < Dp ? j : (j < D2 ? j - Dp : 0);
+            gmm_f16 acc = {};
+#pragma unroll 8
+            for (int s = 0; s < TILE / 2; ++s) {
+                const int p = 2 * s + h;
+                const float a = rowok ? sR[p * MRS + ki] : 0.f;
+                const float xv = sX[p * MXS + jx];
+                const float b = jlo ? xv : (jhi ? xv * xv : 0.f);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) dacc[q][r] += (double)acc[4 * q + r];
+        } else {
+            // the wave's four 16 x 16 output tiles (tt = wv + 4q) accumulated together, their
+            // products interleaved; a tile past the last (tt >= KT JT) multiplies zeros and is
+            // not stored
             int ki[4], jx[4];
             bool rowok[4], jlo[4], jhi[4];
 #pragma unroll
@@ -250,15 +277,26 @@ __device__ __forceinline__ void gmm_mfma_tiles(const float *__restrict__ X, cons
 
     // ---- the block's moments about the table centres, one partial row
     double *sAcc = sh.u.sAcc;
+    if (m32) {
+        const int kt2 = wv / JT2, jt2 = wv - kt2 * JT2, j = jt2 * 32 + (lane & 31);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int tt = wv + 4 * q;
-        if (tt < KT * JT) {
-            const int kt = tt / JT, jt = tt - kt * JT, j = jt * 16 + l16;
+        for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int k = kt * 16 + 4 * l4 + r;
+                const int k = kt2 * 32 + 8 * q + 4 * (lane >> 5) + r;
                 if (k < K && j < D2) sAcc[k * D2 + j] = dacc[q][r];
+            }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int tt = wv + 4 * q;
+            if (tt < KT * JT) {
+                const int kt = tt / JT, jt = tt - kt * JT, j = jt * 16 + l16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int k = kt * 16 + 4 * l4 + r;
+                    if (k < K && j < D2) sAcc[k * D2 + j] = dacc[q][r];
+                }
             }
         }
     }
