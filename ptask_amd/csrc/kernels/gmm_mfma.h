@@ -1,4 +1,5 @@
-// gmm_mfma.h -- GmmAccumulate with its dense parts on the matrix cores (v_mfma_f32_16x16x4f32).
+// gmm_mfma.h -- GmmAccumulate with its dense parts on the matrix cores (v_mfma_f32_16x16x4f32,
+// and v_mfma_f32_32x32x2f32 for the M-step when its moments fill four 32 x 32 tiles).
 //
 //   GmmAccumulateMfma(const float* X, GMM_DIMS d, const double* model, double* partials)
 //     Same arguments, partial-row format and tile-to-row assignment as GmmAccumulate
