@@ -301,7 +301,20 @@ __device__ inline void gmm_finalize_block(double *__restrict__ model, const doub
             const long long col = c == 0   ? (long long)k
                                   : c <= D ? K + (long long)k * D + (c - 1)
                                            : K + (long long)K * D + (long long)k * D + (c - 1 - D);
-            for (int b = g; b < P; b += G) acc += partials[(long long)b * S + col];
+            // rows g, g + G, ... in order; sixteen rows' loads issued together (a row past the
+            // last reads row g and is not added)
+            const double *src = partials + col;
+            for (int b0 = g; b0 < P; b0 += 16 * G) {
+                double v[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int b = b0 + u * G;
+                    v[u] = src[(long long)(b < P ? b : g) * S];
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (b0 + u * G < P) acc += v[u];
+            }
         }
         sTot[tid] = acc;
         __syncthreads();
