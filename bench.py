@@ -349,7 +349,7 @@ def leg_gmm() -> dict:
 
 
 def leg_pca() -> dict:
-    """The unrolled PCA graph (386 tasks per run) dispatched task by task and with launch capture,
+    """The unrolled PCA graph (498 tasks per run) dispatched task by task and with launch capture,
     each timed without per-task profiling; a third, profiled run gives PcaGram's kernel time."""
     from ptask_amd import paths
     base = ["--kernels", paths.kernel_path("pca.hsaco"), "--m", 65536, "--n", 2048, "--k", 16, "--j", 10,

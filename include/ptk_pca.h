@@ -15,14 +15,16 @@
  *
  * Task kernels (PCA_DIMS by value; k selects the component a task works on):
  *   PcaLoad(const float* X, PCA_DIMS d, float* R, float* T, float* S)
- *   PcaStart(float* R, float* T, float* S, PCA_DIMS d)      deflate by component k-1, R^T R[:,k]
+ *   PcaStart(float* R, float* T, float* S, PCA_DIMS d)      for k > 0: t_{k-1} = R p_{k-1} into
+ *                                                           T[:,k-1] and the |t|^2 partials,
+ *                                                           R -= t p_{k-1}^T; then R^T R[:,k]
  *   PcaGram(float* R, float* S, PCA_DIMS d)                 R^T (R p_k)
- *   PcaProject(float* R, float* T, float* S, PCA_DIMS d)    t = R p_k, partial |t|^2
- *   PcaDeflate(float* R, float* T, float* S, PCA_DIMS d)    R -= L_k t_k p_k^T (after the last k)
+ *   PcaDeflate(float* R, float* T, float* S, PCA_DIMS d)    t_k = R p_k into T[:,k] and the |t|^2
+ *                                                           partials, R -= t_k p_k^T (the last k)
  *   PcaReduce(float* S, PCA_DIMS d)                         partial sums, first level
  *   PcaOrthonormalize(float* S, PCA_DIMS d)                 p_k = GS(p), p_k /= |p_k|
- *   PcaFinish(float* T, float* S, PCA_DIMS d)               L_k = |t|, T[:,k] = t / L_k
- *   PcaScaleT(float* T, float* S, PCA_DIMS d)               T[:,k] *= L_k for every k
+ *   PcaFinish(float* S, PCA_DIMS d)                         L_k = |t_k| from the partials
+ * T[:,k] = t_k = R p_k is the reference's final L_k (t / L_k), formed without the two roundings.
  */
 #ifndef PTK_PCA_H
 #define PTK_PCA_H

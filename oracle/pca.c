@@ -131,22 +131,33 @@ static void pca_host_coldot(const float *R, const PCA_DIMS *d, const double *t, 
     }
 }
 
+/* t = R p_c into T[:, c] (binary32), |t|^2 into tn[0] (the other G-1 partials zero), then
+ * R -= t p_c^T: the reference's t = R p, L = |t|, t /= L, R -= L t p^T (pca_cpu.cpp:380-401) with
+ * L t formed as t itself */
+static void pca_host_score_deflate(float *R, float *T, float *S, const PCA_DIMS *d, int c) {
+    const float *q = PCA_P(S, d) + (size_t)c * d->ld;
+    double *t = (double *)malloc(sizeof(double) * (size_t)d->Mp);
+    pca_host_rowdot(R, d, q, t);
+    double *tn = PCA_TN(S, d);
+    memset(tn, 0, sizeof(double) * (size_t)d->G);
+    for (int m = 0; m < d->Mp; ++m) {
+        const float tm = (float)t[m];
+        T[(size_t)c * d->Mp + m] = tm;
+        tn[0] += (double)tm * (double)tm;
+        for (int col = 0; col < d->ld; ++col) {
+            float *r = R + (size_t)m * d->ld + col;
+            *r = (float)((double)*r - (double)tm * q[col]);
+        }
+    }
+    free(t);
+}
+
 void PcaStart(unsigned int n, void **a) {
     (void)n;
     float *R = (float *)a[0], *T = (float *)a[1], *S = (float *)a[2];
     const PCA_DIMS *d = (const PCA_DIMS *)a[3];
     const int k = d->k;
-    if (k > 0) {
-        const double Lp = PCA_L(S, d)[k - 1];
-        const float *q = PCA_P(S, d) + (size_t)(k - 1) * d->ld, *tp = T + (size_t)(k - 1) * d->Mp;
-        for (int m = 0; m < d->Mp; ++m) {
-            const double am = Lp * (double)tp[m];
-            for (int c = 0; c < d->ld; ++c) {
-                float *r = R + (size_t)m * d->ld + c;
-                *r = (float)((double)*r - am * q[c]);
-            }
-        }
-    }
+    if (k > 0) pca_host_score_deflate(R, T, S, d, k - 1);
     double *t = (double *)malloc(sizeof(double) * (size_t)d->Mp);
     for (int m = 0; m < d->Mp; ++m) t[m] = R[(size_t)m * d->ld + k];
     pca_host_coldot(R, d, t, S);
@@ -155,17 +166,8 @@ void PcaStart(unsigned int n, void **a) {
 
 void PcaDeflate(unsigned int n, void **a) {
     (void)n;
-    float *R = (float *)a[0], *T = (float *)a[1], *S = (float *)a[2];
     const PCA_DIMS *d = (const PCA_DIMS *)a[3];
-    const double Lk = PCA_L(S, d)[d->k];
-    const float *q = PCA_P(S, d) + (size_t)d->k * d->ld, *tk = T + (size_t)d->k * d->Mp;
-    for (int m = 0; m < d->Mp; ++m) {
-        const double am = Lk * (double)tk[m];
-        for (int c = 0; c < d->ld; ++c) {
-            float *r = R + (size_t)m * d->ld + c;
-            *r = (float)((double)*r - am * q[c]);
-        }
-    }
+    pca_host_score_deflate((float *)a[0], (float *)a[1], (float *)a[2], d, d->k);
 }
 
 void PcaGram(unsigned int n, void **a) {
@@ -175,21 +177,6 @@ void PcaGram(unsigned int n, void **a) {
     double *t = (double *)malloc(sizeof(double) * (size_t)d->Mp);
     pca_host_rowdot(R, d, PCA_P(S, d) + (size_t)d->k * d->ld, t);
     pca_host_coldot(R, d, t, S);
-    free(t);
-}
-
-void PcaProject(unsigned int n, void **a) {
-    (void)n;
-    float *R = (float *)a[0], *T = (float *)a[1], *S = (float *)a[2];
-    const PCA_DIMS *d = (const PCA_DIMS *)a[3];
-    double *t = (double *)malloc(sizeof(double) * (size_t)d->Mp);
-    pca_host_rowdot(R, d, PCA_P(S, d) + (size_t)d->k * d->ld, t);
-    double *tn = PCA_TN(S, d);
-    memset(tn, 0, sizeof(double) * (size_t)d->G);
-    for (int m = 0; m < d->Mp; ++m) {
-        T[(size_t)d->k * d->Mp + m] = (float)t[m];
-        tn[0] += (double)(float)t[m] * (double)(float)t[m];
-    }
     free(t);
 }
 
@@ -237,25 +224,11 @@ void PcaOrthonormalize(unsigned int n, void **a) {
 
 void PcaFinish(unsigned int n, void **a) {
     (void)n;
-    float *T = (float *)a[0], *S = (float *)a[1];
-    const PCA_DIMS *d = (const PCA_DIMS *)a[2];
+    float *S = (float *)a[0];
+    const PCA_DIMS *d = (const PCA_DIMS *)a[1];
     double s = 0;
     for (int b = 0; b < d->G; ++b) s += PCA_TN(S, d)[b];
-    const double Lk = sqrt(s);
-    float *t = T + (size_t)d->k * d->Mp;
-    for (int m = 0; m < d->Mp; ++m) t[m] = (float)((double)t[m] * (1.0 / Lk));
-    PCA_L(S, d)[d->k] = Lk;
-}
-
-void PcaScaleT(unsigned int n, void **a) {
-    (void)n;
-    float *T = (float *)a[0], *S = (float *)a[1];
-    const PCA_DIMS *d = (const PCA_DIMS *)a[2];
-    for (int k = 0; k < d->K; ++k)
-        for (int m = 0; m < d->Mp; ++m) {
-            float *t = T + (size_t)k * d->Mp + m;
-            *t = (float)((double)*t * PCA_L(S, d)[k]);
-        }
+    PCA_L(S, d)[d->k] = sqrt(s);
 }
 
 /* ---- host-task forms of the reference PCA graph's BLAS tasks (ptaskUnitTest/graphcupca.cpp:294-299)

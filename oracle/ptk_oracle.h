@@ -90,12 +90,10 @@ void GmmFinalize(unsigned int n, void **a);
 void PcaLoad(unsigned int n, void **a);
 void PcaStart(unsigned int n, void **a);
 void PcaGram(unsigned int n, void **a);
-void PcaProject(unsigned int n, void **a);
 void PcaDeflate(unsigned int n, void **a);
 void PcaReduce(unsigned int n, void **a);
 void PcaOrthonormalize(unsigned int n, void **a);
 void PcaFinish(unsigned int n, void **a);
-void PcaScaleT(unsigned int n, void **a);
 
 #ifdef __cplusplus
 }

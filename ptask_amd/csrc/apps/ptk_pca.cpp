@@ -3,9 +3,10 @@
 //
 // The reference unrolls one task per BLAS call (graphcupca.cpp:317-1540).  This graph unrolls the
 // fused task kernels of pca.hsaco: for each component k
-//   PcaStart -> J x (PcaReduce -> PcaOrthonormalize [-> PcaGram unless last]) -> PcaProject ->
-//   PcaFinish
-// between PcaLoad and PcaDeflate (the last component's deflation) + PcaScaleT.  The residual R, the scores T and the state S (loads, singular
+//   PcaStart [-> PcaFinish of k-1] -> J x (PcaReduce -> PcaOrthonormalize [-> PcaGram unless last])
+// between PcaLoad and PcaDeflate + PcaFinish of the last component.  PcaStart of k forms component
+// k-1's scores t = R p in the read of R that deflates it, so no separate projection pass reads R.
+// The residual R, the scores T and the state S (loads, singular
 // values, partial sums) are datablocks chained through in/out port pairs, so every task runs on
 // the device-resident copy; each task has its own sticky PCA_DIMS carrying its component index.
 // X (column-major M x N binary32) is N(0,1) or U[0,1) from the shared generator, or --data.
@@ -213,24 +214,23 @@ int main(int argc, char **argv) {
     for (int b = 0; b < 3; ++b) G.cur[b] = static_cast<OutputPort *>(lOut[b]);
 
     const PTASKDIM3 b256(256, 1, 1), b1024(1024, 1, 1);
-    const PTASKDIM3 gRows(d.G, 1, 1), gRed((d.ld + 63) / 64, d.GR, 1), gOne(1, 1, 1), gT((d.Mp + 255) / 256, 1, 1);
+    const PTASKDIM3 gRows(d.G, 1, 1), gRed((d.ld + 63) / 64, d.GR, 1), gOne(1, 1, 1);
     bool built = true;
     for (int k = 0; k < o.k && built; ++k) {
-        PCA_DIMS dk = d;
+        PCA_DIMS dk = d, dp = d;
         dk.k = k;
+        dp.k = k - 1;
         built = G.Step("PcaStart", {BR, BT, BS}, dk, gRows, b256);
+        if (built && k > 0) built = G.Step("PcaFinish", {BS}, dp, gOne, b256);
         for (int j = 1; j <= o.j && built; ++j) {
             built = G.Step("PcaReduce", {BS}, dk, gRed, b256) && G.Step("PcaOrthonormalize", {BS}, dk, gOne, b1024);
             if (built && j < o.j) built = G.Step("PcaGram", {BR, BS}, dk, gRows, b256);
         }
-        built = built && G.Step("PcaProject", {BR, BT, BS}, dk, gRows, b256) &&
-                G.Step("PcaFinish", {BT, BS}, dk, gT, b256);
     }
     PCA_DIMS dl = d;
     dl.k = o.k - 1;
-    built = built && G.Step("PcaDeflate", {BR, BT, BS}, dl, gRows, b256);
-    const int scaleGrid = (int)std::min<long long>(((long long)d.K * d.Mp + 255) / 256, 4096);
-    built = built && G.Step("PcaScaleT", {BT, BS}, dl, PTASKDIM3(scaleGrid, 1, 1), b256);
+    built = built && G.Step("PcaDeflate", {BR, BT, BS}, dl, gRows, b256) &&
+            G.Step("PcaFinish", {BS}, dl, gOne, b256);
     if (!built) {
         Runtime::Terminate();
         return 1;
@@ -367,10 +367,10 @@ int main(int argc, char **argv) {
         }
     }
     // algorithmic HBM bytes of one run: PcaLoad reads X (twice when centring) and writes R; per
-    // component PcaStart reads R (and writes it back when deflating), J-1 PcaGram and one
-    // PcaProject read it; PcaDeflate reads and writes it once.
+    // component PcaStart reads R (and writes it back when deflating) and J-1 PcaGram read it;
+    // PcaDeflate reads and writes it once.
     const double xB = 4.0 * o.m * o.n, rB = (double)rBytes;
-    const double runBytes = xB * (o.center ? 2 : 1) + rB + (double)o.k * o.j * rB + (o.k - 1) * 2 * rB + rB + 2 * rB;
+    const double runBytes = xB * (o.center ? 2 : 1) + rB + (double)o.k * o.j * rB + (o.k - 1) * rB + 2 * rB;
     const double msRun = o.repeat ? timedMs / o.repeat : 0;
     std::string lj;
     for (int k = 0; k < o.k; ++k) {

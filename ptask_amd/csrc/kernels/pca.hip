@@ -6,16 +6,19 @@
 // (dgemv R^T t, dgemv for the Gram-Schmidt step, dnrm2, dscal, dgemv R p, dnrm2, dscal) and a
 // rank-1 dger per component.  Here the matrix R lives row-major in HBM and every pass over it
 // does as much as one read allows:
-//   PcaStart    deflation by component k-1 (R -= L t p^T, written back) fused with p = R^T R[:,k]
+//   PcaStart    t_{k-1} = R p_{k-1} and deflation by component k-1 (R -= t p^T, written back),
+//               fused with p = R^T R[:,k]: the last inner iteration's t = R p of component k-1
+//               is formed in the same read of R that deflates it
 //   PcaGram     p = R^T (R p) in one read of R: each 8-row slab's t = R_slab p is reduced inside
 //               the block and immediately used for R_slab^T t (the reference's t = R p, |t|,
 //               t /= |t|, p = R^T t -- the scaling by 1/|t| is undone by the next normalisation)
-//   PcaProject  t = R p and |t|^2 partials (the last inner iteration)
-//   PcaDeflate  R -= L t p^T by the last component (the residual the reference returns)
-// The small steps are single kernels: PcaReduce + PcaOrthonormalize (sum the per-block p
-// partials in a fixed order, p -= P_<k (P_<k^T p), p /= |p|), PcaFinish (L_k = |t|,
-// T[:,k] = t / L_k) and PcaScaleT (T[:,k] *= L_k).  All reductions are fixed-order, so results
-// are bitwise reproducible.  Dot products and norms of length-N and length-M vectors are binary64.
+//   PcaDeflate  t_K-1 = R p and R -= t p^T by the last component (the residual the reference returns)
+// T[:,k] holds t = R p_k, which is the reference's final L_k (t / L_k) (pca_cpu.cpp:385-386 and
+// 404-408) without the two rounding steps.  The small steps are single kernels: PcaReduce +
+// PcaOrthonormalize (sum the per-block p partials in a fixed order, p -= P_<k (P_<k^T p),
+// p /= |p|) and PcaFinish (L_k = |t| from the per-block |t|^2 partials).  All reductions are
+// fixed-order, so results are bitwise reproducible.  Dot products and norms of length-N and
+// length-M vectors are binary64.
 #include <hip/hip_runtime.h>
 
 #include "ptk_pca.h"
@@ -23,7 +26,7 @@
 // Tolerance-checked binary32 work: let the compiler fuse multiply-adds (the build default is off).
 #pragma clang fp contract(fast)
 
-enum { PCA_START = 0, PCA_GRAM = 1, PCA_PROJECT = 2, PCA_DEFLATE = 3 };
+enum { PCA_START = 0, PCA_GRAM = 1, PCA_DEFLATE = 3 };
 
 __device__ inline bool pca_ok(const PCA_DIMS &d) {
     return d.M >= 1 && d.N >= 1 && d.N <= PTK_PCA_MAX_N && d.K >= 1 && d.K <= PTK_PCA_MAX_K &&
@@ -84,30 +87,38 @@ PcaLoad(const float *__restrict__ X, PCA_DIMS d, float *__restrict__ R, float *_
 // ---- the row passes.  Block of 256; thread t owns float4 column chunks t + 256 c (c < NCH); a
 // slab of RB rows is held in registers, its RB row dot products are reduced across the block
 // (wave shuffles, then a double-buffered LDS exchange, one barrier per slab).
+//
+// Deflation by component dk (PcaStart of component k deflates k - 1; PcaDeflate deflates the
+// last): the rows' scores t = R p_dk are the block-reduced dot products of the rows about to be
+// deflated (pca_cpu.cpp:380-401 computes t = R p, L = |t|, t /= L and R -= L t p^T; L t is t
+// before the scaling, so no separate pass computes it); t goes to T[:, dk] and its squares to
+// the block's tn partial (PcaFinish takes L_dk = |t| from them); then R -= t p_dk^T.
 template <int NCH, int RB, int MODE>
 __device__ __forceinline__ void pca_row_pass(float *__restrict__ R, float *__restrict__ T,
                                              float *__restrict__ S, const PCA_DIMS &d) {
     __shared__ float sRed[2][4][RB];
+    __shared__ float sCol[2][RB];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int ld = d.ld, ld4 = ld >> 2, k = d.k;
     const long long Mp = d.Mp;
     float *P = pca_P(S, d);
     int cix[NCH];
     bool cv[NCH];
-    float4 pv[NCH], qv[NCH], acc[NCH];
-    // deflation by component dk: the previous one when starting component k, k itself at the end
+    float4 wv4[NCH], acc[NCH];  // the vector the rows are dotted with: p_k (Gram) or p_dk (deflation)
     const bool deflate = (MODE == PCA_START && k > 0) || MODE == PCA_DEFLATE;
+    const bool rowdot = MODE == PCA_GRAM || deflate;
     const int dk = MODE == PCA_START ? k - 1 : k;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         cix[c] = tid + 256 * c;
         cv[c] = cix[c] < ld4;
         const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-        pv[c] = ((MODE == PCA_GRAM || MODE == PCA_PROJECT) && cv[c]) ? ((const float4 *)(P + (long long)k * ld))[cix[c]] : z;
-        qv[c] = (deflate && cv[c]) ? ((const float4 *)(P + (long long)dk * ld))[cix[c]] : z;
+        wv4[c] = (rowdot && cv[c]) ? ((const float4 *)(P + (long long)(MODE == PCA_GRAM ? k : dk) * ld))[cix[c]] : z;
         acc[c] = z;
     }
-    const double Lprev = deflate ? pca_L(S, d)[dk] : 0.0;
+    // p_dk at column k: every thread forms column k of the deflated rows from the exchanged
+    // pre-deflation values, as the column's owner deflates it (the same fused multiply-add)
+    const float pk = (MODE == PCA_START && deflate) ? P[(long long)dk * ld + k] : 0.f;
     const int ck = k >> 2, ce = k & 3, cowner = ck & 255, creg = ck >> 8;
     const long long nslab = Mp / RB;
     int buf = 0;
@@ -120,46 +131,13 @@ __device__ __forceinline__ void pca_row_pass(float *__restrict__ R, float *__res
 #pragma unroll
             for (int c = 0; c < NCH; ++c)
                 x[i][c] = cv[c] ? ((const float4 *)(R + (r0 + i) * ld))[cix[c]] : make_float4(0.f, 0.f, 0.f, 0.f);
-        if (deflate) {
-            // R -= (L_dk t_dk) p_dk^T  (dger, pca_cpu.cpp:401), t_dk normalised
-#pragma unroll
-            for (int i = 0; i < RB; ++i) {
-                const float a = (float)(Lprev * (double)T[(long long)dk * Mp + r0 + i]);
-#pragma unroll
-                for (int c = 0; c < NCH; ++c) {
-                    if (!cv[c]) continue;
-                    x[i][c].x = fmaf(-a, qv[c].x, x[i][c].x);
-                    x[i][c].y = fmaf(-a, qv[c].y, x[i][c].y);
-                    x[i][c].z = fmaf(-a, qv[c].z, x[i][c].z);
-                    x[i][c].w = fmaf(-a, qv[c].w, x[i][c].w);
-                    ((float4 *)(R + (r0 + i) * ld))[cix[c]] = x[i][c];
-                }
-            }
-        }
-        if (MODE == PCA_DEFLATE) continue;
-        float t[RB];
-        if (MODE == PCA_START) {
-            // t = R[:,k] of the (deflated) residual, from the thread that owns column k
-            if (tid == cowner) {
-#pragma unroll
-                for (int i = 0; i < RB; ++i) {
-                    float v = 0.f;
-#pragma unroll
-                    for (int c = 0; c < NCH; ++c)
-                        if (c == creg) v = f4get(x[i][c], ce);
-                    sRed[buf][0][i] = v;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < RB; ++i) t[i] = sRed[buf][0][i];
-        } else {
+        if (rowdot) {
             float v[RB];
 #pragma unroll
             for (int i = 0; i < RB; ++i) {
                 v[i] = 0.f;
 #pragma unroll
-                for (int c = 0; c < NCH; ++c) v[i] += f4dot(x[i][c], pv[c]);
+                for (int c = 0; c < NCH; ++c) v[i] += f4dot(x[i][c], wv4[c]);
 #pragma unroll
                 for (int off = 32; off >= 1; off >>= 1) v[i] += __shfl_xor(v[i], off, 64);
             }
@@ -167,34 +145,66 @@ __device__ __forceinline__ void pca_row_pass(float *__restrict__ R, float *__res
 #pragma unroll
                 for (int i = 0; i < RB; ++i) sRed[buf][wv][i] = v[i];
             }
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < RB; ++i) t[i] = sRed[buf][0][i] + sRed[buf][1][i] + sRed[buf][2][i] + sRed[buf][3][i];
         }
-        buf ^= 1;
-        if (MODE == PCA_PROJECT) {
+        if (MODE == PCA_START && tid == cowner) {
+            // column k of the rows (before any deflation), from the thread that owns it
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                float xv = 0.f;
+#pragma unroll
+                for (int c = 0; c < NCH; ++c)
+                    if (c == creg) xv = f4get(x[i][c], ce);
+                sCol[buf][i] = xv;
+            }
+        }
+        __syncthreads();
+        float red[RB];
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+            red[i] = rowdot ? sRed[buf][0][i] + sRed[buf][1][i] + sRed[buf][2][i] + sRed[buf][3][i] : 0.f;
+        if (deflate) {
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                const float a = red[i];  // t of row r0 + i
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) {
+                    if (!cv[c]) continue;
+                    x[i][c].x = fmaf(-a, wv4[c].x, x[i][c].x);
+                    x[i][c].y = fmaf(-a, wv4[c].y, x[i][c].y);
+                    x[i][c].z = fmaf(-a, wv4[c].z, x[i][c].z);
+                    x[i][c].w = fmaf(-a, wv4[c].w, x[i][c].w);
+                    ((float4 *)(R + (r0 + i) * ld))[cix[c]] = x[i][c];
+                }
+            }
             if (tid == 0) {
 #pragma unroll
                 for (int i = 0; i < RB; ++i) {
-                    T[(long long)k * Mp + r0 + i] = t[i];
-                    tsq += (double)t[i] * (double)t[i];
+                    T[(long long)dk * Mp + r0 + i] = red[i];
+                    tsq += (double)red[i] * (double)red[i];
                 }
             }
-        } else {
-#pragma unroll
-            for (int c = 0; c < NCH; ++c)
-#pragma unroll
-                for (int i = 0; i < RB; ++i) {
-                    acc[c].x = fmaf(x[i][c].x, t[i], acc[c].x);
-                    acc[c].y = fmaf(x[i][c].y, t[i], acc[c].y);
-                    acc[c].z = fmaf(x[i][c].z, t[i], acc[c].z);
-                    acc[c].w = fmaf(x[i][c].w, t[i], acc[c].w);
-                }
         }
+        if (MODE == PCA_DEFLATE) {
+            buf ^= 1;
+            continue;
+        }
+        float t[RB];  // Start: column k of the (deflated) residual; Gram: R_slab p_k
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+            t[i] = MODE == PCA_GRAM ? red[i] : (deflate ? fmaf(-red[i], pk, sCol[buf][i]) : sCol[buf][i]);
+        buf ^= 1;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                acc[c].x = fmaf(x[i][c].x, t[i], acc[c].x);
+                acc[c].y = fmaf(x[i][c].y, t[i], acc[c].y);
+                acc[c].z = fmaf(x[i][c].z, t[i], acc[c].z);
+                acc[c].w = fmaf(x[i][c].w, t[i], acc[c].w);
+            }
     }
-    if (MODE == PCA_PROJECT) {
-        if (tid == 0) pca_tn(S, d)[blockIdx.x] = tsq;
-    } else if (MODE != PCA_DEFLATE) {
+    if (deflate && tid == 0) pca_tn(S, d)[blockIdx.x] = tsq;
+    if (MODE != PCA_DEFLATE) {
         float *part = pca_part(S, d) + (long long)blockIdx.x * ld;
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
@@ -224,13 +234,8 @@ PcaGram(float *__restrict__ R, float *__restrict__ S, PCA_DIMS d) {
     pca_row_dispatch<PCA_GRAM>(R, nullptr, S, d);
 }
 
-extern "C" __global__ void __launch_bounds__(256)
-PcaProject(float *__restrict__ R, float *__restrict__ T, float *__restrict__ S, PCA_DIMS d) {
-    if (!pca_ok(d) || blockDim.x != 256 || (int)gridDim.x != d.G) return;
-    pca_row_dispatch<PCA_PROJECT>(R, T, S, d);
-}
-
-// After the last component: R -= L_k t_k p_k^T for k = d.k (pca_cpu.cpp:401 of the final k).
+// After the last component: t_k = R p_k into T[:, k] and the tn partials, R -= t_k p_k^T for
+// k = d.k (pca_cpu.cpp:380-401 of the final k).
 extern "C" __global__ void __launch_bounds__(256)
 PcaDeflate(float *__restrict__ R, float *__restrict__ T, float *__restrict__ S, PCA_DIMS d) {
     if (!pca_ok(d) || blockDim.x != 256 || (int)gridDim.x != d.G) return;
@@ -384,12 +389,12 @@ PcaOrthonormalize(float *__restrict__ S, PCA_DIMS d) {
     else pca_orthonormalize<4>(S, d, sW, sU);
 }
 
-// ---- PcaFinish: L_k = sqrt(sum of tn) (pca_cpu.cpp:385), T[:,k] *= 1/L_k (:386).  Grid
-// ceil(Mp/256) blocks of 256; each block sums tn in the same fixed order.
+// ---- PcaFinish: L_k = sqrt(sum of the G tn partials) (pca_cpu.cpp:385), one block of 256
+// summing in a fixed order.
 extern "C" __global__ void __launch_bounds__(256)
-PcaFinish(float *__restrict__ T, float *__restrict__ S, PCA_DIMS d) {
+PcaFinish(float *__restrict__ S, PCA_DIMS d) {
     __shared__ double sh[256];
-    if (!pca_ok(d) || blockDim.x != 256 || (int)gridDim.x != (d.Mp + 255) / 256) return;
+    if (!pca_ok(d) || blockDim.x != 256 || gridDim.x != 1) return;
     const int tid = threadIdx.x;
     const double *tn = pca_tn(S, d);
     double s = 0;
@@ -400,22 +405,5 @@ PcaFinish(float *__restrict__ T, float *__restrict__ S, PCA_DIMS d) {
         if (tid < w) sh[tid] += sh[tid + w];
         __syncthreads();
     }
-    const double Lk = sqrt(sh[0]);
-    const long long m = (long long)blockIdx.x * 256 + tid;
-    if (m < d.Mp) {
-        float *t = T + (long long)d.k * d.Mp;
-        t[m] = (float)((double)t[m] * (1.0 / Lk));
-    }
-    if (blockIdx.x == 0 && tid == 0) pca_L(S, d)[d.k] = Lk;
+    if (tid == 0) pca_L(S, d)[d.k] = sqrt(sh[0]);
 }
-
-// ---- PcaScaleT: T[:,k] *= L_k for every component (pca_cpu.cpp:404-408).
-extern "C" __global__ void __launch_bounds__(256)
-PcaScaleT(float *__restrict__ T, float *__restrict__ S, PCA_DIMS d) {
-    if (!pca_ok(d) || blockDim.x != 256) return;
-    const long long n = (long long)d.K * d.Mp;
-    const double *L = pca_L(S, d);
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-        T[i] = (float)((double)T[i] * L[i / d.Mp]);
-}
-#include "pca_blas.h"

@@ -68,7 +68,7 @@ def test_host_graph_equals_oracle(tmp_path, M, N, K, J, center):
     extra = [] if center else ["--no-center"]
     js, X, T, P, R, L = run_pca(HOSTLIB, tmp_path, M, N, K, J, *extra, "--dist", "uniform")
     assert js["ok"] and js["backend"] == "host"
-    assert js["tasks"] == 3 + K * (3 * J + 2)
+    assert js["tasks"] == 2 + K * (3 * J + 1)
     compare(X, T, P, R, L, K, J, center, HOST_TOL)
 
 
