@@ -407,3 +407,4 @@ PcaFinish(float *__restrict__ S, PCA_DIMS d) {
     }
     if (tid == 0) pca_L(S, d)[d.k] = sqrt(sh[0]);
 }
+#include "pca_blas.h"
