@@ -55,28 +55,58 @@ PcaLoad(const float *__restrict__ X, PCA_DIMS d, float *__restrict__ R, float *_
     const int tid = threadIdx.x, M = d.M, N = d.N, ld = d.ld;
     const long long r0 = (long long)blockIdx.x * 64;
     {
+        // row sums: eight columns' loads issued together (row and column indexes clamped in
+        // range), added in column order
         const int r = tid & 63, grp = tid >> 6;
+        const long long m = r0 + r < M ? r0 + r : M - 1;
         double s = 0;
         if (d.center && r0 + r < M)
-            for (int n = grp; n < N; n += 4) s += (double)X[(long long)n * M + r0 + r];
+            for (int n0 = grp; n0 < N; n0 += 32) {
+                float v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int n = n0 + 4 * u < N ? n0 + 4 * u : N - 1;
+                    v[u] = X[(long long)n * M + m];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (n0 + 4 * u < N) s += (double)v[u];
+            }
         sSum[grp][r] = s;
     }
     __syncthreads();
     if (tid < 64) sMean[tid] = d.center ? (sSum[0][tid] + sSum[1][tid] + sSum[2][tid] + sSum[3][tid]) / N : 0.0;
     __syncthreads();
+    // 64 x 64 tiles through LDS; the next tile's sixteen loads per thread are issued before the
+    // current tile is transposed and stored
+    const int r = tid & 63, c0 = tid >> 6;
+    const long long m = r0 + r < M ? r0 + r : M - 1;
+    const bool mv = r0 + r < M;
+    float cur[16], nxt[16];
+    auto load = [&](float (&v)[16], int n0) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int n = n0 + c0 + 4 * j;
+            v[j] = X[(long long)(n < N ? n : N - 1) * M + m];
+        }
+    };
+    load(cur, 0);
     for (int n0 = 0; n0 < ld; n0 += 64) {
-        for (int i = tid; i < 64 * 64; i += 256) {
-            const int c = i >> 6, r = i & 63;
-            const int n = n0 + c;
-            const long long m = r0 + r;
-            tile[c][r] = (n < N && m < M) ? (float)((double)X[(long long)n * M + m] - sMean[r]) : 0.f;
+        if (n0 + 64 < ld) load(nxt, n0 + 64);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int c = c0 + 4 * j;
+            tile[c][r] = (n0 + c < N && mv) ? (float)((double)cur[j] - sMean[r]) : 0.f;
         }
         __syncthreads();
         for (int i = tid; i < 64 * 64; i += 256) {
-            const int r = i >> 6, c = i & 63;
-            if (n0 + c < ld) R[(r0 + r) * ld + n0 + c] = tile[c][r];
+            const int rr = i >> 6, c = i & 63;
+            if (n0 + c < ld) R[(r0 + rr) * ld + n0 + c] = tile[c][rr];
         }
         __syncthreads();
+        if (n0 + 64 < ld)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) cur[j] = nxt[j];
     }
     const long long nT = (long long)d.K * d.Mp, nS = ptk_pca_state_bytes(&d) / 4;
     const long long step = (long long)gridDim.x * 256;
